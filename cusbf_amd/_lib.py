@@ -1,0 +1,103 @@
+"""ctypes binding of libcusbf_hip.so (the C ABI declared in include/cusbf_hip.h).
+
+The library is built in-tree by cusbf_amd.build. Loading fails loudly when it
+is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcusbf_hip.so")
+
+P = ctypes.c_void_p
+u8 = ctypes.c_uint8
+u32 = ctypes.c_uint32
+u64 = ctypes.c_uint64
+i32 = ctypes.c_int
+sz = ctypes.c_size_t
+
+
+class CConfig(ctypes.Structure):
+    _fields_ = [
+        ("k", u32),
+        ("s", u32),
+        ("m", u32),
+        ("hash_count", u32),
+        ("alphabet", u32),
+        ("symbol_count", u32),
+        ("invalid_symbol", u8),
+        ("separator", u8),
+        ("reserved", u8 * 2),
+        ("lut", u8 * 256),
+    ]
+
+
+class CRecordRange(ctypes.Structure):
+    _fields_ = [("sequence_offset", u64), ("sequence_bytes", u64)]
+
+
+class CInsertReport(ctypes.Structure):
+    _fields_ = [("records_indexed", u64), ("indexed_bases", u64), ("inserted_kmers", u64)]
+
+
+class CQueryReport(ctypes.Structure):
+    _fields_ = [("records_queried", u64), ("queried_bases", u64), ("queried_kmers", u64),
+                ("positive_kmers", u64)]
+
+
+# (name, restype, argtypes) of every exported entry point of include/cusbf_hip.h.
+CFG = ctypes.POINTER(CConfig)
+RR = ctypes.POINTER(CRecordRange)
+SIGNATURES = [
+    ("cusbf_last_error", i32, [ctypes.c_char_p, sz]),
+    ("cusbf_abi_version", i32, []),
+    ("cusbf_config_init", i32, [CFG, u32, u32, u32, u32, u32]),
+    ("cusbf_config_symbol_width", i32, [CFG, ctypes.POINTER(u32)]),
+    ("cusbf_config_symbol_bits", i32, [CFG, ctypes.POINTER(u32)]),
+    ("cusbf_create", i32, [CFG, u64, ctypes.POINTER(P)]),
+    ("cusbf_destroy", None, [P]),
+    ("cusbf_clear", i32, [P, P]),
+    ("cusbf_load_factor", i32, [P, ctypes.POINTER(ctypes.c_float)]),
+    ("cusbf_filter_bits", u64, [P]),
+    ("cusbf_num_shards", u64, [P]),
+    ("cusbf_words_device_ptr", i32, [P, ctypes.POINTER(P)]),
+    ("cusbf_copy_words_to_host", i32, [P, P, u64]),
+    ("cusbf_insert_sequence_async", i32, [P, P, u64, P, ctypes.POINTER(u64)]),
+    ("cusbf_insert_sequence", i32, [P, P, u64, P, ctypes.POINTER(u64)]),
+    ("cusbf_contains_sequence_async", i32, [P, P, u64, P, u64, P]),
+    ("cusbf_contains_sequence", i32, [P, P, u64, P, u64, P]),
+    ("cusbf_dense_packed_word_count", u64, [CFG, u64]),
+    ("cusbf_dense_packed_kmer_count", u64, [CFG, u64]),
+    ("cusbf_pack_dense_sequence_async", i32, [CFG, P, u64, P, u64, P]),
+    ("cusbf_insert_dense_packed_async", i32, [P, P, u64, u64, P, ctypes.POINTER(u64)]),
+    ("cusbf_contains_dense_packed_async", i32, [P, P, u64, u64, P, u64, P]),
+    ("cusbf_insert_record_batch_async", i32, [P, P, u64, RR, u64, P, ctypes.POINTER(u64)]),
+    ("cusbf_contains_record_batch_async", i32, [P, P, u64, RR, u64, P, u64, P]),
+    ("cusbf_insert_record_batch", i32, [P, P, u64, RR, u64, P, ctypes.POINTER(CInsertReport)]),
+    ("cusbf_query_record_batch", i32, [P, P, u64, RR, u64, P, ctypes.POINTER(CQueryReport)]),
+    ("cusbf_count_positive_async", i32, [P, u64, P, P]),
+    ("cusbf_count_positive_per_record_async", i32, [P, P, P, u64, P, P]),
+    ("cusbf_count_windows_runs_async", i32, [P, P, u64, P, P]),
+    ("cusbf_kernel_family", ctypes.c_char_p, [P]),
+]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libcusbf_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build the HIP library first (python -m cusbf_amd.build)")
+    lib = ctypes.CDLL(path)
+    for name, restype, argtypes in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib

@@ -1,0 +1,78 @@
+"""Builds libcusbf_hip.so (HIP kernels + C ABI) in-tree for gfx950 with hipcc.
+
+Usage: python -m cusbf_amd.build  (or cusbf_amd.build.build()).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB_PATH = os.path.join(HERE, "libcusbf_hip.so")
+SYNTH_LIB_PATH = os.path.join(HERE, "libcusbf_synth.so")
+
+HIP_SOURCES = ["kernels_dna.hip", "kernels_generic.hip", "capi.hip"]
+SYNTH_SOURCES = ["synth.hip"]
+
+# -packed-fp32-ops: the kernels are integer code, and packed FP32 forms stay off.
+HIPCC_FLAGS = [
+    "-O3",
+    "-std=c++17",
+    "--offload-arch=gfx950",
+    "-fPIC",
+    "-Wall",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP library cannot be built")
+
+
+def _compile(src: str, obj: str) -> None:
+    cmd = [hipcc(), *HIPCC_FLAGS, "-c", src, "-o", obj]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr[-6000:]}")
+
+
+def _link(objs, path: str) -> None:
+    tmp = path + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed:\n{res.stderr[-4000:]}")
+    os.replace(tmp, path)
+
+
+def build(verbose: bool = False, jobs: int = 4) -> str:
+    """Compile every HIP source (always) and link libcusbf_hip.so and libcusbf_synth.so."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    out_dir = os.path.join(HERE, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+
+    def objs_of(names):
+        return [os.path.join(out_dir, n.replace(".hip", ".o")) for n in names]
+
+    work = [(os.path.join(CSRC, n), o)
+            for n, o in zip(HIP_SOURCES + SYNTH_SOURCES, objs_of(HIP_SOURCES + SYNTH_SOURCES))]
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for fut in [ex.submit(_compile, s, o) for s, o in work]:
+            fut.result()
+    _link(objs_of(HIP_SOURCES), LIB_PATH)
+    _link(objs_of(SYNTH_SOURCES), SYNTH_LIB_PATH)
+    if verbose:
+        print(f"built {LIB_PATH} and {SYNTH_LIB_PATH}")
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

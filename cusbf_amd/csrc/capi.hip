@@ -1,0 +1,658 @@
+// capi.hip -- extern "C" boundary of libcusbf_hip.so (declared in include/cusbf_hip.h).
+//
+// Mirrors the hot-path methods of the reference's cusbf::filter<Config>
+// (include/cusbf/filter.cuh:68-1848): same counting conventions, same argument
+// checks and error strings, Result<T> categories mapped onto status codes.
+// Kernel dispatch: DNA configurations with a compiled fast kernel set use the
+// wave64 lane-strip kernels (kernels_dna.hip), everything else the generic
+// kernels (kernels_generic.hip). There is no CPU fallback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cusbf_hip.h"
+#include "launch.hpp"
+#include "sbf_common.hpp"
+
+namespace {
+
+thread_local int g_last_status = CUSBF_OK;
+thread_local std::string g_last_message;
+
+int fail(int status, std::string message) {
+    g_last_status = status;
+    g_last_message = std::move(message);
+    return status;
+}
+
+// Error::cuda (error.hpp:109-117): "<site>: <runtime error string>".
+int fail_hip(hipError_t err, const char* file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s:%d: %s", file, line, hipGetErrorString(err));
+    return fail(CUSBF_ERR_DEVICE, buf);
+}
+
+#define CUSBF_HIP_TRY(expr)                                               \
+    do {                                                                  \
+        const hipError_t _e = (expr);                                     \
+        if (_e != hipSuccess) return fail_hip(_e, __FILE__, __LINE__);    \
+    } while (0)
+
+uint32_t bit_width_u32(uint32_t x) {
+    uint32_t w = 0;
+    while (x) {
+        ++w;
+        x >>= 1;
+    }
+    return w;
+}
+
+struct Derived {
+    uint32_t symbol_width;
+    uint32_t symbol_count;
+    uint32_t symbol_bits;
+    uint8_t invalid_symbol;
+    uint8_t separator;
+};
+
+// Alphabet.cuh:113-218 constants; config.cuh:51 symbolBits.
+int derive(const cusbf_config* cfg, Derived* d) {
+    switch (cfg->alphabet) {
+        case CUSBF_ALPHABET_DNA:
+            *d = Derived{1, 4, 0, 0xFF, 'N'};
+            break;
+        case CUSBF_ALPHABET_DNA_TRIPLET:
+            *d = Derived{3, 64, 0, 0xFF, 'N'};
+            break;
+        case CUSBF_ALPHABET_PROTEIN:
+            *d = Derived{1, 26, 0, 0xFF, '*'};
+            break;
+        case CUSBF_ALPHABET_CUSTOM:
+            if (cfg->symbol_count == 0 || cfg->symbol_count > 255) {
+                return fail(CUSBF_ERR_INVALID_ARGUMENT, "alphabet symbolCount must be in [1, 255]");
+            }
+            *d = Derived{1, cfg->symbol_count, 0, cfg->invalid_symbol, cfg->separator};
+            break;
+        default:
+            return fail(CUSBF_ERR_INVALID_ARGUMENT, "unknown alphabet");
+    }
+    d->symbol_bits = bit_width_u32(d->symbol_count - 1);
+    return CUSBF_OK;
+}
+
+// config.cuh:74-117, as runtime checks with the static_assert messages.
+int validate(const cusbf_config* cfg, const Derived& d) {
+    if (cfg->k == 0) return fail(CUSBF_ERR_INVALID_ARGUMENT, "k must be positive");
+    if (cfg->m == 0 || cfg->m > cfg->k) return fail(CUSBF_ERR_INVALID_ARGUMENT, "m must satisfy 0 < m <= k");
+    if (cfg->s == 0 || cfg->s > cfg->k) return fail(CUSBF_ERR_INVALID_ARGUMENT, "s must satisfy 0 < s <= k");
+    if (d.symbol_bits == 0) return fail(CUSBF_ERR_INVALID_ARGUMENT, "alphabet needs at least two symbols");
+    if (cfg->k * d.symbol_bits > 64) return fail(CUSBF_ERR_INVALID_ARGUMENT, "k-mer must fit in one packed uint64_t");
+    if (cfg->hash_count == 0) return fail(CUSBF_ERR_INVALID_ARGUMENT, "At least one Bloom hash is required");
+    if (cfg->hash_count > 16) return fail(CUSBF_ERR_INVALID_ARGUMENT, "This implementation provides 16 multiplicative salts");
+    if (cfg->hash_count < 4) return fail(CUSBF_ERR_INVALID_ARGUMENT, "Sectorized layout requires hashCount >= blockWordCount");
+    if (cfg->hash_count % 4 != 0) return fail(CUSBF_ERR_INVALID_ARGUMENT, "Hash count must distribute evenly across shard words");
+    return CUSBF_OK;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+
+struct cusbf_filter {
+    cusbf_config cfg{};
+    Derived d{};
+    sbf::KernelConfig kc{};
+    const sbf::DnaKernelSet* dna = nullptr;
+    uint64_t num_shards = 0;
+    uint64_t filter_bits = 0;
+    uint64_t* d_words = nullptr;
+    int device = 0;
+    // grow-only staging (fastx_pipeline_state.cuh:145-201)
+    char* d_stage = nullptr;
+    uint64_t stage_cap = 0;
+    uint8_t* d_result = nullptr;
+    uint64_t result_cap = 0;
+    uint64_t* d_scalar = nullptr;  // reductions
+};
+
+namespace {
+
+uint64_t symbols_of(const cusbf_filter* f, uint64_t bytes) { return bytes / f->d.symbol_width; }
+
+uint64_t windows_of(const cusbf_filter* f, uint64_t bytes) {
+    const uint64_t sym = symbols_of(f, bytes);
+    return sym < f->cfg.k ? 0 : sym - f->cfg.k + 1;
+}
+
+int ensure_stage(cusbf_filter* f, uint64_t bytes) {
+    if (bytes <= f->stage_cap) return CUSBF_OK;
+    if (f->d_stage) CUSBF_HIP_TRY(hipFree(f->d_stage));
+    f->d_stage = nullptr;
+    f->stage_cap = 0;
+    const hipError_t e = hipMalloc(&f->d_stage, bytes);
+    if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+    f->stage_cap = bytes;
+    return CUSBF_OK;
+}
+
+int ensure_result(cusbf_filter* f, uint64_t bytes) {
+    if (bytes <= f->result_cap) return CUSBF_OK;
+    if (f->d_result) CUSBF_HIP_TRY(hipFree(f->d_result));
+    f->d_result = nullptr;
+    f->result_cap = 0;
+    const hipError_t e = hipMalloc(&f->d_result, bytes);
+    if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+    f->result_cap = bytes;
+    return CUSBF_OK;
+}
+
+// fastx_pipeline_state.cuh:230-234 (release_all).
+void release_staging(cusbf_filter* f) {
+    if (f->d_stage) (void)hipFree(f->d_stage);
+    if (f->d_result) (void)hipFree(f->d_result);
+    f->d_stage = nullptr;
+    f->d_result = nullptr;
+    f->stage_cap = f->result_cap = 0;
+}
+
+sbf::FilterView view(const cusbf_filter* f) { return sbf::FilterView{f->d_words, f->num_shards}; }
+
+int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
+    const hipError_t e = f->dna ? f->dna->insert(in, view(f), stream)
+                                : sbf::generic_insert(f->kc, in, view(f), stream);
+    if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+    return CUSBF_OK;
+}
+
+int launch_contains(const cusbf_filter* f, const sbf::SequenceInput& in, uint8_t* out,
+                    hipStream_t stream) {
+    const hipError_t e = f->dna ? f->dna->contains(in, view(f), out, stream)
+                                : sbf::generic_contains(f->kc, in, view(f), out, stream);
+    if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+    return CUSBF_OK;
+}
+
+sbf::SequenceInput byte_input(const cusbf_filter* f, const char* seq, uint64_t bytes) {
+    sbf::SequenceInput in;
+    in.seq = seq;
+    in.num_symbols = symbols_of(f, bytes);
+    return in;
+}
+
+// filter.cuh:429-457 (validate_device_record_batch) / normalized_record_batch.hpp:123-147.
+int validate_ranges(const cusbf_filter* f, uint64_t bytes, const cusbf_record_range* ranges,
+                    uint64_t n, const char* exceeds_message) {
+    if (n != 0 && ranges == nullptr) return fail(CUSBF_ERR_INVALID_ARGUMENT, "record ranges pointer is null");
+    uint64_t next_offset = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const cusbf_record_range& r = ranges[i];
+        if (r.sequence_offset < next_offset) {
+            return fail(CUSBF_ERR_INVALID_ARGUMENT, "record batch ranges must be ordered and non-overlapping");
+        }
+        if (r.sequence_offset > bytes || r.sequence_bytes > bytes - r.sequence_offset) {
+            return fail(CUSBF_ERR_INVALID_ARGUMENT, exceeds_message);
+        }
+        if (r.sequence_offset % f->d.symbol_width != 0 || r.sequence_bytes % f->d.symbol_width != 0) {
+            return fail(CUSBF_ERR_INVALID_ARGUMENT,
+                        "record batch ranges must align to the configured alphabet symbol width");
+        }
+        next_offset = r.sequence_offset + r.sequence_bytes;
+    }
+    return CUSBF_OK;
+}
+
+struct NormRecord {
+    uint64_t size;
+    uint64_t valid_kmers;
+};
+
+uint64_t record_kmers(const cusbf_filter* f, uint64_t bytes) { return windows_of(f, bytes); }
+
+// normalized_record_batch.hpp:87-110 (valid_record_kmer_count).
+uint64_t valid_record_kmers(const cusbf_filter* f, const char* seq, uint64_t bytes) {
+    const uint64_t sym = symbols_of(f, bytes);
+    if (sym < f->cfg.k) return 0;
+    const uint8_t inv = f->d.invalid_symbol;
+    const uint32_t w = f->d.symbol_width;
+    uint64_t invalid = 0;
+    for (uint64_t i = 0; i < f->cfg.k; ++i) invalid += sbf::encode_symbol(f->kc, seq + i * w) == inv;
+    uint64_t valid = invalid == 0;
+    const uint64_t windows = sym - f->cfg.k + 1;
+    for (uint64_t start = 1; start < windows; ++start) {
+        invalid -= sbf::encode_symbol(f->kc, seq + (start - 1) * w) == inv;
+        invalid += sbf::encode_symbol(f->kc, seq + (start + f->cfg.k - 1) * w) == inv;
+        valid += invalid == 0;
+    }
+    return valid;
+}
+
+// normalized_record_batch.hpp:208-276: records joined by the alphabet
+// separator, padded to the symbol width.
+void normalize(const cusbf_filter* f, const char* seq, const cusbf_record_range* ranges, uint64_t n,
+               std::vector<char>& out, std::vector<NormRecord>& records) {
+    const uint32_t w = f->d.symbol_width;
+    const char sep = static_cast<char>(f->d.separator);
+    out.clear();
+    records.clear();
+    records.reserve(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!out.empty()) {
+            const uint64_t rem = out.size() % w;
+            if (rem != 0) out.insert(out.end(), w - rem, sep);
+            out.insert(out.end(), w, sep);
+        }
+        const char* rec = seq + ranges[i].sequence_offset;
+        out.insert(out.end(), rec, rec + ranges[i].sequence_bytes);
+        records.push_back(NormRecord{ranges[i].sequence_bytes, valid_record_kmers(f, rec, ranges[i].sequence_bytes)});
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cusbf_abi_version(void) { return CUSBF_ABI_VERSION; }
+
+int cusbf_last_error(char* buf, size_t buf_len) {
+    if (buf && buf_len > 0) {
+        const size_t n = std::min(buf_len - 1, g_last_message.size());
+        std::memcpy(buf, g_last_message.data(), n);
+        buf[n] = '\0';
+    }
+    return g_last_status;
+}
+
+int cusbf_config_init(cusbf_config* cfg, uint32_t k, uint32_t s, uint32_t m, uint32_t hash_count,
+                      uint32_t alphabet) {
+    if (!cfg) return fail(CUSBF_ERR_INVALID_ARGUMENT, "config pointer is null");
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->k = k;
+    cfg->s = s;
+    cfg->m = m;
+    cfg->hash_count = hash_count;
+    cfg->alphabet = alphabet;
+    if (alphabet == CUSBF_ALPHABET_CUSTOM) return fail(CUSBF_ERR_INVALID_ARGUMENT, "custom alphabets need a table");
+    Derived d;
+    int rc = derive(cfg, &d);
+    if (rc != CUSBF_OK) return rc;
+    cfg->symbol_count = d.symbol_count;
+    cfg->invalid_symbol = d.invalid_symbol;
+    cfg->separator = d.separator;
+    return validate(cfg, d);
+}
+
+int cusbf_config_symbol_width(const cusbf_config* cfg, uint32_t* out) {
+    Derived d;
+    const int rc = derive(cfg, &d);
+    if (rc == CUSBF_OK) *out = d.symbol_width;
+    return rc;
+}
+
+int cusbf_config_symbol_bits(const cusbf_config* cfg, uint32_t* out) {
+    Derived d;
+    const int rc = derive(cfg, &d);
+    if (rc == CUSBF_OK) *out = d.symbol_bits;
+    return rc;
+}
+
+int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_filter** out) {
+    if (!cfg || !out) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    Derived d;
+    int rc = derive(cfg, &d);
+    if (rc != CUSBF_OK) return rc;
+    rc = validate(cfg, d);
+    if (rc != CUSBF_OK) return rc;
+
+    cusbf_filter* f = new cusbf_filter();
+    f->cfg = *cfg;
+    f->d = d;
+    f->kc.k = cfg->k;
+    f->kc.s = cfg->s;
+    f->kc.m = cfg->m;
+    f->kc.h = cfg->hash_count;
+    f->kc.alphabet = cfg->alphabet;
+    f->kc.symbol_width = d.symbol_width;
+    f->kc.symbol_bits = d.symbol_bits;
+    f->kc.invalid_symbol = d.invalid_symbol;
+    std::memcpy(f->kc.lut, cfg->lut, 256);
+    const char* force_generic = std::getenv("CUSBF_FORCE_GENERIC");
+    if (cfg->alphabet == CUSBF_ALPHABET_DNA && !(force_generic && force_generic[0] == '1')) {
+        f->dna = sbf::find_dna_kernels(cfg->k, cfg->s, cfg->m, cfg->hash_count);
+    }
+    // filter.cuh:108-117
+    uint64_t shards = requested_filter_bits / 256 + (requested_filter_bits % 256 != 0);
+    if (shards < 1) shards = 1;
+    uint64_t p = 1;
+    while (p < shards) p <<= 1;
+    f->num_shards = p;
+    f->filter_bits = p * 256;
+    hipError_t e = hipGetDevice(&f->device);
+    if (e == hipSuccess) e = hipMalloc(&f->d_words, f->num_shards * 32);
+    if (e == hipSuccess) e = hipMalloc(&f->d_scalar, 4 * sizeof(uint64_t));
+    if (e != hipSuccess) {
+        const int st = fail_hip(e, __FILE__, __LINE__);
+        cusbf_destroy(f);
+        return st;
+    }
+    rc = cusbf_clear(f, nullptr);
+    if (rc != CUSBF_OK) {
+        cusbf_destroy(f);
+        return rc;
+    }
+    *out = f;
+    return CUSBF_OK;
+}
+
+void cusbf_destroy(cusbf_filter* f) {
+    if (!f) return;
+    release_staging(f);
+    if (f->d_words) (void)hipFree(f->d_words);
+    if (f->d_scalar) (void)hipFree(f->d_scalar);
+    delete f;
+}
+
+// filter.cuh:793-803. A fill kernel rather than a memset node, so the clear
+// can also sit inside captured graphs.
+int cusbf_clear(cusbf_filter* f, void* stream) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    CUSBF_HIP_TRY(sbf::fill_u64(f->d_words, 0, f->num_shards * 4, as_stream(stream)));
+    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return CUSBF_OK;
+}
+
+// filter.cuh:810-823
+int cusbf_load_factor(const cusbf_filter* f, float* out) {
+    if (!f || !out) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    CUSBF_HIP_TRY(sbf::fill_u64(f->d_scalar, 0, 1, nullptr));
+    CUSBF_HIP_TRY(sbf::popcount_words(f->d_words, f->num_shards * 4, f->d_scalar, nullptr));
+    uint64_t set_bits = 0;
+    CUSBF_HIP_TRY(hipMemcpy(&set_bits, f->d_scalar, sizeof(set_bits), hipMemcpyDeviceToHost));
+    *out = static_cast<float>(set_bits) / static_cast<float>(f->filter_bits);
+    return CUSBF_OK;
+}
+
+uint64_t cusbf_filter_bits(const cusbf_filter* f) { return f ? f->filter_bits : 0; }
+uint64_t cusbf_num_shards(const cusbf_filter* f) { return f ? f->num_shards : 0; }
+
+int cusbf_words_device_ptr(const cusbf_filter* f, uint64_t** out) {
+    if (!f || !out) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    *out = f->d_words;
+    return CUSBF_OK;
+}
+
+int cusbf_copy_words_to_host(const cusbf_filter* f, uint64_t* host_words, uint64_t num_words) {
+    if (!f || (!host_words && num_words)) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_words < f->num_shards * 4) return fail(CUSBF_ERR_INVALID_ARGUMENT, "host word buffer is too small");
+    CUSBF_HIP_TRY(hipDeviceSynchronize());
+    CUSBF_HIP_TRY(hipMemcpy(host_words, f->d_words, f->num_shards * 32, hipMemcpyDeviceToHost));
+    return CUSBF_OK;
+}
+
+const char* cusbf_kernel_family(const cusbf_filter* f) {
+    if (!f) return "";
+    return f->dna ? "dna_fast" : "generic";
+}
+
+// filter.cuh:179-190
+int cusbf_insert_sequence_async(cusbf_filter* f, const char* d_sequence, uint64_t bytes,
+                                void* stream, uint64_t* inserted) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    const uint64_t total = windows_of(f, bytes);
+    if (inserted) *inserted = total;
+    if (total == 0) return CUSBF_OK;
+    if (!d_sequence) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device sequence");
+    return launch_insert(f, byte_input(f, d_sequence, bytes), as_stream(stream));
+}
+
+// filter.cuh:154-167
+int cusbf_insert_sequence(cusbf_filter* f, const char* h_sequence, uint64_t bytes, void* stream,
+                          uint64_t* inserted) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (inserted) *inserted = 0;
+    if (symbols_of(f, bytes) < f->cfg.k) return CUSBF_OK;
+    int rc = ensure_stage(f, bytes);
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, h_sequence, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    rc = launch_insert(f, byte_input(f, f->d_stage, bytes), as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    if (inserted) *inserted = windows_of(f, bytes);
+    return CUSBF_OK;
+}
+
+// filter.cuh:464-474 (no output-size check, as in the reference)
+int cusbf_contains_sequence_async(const cusbf_filter* f, const char* d_sequence, uint64_t bytes,
+                                  uint8_t* d_output, uint64_t d_output_len, void* stream) {
+    (void)d_output_len;
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (windows_of(f, bytes) == 0) return CUSBF_OK;
+    if (!d_sequence || !d_output) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device buffer");
+    return launch_contains(f, byte_input(f, d_sequence, bytes), d_output, as_stream(stream));
+}
+
+// filter.cuh:514-533
+int cusbf_contains_sequence(const cusbf_filter* f_const, const char* h_sequence, uint64_t bytes,
+                            uint8_t* h_output, uint64_t h_output_len, void* stream) {
+    cusbf_filter* f = const_cast<cusbf_filter*>(f_const);  // staging is mutable state
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    const uint64_t n = windows_of(f, bytes);
+    if (n == 0) return CUSBF_OK;
+    if (h_output_len < n) return fail(CUSBF_ERR_INVALID_ARGUMENT, "host output buffer is too small");
+    int rc = ensure_stage(f, bytes);
+    if (rc == CUSBF_OK) rc = ensure_result(f, n);
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, h_sequence, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    rc = launch_contains(f, byte_input(f, f->d_stage, bytes), f->d_result, as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipMemcpyAsync(h_output, f->d_result, n, hipMemcpyDeviceToHost, as_stream(stream)));
+    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return CUSBF_OK;
+}
+
+uint64_t cusbf_dense_packed_word_count(const cusbf_config* cfg, uint64_t num_symbols) {
+    Derived d;
+    if (!cfg || derive(cfg, &d) != CUSBF_OK || d.symbol_bits == 0) return 0;
+    const uint64_t per_word = 64u / d.symbol_bits;
+    return (num_symbols + per_word - 1) / per_word;
+}
+
+uint64_t cusbf_dense_packed_kmer_count(const cusbf_config* cfg, uint64_t num_symbols) {
+    if (!cfg) return 0;
+    return num_symbols < cfg->k ? 0 : num_symbols - cfg->k + 1;
+}
+
+// dense_packed.hpp:57-73
+int cusbf_pack_dense_sequence_async(const cusbf_config* cfg, const char* d_sequence,
+                                    uint64_t num_symbols, uint64_t* d_words, uint64_t num_words,
+                                    void* stream) {
+    if (!cfg) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null config");
+    Derived d;
+    int rc = derive(cfg, &d);
+    if (rc != CUSBF_OK) return rc;
+    if (num_symbols == 0) return CUSBF_OK;
+    if (num_words < cusbf_dense_packed_word_count(cfg, num_symbols)) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "dense packed span is too small for num_symbols");
+    }
+    sbf::KernelConfig kc{};
+    kc.k = cfg->k;
+    kc.alphabet = cfg->alphabet;
+    kc.symbol_width = d.symbol_width;
+    kc.symbol_bits = d.symbol_bits;
+    kc.invalid_symbol = d.invalid_symbol;
+    std::memcpy(kc.lut, cfg->lut, 256);
+    CUSBF_HIP_TRY(sbf::pack_dense(kc, d_sequence, num_symbols, d_words, as_stream(stream)));
+    return CUSBF_OK;
+}
+
+// filter.cuh:207-219 + launch checks of :1742-1764
+int cusbf_insert_dense_packed_async(cusbf_filter* f, const uint64_t* d_words, uint64_t num_words,
+                                    uint64_t num_symbols, void* stream, uint64_t* inserted) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    const uint64_t total = cusbf_dense_packed_kmer_count(&f->cfg, num_symbols);
+    if (inserted) *inserted = total;
+    if (total == 0) return CUSBF_OK;
+    if (num_words < cusbf_dense_packed_word_count(&f->cfg, num_symbols)) {
+        if (inserted) *inserted = 0;
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "dense packed span is too small for num_symbols");
+    }
+    sbf::SequenceInput in;
+    in.words = d_words;
+    in.num_words = num_words;
+    in.num_symbols = num_symbols;
+    in.dense = true;
+    return launch_insert(f, in, as_stream(stream));
+}
+
+// filter.cuh:248-259 + launch checks of :1766-1796
+int cusbf_contains_dense_packed_async(const cusbf_filter* f, const uint64_t* d_words,
+                                      uint64_t num_words, uint64_t num_symbols, uint8_t* d_output,
+                                      uint64_t d_output_len, void* stream) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    const uint64_t n = cusbf_dense_packed_kmer_count(&f->cfg, num_symbols);
+    if (n == 0) return CUSBF_OK;
+    if (num_words < cusbf_dense_packed_word_count(&f->cfg, num_symbols)) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "dense packed span is too small for num_symbols");
+    }
+    if (d_output_len < n) return fail(CUSBF_ERR_INVALID_ARGUMENT, "dense packed query output span is too small");
+    sbf::SequenceInput in;
+    in.words = d_words;
+    in.num_words = num_words;
+    in.num_symbols = num_symbols;
+    in.dense = true;
+    return launch_contains(f, in, d_output, as_stream(stream));
+}
+
+// filter.cuh:385-400
+int cusbf_insert_record_batch_async(cusbf_filter* f, const char* d_sequence, uint64_t bytes,
+                                    const cusbf_record_range* ranges, uint64_t num_ranges,
+                                    void* stream, uint64_t* inserted) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (inserted) *inserted = 0;
+    int rc = validate_ranges(f, bytes, ranges, num_ranges, "record batch range exceeds device sequence");
+    if (rc != CUSBF_OK) return rc;
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < num_ranges; ++i) total += record_kmers(f, ranges[i].sequence_bytes);
+    if (total == 0) return CUSBF_OK;
+    rc = launch_insert(f, byte_input(f, d_sequence, bytes), as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
+    if (inserted) *inserted = total;
+    return CUSBF_OK;
+}
+
+// filter.cuh:603-618
+int cusbf_contains_record_batch_async(const cusbf_filter* f, const char* d_sequence,
+                                      uint64_t bytes, const cusbf_record_range* ranges,
+                                      uint64_t num_ranges, uint8_t* d_output,
+                                      uint64_t d_output_len, void* stream) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    const int rc = validate_ranges(f, bytes, ranges, num_ranges, "record batch range exceeds device sequence");
+    if (rc != CUSBF_OK) return rc;
+    const uint64_t total = windows_of(f, bytes);
+    if (total == 0) return CUSBF_OK;
+    if (d_output_len < total) return fail(CUSBF_ERR_INVALID_ARGUMENT, "record batch query output span is too small");
+    return launch_contains(f, byte_input(f, d_sequence, bytes), d_output, as_stream(stream));
+}
+
+// filter.cuh:338-364
+int cusbf_insert_record_batch(cusbf_filter* f, const char* h_sequence, uint64_t bytes,
+                              const cusbf_record_range* ranges, uint64_t num_ranges, void* stream,
+                              cusbf_insert_report* report) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    int rc = validate_ranges(f, bytes, ranges, num_ranges, "record batch range exceeds the source sequence");
+    if (rc != CUSBF_OK) return rc;
+    std::vector<char> norm;
+    std::vector<NormRecord> records;
+    normalize(f, h_sequence, ranges, num_ranges, norm, records);
+    cusbf_insert_report rep{records.size(), 0, 0};
+    for (const NormRecord& r : records) {
+        rep.indexed_bases += r.size;
+        rep.inserted_kmers += r.valid_kmers;
+    }
+    if (!norm.empty()) {
+        rc = ensure_stage(f, norm.size());
+        if (rc != CUSBF_OK) return rc;
+        CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, norm.data(), norm.size(), hipMemcpyHostToDevice, as_stream(stream)));
+        rc = launch_insert(f, byte_input(f, f->d_stage, norm.size()), as_stream(stream));
+        if (rc != CUSBF_OK) return rc;
+        CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    }
+    release_staging(f);
+    if (report) *report = rep;
+    return CUSBF_OK;
+}
+
+// filter.cuh:930-990 (aggregate query of a host record batch).
+int cusbf_query_record_batch(const cusbf_filter* f_const, const char* h_sequence, uint64_t bytes,
+                             const cusbf_record_range* ranges, uint64_t num_ranges, void* stream,
+                             cusbf_query_report* report) {
+    cusbf_filter* f = const_cast<cusbf_filter*>(f_const);
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    int rc = validate_ranges(f, bytes, ranges, num_ranges, "record batch range exceeds the source sequence");
+    if (rc != CUSBF_OK) return rc;
+    std::vector<char> norm;
+    std::vector<NormRecord> records;
+    normalize(f, h_sequence, ranges, num_ranges, norm, records);
+    cusbf_query_report rep{records.size(), 0, 0, 0};
+    for (const NormRecord& r : records) {
+        rep.queried_bases += r.size;
+        rep.queried_kmers += r.valid_kmers;
+    }
+    if (!norm.empty()) {
+        // query_layout.cuh:66-67: total hits = windows of the normalised buffer.
+        const uint64_t hits = windows_of(f, norm.size());
+        rc = ensure_stage(f, norm.size());
+        if (rc == CUSBF_OK && hits) rc = ensure_result(f, hits);
+        if (rc != CUSBF_OK) return rc;
+        CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, norm.data(), norm.size(), hipMemcpyHostToDevice, as_stream(stream)));
+        if (hits) {
+            rc = launch_contains(f, byte_input(f, f->d_stage, norm.size()), f->d_result, as_stream(stream));
+            if (rc != CUSBF_OK) return rc;
+            CUSBF_HIP_TRY(sbf::fill_u64(f->d_scalar, 0, 1, as_stream(stream)));
+            CUSBF_HIP_TRY(sbf::count_ones(f->d_result, hits, f->d_scalar, as_stream(stream)));
+            CUSBF_HIP_TRY(hipMemcpyAsync(&rep.positive_kmers, f->d_scalar, sizeof(uint64_t), hipMemcpyDeviceToHost, as_stream(stream)));
+        }
+        CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    }
+    release_staging(f);
+    if (report) *report = rep;
+    return CUSBF_OK;
+}
+
+int cusbf_count_positive_async(const uint8_t* d_hits, uint64_t num_hits, uint64_t* d_count,
+                               void* stream) {
+    if (!d_count) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null count pointer");
+    CUSBF_HIP_TRY(sbf::fill_u64(d_count, 0, 1, as_stream(stream)));
+    if (num_hits == 0) return CUSBF_OK;
+    if (!d_hits) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null hits pointer");
+    CUSBF_HIP_TRY(sbf::count_ones(d_hits, num_hits, d_count, as_stream(stream)));
+    return CUSBF_OK;
+}
+
+int cusbf_count_positive_per_record_async(const uint8_t* d_hits, const uint64_t* d_offsets,
+                                          const uint64_t* d_counts, uint64_t num_records,
+                                          uint64_t* d_positive, void* stream) {
+    if (num_records == 0) return CUSBF_OK;
+    if (!d_hits || !d_offsets || !d_counts || !d_positive) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device buffer");
+    }
+    CUSBF_HIP_TRY(sbf::count_ones_per_record(d_hits, d_offsets, d_counts, num_records, d_positive, as_stream(stream)));
+    return CUSBF_OK;
+}
+
+int cusbf_count_windows_runs_async(const cusbf_filter* f, const char* d_sequence, uint64_t bytes,
+                                   uint64_t* d_counts, void* stream) {
+    if (!f || !d_counts) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    CUSBF_HIP_TRY(sbf::fill_u64(d_counts, 0, 2, as_stream(stream)));
+    if (windows_of(f, bytes) == 0) return CUSBF_OK;
+    CUSBF_HIP_TRY(sbf::generic_count(f->kc, byte_input(f, d_sequence, bytes), f->num_shards, d_counts, as_stream(stream)));
+    return CUSBF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,343 @@
+// kernels_generic.hip -- Super Bloom kernels for any configuration and alphabet,
+// plus the packing and reduction kernels.
+//
+// One window per thread over an LDS tile of encoded symbols, computed the way
+// the reference computes each window (include/cusbf/detail/kernels.cuh:40-172:
+// pack the k-mer, minimum of the k-m+1 m-mer hashes, k-s+1 s-mer hashes).
+// Configurations without a fast DNA kernel (protein, triplets, custom
+// alphabets, other k/s/m/h) run here; the fast DNA kernels are checked against
+// these and against the CPU oracle.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "launch.hpp"
+#include "sbf_common.hpp"
+
+namespace sbf::generic {
+
+constexpr int kBlock = 256;
+constexpr int kMaxK = 64;
+
+// Tile of encoded symbols [first, first + count); symbols outside [0, n) get
+// the invalid sentinel (they only occur in windows that are never reported).
+__device__ void load_tile(const KernelConfig& c, const SequenceInput& in, int64_t first,
+                          int count, uint8_t* tile) {
+    const uint64_t per_word = 64u / c.symbol_bits;
+    const uint64_t sym_mask = (1ull << c.symbol_bits) - 1ull;
+    for (int i = static_cast<int>(threadIdx.x); i < count; i += kBlock) {
+        const int64_t p = first + i;
+        uint8_t v = static_cast<uint8_t>(c.invalid_symbol);
+        if (p >= 0 && static_cast<uint64_t>(p) < in.num_symbols) {
+            const uint64_t up = static_cast<uint64_t>(p);
+            if (in.dense) {
+                // dense_packed.cuh:56-65: LSB-first decode, masked to symbolBits.
+                v = static_cast<uint8_t>((in.words[up / per_word] >> ((up % per_word) * c.symbol_bits)) &
+                                         sym_mask);
+            } else {
+                v = encode_symbol(c, in.seq + up * c.symbol_width);
+            }
+        }
+        tile[i] = v;
+    }
+}
+
+struct Window {
+    bool valid;
+    uint64_t packed;
+};
+
+// sequence_kmer.cuh:74-82 (pack) and :92-101 (validity).
+__device__ __forceinline__ Window read_window(const KernelConfig& c, const uint8_t* t) {
+    const uint64_t sym_mask = (1ull << c.symbol_bits) - 1ull;
+    Window w{true, 0};
+    for (uint32_t i = 0; i < c.k; ++i) {
+        const uint8_t sym = t[i];
+        w.valid &= sym != static_cast<uint8_t>(c.invalid_symbol);
+        w.packed = (w.packed << c.symbol_bits) | (sym & sym_mask);
+    }
+    return w;
+}
+
+// filter_common.cuh:128-134
+__device__ __forceinline__ uint64_t subwindow(const KernelConfig& c, uint64_t packed, uint32_t len,
+                                              uint32_t start) {
+    return (packed >> (c.symbol_bits * (c.k - (start + len)))) & window_mask(c.symbol_bits, len);
+}
+
+// sequence_kmer.cuh:40-50
+__device__ __forceinline__ uint64_t minimizer(const KernelConfig& c, uint64_t packed) {
+    uint64_t best = ~0ull;
+    for (uint32_t o = 0; o + c.m <= c.k; ++o) {
+        const uint64_t h = minimizer_hash64(subwindow(c, packed, c.m, o));
+        best = h < best ? h : best;
+    }
+    return best;
+}
+
+// kernels.cuh:294-304 with filter_impl.cuh:54-77.
+__device__ __forceinline__ void masks(const KernelConfig& c, uint64_t packed, uint64_t (&m)[4]) {
+    m[0] = m[1] = m[2] = m[3] = 0;
+    for (uint32_t o = 0; o + c.s <= c.k; ++o) {
+        const uint64_t h = hash64(subwindow(c, packed, c.s, o));
+        for (uint32_t i = 0; i < c.h; ++i) m[i & 3u] |= 1ull << bit_address(c.h, i, h);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, SequenceInput in,
+                                                        FilterView f, uint64_t num_windows) {
+    __shared__ uint8_t tile[kBlock + kMaxK];
+    const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock;
+    load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, tile);
+    __syncthreads();
+    const uint64_t j = first + threadIdx.x;
+    if (j >= num_windows) return;
+    const Window w = read_window(c, tile + threadIdx.x);
+    if (!w.valid) return;
+    const uint32_t shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
+    uint64_t m[4];
+    masks(c, w.packed, m);
+    uint64_t* dst = f.words + 4ull * shard;
+    for (int q = 0; q < 4; ++q) {
+        if (m[q] != 0) __hip_atomic_fetch_or(dst + q, m[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void contains_kernel(KernelConfig c, SequenceInput in,
+                                                          FilterView f, uint64_t num_windows,
+                                                          uint8_t* out) {
+    __shared__ uint8_t tile[kBlock + kMaxK];
+    const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock;
+    load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, tile);
+    __syncthreads();
+    const uint64_t j = first + threadIdx.x;
+    if (j >= num_windows) return;
+    const Window w = read_window(c, tile + threadIdx.x);
+    if (!w.valid) {
+        out[j] = 0;
+        return;
+    }
+    const uint32_t shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
+    uint64_t m[4];
+    masks(c, w.packed, m);
+    const uint64_t* src = f.words + 4ull * shard;
+    uint64_t miss = 0;
+    for (int q = 0; q < 4; ++q) miss |= m[q] & ~src[q];
+    out[j] = miss == 0 ? 1 : 0;
+}
+
+// Valid windows and shard-run heads; the tile starts one window early so each
+// thread can compare with its predecessor.
+__global__ __launch_bounds__(kBlock) void count_kernel(KernelConfig c, SequenceInput in,
+                                                       uint64_t num_shards, uint64_t num_windows,
+                                                       unsigned long long* counts) {
+    __shared__ uint8_t tile[kBlock + 1 + kMaxK];
+    __shared__ uint32_t shard_of[kBlock + 1];
+    __shared__ uint8_t valid_of[kBlock + 1];
+    __shared__ unsigned long long totals[2];
+    const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock;
+    if (threadIdx.x < 2) totals[threadIdx.x] = 0;
+    load_tile(c, in, static_cast<int64_t>(first) - 1, kBlock + static_cast<int>(c.k), tile);
+    __syncthreads();
+    for (int t = static_cast<int>(threadIdx.x); t < kBlock + 1; t += kBlock) {
+        const int64_t j = static_cast<int64_t>(first) - 1 + t;
+        bool valid = false;
+        uint32_t shard = 0;
+        if (j >= 0 && static_cast<uint64_t>(j) < num_windows) {
+            const Window w = read_window(c, tile + t);
+            valid = w.valid;
+            if (valid) shard = static_cast<uint32_t>(minimizer(c, w.packed) & (num_shards - 1));
+        }
+        shard_of[t] = shard;
+        valid_of[t] = valid;
+    }
+    __syncthreads();
+    const uint64_t j = first + threadIdx.x;
+    if (j < num_windows && valid_of[threadIdx.x + 1]) {
+        const bool head = !valid_of[threadIdx.x] || shard_of[threadIdx.x] != shard_of[threadIdx.x + 1];
+        atomicAdd(&totals[0], 1ull);
+        if (head) atomicAdd(&totals[1], 1ull);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && totals[threadIdx.x] != 0) atomicAdd(&counts[threadIdx.x], totals[threadIdx.x]);
+}
+
+// dense_packed.cuh:126-149, staged through LDS: the block reads its input
+// bytes with coalesced loads, then each thread packs one output word.
+constexpr int kPackMaxBytes = kBlock * 32;
+
+__global__ __launch_bounds__(kBlock) void pack_kernel(KernelConfig c, const char* seq,
+                                                      uint64_t num_symbols, uint64_t* words) {
+    __shared__ uint8_t codes[kBlock * 64];
+    const uint32_t per_word = 64u / c.symbol_bits;
+    const uint64_t sym_mask = (1ull << c.symbol_bits) - 1ull;
+    const uint64_t first_symbol = static_cast<uint64_t>(blockIdx.x) * kBlock * per_word;
+    const uint32_t tile_symbols = kBlock * per_word;
+    for (uint32_t i = threadIdx.x; i < tile_symbols; i += kBlock) {
+        const uint64_t p = first_symbol + i;
+        uint8_t v = 0;
+        if (p < num_symbols) v = static_cast<uint8_t>(encode_symbol(c, seq + p * c.symbol_width) & sym_mask);
+        codes[i] = v;
+    }
+    __syncthreads();
+    const uint64_t word = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const uint64_t num_words = (num_symbols + per_word - 1) / per_word;
+    if (word >= num_words) return;
+    uint64_t packed = 0;
+    const uint8_t* src = codes + threadIdx.x * per_word;
+    for (uint32_t i = 0; i < per_word; ++i) packed |= static_cast<uint64_t>(src[i]) << (i * c.symbol_bits);
+    words[word] = packed;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kBlock) void popcount_kernel(const uint64_t* words, uint64_t n,
+                                                          unsigned long long* out) {
+    unsigned long long acc = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+        acc += static_cast<unsigned long long>(__popcll(words[i]));
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
+}
+
+// count_positive_kmers.cuh:47-62: number of bytes equal to 1.
+__global__ __launch_bounds__(kBlock) void count_ones_kernel(const uint8_t* hits, uint64_t n,
+                                                            unsigned long long* out) {
+    unsigned long long acc = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    const uint64_t vec = n / 16;
+    const uint4* h4 = reinterpret_cast<const uint4*>(hits);
+    const bool aligned = (reinterpret_cast<uintptr_t>(hits) & 15u) == 0;
+    if (aligned) {
+        for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < vec; i += stride) {
+            const uint4 v = h4[i];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            for (int q = 0; q < 4; ++q) {
+                // bytes equal to 1 (hit bytes are 0 or 1, but count exactly "== 1")
+                const uint32_t x = w[q] ^ 0x01010101u;
+                const uint32_t nz = (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+                acc += 4u - static_cast<unsigned>(__popc(nz));
+            }
+        }
+    }
+    const uint64_t tail_start = aligned ? vec * 16 : 0;
+    for (uint64_t i = tail_start + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+        acc += hits[i] == 1;
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
+}
+
+// count_positive_kmers.cuh:21-45, one wave per record.
+__global__ __launch_bounds__(kBlock) void count_ones_per_record_kernel(const uint8_t* hits,
+                                                                       const uint64_t* offsets,
+                                                                       const uint64_t* counts,
+                                                                       uint64_t num_records,
+                                                                       uint64_t* out) {
+    const uint64_t record = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / 64;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    if (record >= num_records) return;
+    const uint64_t begin = offsets[record];
+    const uint64_t n = counts[record];
+    unsigned long long acc = 0;
+    for (uint64_t i = lane; i < n; i += 64) acc += hits[begin + i];
+    acc = wave_sum(acc);
+    if (lane == 0) out[record] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void fill_kernel(uint64_t* dst, uint64_t value, uint64_t n) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) dst[i] = value;
+}
+
+uint32_t grid_for(uint64_t items, uint64_t per_block, uint64_t cap) {
+    uint64_t g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return static_cast<uint32_t>(g);
+}
+
+}  // namespace sbf::generic
+
+namespace sbf {
+
+using namespace generic;
+
+static uint64_t window_count(const KernelConfig& c, const SequenceInput& in) {
+    return in.num_symbols < c.k ? 0 : in.num_symbols - c.k + 1;
+}
+
+hipError_t generic_insert(const KernelConfig& c, const SequenceInput& in, FilterView f,
+                          hipStream_t stream) {
+    const uint64_t n = window_count(c, in);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(insert_kernel, dim3(grid_for(n, kBlock, 0xFFFFFFFFull)), dim3(kBlock), 0,
+                       stream, c, in, f, n);
+    return hipGetLastError();
+}
+
+hipError_t generic_contains(const KernelConfig& c, const SequenceInput& in, FilterView f,
+                            uint8_t* out, hipStream_t stream) {
+    const uint64_t n = window_count(c, in);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(contains_kernel, dim3(grid_for(n, kBlock, 0xFFFFFFFFull)), dim3(kBlock), 0,
+                       stream, c, in, f, n, out);
+    return hipGetLastError();
+}
+
+hipError_t generic_count(const KernelConfig& c, const SequenceInput& in, uint64_t num_shards,
+                         uint64_t* d_counts, hipStream_t stream) {
+    const uint64_t n = window_count(c, in);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(count_kernel, dim3(grid_for(n, kBlock, 0xFFFFFFFFull)), dim3(kBlock), 0,
+                       stream, c, in, num_shards, n,
+                       reinterpret_cast<unsigned long long*>(d_counts));
+    return hipGetLastError();
+}
+
+hipError_t pack_dense(const KernelConfig& c, const char* seq, uint64_t num_symbols,
+                      uint64_t* words, hipStream_t stream) {
+    if (num_symbols == 0) return hipSuccess;
+    const uint64_t per_word = 64u / c.symbol_bits;
+    const uint64_t num_words = (num_symbols + per_word - 1) / per_word;
+    hipLaunchKernelGGL(pack_kernel, dim3(grid_for(num_words, kBlock, 0xFFFFFFFFull)), dim3(kBlock),
+                       0, stream, c, seq, num_symbols, words);
+    return hipGetLastError();
+}
+
+hipError_t popcount_words(const uint64_t* words, uint64_t n, uint64_t* d_out, hipStream_t stream) {
+    hipLaunchKernelGGL(popcount_kernel, dim3(grid_for(n, kBlock * 8, 4096)), dim3(kBlock), 0,
+                       stream, words, n, reinterpret_cast<unsigned long long*>(d_out));
+    return hipGetLastError();
+}
+
+hipError_t count_ones(const uint8_t* hits, uint64_t n, uint64_t* d_out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(count_ones_kernel, dim3(grid_for(n, kBlock * 64, 4096)), dim3(kBlock), 0,
+                       stream, hits, n, reinterpret_cast<unsigned long long*>(d_out));
+    return hipGetLastError();
+}
+
+hipError_t count_ones_per_record(const uint8_t* hits, const uint64_t* offsets,
+                                 const uint64_t* counts, uint64_t num_records, uint64_t* d_out,
+                                 hipStream_t stream) {
+    if (num_records == 0) return hipSuccess;
+    hipLaunchKernelGGL(count_ones_per_record_kernel,
+                       dim3(grid_for(num_records, kBlock / 64, 0xFFFFFFFFull)), dim3(kBlock), 0,
+                       stream, hits, offsets, counts, num_records, d_out);
+    return hipGetLastError();
+}
+
+hipError_t fill_u64(uint64_t* dst, uint64_t value, uint64_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, kBlock * 4, 65536)), dim3(kBlock), 0, stream,
+                       dst, value, n);
+    return hipGetLastError();
+}
+
+}  // namespace sbf

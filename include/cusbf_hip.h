@@ -1,0 +1,210 @@
+/*
+ * cusbf_hip.h -- C ABI of the MI355X-native Super Bloom filter (libcusbf_hip.so).
+ *
+ * Plain pointers and sizes only. Device pointers are HIP device allocations on
+ * the filter's device; streams are hipStream_t passed as void* (NULL = the
+ * default stream). Every entry point returns a cusbf_status; on failure the
+ * calling thread's last error (category + message) is readable through
+ * cusbf_last_error().
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference checkout; all in include/cusbf/).
+ */
+#ifndef CUSBF_HIP_H
+#define CUSBF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUSBF_ABI_VERSION 1
+
+/* Status codes. A failing status minus one is the reference's ErrorCategory
+ * index (error.hpp:67-73, pinned by :139-143): cuda=0, io=1, fastx_parse=2,
+ * invalid_argument=3, resource=4. */
+typedef enum cusbf_status {
+    CUSBF_OK = 0,
+    CUSBF_ERR_DEVICE = 1,           /* ErrorCategory::cuda (a HIP runtime error here) */
+    CUSBF_ERR_IO = 2,               /* ErrorCategory::io */
+    CUSBF_ERR_FASTX_PARSE = 3,      /* ErrorCategory::fastx_parse */
+    CUSBF_ERR_INVALID_ARGUMENT = 4, /* ErrorCategory::invalid_argument */
+    CUSBF_ERR_RESOURCE = 5          /* ErrorCategory::resource */
+} cusbf_status;
+
+/* Alphabets: Alphabet.cuh:113-142 (DNA), :150-178 (DnaTriplet), :189-218
+ * (Protein); CUSTOM = any width-1 alphabet given as a byte->symbol table
+ * (the Alphabet concept of Alphabet.cuh:94-107 evaluated for all 256 bytes). */
+typedef enum cusbf_alphabet {
+    CUSBF_ALPHABET_DNA = 0,
+    CUSBF_ALPHABET_DNA_TRIPLET = 1,
+    CUSBF_ALPHABET_PROTEIN = 2,
+    CUSBF_ALPHABET_CUSTOM = 3
+} cusbf_alphabet;
+
+/* Runtime form of Config<K,S,M,HashCount,CudaBlockSize,Alphabet> (config.cuh:28-118).
+ * symbol_count/invalid_symbol/separator/lut are read only for CUSTOM. */
+typedef struct cusbf_config {
+    uint32_t k;
+    uint32_t s;
+    uint32_t m;
+    uint32_t hash_count;
+    uint32_t alphabet; /* cusbf_alphabet */
+    uint32_t symbol_count;
+    uint8_t invalid_symbol;
+    uint8_t separator;
+    uint8_t reserved[2];
+    uint8_t lut[256];
+} cusbf_config;
+
+/* RecordRange (Fastx.hpp:24-29). */
+typedef struct cusbf_record_range {
+    uint64_t sequence_offset;
+    uint64_t sequence_bytes;
+} cusbf_record_range;
+
+/* FastxInsertReport (Fastx.hpp:150-158). */
+typedef struct cusbf_insert_report {
+    uint64_t records_indexed;
+    uint64_t indexed_bases;
+    uint64_t inserted_kmers;
+} cusbf_insert_report;
+
+/* FastxQueryReport (Fastx.hpp:161-170). */
+typedef struct cusbf_query_report {
+    uint64_t records_queried;
+    uint64_t queried_bases;
+    uint64_t queried_kmers;
+    uint64_t positive_kmers;
+} cusbf_query_report;
+
+typedef struct cusbf_filter cusbf_filter;
+
+/* --- errors --------------------------------------------------------------- */
+
+/* Error::category()/message() of the last failure on this thread
+ * (error.hpp:76-87). Returns the status of that failure (CUSBF_OK if none);
+ * copies the NUL-terminated message into buf when buf_len > 0. */
+int cusbf_last_error(char* buf, size_t buf_len);
+int cusbf_abi_version(void);
+
+/* --- configuration helpers (config.cuh:28-118) ------------------------------ */
+
+/* Fills *cfg for a built-in alphabet; validates the static_asserts of
+ * config.cuh:74-117 (returns CUSBF_ERR_INVALID_ARGUMENT when violated). */
+int cusbf_config_init(cusbf_config* cfg, uint32_t k, uint32_t s, uint32_t m,
+                      uint32_t hash_count, uint32_t alphabet);
+/* Derived constants: symbolWidth (:49), symbolBits (:51). */
+int cusbf_config_symbol_width(const cusbf_config* cfg, uint32_t* out);
+int cusbf_config_symbol_bits(const cusbf_config* cfg, uint32_t* out);
+
+/* --- lifetime (filter.cuh:108-130) ----------------------------------------- */
+
+/* filter<Config>::filter(requestedFilterBits) (filter.cuh:108-120): shards =
+ * bit_ceil(max(1, ceil(bits/256))), zeroed. Uses the current HIP device. */
+int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_filter** out);
+/* ~filter() (filter.cuh:130). NULL is a no-op. */
+void cusbf_destroy(cusbf_filter* f);
+
+/* clear(stream) (filter.cuh:793-803): zero all shards, synchronise. */
+int cusbf_clear(cusbf_filter* f, void* stream);
+/* load_factor() (filter.cuh:810-823). */
+int cusbf_load_factor(const cusbf_filter* f, float* out);
+/* filter_bits() / num_shards() (filter.cuh:826-833). */
+uint64_t cusbf_filter_bits(const cusbf_filter* f);
+uint64_t cusbf_num_shards(const cusbf_filter* f);
+/* ref().blocks() (filter.cuh:137-142, filter_ref.cuh:56-64): device address of
+ * shard 0; shard i occupies words [4i, 4i+4). */
+int cusbf_words_device_ptr(const cusbf_filter* f, uint64_t** out);
+/* Parity dump of all num_shards*4 filter words into host memory (synchronous). */
+int cusbf_copy_words_to_host(const cusbf_filter* f, uint64_t* host_words, uint64_t num_words);
+
+/* --- byte-sequence path ---------------------------------------------------- */
+
+/* insert_sequence_async(device_span<const char>, stream) (filter.cuh:179-190):
+ * returns all k-mer windows (invalid ones are skipped by the kernel). */
+int cusbf_insert_sequence_async(cusbf_filter* f, const char* d_sequence, uint64_t bytes,
+                                void* stream, uint64_t* inserted);
+/* insert_sequence(std::string_view, stream) (filter.cuh:154-167): host bytes,
+ * staged through library-owned device memory, synchronises. */
+int cusbf_insert_sequence(cusbf_filter* f, const char* h_sequence, uint64_t bytes, void* stream,
+                          uint64_t* inserted);
+/* contains_sequence_async(device_span<const char>, device_span<uint8_t>, stream)
+ * (filter.cuh:464-474): one byte per window, 1 = present, 0 = absent/invalid.
+ * Like the reference it does not check d_output_len against the window count;
+ * the caller provides at least that many bytes. */
+int cusbf_contains_sequence_async(const cusbf_filter* f, const char* d_sequence, uint64_t bytes,
+                                  uint8_t* d_output, uint64_t d_output_len, void* stream);
+/* contains_sequence(std::string_view, std::vector out) (filter.cuh:514-533):
+ * host bytes in, host hits out (h_output must hold the window count), synchronises. */
+int cusbf_contains_sequence(const cusbf_filter* f, const char* h_sequence, uint64_t bytes,
+                            uint8_t* h_output, uint64_t h_output_len, void* stream);
+
+/* --- dense packed path (dense_packed.hpp, filter.cuh:207-312) --------------- */
+
+/* dense_packed_word_count / dense_packed_kmer_count (dense_packed.cuh:19-28). */
+uint64_t cusbf_dense_packed_word_count(const cusbf_config* cfg, uint64_t num_symbols);
+uint64_t cusbf_dense_packed_kmer_count(const cusbf_config* cfg, uint64_t num_symbols);
+/* pack_dense_sequence_device (dense_packed.hpp:57-73). */
+int cusbf_pack_dense_sequence_async(const cusbf_config* cfg, const char* d_sequence,
+                                    uint64_t num_symbols, uint64_t* d_words, uint64_t num_words,
+                                    void* stream);
+/* insert_dense_packed_async (filter.cuh:207-219). */
+int cusbf_insert_dense_packed_async(cusbf_filter* f, const uint64_t* d_words, uint64_t num_words,
+                                    uint64_t num_symbols, void* stream, uint64_t* inserted);
+/* contains_dense_packed_async (filter.cuh:248-259). */
+int cusbf_contains_dense_packed_async(const cusbf_filter* f, const uint64_t* d_words,
+                                      uint64_t num_words, uint64_t num_symbols, uint8_t* d_output,
+                                      uint64_t d_output_len, void* stream);
+
+/* --- device record batches (filter.cuh:385-400, 603-618) ------------------- */
+
+/* insert_record_batch_async: separators already embedded; ranges validated
+ * (filter.cuh:848-876); returns the sum of per-record window counts. */
+int cusbf_insert_record_batch_async(cusbf_filter* f, const char* d_sequence, uint64_t bytes,
+                                    const cusbf_record_range* ranges, uint64_t num_ranges,
+                                    void* stream, uint64_t* inserted);
+/* contains_record_batch_async: output must hold every window of d_sequence. */
+int cusbf_contains_record_batch_async(const cusbf_filter* f, const char* d_sequence,
+                                      uint64_t bytes, const cusbf_record_range* ranges,
+                                      uint64_t num_ranges, uint8_t* d_output,
+                                      uint64_t d_output_len, void* stream);
+
+/* --- host record batches with separator injection --------------------------- */
+
+/* insert_record_batch(RecordBatchView) (filter.cuh:338-364): normalises
+ * (normalized_record_batch.hpp:288-304), stages, inserts, synchronises. */
+int cusbf_insert_record_batch(cusbf_filter* f, const char* h_sequence, uint64_t bytes,
+                              const cusbf_record_range* ranges, uint64_t num_ranges,
+                              void* stream, cusbf_insert_report* report);
+/* query_record_batch(RecordBatchView) aggregate form (filter.cuh:549-552, 951-990). */
+int cusbf_query_record_batch(const cusbf_filter* f, const char* h_sequence, uint64_t bytes,
+                             const cusbf_record_range* ranges, uint64_t num_ranges, void* stream,
+                             cusbf_query_report* report);
+
+/* --- counting and diagnostics ---------------------------------------------- */
+
+/* thrust::count(hits == 1) (count_positive_kmers.cuh:47-62) as a HIP reduction;
+ * d_count receives one uint64 (device memory). */
+int cusbf_count_positive_async(const uint8_t* d_hits, uint64_t num_hits, uint64_t* d_count,
+                               void* stream);
+/* Per-record positive counts (count_positive_kmers.cuh:21-45, 69-90):
+ * d_offsets/d_counts are per-record hit offsets and hit counts. */
+int cusbf_count_positive_per_record_async(const uint8_t* d_hits, const uint64_t* d_offsets,
+                                          const uint64_t* d_counts, uint64_t num_records,
+                                          uint64_t* d_positive, void* stream);
+/* No reference counterpart: counts valid windows and shard runs (maximal runs
+ * of consecutive valid windows sharing a shard) of a device sequence, for the
+ * byte model of the roofline. d_counts receives two uint64 (device memory). */
+int cusbf_count_windows_runs_async(const cusbf_filter* f, const char* d_sequence, uint64_t bytes,
+                                   uint64_t* d_counts, void* stream);
+
+/* Name of the kernel family the filter dispatches to ("dna_fast" or "generic"). */
+const char* cusbf_kernel_family(const cusbf_filter* f);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUSBF_HIP_H */

@@ -1,0 +1,99 @@
+"""Shared helpers for the tests: configurations, seeded inputs, oracle construction."""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+from oracle import oracle as orc
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+with open(os.path.join(GOLDEN_DIR, "reference_known_answers.json")) as fh:
+    KNOWN = json.load(fh)
+
+XYZ = KNOWN["custom_xyz"]
+
+
+def xyz_encode(b: int) -> int:
+    return XYZ["symbols"].get(chr(b), XYZ["invalid_symbol"]) if b < 128 else XYZ["invalid_symbol"]
+
+
+def xyz_lut() -> list:
+    return [xyz_encode(b) for b in range(256)]
+
+
+ALPHA_CODES = {"dna": orc.ALPHA_DNA, "triplet": orc.ALPHA_DNA_TRIPLET,
+               "protein": orc.ALPHA_PROTEIN, "custom_xyz": orc.ALPHA_CUSTOM}
+
+
+def make_oracle(k, s, m, h=4, alphabet="dna") -> orc.Oracle:
+    if alphabet == "custom_xyz":
+        return orc.Oracle(k, s, m, h, orc.ALPHA_CUSTOM, xyz_lut(), XYZ["symbol_count"],
+                          XYZ["invalid_symbol"], ord(XYZ["separator"]))
+    return orc.Oracle(k, s, m, h, ALPHA_CODES[alphabet])
+
+
+def oracle_for(name: str) -> orc.Oracle:
+    c = KNOWN["configs"][name]
+    return make_oracle(c["k"], c["s"], c["m"], c["h"], c["alphabet"])
+
+
+def make_config(k, s, m, h=4, alphabet="dna"):
+    """Product-side Config for the same parameters."""
+    import cusbf_amd as cs
+
+    if alphabet == "dna":
+        return cs.Config(k, s, m, h)
+    if alphabet == "protein":
+        return cs.Config(k, s, m, h, alphabet=cs.ProteinAlphabet)
+    if alphabet == "triplet":
+        return cs.Config(k, s, m, h, alphabet=cs.DnaTripletAlphabet)
+    if alphabet == "custom_xyz":
+        alpha = cs.Alphabet.custom("xyz", XYZ["symbol_count"], XYZ["invalid_symbol"],
+                                   XYZ["separator"], b"xyz", xyz_encode)
+        return cs.Config(k, s, m, h, alphabet=alpha)
+    raise ValueError(alphabet)
+
+
+def config_for(name: str):
+    c = KNOWN["configs"][name]
+    return make_config(c["k"], c["s"], c["m"], c["h"], c["alphabet"])
+
+
+def random_dna(n: int, seed: int, n_rate: float = 0.0, lower_rate: float = 0.0) -> bytes:
+    rng = np.random.default_rng(seed)
+    seq = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, n)].copy()
+    if n_rate > 0:
+        seq[rng.random(n) < n_rate] = ord("N")
+    if lower_rate > 0:
+        lo = rng.random(n) < lower_rate
+        seq[lo] = seq[lo] | 0x20
+    return seq.tobytes()
+
+
+def random_protein(n: int, seed: int, invalid_rate: float = 0.0) -> bytes:
+    rng = np.random.default_rng(seed)
+    letters = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY", dtype=np.uint8)
+    seq = letters[rng.integers(0, letters.size, n)].copy()
+    if invalid_rate > 0:
+        seq[rng.random(n) < invalid_rate] = ord("*")
+    return seq.tobytes()
+
+
+def random_bytes_for(alphabet: str, n: int, seed: int, invalid_rate: float = 0.0) -> bytes:
+    if alphabet == "dna":
+        return random_dna(n, seed, invalid_rate)
+    if alphabet == "triplet":
+        return random_dna(n, seed, invalid_rate)
+    if alphabet == "protein":
+        return random_protein(n, seed, invalid_rate)
+    rng = np.random.default_rng(seed)
+    seq = np.frombuffer(b"xyz", dtype=np.uint8)[rng.integers(0, 3, n)].copy()
+    if invalid_rate > 0:
+        seq[rng.random(n) < invalid_rate] = ord("!")
+    return seq.tobytes()
+
+
+BASE_CODE = {"A": 0, "C": 1, "T": 2, "G": 3}
