@@ -1,0 +1,213 @@
+"""Benchmark: Super Bloom insert & query k-mers/s on one MI355X, with HBM roofline.
+
+One step = clear the filter, insert every k-mer of the synthetic genome
+(one record batch, chromosomes joined by separators), then query every k-mer
+window of the read set (150 bp reads joined by separators). Inputs are
+resident in HBM before timing starts; HIP events on the launch stream bracket
+the insert and query kernels (the clear is outside the event pair, as the
+reference clears outside its timed region).
+
+value = (valid inserted k-mers + valid queried k-mers) / (insert + query kernel time)
+
+Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "insert & query k-mers/s at 1 GPU (% HBM roofline) + p50/p99 serve latency"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="human", choices=["human", "nematode"])
+    ap.add_argument("--reads", type=int, default=0, help="override the number of reads")
+    ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(genome, reads, wl, threads: int):
+    """CPU Super Bloom (the oracle's multithreaded sliding form) on a bounded sample."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    t_avail = min(16, os.cpu_count() or 1)
+    threads = threads or t_avail
+    ins_bytes = min(genome.numel(), 400_000_000)
+    read_stride = wl.reads.read_len + 1
+    q_reads = min(wl.reads.n_reads, 4_000_000)
+    g = genome[:ins_bytes].cpu().numpy()
+    r = reads[: q_reads * read_stride - 1].cpu().numpy()
+    o = orc.Oracle(wl.k, wl.s, wl.m, wl.h)
+    n_ins = o.kmer_count(g.size)
+    shards = o.num_shards(int(wl.bits_per_kmer * n_ins))
+    words = o.new_filter(shards)
+    t0 = time.perf_counter()
+    o.mt_insert(g, words, threads)
+    t1 = time.perf_counter()
+    hits = o.mt_contains(r, words, threads)
+    t2 = time.perf_counter()
+    v_ins, _ = o.count(g[: min(g.size, 20_000_000)], shards)  # valid fraction estimate
+    frac = v_ins / max(1, o.kmer_count(min(g.size, 20_000_000)))
+    ins_valid = n_ins * frac
+    q_valid = q_reads * (wl.reads.read_len - wl.k + 1)
+    ins_rate = ins_valid / (t1 - t0)
+    q_rate = q_valid / (t2 - t1)
+    return {
+        "value": (ins_valid + q_valid) / ((t1 - t0) + (t2 - t1)),
+        "unit": "k-mers/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"insert first {ins_bytes / 1e6:.0f} Mbp of the genome, query first "
+                   f"{q_reads / 1e6:.0f}M reads; {threads} threads; "
+                   f"insert {ins_rate / 1e6:.0f} M/s, query {q_rate / 1e6:.0f} M/s; "
+                   f"{int(hits.sum())} positives"),
+        "insert_kmers_per_s": ins_rate,
+        "query_kmers_per_s": q_rate,
+        "seconds": (t2 - t0),
+    }
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import torch
+
+    import cusbf_amd as cs
+    from cusbf_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        return 1
+    wl = W.WORKLOADS[args.workload]
+    if args.reads:
+        wl.reads.n_reads = args.reads
+    log(f"[bench] workload {wl.name}: generating genome ({wl.genome.total_len / 1e9:.2f} Gbp)")
+    t0 = time.perf_counter()
+    genome, chrom_lens = W.generate_genome(wl.genome)
+    log(f"[bench] generating {wl.reads.n_reads / 1e6:.0f}M reads")
+    reads = W.generate_reads(wl.reads, genome)
+    log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f}s "
+        f"(genome {genome.numel() / 1e9:.2f} GB, reads {reads.numel() / 1e9:.2f} GB)")
+
+    cfg = cs.Config(wl.k, wl.s, wl.m, wl.h)
+    genome_windows = genome.numel() - wl.k + 1
+    filt = cs.Filter(cfg, int(wl.bits_per_kmer * genome_windows))
+    out = torch.empty(reads.numel() - wl.k + 1, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    # exact work counts for the byte model (valid windows, shard runs)
+    v_ins, s_ins = filt.count_windows_runs(genome)
+    v_q, s_q = filt.count_windows_runs(reads)
+    log(f"[bench] insert: {v_ins} valid k-mers, {s_ins} shard runs; "
+        f"query: {v_q} valid k-mers, {s_q} shard runs; kernels={filt.kernel_family()}")
+
+    def step(record=None):
+        filt.clear(sh)
+        if record:
+            record[0].record(stream)
+        filt.insert_sequence_async(genome, stream=sh)
+        if record:
+            record[1].record(stream)
+        filt.contains_sequence_async(reads, out, stream=sh)
+        if record:
+            record[2].record(stream)
+
+    for i in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"[bench] warmup done ({args.warmup} steps)")
+
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    for i in range(args.steps):
+        step(events[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall0
+    t_ins = [e[0].elapsed_time(e[1]) / 1e3 for e in events]
+    t_q = [e[1].elapsed_time(e[2]) / 1e3 for e in events]
+    ti, tq = statistics.median(t_ins), statistics.median(t_q)
+    log(f"[bench] timed {args.steps} steps in {wall:.3f}s; insert median {ti * 1e3:.2f} ms, "
+        f"query median {tq * 1e3:.2f} ms")
+
+    positives = int(cs.count_positive(out))
+    ins_rate = v_ins / ti
+    q_rate = v_q / tq
+    value = (v_ins + v_q) / (ti + tq)
+
+    # byte model: insert = input + 64 B per shard run; query = input + hit bytes + 32 B per run
+    bytes_ins = genome.numel() + 64 * s_ins
+    bytes_q = reads.numel() + out.numel() + 32 * s_q
+    dom = "query" if tq >= ti else "insert"
+    t_dom, b_dom = (tq, bytes_q) if dom == "query" else (ti, bytes_ins)
+    achieved = b_dom / t_dom
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "k-mers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded genome and reads generated on the GPU)",
+        "config": {"workload": f"{wl.name}-scale plain GPU path",
+                   "genome_bp": wl.genome.total_len, "chromosomes": wl.genome.n_chrom,
+                   "reads": wl.reads.n_reads, "read_len": wl.reads.read_len,
+                   "k": wl.k, "s": wl.s, "m": wl.m, "hash_count": wl.h,
+                   "bits_per_kmer_requested": wl.bits_per_kmer,
+                   "filter_bits": filt.filter_bits(),
+                   "bits_per_kmer_effective": filt.filter_bits() / max(1, v_ins)},
+        "insert_kmers_per_s": ins_rate,
+        "query_kmers_per_s": q_rate,
+        "insert_ms": ti * 1e3,
+        "query_ms": tq * 1e3,
+        "insert_roofline_frac": bytes_ins / ti / HBM_PEAK,
+        "query_roofline_frac": bytes_q / tq / HBM_PEAK,
+        "query_positive_kmers": positives,
+        "query_positive_frac": positives / max(1, v_q),
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": dom, "alg_bytes_per_launch": b_dom},
+        "serve_p50_ms": None,
+        "serve_p99_ms": None,
+    }
+    if args.cpu_baseline == "on":
+        log("[bench] CPU baseline on a bounded sample")
+        result["cpu_baseline"] = cpu_baseline(genome, reads, wl, args.cpu_threads)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -86,6 +86,20 @@ SIGNATURES = [
 _lib = None
 
 
+def share_hip_runtime() -> None:
+    """Load torch's HIP runtime before any library of this package.
+
+    torch ships its own libamdhip64 with the soname libamdhip64.so.7. Loaded
+    first, it satisfies this package's libraries by soname, so the process
+    holds one HIP runtime and device pointers and streams are shared. Loading
+    these libraries first would bring in a second runtime next to torch's.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # C/C++ callers without torch use the system runtime
+        pass
+
+
 def load(path: str = LIB_PATH):
     """Load libcusbf_hip.so; raises if it has not been built."""
     global _lib
@@ -94,6 +108,7 @@ def load(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError(
             f"{path} is missing: build the HIP library first (python -m cusbf_amd.build)")
+    share_hip_runtime()
     lib = ctypes.CDLL(path)
     for name, restype, argtypes in SIGNATURES:
         fn = getattr(lib, name)

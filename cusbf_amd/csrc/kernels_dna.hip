@@ -34,6 +34,11 @@
 #include "launch.hpp"
 #include "sbf_common.hpp"
 
+// Minimum waves per SIMD the register allocation must allow.
+#ifndef SBF_DNA_WAVES_PER_EU
+#define SBF_DNA_WAVES_PER_EU 4
+#endif
+
 namespace sbf::dna {
 
 constexpr int kWave = 64;
@@ -55,7 +60,7 @@ struct Shape {
     static_assert(K >= 1 && K <= 32, "DNA k-mers hold at most 32 symbols");
     static_assert(M >= 1 && M <= K && S >= 1 && S <= K, "m and s must be in [1, k]");
     static_assert(kMinSpan <= kLaneWindows + 1, "minimizer span must fit two lanes");
-    static_assert(kFindereSpan <= kLaneWindows + 1, "findere span must fit two lanes");
+    static_assert(kFindereSpan <= 8, "findere span of the fast kernels is at most 8");
     static_assert(H == 4 || H == 8 || H == 12 || H == 16, "hash count must be 4, 8, 12 or 16");
 };
 
@@ -160,10 +165,19 @@ __device__ __forceinline__ uint64_t next_lane(uint64_t v) { return __shfl_down(v
 
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
-// s-mer hash -> four word masks (filter_impl.cuh:54-77).
+// Bit positions of one s-mer: 6 bits per hash index, hash i at bits [6i, 6i+6).
 template <int H>
-__device__ __forceinline__ void smer_masks(uint64_t h, uint64_t (&m)[4]) {
-    m[0] = m[1] = m[2] = m[3] = 0;
+struct PosCode {
+    static constexpr int kWords = (6 * H + 31) / 32;
+    uint32_t w[kWords];
+};
+
+// filter_impl.cuh:36-47: the H bit positions of an s-mer hash.
+template <int H>
+__device__ __forceinline__ PosCode<H> pos_code(uint64_t h) {
+    PosCode<H> c;
+#pragma unroll
+    for (int j = 0; j < PosCode<H>::kWords; ++j) c.w[j] = 0;
 #pragma unroll
     for (int i = 0; i < H; ++i) {
         uint32_t pos;
@@ -173,16 +187,43 @@ __device__ __forceinline__ void smer_masks(uint64_t h, uint64_t (&m)[4]) {
         } else {
             pos = static_cast<uint32_t>((h * salt(static_cast<uint32_t>(i))) >> 58);
         }
-        m[i & 3] |= 1ull << pos;
+        const int bit = 6 * i;
+        c.w[bit / 32] |= pos << (bit % 32);
+        if (bit % 32 > 26) c.w[bit / 32 + 1] |= pos >> (32 - bit % 32);
     }
+    return c;
 }
 
-// Shared front end: per-lane shard indices, window validity and s-mer hashes.
+template <int H>
+__device__ __forceinline__ uint32_t code_pos(const PosCode<H>& c, int i) {
+    const int bit = 6 * i;
+    uint32_t v = c.w[bit / 32] >> (bit % 32);
+    if (bit % 32 > 26) v |= c.w[bit / 32 + 1] << (32 - bit % 32);
+    return v & 63u;
+}
+
+// filter_impl.cuh:54-77: hash i sets bit pos_i of word i % 4.
+template <int H>
+__device__ __forceinline__ void code_masks(const PosCode<H>& c, uint64_t (&m)[4]) {
+    m[0] = m[1] = m[2] = m[3] = 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) m[i & 3] |= 1ull << code_pos<H>(c, i);
+}
+
+template <int H>
+__device__ __forceinline__ PosCode<H> next_lane_code(const PosCode<H>& c) {
+    PosCode<H> r;
+#pragma unroll
+    for (int j = 0; j < PosCode<H>::kWords; ++j) r.w[j] = __shfl_down(c.w[j], 1, kWave);
+    return r;
+}
+
+// Shared front end: per-lane shard indices, window validity and s-mer bit positions.
 template <int K, int S, int M, int H>
 struct LaneWindows {
     uint32_t shard[kLaneWindows];
-    uint32_t valid;                                   // bit o: window o valid and in range
-    uint64_t sh[kLaneWindows + Shape<K, S, M, H>::kFindereSpan - 1];  // s-mer hashes
+    uint32_t valid;  // bit o: window o valid and in range
+    PosCode<H> code[kLaneWindows + Shape<K, S, M, H>::kFindereSpan - 1];
 };
 
 template <int K, int S, int M, int H>
@@ -217,13 +258,8 @@ __device__ __forceinline__ void lane_windows(const uint32_t* lds_p, const uint32
 #pragma unroll
     for (int p = 0; p < kLaneWindows; ++p) {
         const uint64_t x = p == 0 ? top : ((top << (2 * p)) | (w2 >> (32 - 2 * p)));
-        const uint64_t mmer = M == 32 ? x : (x >> (64 - 2 * M));
-        const uint64_t smer = S == 32 ? x : (x >> (64 - 2 * S));
-        mh[p] = minimizer_hash64(mmer);
-        lw.sh[p] = hash64(smer);
+        mh[p] = minimizer_hash64(M == 32 ? x : (x >> (64 - 2 * M)));
     }
-#pragma unroll
-    for (int i = 0; i < F - 1; ++i) lw.sh[kLaneWindows + i] = next_lane(lw.sh[i]);
 
     uint64_t mn[kLaneWindows];
     if constexpr (W == kLaneWindows) {
@@ -260,6 +296,15 @@ __device__ __forceinline__ void lane_windows(const uint32_t* lds_p, const uint32
     // filter_ref.cuh:52-55 with the uint32_t narrowing of kernels.cuh:240-241.
 #pragma unroll
     for (int o = 0; o < kLaneWindows; ++o) lw.shard[o] = static_cast<uint32_t>(mn[o]) & shard_mask;
+
+    // s-mer bit positions (second pass keeps the minimizer temporaries dead)
+#pragma unroll
+    for (int p = 0; p < kLaneWindows; ++p) {
+        const uint64_t x = p == 0 ? top : ((top << (2 * p)) | (w2 >> (32 - 2 * p)));
+        lw.code[p] = pos_code<H>(hash64(S == 32 ? x : (x >> (64 - 2 * S))));
+    }
+#pragma unroll
+    for (int i = 0; i < F - 1; ++i) lw.code[kLaneWindows + i] = next_lane_code<H>(lw.code[i]);
 }
 
 template <bool Dense>
@@ -280,58 +325,164 @@ __device__ __forceinline__ void stage_tile(const SequenceInput& in, uint64_t fir
     }
 }
 
+// --- per-wave run slots ---------------------------------------------------------
+//
+// A run is a maximal sequence of consecutive valid windows of the wave tile
+// with the same shard (a super-k-mer cut at the tile edges). The first kSlots
+// runs of a tile get a slot in the wave's LDS: the query loads each run's
+// 32-byte shard once into its slot with dense, all-lane loads; the insert ORs
+// each run's masks into its slot and then issues one 64-bit atomic OR per
+// non-zero word. A tile with more runs than slots (possible only for
+// sequences whose minimizer changes at nearly every position) handles the
+// excess runs with direct global loads / atomics in the same pass.
+
+constexpr int kSlots = 256;
+
+struct WaveLds {
+    uint32_t p[kLdsWords];
+    uint32_t inv[kLdsWords];
+    uint32_t slot_shard[kSlots];
+    uint64_t slot_w[kSlots][4];
+};
+
+// Orders this wave's LDS writes before later LDS reads by its other lanes. LDS
+// operations of one wave execute in order, so it is enough to wait for them
+// and keep the compiler from moving accesses across this point; outstanding
+// global loads and atomics are not waited for.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), vmcnt and expcnt unconstrained
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
+// Exclusive prefix sum over the wave of c (< 64) and the wave total.
+__device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t c, int lane, uint32_t& total) {
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t base = 0;
+    total = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const uint64_t bal = __ballot((c >> b) & 1u);
+        base += static_cast<uint32_t>(__popcll(bal & below)) << b;
+        total += static_cast<uint32_t>(__popcll(bal)) << b;
+    }
+    return base;
+}
+
+// Bit o of the result: window o starts a run. Bit o of `cont`: window o
+// continues the run of the window before it (for o = 0, the previous lane's
+// last window).
+template <int K, int S, int M, int H>
+__device__ __forceinline__ uint32_t run_heads(const LaneWindows<K, S, M, H>& lw, int lane,
+                                              uint32_t& cont) {
+    const uint32_t prev_shard = __shfl_up(lw.shard[kLaneWindows - 1], 1, kWave);
+    const uint32_t prev_valid = __shfl_up((lw.valid >> (kLaneWindows - 1)) & 1u, 1, kWave);
+    uint32_t same = 0;
+#pragma unroll
+    for (int o = 1; o < kLaneWindows; ++o) same |= (lw.shard[o] == lw.shard[o - 1] ? 1u : 0u) << o;
+    same &= lw.valid & (lw.valid << 1);
+    if (lane > 0 && prev_valid && (lw.valid & 1u) && lw.shard[0] == prev_shard) same |= 1u;
+    cont = same;
+    return lw.valid & ~same;
+}
+
+// Run-slot bookkeeping shared by query and insert: publishes the shard of
+// every run head that has an LDS slot.
+template <int K, int S, int M, int H>
+__device__ __forceinline__ void publish_heads(const LaneWindows<K, S, M, H>& lw, uint32_t heads,
+                                              uint32_t base, WaveLds& L) {
+#pragma unroll
+    for (int o = 0; o < kLaneWindows; ++o) {
+        if ((heads >> o) & 1u) {
+            const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((1u << o) - 1u)));
+            if (slot < kSlots) L.slot_shard[slot] = lw.shard[o];
+        }
+    }
+}
+
 // --- kernels -------------------------------------------------------------------
 
 template <int K, int S, int M, int H, bool Dense>
-__global__ __launch_bounds__(kBlock) void contains_kernel(SequenceInput in, FilterView filter,
-                                                          uint64_t num_windows, uint8_t* out) {
+__device__ __forceinline__ void contains_tile(const SequenceInput& in, const FilterView& filter,
+                                              uint64_t num_windows, uint8_t* out, uint64_t tile,
+                                              int lane, WaveLds& L) {
     using Sh = Shape<K, S, M, H>;
     constexpr int F = Sh::kFindereSpan;
-    __shared__ uint32_t lds_p[kWavesPerBlock][kLdsWords];
-    __shared__ uint32_t lds_i[kWavesPerBlock][kLdsWords];
-
-    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
-    const uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wave;
     const uint64_t tile_first = tile * Sh::kWaveWindows;
 
-    stage_tile<Dense>(in, tile_first, lane, lds_p[wave], lds_i[wave]);
-    __syncthreads();
-    if (tile_first >= num_windows) return;
+    stage_tile<Dense>(in, tile_first, lane, L.p, L.inv);
+    wave_lds_sync();
 
     const bool owns = lane < Sh::kWindowLanes;
     const uint64_t first_window = tile_first + 16ull * static_cast<uint64_t>(lane);
     LaneWindows<K, S, M, H> lw;
-    lane_windows<K, S, M, H>(lds_p[wave], lds_i[wave], lane, owns, first_window, num_windows,
+    lane_windows<K, S, M, H>(L.p, L.inv, lane, owns, first_window, num_windows,
                              static_cast<uint32_t>(filter.num_shards - 1), lw);
-    if (!owns || first_window >= num_windows) return;
+    uint32_t cont;
+    const uint32_t heads = run_heads<K, S, M, H>(lw, lane, cont);
+    uint32_t total;
+    const uint32_t base = wave_exclusive_sum(static_cast<uint32_t>(__popc(heads)), lane, total);
+    publish_heads<K, S, M, H>(lw, heads, base, L);
+    wave_lds_sync();
 
-    const uint4* shards = reinterpret_cast<const uint4*>(filter.words);
+    // dense shard loads: lane q fetches slots q, q+64, q+128, q+192
+    const u32x4* shards = reinterpret_cast<const u32x4*>(filter.words);
+    const uint32_t n_slots = total < kSlots ? total : kSlots;
+    u32x4 wa[kSlots / kWave], wb[kSlots / kWave];
+#pragma unroll
+    for (int j = 0; j < kSlots / kWave; ++j) {
+        const uint32_t q = static_cast<uint32_t>(lane + j * kWave);
+        if (q < n_slots) {
+            const uint64_t sidx = L.slot_shard[q];
+            wa[j] = shards[2 * sidx];
+            wb[j] = shards[2 * sidx + 1];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kSlots / kWave; ++j) {
+        const uint32_t q = static_cast<uint32_t>(lane + j * kWave);
+        if (q < n_slots) {
+            u32x4* dst = reinterpret_cast<u32x4*>(L.slot_w[q]);
+            dst[0] = wa[j];
+            dst[1] = wb[j];
+        }
+    }
+    wave_lds_sync();
+
+    // findere test of every window against its run's shard words
     uint32_t result[4] = {0, 0, 0, 0};
     uint64_t ring[F][4];
 #pragma unroll
-    for (int p = 0; p < F - 1; ++p) smer_masks<H>(lw.sh[p], ring[p]);
+    for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
 #pragma unroll
     for (int o = 0; o < kLaneWindows; ++o) {
-        smer_masks<H>(lw.sh[o + F - 1], ring[(o + F - 1) % F]);
-        uint64_t wm[4] = {0, 0, 0, 0};
+        code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
+        if ((lw.valid >> o) & 1u) {
+            const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u;
+            uint64_t wm[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int d = 0; d < F; ++d) {
+            for (int d = 0; d < F; ++d) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) wm[q] |= ring[(o + d) % F][q];
+                for (int q = 0; q < 4; ++q) wm[q] |= ring[(o + d) % F][q];
+            }
+            uint64_t w[4];
+            if (slot < kSlots) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) w[q] = L.slot_w[slot][q];
+            } else {
+                const uint64_t* src = filter.words + 4ull * lw.shard[o];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) w[q] = src[q];
+            }
+            const uint64_t miss =
+                (wm[0] & ~w[0]) | (wm[1] & ~w[1]) | (wm[2] & ~w[2]) | (wm[3] & ~w[3]);
+            result[o >> 2] |= (miss == 0 ? 1u : 0u) << (8 * (o & 3));
         }
-        const uint64_t base = 2ull * lw.shard[o];
-        const uint4 a = shards[base];
-        const uint4 b = shards[base + 1];
-        const uint64_t s0 = (static_cast<uint64_t>(a.y) << 32) | a.x;
-        const uint64_t s1 = (static_cast<uint64_t>(a.w) << 32) | a.z;
-        const uint64_t s2 = (static_cast<uint64_t>(b.y) << 32) | b.x;
-        const uint64_t s3 = (static_cast<uint64_t>(b.w) << 32) | b.z;
-        const uint64_t miss = (wm[0] & ~s0) | (wm[1] & ~s1) | (wm[2] & ~s2) | (wm[3] & ~s3);
-        const uint32_t hit = (miss == 0) & ((lw.valid >> o) & 1u);
-        result[o >> 2] |= hit << (8 * (o & 3));
     }
+    wave_lds_sync();  // the next tile of this wave reuses the LDS
 
+    if (!owns || first_window >= num_windows) return;
     uint8_t* dst = out + first_window;
     if (first_window + kLaneWindows <= num_windows &&
         (reinterpret_cast<uintptr_t>(dst) & 15u) == 0) {
@@ -348,54 +499,47 @@ __global__ __launch_bounds__(kBlock) void contains_kernel(SequenceInput in, Filt
     }
 }
 
-__device__ __forceinline__ void flush_run(uint64_t* words, uint32_t shard, const uint64_t (&acc)[4]) {
-    uint64_t* dst = words + 4ull * shard;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (acc[q] != 0) {
-            __hip_atomic_fetch_or(dst + q, acc[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-}
-
 template <int K, int S, int M, int H, bool Dense>
-__global__ __launch_bounds__(kBlock) void insert_kernel(SequenceInput in, FilterView filter,
-                                                        uint64_t num_windows) {
+__device__ __forceinline__ void insert_tile(const SequenceInput& in, const FilterView& filter,
+                                            uint64_t num_windows, uint64_t tile, int lane,
+                                            WaveLds& L) {
     using Sh = Shape<K, S, M, H>;
     constexpr int F = Sh::kFindereSpan;
-    __shared__ uint32_t lds_p[kWavesPerBlock][kLdsWords];
-    __shared__ uint32_t lds_i[kWavesPerBlock][kLdsWords];
-
-    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
-    const uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wave;
     const uint64_t tile_first = tile * Sh::kWaveWindows;
 
-    stage_tile<Dense>(in, tile_first, lane, lds_p[wave], lds_i[wave]);
-    __syncthreads();
-    if (tile_first >= num_windows) return;
+    stage_tile<Dense>(in, tile_first, lane, L.p, L.inv);
+    // zero the mask slots (each lane its four)
+#pragma unroll
+    for (int j = 0; j < kSlots / kWave; ++j) {
+        const u32x4 z = {0u, 0u, 0u, 0u};
+        u32x4* dst = reinterpret_cast<u32x4*>(L.slot_w[lane + j * kWave]);
+        dst[0] = z;
+        dst[1] = z;
+    }
+    wave_lds_sync();
 
     const bool owns = lane < Sh::kWindowLanes;
     const uint64_t first_window = tile_first + 16ull * static_cast<uint64_t>(lane);
     LaneWindows<K, S, M, H> lw;
-    lane_windows<K, S, M, H>(lds_p[wave], lds_i[wave], lane, owns, first_window, num_windows,
+    lane_windows<K, S, M, H>(L.p, L.inv, lane, owns, first_window, num_windows,
                              static_cast<uint32_t>(filter.num_shards - 1), lw);
-    if (!owns || first_window >= num_windows) return;
+    uint32_t cont;
+    const uint32_t heads = run_heads<K, S, M, H>(lw, lane, cont);
+    uint32_t total;
+    const uint32_t base = wave_exclusive_sum(static_cast<uint32_t>(__popc(heads)), lane, total);
+    publish_heads<K, S, M, H>(lw, heads, base, L);
+    // bit o: window o is the last window of its run inside this lane
+    const uint32_t ends = lw.valid & ~(cont >> 1);
 
+    // each lane ORs the masks of its piece of every run into the run's slot
     uint64_t ring[F][4];
 #pragma unroll
-    for (int p = 0; p < F - 1; ++p) smer_masks<H>(lw.sh[p], ring[p]);
-
+    for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
     uint64_t acc[4] = {0, 0, 0, 0};
-    uint32_t cur = 0;
-    bool open = false;
 #pragma unroll
     for (int o = 0; o < kLaneWindows; ++o) {
-        smer_masks<H>(lw.sh[o + F - 1], ring[(o + F - 1) % F]);
-        const bool v = (lw.valid >> o) & 1u;
-        const bool cont = open && v && lw.shard[o] == cur;
-        if (open && !cont) flush_run(filter.words, cur, acc);
-        if (cont) {
+        code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
+        if ((cont >> o) & 1u) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc[q] |= ring[(o + F - 1) % F][q];
         } else {
@@ -407,26 +551,100 @@ __global__ __launch_bounds__(kBlock) void insert_kernel(SequenceInput in, Filter
                 acc[q] = x;
             }
         }
-        open = v;
-        cur = lw.shard[o];
+        if ((ends >> o) & 1u) {
+            const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u;
+            if (slot < kSlots) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    __hip_atomic_fetch_or(&L.slot_w[slot][q], acc[q], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            } else {
+                uint64_t* dst = filter.words + 4ull * lw.shard[o];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (acc[q] != 0) {
+                        __hip_atomic_fetch_or(dst + q, acc[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+            }
+        }
     }
-    if (open) flush_run(filter.words, cur, acc);
+    wave_lds_sync();
+
+    // dense global atomics: one 64-bit OR per non-zero word of each slotted run
+    const uint32_t n_slots = total < kSlots ? total : kSlots;
+#pragma unroll
+    for (int j = 0; j < kSlots / kWave; ++j) {
+        const uint32_t q = static_cast<uint32_t>(lane + j * kWave);
+        if (q < n_slots) {
+            uint64_t* dst = filter.words + 4ull * L.slot_shard[q];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const uint64_t m = L.slot_w[q][w];
+                if (m != 0) __hip_atomic_fetch_or(dst + w, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    wave_lds_sync();  // the next tile of this wave reuses the LDS
+}
+
+template <int K, int S, int M, int H, bool Dense>
+__global__ __launch_bounds__(kBlock, SBF_DNA_WAVES_PER_EU) void contains_kernel(SequenceInput in, FilterView filter,
+                                                          uint64_t num_windows, uint64_t num_tiles,
+                                                          uint8_t* out) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+    for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wave; tile < num_tiles;
+         tile += stride) {
+        contains_tile<K, S, M, H, Dense>(in, filter, num_windows, out, tile, lane, lds[wave]);
+    }
+}
+
+template <int K, int S, int M, int H, bool Dense>
+__global__ __launch_bounds__(kBlock, SBF_DNA_WAVES_PER_EU) void insert_kernel(SequenceInput in, FilterView filter,
+                                                        uint64_t num_windows, uint64_t num_tiles) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+    for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wave; tile < num_tiles;
+         tile += stride) {
+        insert_tile<K, S, M, H, Dense>(in, filter, num_windows, tile, lane, lds[wave]);
+    }
 }
 
 // --- host launchers ----------------------------------------------------------
 
+// Grid of at most kMaxBlocks blocks (a dispatch must stay below 2^32 work-items);
+// waves stride over the tiles.
+constexpr uint64_t kMaxBlocks = 1ull << 16;
+
+template <int K, int S, int M, int H>
+uint32_t tile_grid(uint64_t num_windows, uint64_t& num_tiles) {
+    using Sh = Shape<K, S, M, H>;
+    num_tiles = (num_windows + Sh::kWaveWindows - 1) / Sh::kWaveWindows;
+    uint64_t blocks = (num_tiles + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+    return static_cast<uint32_t>(blocks);
+}
+
 template <int K, int S, int M, int H>
 hipError_t launch_insert(const SequenceInput& in, FilterView f, hipStream_t stream) {
-    using Sh = Shape<K, S, M, H>;
     const uint64_t n = in.num_symbols < static_cast<uint64_t>(K) ? 0 : in.num_symbols - K + 1;
     if (n == 0) return hipSuccess;
-    const uint64_t grid = (n + Sh::kBlockWindows - 1) / Sh::kBlockWindows;
+    uint64_t tiles;
+    const uint32_t grid = tile_grid<K, S, M, H>(n, tiles);
     if (in.dense) {
-        hipLaunchKernelGGL((insert_kernel<K, S, M, H, true>), dim3(static_cast<uint32_t>(grid)),
-                           dim3(kBlock), 0, stream, in, f, n);
+        (void)hipGetLastError();  // report only this launch
+        hipLaunchKernelGGL((insert_kernel<K, S, M, H, true>), dim3(grid), dim3(kBlock), 0, stream,
+                           in, f, n, tiles);
     } else {
-        hipLaunchKernelGGL((insert_kernel<K, S, M, H, false>), dim3(static_cast<uint32_t>(grid)),
-                           dim3(kBlock), 0, stream, in, f, n);
+        (void)hipGetLastError();  // report only this launch
+        hipLaunchKernelGGL((insert_kernel<K, S, M, H, false>), dim3(grid), dim3(kBlock), 0, stream,
+                           in, f, n, tiles);
     }
     return hipGetLastError();
 }
@@ -434,17 +652,18 @@ hipError_t launch_insert(const SequenceInput& in, FilterView f, hipStream_t stre
 template <int K, int S, int M, int H>
 hipError_t launch_contains(const SequenceInput& in, FilterView f, uint8_t* out,
                            hipStream_t stream) {
-    using Sh = Shape<K, S, M, H>;
     const uint64_t n = in.num_symbols < static_cast<uint64_t>(K) ? 0 : in.num_symbols - K + 1;
     if (n == 0) return hipSuccess;
-    const uint64_t grid = (n + Sh::kBlockWindows - 1) / Sh::kBlockWindows;
+    uint64_t tiles;
+    const uint32_t grid = tile_grid<K, S, M, H>(n, tiles);
     if (in.dense) {
-        hipLaunchKernelGGL((contains_kernel<K, S, M, H, true>), dim3(static_cast<uint32_t>(grid)),
-                           dim3(kBlock), 0, stream, in, f, n, out);
+        (void)hipGetLastError();  // report only this launch
+        hipLaunchKernelGGL((contains_kernel<K, S, M, H, true>), dim3(grid), dim3(kBlock), 0, stream,
+                           in, f, n, tiles, out);
     } else {
-        hipLaunchKernelGGL((contains_kernel<K, S, M, H, false>),
-                           dim3(static_cast<uint32_t>(grid)), dim3(kBlock), 0, stream, in, f, n,
-                           out);
+        (void)hipGetLastError();  // report only this launch
+        hipLaunchKernelGGL((contains_kernel<K, S, M, H, false>), dim3(grid), dim3(kBlock), 0,
+                           stream, in, f, n, tiles, out);
     }
     return hipGetLastError();
 }

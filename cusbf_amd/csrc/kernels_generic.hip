@@ -18,6 +18,7 @@ namespace sbf::generic {
 
 constexpr int kBlock = 256;
 constexpr int kMaxK = 64;
+constexpr uint64_t kMaxBlocks = 1ull << 16;  // 2^24 work-items per dispatch at most
 
 // Tile of encoded symbols [first, first + count); symbols outside [0, n) get
 // the invalid sentinel (they only occur in windows that are never reported).
@@ -84,22 +85,27 @@ __device__ __forceinline__ void masks(const KernelConfig& c, uint64_t packed, ui
     }
 }
 
+// Blocks stride over tiles of kBlock windows: a dispatch must stay below 2^32
+// work-items, so grids are capped and large inputs loop.
 __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, SequenceInput in,
                                                         FilterView f, uint64_t num_windows) {
     __shared__ uint8_t tile[kBlock + kMaxK];
-    const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock;
-    load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, tile);
-    __syncthreads();
-    const uint64_t j = first + threadIdx.x;
-    if (j >= num_windows) return;
-    const Window w = read_window(c, tile + threadIdx.x);
-    if (!w.valid) return;
-    const uint32_t shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
-    uint64_t m[4];
-    masks(c, w.packed, m);
-    uint64_t* dst = f.words + 4ull * shard;
-    for (int q = 0; q < 4; ++q) {
-        if (m[q] != 0) __hip_atomic_fetch_or(dst + q, m[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
+         first += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        __syncthreads();
+        load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, tile);
+        __syncthreads();
+        const uint64_t j = first + threadIdx.x;
+        if (j >= num_windows) continue;
+        const Window w = read_window(c, tile + threadIdx.x);
+        if (!w.valid) continue;
+        const uint32_t shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
+        uint64_t m[4];
+        masks(c, w.packed, m);
+        uint64_t* dst = f.words + 4ull * shard;
+        for (int q = 0; q < 4; ++q) {
+            if (m[q] != 0) __hip_atomic_fetch_or(dst + q, m[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -107,23 +113,26 @@ __global__ __launch_bounds__(kBlock) void contains_kernel(KernelConfig c, Sequen
                                                           FilterView f, uint64_t num_windows,
                                                           uint8_t* out) {
     __shared__ uint8_t tile[kBlock + kMaxK];
-    const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock;
-    load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, tile);
-    __syncthreads();
-    const uint64_t j = first + threadIdx.x;
-    if (j >= num_windows) return;
-    const Window w = read_window(c, tile + threadIdx.x);
-    if (!w.valid) {
-        out[j] = 0;
-        return;
+    for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
+         first += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        __syncthreads();
+        load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, tile);
+        __syncthreads();
+        const uint64_t j = first + threadIdx.x;
+        if (j >= num_windows) continue;
+        const Window w = read_window(c, tile + threadIdx.x);
+        if (!w.valid) {
+            out[j] = 0;
+            continue;
+        }
+        const uint32_t shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
+        uint64_t m[4];
+        masks(c, w.packed, m);
+        const uint64_t* src = f.words + 4ull * shard;
+        uint64_t miss = 0;
+        for (int q = 0; q < 4; ++q) miss |= m[q] & ~src[q];
+        out[j] = miss == 0 ? 1 : 0;
     }
-    const uint32_t shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
-    uint64_t m[4];
-    masks(c, w.packed, m);
-    const uint64_t* src = f.words + 4ull * shard;
-    uint64_t miss = 0;
-    for (int q = 0; q < 4; ++q) miss |= m[q] & ~src[q];
-    out[j] = miss == 0 ? 1 : 0;
 }
 
 // Valid windows and shard-run heads; the tile starts one window early so each
@@ -135,28 +144,32 @@ __global__ __launch_bounds__(kBlock) void count_kernel(KernelConfig c, SequenceI
     __shared__ uint32_t shard_of[kBlock + 1];
     __shared__ uint8_t valid_of[kBlock + 1];
     __shared__ unsigned long long totals[2];
-    const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock;
     if (threadIdx.x < 2) totals[threadIdx.x] = 0;
-    load_tile(c, in, static_cast<int64_t>(first) - 1, kBlock + static_cast<int>(c.k), tile);
-    __syncthreads();
-    for (int t = static_cast<int>(threadIdx.x); t < kBlock + 1; t += kBlock) {
-        const int64_t j = static_cast<int64_t>(first) - 1 + t;
-        bool valid = false;
-        uint32_t shard = 0;
-        if (j >= 0 && static_cast<uint64_t>(j) < num_windows) {
-            const Window w = read_window(c, tile + t);
-            valid = w.valid;
-            if (valid) shard = static_cast<uint32_t>(minimizer(c, w.packed) & (num_shards - 1));
+    for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
+         first += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        __syncthreads();
+        load_tile(c, in, static_cast<int64_t>(first) - 1, kBlock + static_cast<int>(c.k), tile);
+        __syncthreads();
+        for (int t = static_cast<int>(threadIdx.x); t < kBlock + 1; t += kBlock) {
+            const int64_t j = static_cast<int64_t>(first) - 1 + t;
+            bool valid = false;
+            uint32_t shard = 0;
+            if (j >= 0 && static_cast<uint64_t>(j) < num_windows) {
+                const Window w = read_window(c, tile + t);
+                valid = w.valid;
+                if (valid) shard = static_cast<uint32_t>(minimizer(c, w.packed) & (num_shards - 1));
+            }
+            shard_of[t] = shard;
+            valid_of[t] = valid;
         }
-        shard_of[t] = shard;
-        valid_of[t] = valid;
-    }
-    __syncthreads();
-    const uint64_t j = first + threadIdx.x;
-    if (j < num_windows && valid_of[threadIdx.x + 1]) {
-        const bool head = !valid_of[threadIdx.x] || shard_of[threadIdx.x] != shard_of[threadIdx.x + 1];
-        atomicAdd(&totals[0], 1ull);
-        if (head) atomicAdd(&totals[1], 1ull);
+        __syncthreads();
+        const uint64_t j = first + threadIdx.x;
+        if (j < num_windows && valid_of[threadIdx.x + 1]) {
+            const bool head =
+                !valid_of[threadIdx.x] || shard_of[threadIdx.x] != shard_of[threadIdx.x + 1];
+            atomicAdd(&totals[0], 1ull);
+            if (head) atomicAdd(&totals[1], 1ull);
+        }
     }
     __syncthreads();
     if (threadIdx.x < 2 && totals[threadIdx.x] != 0) atomicAdd(&counts[threadIdx.x], totals[threadIdx.x]);
@@ -171,22 +184,26 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(KernelConfig c, const char
     __shared__ uint8_t codes[kBlock * 64];
     const uint32_t per_word = 64u / c.symbol_bits;
     const uint64_t sym_mask = (1ull << c.symbol_bits) - 1ull;
-    const uint64_t first_symbol = static_cast<uint64_t>(blockIdx.x) * kBlock * per_word;
     const uint32_t tile_symbols = kBlock * per_word;
-    for (uint32_t i = threadIdx.x; i < tile_symbols; i += kBlock) {
-        const uint64_t p = first_symbol + i;
-        uint8_t v = 0;
-        if (p < num_symbols) v = static_cast<uint8_t>(encode_symbol(c, seq + p * c.symbol_width) & sym_mask);
-        codes[i] = v;
-    }
-    __syncthreads();
-    const uint64_t word = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     const uint64_t num_words = (num_symbols + per_word - 1) / per_word;
-    if (word >= num_words) return;
-    uint64_t packed = 0;
-    const uint8_t* src = codes + threadIdx.x * per_word;
-    for (uint32_t i = 0; i < per_word; ++i) packed |= static_cast<uint64_t>(src[i]) << (i * c.symbol_bits);
-    words[word] = packed;
+    for (uint64_t first_word = static_cast<uint64_t>(blockIdx.x) * kBlock; first_word < num_words;
+         first_word += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint64_t first_symbol = first_word * per_word;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < tile_symbols; i += kBlock) {
+            const uint64_t p = first_symbol + i;
+            uint8_t v = 0;
+            if (p < num_symbols) v = static_cast<uint8_t>(encode_symbol(c, seq + p * c.symbol_width) & sym_mask);
+            codes[i] = v;
+        }
+        __syncthreads();
+        const uint64_t word = first_word + threadIdx.x;
+        if (word >= num_words) continue;
+        uint64_t packed = 0;
+        const uint8_t* src = codes + threadIdx.x * per_word;
+        for (uint32_t i = 0; i < per_word; ++i) packed |= static_cast<uint64_t>(src[i]) << (i * c.symbol_bits);
+        words[word] = packed;
+    }
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
@@ -233,21 +250,23 @@ __global__ __launch_bounds__(kBlock) void count_ones_kernel(const uint8_t* hits,
     if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
 }
 
-// count_positive_kmers.cuh:21-45, one wave per record.
+// count_positive_kmers.cuh:21-45, one wave per record (waves stride over records).
 __global__ __launch_bounds__(kBlock) void count_ones_per_record_kernel(const uint8_t* hits,
                                                                        const uint64_t* offsets,
                                                                        const uint64_t* counts,
                                                                        uint64_t num_records,
                                                                        uint64_t* out) {
-    const uint64_t record = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / 64;
     const int lane = static_cast<int>(threadIdx.x & 63);
-    if (record >= num_records) return;
-    const uint64_t begin = offsets[record];
-    const uint64_t n = counts[record];
-    unsigned long long acc = 0;
-    for (uint64_t i = lane; i < n; i += 64) acc += hits[begin + i];
-    acc = wave_sum(acc);
-    if (lane == 0) out[record] = acc;
+    const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / 64);
+    for (uint64_t record = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / 64;
+         record < num_records; record += waves) {
+        const uint64_t begin = offsets[record];
+        const uint64_t n = counts[record];
+        unsigned long long acc = 0;
+        for (uint64_t i = lane; i < n; i += 64) acc += hits[begin + i];
+        acc = wave_sum(acc);
+        if (lane == 0) out[record] = acc;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void fill_kernel(uint64_t* dst, uint64_t value, uint64_t n) {
@@ -276,7 +295,8 @@ hipError_t generic_insert(const KernelConfig& c, const SequenceInput& in, Filter
                           hipStream_t stream) {
     const uint64_t n = window_count(c, in);
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(insert_kernel, dim3(grid_for(n, kBlock, 0xFFFFFFFFull)), dim3(kBlock), 0,
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(insert_kernel, dim3(grid_for(n, kBlock, kMaxBlocks)), dim3(kBlock), 0,
                        stream, c, in, f, n);
     return hipGetLastError();
 }
@@ -285,7 +305,8 @@ hipError_t generic_contains(const KernelConfig& c, const SequenceInput& in, Filt
                             uint8_t* out, hipStream_t stream) {
     const uint64_t n = window_count(c, in);
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(contains_kernel, dim3(grid_for(n, kBlock, 0xFFFFFFFFull)), dim3(kBlock), 0,
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(contains_kernel, dim3(grid_for(n, kBlock, kMaxBlocks)), dim3(kBlock), 0,
                        stream, c, in, f, n, out);
     return hipGetLastError();
 }
@@ -294,7 +315,8 @@ hipError_t generic_count(const KernelConfig& c, const SequenceInput& in, uint64_
                          uint64_t* d_counts, hipStream_t stream) {
     const uint64_t n = window_count(c, in);
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(count_kernel, dim3(grid_for(n, kBlock, 0xFFFFFFFFull)), dim3(kBlock), 0,
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(count_kernel, dim3(grid_for(n, kBlock, kMaxBlocks)), dim3(kBlock), 0,
                        stream, c, in, num_shards, n,
                        reinterpret_cast<unsigned long long*>(d_counts));
     return hipGetLastError();
@@ -305,12 +327,14 @@ hipError_t pack_dense(const KernelConfig& c, const char* seq, uint64_t num_symbo
     if (num_symbols == 0) return hipSuccess;
     const uint64_t per_word = 64u / c.symbol_bits;
     const uint64_t num_words = (num_symbols + per_word - 1) / per_word;
-    hipLaunchKernelGGL(pack_kernel, dim3(grid_for(num_words, kBlock, 0xFFFFFFFFull)), dim3(kBlock),
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(pack_kernel, dim3(grid_for(num_words, kBlock, kMaxBlocks)), dim3(kBlock),
                        0, stream, c, seq, num_symbols, words);
     return hipGetLastError();
 }
 
 hipError_t popcount_words(const uint64_t* words, uint64_t n, uint64_t* d_out, hipStream_t stream) {
+    (void)hipGetLastError();  // report only this launch
     hipLaunchKernelGGL(popcount_kernel, dim3(grid_for(n, kBlock * 8, 4096)), dim3(kBlock), 0,
                        stream, words, n, reinterpret_cast<unsigned long long*>(d_out));
     return hipGetLastError();
@@ -318,6 +342,7 @@ hipError_t popcount_words(const uint64_t* words, uint64_t n, uint64_t* d_out, hi
 
 hipError_t count_ones(const uint8_t* hits, uint64_t n, uint64_t* d_out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
+    (void)hipGetLastError();  // report only this launch
     hipLaunchKernelGGL(count_ones_kernel, dim3(grid_for(n, kBlock * 64, 4096)), dim3(kBlock), 0,
                        stream, hits, n, reinterpret_cast<unsigned long long*>(d_out));
     return hipGetLastError();
@@ -327,14 +352,16 @@ hipError_t count_ones_per_record(const uint8_t* hits, const uint64_t* offsets,
                                  const uint64_t* counts, uint64_t num_records, uint64_t* d_out,
                                  hipStream_t stream) {
     if (num_records == 0) return hipSuccess;
+    (void)hipGetLastError();  // report only this launch
     hipLaunchKernelGGL(count_ones_per_record_kernel,
-                       dim3(grid_for(num_records, kBlock / 64, 0xFFFFFFFFull)), dim3(kBlock), 0,
+                       dim3(grid_for(num_records, kBlock / 64, kMaxBlocks)), dim3(kBlock), 0,
                        stream, hits, offsets, counts, num_records, d_out);
     return hipGetLastError();
 }
 
 hipError_t fill_u64(uint64_t* dst, uint64_t value, uint64_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
+    (void)hipGetLastError();  // report only this launch
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, kBlock * 4, 65536)), dim3(kBlock), 0, stream,
                        dst, value, n);
     return hipGetLastError();

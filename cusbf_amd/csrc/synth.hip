@@ -153,9 +153,10 @@ int cusbf_synth_genome(uint64_t seed, uint32_t gc_ppm, const uint64_t* d_chrom_s
                    reinterpret_cast<const RepeatCopy*>(d_repeats), num_repeats};
     const uint64_t threads = (buffer_len + 63) / 64;
     const uint64_t blocks = (threads + 255) / 256;
+    (void)hipGetLastError();  // report only this launch
     hipLaunchKernelGGL(genome_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), p, d_out);
-    return hipGetLastError() == hipSuccess ? 0 : 1;
+    return static_cast<int>(hipGetLastError());
 }
 
 int cusbf_synth_reads(uint64_t seed, const uint8_t* d_genome, uint64_t genome_len,
@@ -167,9 +168,10 @@ int cusbf_synth_reads(uint64_t seed, const uint8_t* d_genome, uint64_t genome_le
     const uint64_t total = num_reads * (static_cast<uint64_t>(read_len) + 1) - 1;
     const uint64_t threads = (total + 15) / 16;
     const uint64_t blocks = (threads + 255) / 256;
+    (void)hipGetLastError();  // report only this launch
     hipLaunchKernelGGL(reads_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), p, d_out);
-    return hipGetLastError() == hipSuccess ? 0 : 1;
+    return static_cast<int>(hipGetLastError());
 }
 
 uint64_t cusbf_synth_repeat_copy_size(void) { return sizeof(RepeatCopy); }

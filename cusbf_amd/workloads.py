@@ -1,0 +1,211 @@
+"""Seeded synthetic workloads of the benchmark configurations, generated on the GPU.
+
+The host plans chromosome lengths and repeat copies with numpy (small arrays);
+libcusbf_synth.so writes every base on the device as a pure function of
+(seed, position). Chromosomes form one record batch joined by single 'N'
+separators; reads are fixed-length records joined by 'N'.
+
+Configurations (sizes as in the benchmark definitions):
+  small     10 Mbp uniform ACGT; query it and an independent 10 Mbp sequence
+  nematode  100 Mbp, 35% GC, 6 chromosomes of 12-21 Mbp, 5% tandem and
+            interspersed repeats at 5% divergence, 85.7 bits/k-mer;
+            20M 150 bp reads, half from the genome with 0.5% substitutions
+  human     3.1 Gbp, 41% GC, 24 chromosomes of 50-250 Mbp, 45% repeat copies of
+            300 bp-6 kbp at 2-15% divergence, 21.4 bits/k-mer; 100M 150 bp
+            reads, half from the genome with 1% substitutions
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import build as _build
+
+REPEAT_DTYPE = np.dtype([("dst", "<u8"), ("len", "<u4"), ("family", "<u4"), ("offset", "<u4"),
+                         ("period", "<u4"), ("div_ppm", "<u4"), ("pad", "<u4")])
+
+_synth = None
+
+
+def synth_lib():
+    global _synth
+    if _synth is None:
+        path = _build.SYNTH_LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build with python -m cusbf_amd.build")
+        from ._lib import share_hip_runtime
+
+        share_hip_runtime()
+        L = ctypes.CDLL(path)
+        P, u32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+        L.cusbf_synth_genome.argtypes = [u64, u32, P, u32, P, u64, P, u64, P]
+        L.cusbf_synth_genome.restype = ctypes.c_int
+        L.cusbf_synth_reads.argtypes = [u64, P, u64, u64, u64, u32, u32, u32, P, P]
+        L.cusbf_synth_reads.restype = ctypes.c_int
+        L.cusbf_synth_repeat_copy_size.restype = u64
+        assert L.cusbf_synth_repeat_copy_size() == REPEAT_DTYPE.itemsize
+        _synth = L
+    return _synth
+
+
+@dataclass
+class GenomeSpec:
+    name: str
+    total_len: int
+    n_chrom: int
+    chrom_min: int
+    chrom_max: int
+    gc: float
+    repeat_frac: float
+    repeat_len: tuple = (300, 6000)
+    divergence: tuple = (0.02, 0.15)
+    tandem_frac: float = 0.0     # fraction of repeat bases in tandem arrays
+    n_families: int = 500
+    seed: int = 1
+
+
+@dataclass
+class ReadSpec:
+    n_reads: int
+    read_len: int = 150
+    genome_frac: float = 0.5
+    substitution: float = 0.01
+    seed: int = 2
+
+
+@dataclass
+class Workload:
+    name: str
+    k: int = 31
+    s: int = 28
+    m: int = 16
+    h: int = 4
+    bits_per_kmer: float = 21.4
+    genome: Optional[GenomeSpec] = None
+    reads: Optional[ReadSpec] = None
+    extra: dict = field(default_factory=dict)
+
+
+WORKLOADS = {
+    "nematode": Workload(
+        "nematode", bits_per_kmer=85.7,
+        genome=GenomeSpec("nematode", 100_000_000, 6, 12_000_000, 21_000_000, 0.35, 0.05,
+                          repeat_len=(300, 6000), divergence=(0.05, 0.05), tandem_frac=0.5,
+                          seed=11),
+        reads=ReadSpec(20_000_000, 150, 0.5, 0.005, seed=12)),
+    "human": Workload(
+        "human", bits_per_kmer=21.4,
+        genome=GenomeSpec("human", 3_100_000_000, 24, 50_000_000, 250_000_000, 0.41, 0.45,
+                          repeat_len=(300, 6000), divergence=(0.02, 0.15), seed=21),
+        reads=ReadSpec(100_000_000, 150, 0.5, 0.01, seed=22)),
+}
+
+
+def chromosome_lengths(spec: GenomeSpec) -> np.ndarray:
+    """n_chrom lengths in [chrom_min, chrom_max] summing to total_len."""
+    rng = np.random.default_rng(spec.seed)
+    lens = rng.uniform(spec.chrom_min, spec.chrom_max, spec.n_chrom)
+    for _ in range(64):  # rescale the free part, clip to the bounds, repeat
+        residual = spec.total_len - lens.sum()
+        if abs(residual) < 1:
+            break
+        room = (spec.chrom_max - lens) if residual > 0 else (lens - spec.chrom_min)
+        if room.sum() <= 0:
+            break
+        lens = np.clip(lens + residual * room / room.sum(), spec.chrom_min, spec.chrom_max)
+    lens = np.floor(lens).astype(np.int64)
+    lens[np.argmax(lens)] += spec.total_len - int(lens.sum())
+    return lens
+
+
+def plan_repeats(spec: GenomeSpec) -> np.ndarray:
+    """Disjoint repeat copies covering repeat_frac of the genome, sorted by position."""
+    rng = np.random.default_rng(spec.seed + 1000)
+    target = int(spec.repeat_frac * spec.total_len)
+    if target == 0:
+        return np.zeros(0, dtype=REPEAT_DTYPE)
+    lo, hi = spec.repeat_len
+    fam_len = np.exp(rng.uniform(np.log(lo), np.log(hi), spec.n_families)).astype(np.int64)
+    tandem_target = int(spec.tandem_frac * target)
+    # interspersed copies: fragments of family consensus sequences
+    est = max(16, int(1.3 * (target - tandem_target) / max(1.0, 0.75 * fam_len.mean())) + 16)
+    fam = rng.integers(0, spec.n_families, est)
+    length = np.maximum(lo // 2, (fam_len[fam] * rng.uniform(0.5, 1.0, est)).astype(np.int64))
+    keep = np.cumsum(length) <= target - tandem_target
+    fam, length = fam[keep], length[keep]
+    offset = ((fam_len[fam] - length).clip(min=0) * rng.random(fam.size)).astype(np.int64)
+    period = np.zeros(fam.size, dtype=np.int64)
+    # tandem arrays: a 2-100 bp unit repeated over 100-3000 bp
+    if tandem_target:
+        n_t = max(1, int(tandem_target / 1550) + 16)
+        t_len = rng.integers(100, 3000, n_t)
+        keep = np.cumsum(t_len) <= tandem_target
+        t_len = t_len[keep]
+        fam = np.concatenate([fam, spec.n_families + rng.integers(0, 1 << 20, t_len.size)])
+        length = np.concatenate([length, t_len])
+        offset = np.concatenate([offset, np.zeros(t_len.size, dtype=np.int64)])
+        period = np.concatenate([period, rng.integers(2, 101, t_len.size)])
+    n = length.size
+    order = rng.permutation(n)
+    fam, length, offset, period = fam[order], length[order], offset[order], period[order]
+    div = rng.uniform(spec.divergence[0], spec.divergence[1], n)
+    # place copies in order with random gaps summing to the unique sequence length
+    free = spec.total_len - int(length.sum())
+    cuts = np.sort(rng.integers(0, free + 1, n))
+    gaps = np.diff(np.concatenate([[0], cuts]))
+    dst = np.cumsum(gaps) + np.concatenate([[0], np.cumsum(length)[:-1]])
+    out = np.zeros(n, dtype=REPEAT_DTYPE)
+    out["dst"] = dst
+    out["len"] = length
+    out["family"] = fam
+    out["offset"] = offset
+    out["period"] = period
+    out["div_ppm"] = np.round(div * 1e6).astype(np.uint32)
+    return out
+
+
+def generate_genome(spec: GenomeSpec, device="cuda"):
+    """(device uint8 buffer with 'N' between chromosomes, chromosome lengths)."""
+    import torch
+
+    lens = chromosome_lengths(spec)
+    starts = np.concatenate([[0], np.cumsum(lens + 1)[:-1]]).astype(np.uint64)
+    buffer_len = int(lens.sum()) + spec.n_chrom - 1
+    reps = plan_repeats(spec)
+    d_starts = torch.from_numpy(starts.view(np.int64)).to(device)
+    d_reps = torch.from_numpy(np.frombuffer(reps.tobytes() or b"\0" * 32, dtype=np.uint8).copy()).to(device)
+    out = torch.empty(buffer_len, dtype=torch.uint8, device=device)
+    rc = synth_lib().cusbf_synth_genome(spec.seed, int(spec.gc * 1e6), d_starts.data_ptr(),
+                                        spec.n_chrom, d_reps.data_ptr(), reps.size,
+                                        out.data_ptr(), buffer_len, 0)
+    if rc != 0:
+        raise RuntimeError(f"genome generation launch failed (hip error {rc})")
+    torch.cuda.synchronize()
+    return out, lens
+
+
+def generate_reads(spec: ReadSpec, genome, device="cuda", gc: float = 0.5):
+    """Device buffer of spec.n_reads records of read_len bases joined by 'N'."""
+    import torch
+
+    total = spec.n_reads * (spec.read_len + 1) - 1
+    out = torch.empty(total, dtype=torch.uint8, device=device)
+    genome_reads = int(spec.n_reads * spec.genome_frac)
+    rc = synth_lib().cusbf_synth_reads(spec.seed, genome.data_ptr(), genome.numel(),
+                                       spec.n_reads, genome_reads, spec.read_len,
+                                       int(spec.substitution * 1e6), int(gc * 1e6),
+                                       out.data_ptr(), 0)
+    if rc != 0:
+        raise RuntimeError(f"read generation launch failed (hip error {rc})")
+    torch.cuda.synchronize()
+    return out
+
+
+def uniform_dna(n: int, seed: int, device="cuda"):
+    """n uniform random ACGT bases on the device."""
+    spec = GenomeSpec("uniform", n, 1, n, n, 0.5, 0.0, seed=seed)
+    return generate_genome(spec, device)[0]
