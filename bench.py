@@ -41,7 +41,23 @@ def parse_args():
     ap.add_argument("--reads", type=int, default=0, help="override the number of reads")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--probes", default="on", choices=["on", "off"],
+                    help="measure stream-copy / random-sector ceilings after the timed steps")
     return ap.parse_args()
+
+
+def committed_traffic(workload: str, kernel: str):
+    """Calibrated HBM bytes per launch of `kernel` from the committed counter summary."""
+    name = {"insert": "insert_kernel<31, 28, 16, 4, false>",
+            "query": "contains_kernel<31, 28, 16, 4, false>"}[kernel]
+    path = os.path.join(ROOT, "profiles", f"dna_insert_query_{workload}_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.split("|")]
+        if len(cells) >= 5 and cells[1] == name:
+            return float(cells[4])
+    return None
 
 
 def cpu_baseline(genome, reads, wl, threads: int):
@@ -166,6 +182,27 @@ def main() -> int:
     dom = "query" if tq >= ti else "insert"
     t_dom, b_dom = (tq, bytes_q) if dom == "query" else (ti, bytes_ins)
     achieved = b_dom / t_dom
+
+    ceilings = None
+    if args.probes == "on":
+        from cusbf_amd import probes
+
+        log("[bench] measuring memory ceilings")
+        words = filt.words_device_ptr()
+        sectors = filt.num_shards()
+        copy_bytes = min(reads.numel(), 1 << 32)
+        copy_bw = probes.stream_copy(reads.data_ptr(), out.data_ptr(), copy_bytes)
+        gather = probes.random_gather(words, sectors, min(s_q, 2_000_000_000))
+        update = probes.random_sector_update(words, sectors, s_ins)
+        ceilings = {
+            "stream_copy_GBps": copy_bw / 1e9,
+            "random_32B_gather_per_s": gather,
+            "random_32B_gather_GBps": gather * 32 / 1e9,
+            "random_32B_update_per_s": update,
+            "buffer_bytes": sectors * 32,
+        }
+        log(f"[bench] copy {copy_bw / 1e9:.0f} GB/s, random gather {gather / 1e9:.2f} G/s, "
+            f"random update {update / 1e9:.2f} G/s")
     result = {
         "metric": METRIC,
         "value": value,
@@ -190,12 +227,20 @@ def main() -> int:
         "query_kmers_per_s": q_rate,
         "insert_ms": ti * 1e3,
         "query_ms": tq * 1e3,
+        "insert_shard_runs": s_ins,
+        "query_shard_runs": s_q,
         "insert_roofline_frac": bytes_ins / ti / HBM_PEAK,
         "query_roofline_frac": bytes_q / tq / HBM_PEAK,
+        "insert_random_sector_frac": (s_ins / ti / ceilings["random_32B_update_per_s"]
+                                      if ceilings else None),
+        "query_random_sector_frac": (s_q / tq / ceilings["random_32B_gather_per_s"]
+                                     if ceilings else None),
         "query_positive_kmers": positives,
         "query_positive_frac": positives / max(1, v_q),
+        "ceilings": ceilings,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": committed_traffic(wl.name, dom),
                      "kernel": dom, "alg_bytes_per_launch": b_dom},
         "serve_p50_ms": None,
         "serve_p99_ms": None,
