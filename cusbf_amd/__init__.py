@@ -22,10 +22,12 @@ from .filter import (  # noqa: F401
     pack_dense_sequence_device,
 )
 from .normalize import NormalizedRecord, normalize_record_batch, query_layout  # noqa: F401
+from .serve import Server, query_record_batch_hits  # noqa: F401
 
 __all__ = [
     "Alphabet", "Config", "DnaAlphabet", "DnaTripletAlphabet", "ProteinAlphabet", "CusbfError",
     "ErrorCategory", "Result", "Filter", "InsertReport", "QueryReport", "RecordRange",
     "count_positive", "encode_symbols", "pack_dense_sequence", "pack_dense_sequence_device",
-    "NormalizedRecord", "normalize_record_batch", "query_layout",
+    "NormalizedRecord", "normalize_record_batch", "query_layout", "Server",
+    "query_record_batch_hits",
 ]

@@ -42,6 +42,15 @@ class CInsertReport(ctypes.Structure):
     _fields_ = [("records_indexed", u64), ("indexed_bases", u64), ("inserted_kmers", u64)]
 
 
+class CServerOptions(ctypes.Structure):
+    _fields_ = [("staging_bytes", u64), ("slots", u32), ("max_wait_us", u32)]
+
+
+class CServerStats(ctypes.Structure):
+    _fields_ = [("requests", u64), ("batches", u64), ("max_requests_per_batch", u64),
+                ("bytes", u64), ("windows", u64)]
+
+
 class CQueryReport(ctypes.Structure):
     _fields_ = [("records_queried", u64), ("queried_bases", u64), ("queried_kmers", u64),
                 ("positive_kmers", u64)]
@@ -68,6 +77,10 @@ SIGNATURES = [
     ("cusbf_insert_sequence", i32, [P, P, u64, P, ctypes.POINTER(u64)]),
     ("cusbf_contains_sequence_async", i32, [P, P, u64, P, u64, P]),
     ("cusbf_contains_sequence", i32, [P, P, u64, P, u64, P]),
+    ("cusbf_contains_sequence_host_to_device", i32, [P, P, u64, P, u64, P]),
+    ("cusbf_insert_dense_packed", i32, [P, P, u64, u64, P, ctypes.POINTER(u64)]),
+    ("cusbf_contains_dense_packed_host_to_device", i32, [P, P, u64, u64, P, u64, P]),
+    ("cusbf_contains_dense_packed", i32, [P, P, u64, u64, P, u64, P]),
     ("cusbf_dense_packed_word_count", u64, [CFG, u64]),
     ("cusbf_dense_packed_kmer_count", u64, [CFG, u64]),
     ("cusbf_pack_dense_sequence_async", i32, [CFG, P, u64, P, u64, P]),
@@ -77,10 +90,19 @@ SIGNATURES = [
     ("cusbf_contains_record_batch_async", i32, [P, P, u64, RR, u64, P, u64, P]),
     ("cusbf_insert_record_batch", i32, [P, P, u64, RR, u64, P, ctypes.POINTER(CInsertReport)]),
     ("cusbf_query_record_batch", i32, [P, P, u64, RR, u64, P, ctypes.POINTER(CQueryReport)]),
+    ("cusbf_query_record_batch_hits", i32,
+     [P, P, u64, RR, u64, P, P, u64, ctypes.POINTER(CQueryReport)]),
     ("cusbf_count_positive_async", i32, [P, u64, P, P]),
     ("cusbf_count_positive_per_record_async", i32, [P, P, P, u64, P, P]),
     ("cusbf_count_windows_runs_async", i32, [P, P, u64, P, P]),
     ("cusbf_kernel_family", ctypes.c_char_p, [P]),
+    ("cusbf_server_create", i32, [P, ctypes.POINTER(CServerOptions), ctypes.POINTER(P)]),
+    ("cusbf_server_destroy", None, [P]),
+    ("cusbf_server_submit", i32, [P, P, u64, RR, u64, P, u64, ctypes.POINTER(u64)]),
+    ("cusbf_server_request_windows", u64, [P, RR, u64]),
+    ("cusbf_server_wait", i32, [P, u64, ctypes.POINTER(CQueryReport),
+                                ctypes.POINTER(ctypes.c_double)]),
+    ("cusbf_server_stats_get", i32, [P, ctypes.POINTER(CServerStats)]),
 ]
 
 _lib = None

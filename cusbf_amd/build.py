@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_PATH = os.path.join(HERE, "libcusbf_hip.so")
 SYNTH_LIB_PATH = os.path.join(HERE, "libcusbf_synth.so")
 
-HIP_SOURCES = ["kernels_dna.hip", "kernels_generic.hip", "capi.hip"]
+HIP_SOURCES = ["kernels_dna.hip", "kernels_generic.hip", "capi.hip", "serve.hip"]
 SYNTH_SOURCES = ["synth.hip", "probe.hip"]
 
 # -packed-fp32-ops: the kernels are integer code, and packed FP32 forms stay off.
@@ -68,9 +68,29 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
             fut.result()
     _link(objs_of(HIP_SOURCES), LIB_PATH)
     _link(objs_of(SYNTH_SOURCES), SYNTH_LIB_PATH)
+    build_cpp_tests()
     if verbose:
-        print(f"built {LIB_PATH} and {SYNTH_LIB_PATH}")
+        print(f"built {LIB_PATH}, {SYNTH_LIB_PATH} and {CPP_TEST_PATH}")
     return LIB_PATH
+
+
+REPO = os.path.dirname(HERE)
+CPP_TEST_SRC = os.path.join(REPO, "tests", "cpp", "test_filter_api.cpp")
+CPP_TEST_PATH = os.path.join(REPO, "tests", "cpp", "_build", "test_filter_api")
+
+
+def build_cpp_tests() -> str:
+    """The C++ API tests: host g++ against include/cusbf/*.hpp and libcusbf_hip.so."""
+    os.makedirs(os.path.dirname(CPP_TEST_PATH), exist_ok=True)
+    rocm = os.path.dirname(os.path.dirname(hipcc()))
+    cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           f"-I{rocm}/include", f"-I{os.path.join(REPO, 'include')}", CPP_TEST_SRC,
+           "-o", CPP_TEST_PATH, f"-L{HERE}", "-lcusbf_hip", f"-L{rocm}/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../../cusbf_amd", f"-Wl,-rpath,{rocm}/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"C++ test build failed:\n{res.stderr[-6000:]}")
+    return CPP_TEST_PATH
 
 
 if __name__ == "__main__":

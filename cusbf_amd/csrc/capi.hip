@@ -214,20 +214,32 @@ struct NormRecord {
 
 uint64_t record_kmers(const cusbf_filter* f, uint64_t bytes) { return windows_of(f, bytes); }
 
-// normalized_record_batch.hpp:87-110 (valid_record_kmer_count).
+// normalized_record_batch.hpp:87-110 (valid_record_kmer_count): windows of
+// the record with no invalid symbol, in one pass tracking the last invalid symbol.
 uint64_t valid_record_kmers(const cusbf_filter* f, const char* seq, uint64_t bytes) {
     const uint64_t sym = symbols_of(f, bytes);
-    if (sym < f->cfg.k) return 0;
+    const uint64_t k = f->cfg.k;
+    if (sym < k) return 0;
     const uint8_t inv = f->d.invalid_symbol;
     const uint32_t w = f->d.symbol_width;
-    uint64_t invalid = 0;
-    for (uint64_t i = 0; i < f->cfg.k; ++i) invalid += sbf::encode_symbol(f->kc, seq + i * w) == inv;
-    uint64_t valid = invalid == 0;
-    const uint64_t windows = sym - f->cfg.k + 1;
-    for (uint64_t start = 1; start < windows; ++start) {
-        invalid -= sbf::encode_symbol(f->kc, seq + (start - 1) * w) == inv;
-        invalid += sbf::encode_symbol(f->kc, seq + (start + f->cfg.k - 1) * w) == inv;
-        valid += invalid == 0;
+    uint64_t valid = 0;
+    int64_t last_invalid = -1;
+    if (w == 1) {
+        bool bad[256];
+        for (int b = 0; b < 256; ++b) {
+            const char c = static_cast<char>(b);
+            bad[b] = sbf::encode_symbol(f->kc, &c) == inv;
+        }
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(seq);
+        for (uint64_t i = 0; i < sym; ++i) {
+            if (bad[p[i]]) last_invalid = static_cast<int64_t>(i);
+            if (i + 1 >= k) valid += last_invalid < static_cast<int64_t>(i + 1 - k);
+        }
+        return valid;
+    }
+    for (uint64_t i = 0; i < sym; ++i) {
+        if (sbf::encode_symbol(f->kc, seq + i * w) == inv) last_invalid = static_cast<int64_t>(i);
+        if (i + 1 >= k) valid += last_invalid < static_cast<int64_t>(i + 1 - k);
     }
     return valid;
 }
@@ -254,6 +266,28 @@ void normalize(const cusbf_filter* f, const char* seq, const cusbf_record_range*
 }
 
 }  // namespace
+
+namespace cusbf_internal {
+
+int set_error(int status, const char* message) { return fail(status, message); }
+int set_hip_error(hipError_t err, const char* file, int line) { return fail_hip(err, file, line); }
+int device(const cusbf_filter* f) { return f->device; }
+uint32_t symbol_width(const cusbf_filter* f) { return f->d.symbol_width; }
+uint8_t separator(const cusbf_filter* f) { return f->d.separator; }
+uint64_t window_count(const cusbf_filter* f, uint64_t bytes) { return windows_of(f, bytes); }
+uint64_t valid_kmers(const cusbf_filter* f, const char* seq, uint64_t bytes) {
+    return valid_record_kmers(f, seq, bytes);
+}
+int validate_host_ranges(const cusbf_filter* f, uint64_t bytes, const cusbf_record_range* ranges,
+                         uint64_t n) {
+    return validate_ranges(f, bytes, ranges, n, "record batch range exceeds the source sequence");
+}
+int launch_contains_bytes(const cusbf_filter* f, const char* d_seq, uint64_t bytes, uint8_t* d_out,
+                          hipStream_t stream) {
+    return launch_contains(f, byte_input(f, d_seq, bytes), d_out, stream);
+}
+
+}  // namespace cusbf_internal
 
 extern "C" {
 
@@ -456,6 +490,24 @@ int cusbf_contains_sequence(const cusbf_filter* f_const, const char* h_sequence,
     return CUSBF_OK;
 }
 
+// filter.cuh:486-501
+int cusbf_contains_sequence_host_to_device(const cusbf_filter* f_const, const char* h_sequence,
+                                           uint64_t bytes, uint8_t* d_output,
+                                           uint64_t d_output_len, void* stream) {
+    (void)d_output_len;
+    cusbf_filter* f = const_cast<cusbf_filter*>(f_const);
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (windows_of(f, bytes) == 0) return CUSBF_OK;
+    if (!d_output) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device buffer");
+    int rc = ensure_stage(f, bytes);
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, h_sequence, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    rc = launch_contains(f, byte_input(f, f->d_stage, bytes), d_output, as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return CUSBF_OK;
+}
+
 uint64_t cusbf_dense_packed_word_count(const cusbf_config* cfg, uint64_t num_symbols) {
     Derived d;
     if (!cfg || derive(cfg, &d) != CUSBF_OK || d.symbol_bits == 0) return 0;
@@ -527,6 +579,72 @@ int cusbf_contains_dense_packed_async(const cusbf_filter* f, const uint64_t* d_w
     in.num_symbols = num_symbols;
     in.dense = true;
     return launch_contains(f, in, d_output, as_stream(stream));
+}
+
+// Host dense words staged into library-owned device memory (word-aligned part of the stage).
+static int stage_words(cusbf_filter* f, const uint64_t* h_words, uint64_t num_words,
+                       hipStream_t stream) {
+    const int rc = ensure_stage(f, num_words * sizeof(uint64_t));
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, h_words, num_words * sizeof(uint64_t),
+                                 hipMemcpyHostToDevice, stream));
+    return CUSBF_OK;
+}
+
+// filter.cuh:226-240
+int cusbf_insert_dense_packed(cusbf_filter* f, const uint64_t* h_words, uint64_t num_words,
+                              uint64_t num_symbols, void* stream, uint64_t* inserted) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (inserted) *inserted = 0;
+    if (cusbf_dense_packed_kmer_count(&f->cfg, num_symbols) == 0) return CUSBF_OK;
+    if (num_words < cusbf_dense_packed_word_count(&f->cfg, num_symbols)) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "dense packed span is too small for num_symbols");
+    }
+    int rc = stage_words(f, h_words, num_words, as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
+    rc = cusbf_insert_dense_packed_async(f, reinterpret_cast<const uint64_t*>(f->d_stage), num_words,
+                                         num_symbols, stream, inserted);
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return CUSBF_OK;
+}
+
+// filter.cuh:267-282
+int cusbf_contains_dense_packed_host_to_device(const cusbf_filter* f_const, const uint64_t* h_words,
+                                               uint64_t num_words, uint64_t num_symbols,
+                                               uint8_t* d_output, uint64_t d_output_len,
+                                               void* stream) {
+    cusbf_filter* f = const_cast<cusbf_filter*>(f_const);
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (cusbf_dense_packed_kmer_count(&f->cfg, num_symbols) == 0) return CUSBF_OK;
+    if (num_words < cusbf_dense_packed_word_count(&f->cfg, num_symbols)) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "dense packed span is too small for num_symbols");
+    }
+    int rc = stage_words(f, h_words, num_words, as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
+    rc = cusbf_contains_dense_packed_async(f, reinterpret_cast<const uint64_t*>(f->d_stage),
+                                           num_words, num_symbols, d_output, d_output_len, stream);
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return CUSBF_OK;
+}
+
+// filter.cuh:289-312
+int cusbf_contains_dense_packed(const cusbf_filter* f_const, const uint64_t* h_words,
+                                uint64_t num_words, uint64_t num_symbols, uint8_t* h_output,
+                                uint64_t h_output_len, void* stream) {
+    cusbf_filter* f = const_cast<cusbf_filter*>(f_const);
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    const uint64_t n = cusbf_dense_packed_kmer_count(&f->cfg, num_symbols);
+    if (n == 0) return CUSBF_OK;
+    if (h_output_len < n) return fail(CUSBF_ERR_INVALID_ARGUMENT, "host output buffer is too small");
+    int rc = ensure_result(f, n);
+    if (rc != CUSBF_OK) return rc;
+    rc = cusbf_contains_dense_packed_host_to_device(f, h_words, num_words, num_symbols, f->d_result, n,
+                                                    stream);
+    if (rc != CUSBF_OK) return rc;
+    CUSBF_HIP_TRY(hipMemcpy(h_output, f->d_result, n, hipMemcpyDeviceToHost));
+    return CUSBF_OK;
 }
 
 // filter.cuh:385-400
@@ -621,6 +739,34 @@ int cusbf_query_record_batch(const cusbf_filter* f_const, const char* h_sequence
         CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
     }
     release_staging(f);
+    if (report) *report = rep;
+    return CUSBF_OK;
+}
+
+// filter.cuh:566-582, 1007-1131: host record batch in, per-window hits of the
+// normalised batch out (the consumer form reports them per record).
+int cusbf_query_record_batch_hits(const cusbf_filter* f_const, const char* h_sequence, uint64_t bytes,
+                                  const cusbf_record_range* ranges, uint64_t num_ranges, void* stream,
+                                  uint8_t* h_hits, uint64_t hits_len, cusbf_query_report* report) {
+    cusbf_filter* f = const_cast<cusbf_filter*>(f_const);
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    int rc = validate_ranges(f, bytes, ranges, num_ranges, "record batch range exceeds the source sequence");
+    if (rc != CUSBF_OK) return rc;
+    std::vector<char> norm;
+    std::vector<NormRecord> records;
+    normalize(f, h_sequence, ranges, num_ranges, norm, records);
+    const uint64_t hits = windows_of(f, norm.size());
+    if (hits_len < hits) return fail(CUSBF_ERR_INVALID_ARGUMENT, "record batch hit buffer is too small");
+    cusbf_query_report rep{records.size(), 0, 0, 0};
+    for (const NormRecord& r : records) {
+        rep.queried_bases += r.size;
+        rep.queried_kmers += r.valid_kmers;
+    }
+    if (hits) {
+        rc = cusbf_contains_sequence(f, norm.data(), norm.size(), h_hits, hits_len, stream);
+        if (rc != CUSBF_OK) return rc;
+        for (uint64_t i = 0; i < hits; ++i) rep.positive_kmers += h_hits[i] == 1;
+    }
     if (report) *report = rep;
     return CUSBF_OK;
 }

@@ -1,0 +1,29 @@
+// filter_internal.hpp -- pieces of the filter object shared by the C ABI sources
+// (capi.hip) and the request server (serve.hip). Internal to libcusbf_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/cusbf_hip.h"
+
+namespace cusbf_internal {
+
+// Record a failure as the calling thread's last error; returns `status`.
+int set_error(int status, const char* message);
+int set_hip_error(hipError_t err, const char* file, int line);
+
+int device(const cusbf_filter* f);
+uint32_t symbol_width(const cusbf_filter* f);
+uint8_t separator(const cusbf_filter* f);
+uint64_t window_count(const cusbf_filter* f, uint64_t bytes);
+// normalized_record_batch.hpp:87-110 for one record.
+uint64_t valid_kmers(const cusbf_filter* f, const char* seq, uint64_t bytes);
+int validate_host_ranges(const cusbf_filter* f, uint64_t bytes, const cusbf_record_range* ranges,
+                         uint64_t n);
+// Query kernel over a device byte sequence (fast or generic family).
+int launch_contains_bytes(const cusbf_filter* f, const char* d_seq, uint64_t bytes, uint8_t* d_out,
+                          hipStream_t stream);
+
+}  // namespace cusbf_internal

@@ -1,0 +1,362 @@
+// serve.hip -- request server for the query path (continuous batching), libcusbf_hip.so.
+//
+// Requests are host record batches. A scheduler thread takes queued requests
+// in arrival order into a free batch slot until the slot's staging is full,
+// normalises them into the slot's pinned buffer (records and requests joined by
+// the alphabet separator), and issues H2D copy, query kernel and D2H copy on the
+// slot's stream. A completion thread waits for each slot's event, hands every
+// request its windows and report, and returns the slot; the scheduler admits
+// new requests as slots come back. With several slots the copies and kernels
+// of different batches overlap on the device. The filter is read-only here.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cusbf_hip.h"
+#include "filter_internal.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+namespace ci = cusbf_internal;
+
+struct Request {
+    uint64_t ticket = 0;
+    const char* seq = nullptr;
+    std::vector<cusbf_record_range> ranges;
+    uint8_t* hits = nullptr;
+    uint64_t norm_bytes = 0;   // normalised size
+    uint64_t windows = 0;      // windows of the normalised batch
+    uint64_t batch_offset = 0; // byte offset inside the slot's batch
+    cusbf_query_report report{};
+    Clock::time_point submitted;
+    double latency = 0.0;
+    bool done = false;
+    int status = CUSBF_OK;
+    std::string message;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    char* d_seq = nullptr;
+    uint8_t* d_hits = nullptr;
+    char* h_seq = nullptr;
+    uint8_t* h_hits = nullptr;
+    uint64_t batch_bytes = 0;
+    int status = CUSBF_OK;
+    std::string message;
+    std::vector<Request*> batch;
+};
+
+// Separator padding of normalized_record_batch.hpp:208-232.
+uint64_t boundary_bytes(uint64_t current, uint32_t width) {
+    if (current == 0) return 0;
+    const uint64_t rem = current % width;
+    return (rem ? width - rem : 0) + width;
+}
+
+uint64_t normalized_size(const cusbf_record_range* r, uint64_t n, uint32_t width) {
+    uint64_t size = 0;
+    for (uint64_t i = 0; i < n; ++i) size += boundary_bytes(size, width) + r[i].sequence_bytes;
+    return size;
+}
+
+}  // namespace
+
+struct cusbf_server {
+    const cusbf_filter* filter = nullptr;
+    cusbf_server_options options{};
+    uint32_t width = 1;
+    char sep = 'N';
+    uint64_t seq_cap = 0;  // per slot
+    std::vector<Slot> slots;
+
+    std::mutex mu;
+    std::condition_variable cv_queue, cv_slots, cv_complete, cv_done;
+    std::deque<Request*> queue;
+    std::vector<int> free_slots;
+    std::deque<int> completing;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    bool scheduler_done = false;
+    cusbf_server_stats stats{};
+    std::thread scheduler, completer;
+};
+
+namespace {
+
+void fill_batch(cusbf_server* s, Slot& slot) {
+    char* out = slot.h_seq;
+    uint64_t pos = 0;
+    for (Request* r : slot.batch) {
+        const uint64_t b = boundary_bytes(pos, s->width);
+        std::memset(out + pos, s->sep, b);
+        pos += b;
+        r->batch_offset = pos;
+        uint64_t local = 0;
+        for (const cusbf_record_range& rr : r->ranges) {
+            const uint64_t lb = boundary_bytes(local, s->width);
+            std::memset(out + pos, s->sep, lb);
+            pos += lb;
+            local += lb;
+            std::memcpy(out + pos, r->seq + rr.sequence_offset, rr.sequence_bytes);
+            pos += rr.sequence_bytes;
+            local += rr.sequence_bytes;
+        }
+    }
+    slot.batch_bytes = pos;
+}
+
+void launch_batch(cusbf_server* s, Slot& slot) {
+    slot.status = CUSBF_OK;
+    const uint64_t windows = ci::window_count(s->filter, slot.batch_bytes);
+    hipError_t e = hipMemcpyAsync(slot.d_seq, slot.h_seq, slot.batch_bytes, hipMemcpyHostToDevice, slot.stream);
+    if (e == hipSuccess && windows) {
+        const int rc = ci::launch_contains_bytes(s->filter, slot.d_seq, slot.batch_bytes, slot.d_hits, slot.stream);
+        if (rc != CUSBF_OK) {
+            char buf[512];
+            cusbf_last_error(buf, sizeof(buf));
+            slot.status = rc;
+            slot.message = buf;
+        }
+        e = hipMemcpyAsync(slot.h_hits, slot.d_hits, windows, hipMemcpyDeviceToHost, slot.stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(slot.done, slot.stream);
+    if (e != hipSuccess) {
+        slot.status = CUSBF_ERR_DEVICE;
+        slot.message = hipGetErrorString(e);
+    }
+}
+
+void scheduler_loop(cusbf_server* s) {
+    (void)hipSetDevice(ci::device(s->filter));
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_queue.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+        if (s->queue.empty()) break;  // stop requested and drained
+        if (s->options.max_wait_us) {
+            const auto until = s->queue.front()->submitted + std::chrono::microseconds(s->options.max_wait_us);
+            s->cv_queue.wait_until(lk, until, [&] {
+                uint64_t bytes = 0;
+                for (Request* r : s->queue) bytes += r->norm_bytes + s->width * 2;
+                return s->stop || bytes >= s->seq_cap;
+            });
+        }
+        s->cv_slots.wait(lk, [&] { return !s->free_slots.empty(); });
+        const int idx = s->free_slots.back();
+        s->free_slots.pop_back();
+        Slot& slot = s->slots[idx];
+        slot.batch.clear();
+        uint64_t bytes = 0;
+        while (!s->queue.empty()) {
+            Request* r = s->queue.front();
+            const uint64_t need = boundary_bytes(bytes, s->width) + r->norm_bytes;
+            if (bytes + need > s->seq_cap && !slot.batch.empty()) break;
+            s->queue.pop_front();
+            slot.batch.push_back(r);
+            bytes += need;
+        }
+        s->stats.batches += 1;
+        s->stats.requests += slot.batch.size();
+        s->stats.max_requests_per_batch = std::max<uint64_t>(s->stats.max_requests_per_batch, slot.batch.size());
+        s->stats.bytes += bytes;
+        lk.unlock();
+        fill_batch(s, slot);
+        launch_batch(s, slot);
+        lk.lock();
+        s->completing.push_back(idx);
+        s->cv_complete.notify_one();
+    }
+    s->scheduler_done = true;
+    s->cv_complete.notify_one();
+}
+
+void completion_loop(cusbf_server* s) {
+    (void)hipSetDevice(ci::device(s->filter));
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_complete.wait(lk, [&] { return !s->completing.empty() || s->scheduler_done; });
+        if (s->completing.empty()) break;
+        const int idx = s->completing.front();
+        s->completing.pop_front();
+        Slot& slot = s->slots[idx];
+        lk.unlock();
+        if (slot.status == CUSBF_OK) {
+            const hipError_t e = hipEventSynchronize(slot.done);
+            if (e != hipSuccess) {
+                slot.status = CUSBF_ERR_DEVICE;
+                slot.message = hipGetErrorString(e);
+            }
+        }
+        const auto now = Clock::now();
+        uint64_t windows = 0;
+        for (Request* r : slot.batch) {
+            r->status = slot.status;
+            r->message = slot.message;
+            if (slot.status == CUSBF_OK && r->windows) {
+                std::memcpy(r->hits, slot.h_hits + r->batch_offset / s->width, r->windows);
+            }
+            windows += r->windows;
+            r->latency = std::chrono::duration<double>(now - r->submitted).count();
+        }
+        lk.lock();
+        s->stats.windows += windows;
+        for (Request* r : slot.batch) r->done = true;
+        slot.batch.clear();
+        s->free_slots.push_back(idx);
+        s->cv_slots.notify_one();
+        s->cv_done.notify_all();
+    }
+}
+
+void free_slots(cusbf_server* s) {
+    for (Slot& slot : s->slots) {
+        if (slot.stream) (void)hipStreamDestroy(slot.stream);
+        if (slot.done) (void)hipEventDestroy(slot.done);
+        if (slot.d_seq) (void)hipFree(slot.d_seq);
+        if (slot.d_hits) (void)hipFree(slot.d_hits);
+        if (slot.h_seq) (void)hipHostFree(slot.h_seq);
+        if (slot.h_hits) (void)hipHostFree(slot.h_hits);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* options,
+                        cusbf_server** out) {
+    if (!f || !options || !out) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (options->slots == 0 || options->slots > 64) {
+        return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "server slots must be in [1, 64]");
+    }
+    auto s = std::make_unique<cusbf_server>();
+    s->filter = f;
+    s->options = *options;
+    s->width = ci::symbol_width(f);
+    s->sep = static_cast<char>(ci::separator(f));
+    s->seq_cap = options->staging_bytes / options->slots / 2;
+    if (s->seq_cap < 4096) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "staging budget is too small");
+    s->slots.resize(options->slots);
+    for (Slot& slot : s->slots) {
+        hipError_t e = hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&slot.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc(&slot.d_seq, s->seq_cap);
+        if (e == hipSuccess) e = hipMalloc(&slot.d_hits, s->seq_cap);
+        if (e == hipSuccess) e = hipHostMalloc(&slot.h_seq, s->seq_cap, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc(&slot.h_hits, s->seq_cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            free_slots(s.get());
+            return ci::set_hip_error(e, __FILE__, __LINE__);
+        }
+    }
+    for (int i = static_cast<int>(options->slots) - 1; i >= 0; --i) s->free_slots.push_back(i);
+    cusbf_server* raw = s.release();
+    raw->scheduler = std::thread(scheduler_loop, raw);
+    raw->completer = std::thread(completion_loop, raw);
+    *out = raw;
+    return CUSBF_OK;
+}
+
+void cusbf_server_destroy(cusbf_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_queue.notify_all();
+    s->scheduler.join();
+    s->completer.join();
+    free_slots(s);
+    delete s;
+}
+
+uint64_t cusbf_server_request_windows(const cusbf_server* s, const cusbf_record_range* ranges,
+                                      uint64_t num_ranges) {
+    if (!s || (!ranges && num_ranges)) return 0;
+    return ci::window_count(s->filter, normalized_size(ranges, num_ranges, s->width));
+}
+
+int cusbf_server_submit(cusbf_server* s, const char* h_sequence, uint64_t bytes,
+                        const cusbf_record_range* ranges, uint64_t num_ranges, uint8_t* h_hits,
+                        uint64_t hits_len, uint64_t* ticket) {
+    if (!s || !ticket) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    const int rc = ci::validate_host_ranges(s->filter, bytes, ranges, num_ranges);
+    if (rc != CUSBF_OK) return rc;
+    auto r = std::make_unique<Request>();
+    r->seq = h_sequence;
+    r->ranges.assign(ranges, ranges + num_ranges);
+    r->hits = h_hits;
+    r->norm_bytes = normalized_size(ranges, num_ranges, s->width);
+    r->windows = ci::window_count(s->filter, r->norm_bytes);
+    if (hits_len < r->windows) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "request hit buffer is too small");
+    if (r->norm_bytes > s->seq_cap) {
+        return ci::set_error(CUSBF_ERR_RESOURCE, "request exceeds the per-batch staging budget");
+    }
+    r->submitted = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return ci::set_error(CUSBF_ERR_RESOURCE, "server is shutting down");
+    r->ticket = s->next_ticket++;
+    *ticket = r->ticket;
+    Request* raw = r.get();
+    s->requests.emplace(raw->ticket, std::move(r));
+    if (raw->norm_bytes == 0) {
+        raw->done = true;  // nothing to query
+    } else {
+        s->queue.push_back(raw);
+        s->cv_queue.notify_one();
+    }
+    return CUSBF_OK;
+}
+
+int cusbf_server_wait(cusbf_server* s, uint64_t ticket, cusbf_query_report* report,
+                      double* latency_seconds) {
+    if (!s) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null server");
+    std::unique_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->requests.find(ticket);
+        if (it == s->requests.end()) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "unknown ticket");
+        Request* raw = it->second.get();
+        s->cv_done.wait(lk, [&] { return raw->done; });
+        r = std::move(it->second);
+        s->requests.erase(it);
+    }
+    if (latency_seconds) *latency_seconds = r->latency;
+    if (r->status != CUSBF_OK) return ci::set_error(r->status, r->message.c_str());
+    if (report) {
+        // FastxQueryReport of the request (filter.cuh:951-990), derived after the
+        // answer is ready so the scans stay off the serving threads.
+        cusbf_query_report rep{r->ranges.size(), 0, 0, 0};
+        for (const cusbf_record_range& rr : r->ranges) {
+            rep.queried_bases += rr.sequence_bytes;
+            rep.queried_kmers += ci::valid_kmers(s->filter, r->seq + rr.sequence_offset, rr.sequence_bytes);
+        }
+        for (uint64_t i = 0; i < r->windows; ++i) rep.positive_kmers += r->hits[i] == 1;
+        *report = rep;
+    }
+    return CUSBF_OK;
+}
+
+int cusbf_server_stats_get(const cusbf_server* s, cusbf_server_stats* out) {
+    if (!s || !out) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(const_cast<cusbf_server*>(s)->mu);
+    *out = s->stats;
+    return CUSBF_OK;
+}
+
+}  // extern "C"

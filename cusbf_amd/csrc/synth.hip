@@ -142,9 +142,54 @@ __global__ void reads_kernel(ReadParams p, uint8_t* out) {
     }
 }
 
+// Long reads: read r is written at out[offsets[r] .. offsets[r] + len_r) with
+// len_r = offsets[r + 1] - offsets[r] - 1 (an 'N' follows every read but the
+// last, whose length is offsets[r + 1] - offsets[r]). Each output base is,
+// with probability err/3 each, an inserted random base, a base after one
+// deleted genome base, or a substituted genome base; otherwise a copy.
+__global__ void long_reads_kernel(uint64_t seed, const uint8_t* genome, uint64_t genome_len,
+                                  uint64_t num_reads, const uint64_t* offsets, uint32_t err_ppm,
+                                  uint8_t* out) {
+    const uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= num_reads) return;
+    const bool last = r + 1 == num_reads;
+    const uint64_t len = offsets[r + 1] - offsets[r] - (last ? 0 : 1);
+    uint8_t* dst = out + offsets[r];
+    uint64_t pos = rnd(seed, 30, r) % (genome_len - 2 * len);
+    for (uint64_t i = 0; i < len; ++i) {
+        const uint64_t x = rnd(seed, 31, (r << 20) + i);
+        const uint32_t u = static_cast<uint32_t>((x >> 32) % 1000000u);
+        uint8_t b;
+        if (u < err_ppm / 3) {
+            b = "ACGT"[x & 3];  // insertion
+        } else if (u < 2 * (err_ppm / 3)) {
+            pos += 1;  // deletion of one genome base
+            b = genome[pos++];
+        } else if (u < err_ppm) {
+            b = genome[pos++];
+            if (b != 'N') b = substitute(b, x);
+        } else {
+            b = genome[pos++];
+        }
+        dst[i] = b;
+    }
+    if (!last) dst[len] = 'N';
+}
+
 }  // namespace
 
 extern "C" {
+
+int cusbf_synth_long_reads(uint64_t seed, const uint8_t* d_genome, uint64_t genome_len,
+                           uint64_t num_reads, const uint64_t* d_offsets, uint32_t err_ppm,
+                           uint8_t* d_out, void* stream) {
+    if (num_reads == 0) return 0;
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(long_reads_kernel, dim3(static_cast<uint32_t>((num_reads + 63) / 64)), dim3(64),
+                       0, reinterpret_cast<hipStream_t>(stream), seed, d_genome, genome_len, num_reads,
+                       d_offsets, err_ppm, d_out);
+    return static_cast<int>(hipGetLastError());
+}
 
 int cusbf_synth_genome(uint64_t seed, uint32_t gc_ppm, const uint64_t* d_chrom_start,
                        uint32_t num_chrom, const void* d_repeats, uint64_t num_repeats,

@@ -47,6 +47,8 @@ def synth_lib():
         L.cusbf_synth_reads.argtypes = [u64, P, u64, u64, u64, u32, u32, u32, P, P]
         L.cusbf_synth_reads.restype = ctypes.c_int
         L.cusbf_synth_repeat_copy_size.restype = u64
+        L.cusbf_synth_long_reads.argtypes = [u64, P, u64, u64, P, u32, P, P]
+        L.cusbf_synth_long_reads.restype = ctypes.c_int
         assert L.cusbf_synth_repeat_copy_size() == REPEAT_DTYPE.itemsize
         _synth = L
     return _synth
@@ -209,3 +211,29 @@ def uniform_dna(n: int, seed: int, device="cuda"):
     """n uniform random ACGT bases on the device."""
     spec = GenomeSpec("uniform", n, 1, n, n, 0.5, 0.0, seed=seed)
     return generate_genome(spec, device)[0]
+
+
+def generate_long_reads(genome, num_reads: int, min_len: int = 10_000, max_len: int = 50_000,
+                        error: float = 0.05, seed: int = 5, device="cuda"):
+    """(device buffer of long reads joined by 'N', host offsets of the reads).
+
+    Lengths are uniform in [min_len, max_len]; each base carries a mixed
+    substitution / insertion / deletion error with total probability `error`.
+    """
+    import torch
+
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(min_len, max_len + 1, num_reads).astype(np.int64)
+    offsets = np.zeros(num_reads + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(lens + 1)
+    offsets[-1] -= 1  # no separator after the last read
+    if genome.numel() <= 2 * max_len:
+        raise ValueError("genome too short for the requested read lengths")
+    d_off = torch.from_numpy(offsets).to(device)
+    out = torch.empty(int(offsets[-1]), dtype=torch.uint8, device=device)
+    rc = synth_lib().cusbf_synth_long_reads(seed, genome.data_ptr(), genome.numel(), num_reads,
+                                            d_off.data_ptr(), int(error * 1e6), out.data_ptr(), 0)
+    if rc != 0:
+        raise RuntimeError(f"long-read generation launch failed (hip error {rc})")
+    torch.cuda.synchronize()
+    return out, offsets

@@ -1,0 +1,543 @@
+// cusbf/filter.hpp -- cusbf::filter<Config> for MI355X, over the C ABI of libcusbf_hip.so.
+//
+// Host C++ drop-in for the hot-path part of the reference's include/cusbf/filter.cuh
+// (insert/query of byte sequences, dense packed sequences and record batches,
+// clear/load_factor/filter_bits/num_shards) with the same method names,
+// argument order, counting conventions and Result<T> errors. Every call
+// forwards to an extern "C" entry point of include/cusbf_hip.h; the kernels
+// are hand-written HIP for gfx950. Streams are hipStream_t (nullptr = default).
+//
+// Also here, as in the reference: RecordRange / RecordBatchView (Fastx.hpp:23-37),
+// the insert/query reports (Fastx.hpp:135-170), record normalisation
+// (normalized_record_batch.hpp), the query layout (detail/query_layout.cuh) and
+// the host dense packer (dense_packed.hpp:35-49).
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <span>
+#include <string>
+#include <string_view>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#if __has_include(<hip/hip_runtime_api.h>)
+#include <hip/hip_runtime_api.h>
+#else
+typedef struct ihipStream_t* hipStream_t;
+#endif
+
+#include "../cusbf_hip.h"
+#include "config.hpp"
+#include "error.hpp"
+
+namespace cusbf {
+
+/// Non-owning view of device memory (device_span.cuh:20-33).
+template <typename T>
+class device_span {
+   public:
+    constexpr device_span() noexcept = default;
+    constexpr device_span(T* data, size_t size) noexcept : data_(data), size_(size) {}
+    template <typename U>
+        requires std::is_convertible_v<U (*)[], T (*)[]>
+    constexpr device_span(device_span<U> other) noexcept : data_(other.data()), size_(other.size()) {}
+    [[nodiscard]] constexpr T* data() const noexcept { return data_; }
+    [[nodiscard]] constexpr size_t size() const noexcept { return size_; }
+    [[nodiscard]] constexpr bool empty() const noexcept { return size_ == 0; }
+
+   private:
+    T* data_ = nullptr;
+    size_t size_ = 0;
+};
+
+struct RecordRange {
+    uint64_t sequenceOffset{};
+    uint64_t sequenceBytes{};
+};
+static_assert(sizeof(RecordRange) == sizeof(cusbf_record_range));
+
+struct RecordBatchView {
+    std::string_view sequence{};
+    std::span<const RecordRange> records{};
+};
+
+struct FastxInsertReport {
+    uint64_t recordsIndexed{};
+    uint64_t indexedBases{};
+    uint64_t insertedKmers{};
+};
+
+struct FastxQueryReport {
+    uint64_t recordsQueried{};
+    uint64_t queriedBases{};
+    uint64_t queriedKmers{};
+    uint64_t positive_kmers{};
+};
+
+struct RecordQueryView {
+    uint64_t record_index{};
+    std::string_view sequence{};
+    uint64_t queriedBases{};
+    uint64_t queriedKmers{};
+    uint64_t positive_kmers{};
+    std::span<const uint8_t> hits{};
+};
+
+template <typename Functor>
+concept RecordQueryConsumer = requires(Functor f, const RecordQueryView& view) {
+    { f(view) } -> std::same_as<void>;
+};
+
+struct NormalizedRecord {
+    uint64_t record_index{};
+    uint64_t input_offset{};
+    uint64_t size{};
+    uint64_t valid_kmers{};
+};
+
+namespace detail {
+
+template <typename Config>
+[[nodiscard]] constexpr uint64_t record_symbol_count(uint64_t bytes) noexcept {
+    return bytes / Config::symbolWidth;
+}
+
+template <typename Config>
+[[nodiscard]] constexpr uint64_t record_kmer_count(uint64_t bytes) noexcept {
+    const uint64_t symbols = record_symbol_count<Config>(bytes);
+    return symbols < Config::k ? 0 : symbols - Config::k + 1;
+}
+
+/// normalized_record_batch.hpp:87-110
+template <typename Config>
+[[nodiscard]] uint64_t valid_record_kmer_count(std::string_view seq) {
+    const uint64_t n = record_kmer_count<Config>(seq.size());
+    if (n == 0) return 0;
+    using A = typename Config::Alphabet;
+    auto bad = [&](uint64_t i) -> uint64_t {
+        return A::encode(seq.data() + i * Config::symbolWidth) == A::invalidSymbol;
+    };
+    uint64_t invalid = 0;
+    for (uint64_t i = 0; i < Config::k; ++i) invalid += bad(i);
+    uint64_t valid = invalid == 0;
+    for (uint64_t start = 1; start < n; ++start) {
+        invalid -= bad(start - 1);
+        invalid += bad(start + Config::k - 1);
+        valid += invalid == 0;
+    }
+    return valid;
+}
+
+/// normalized_record_batch.hpp:123-147
+template <typename Config>
+[[nodiscard]] Result<void> validate_record_batch(uint64_t bytes, std::span<const RecordRange> records,
+                                                 const char* exceeds) {
+    uint64_t next = 0;
+    for (const RecordRange& r : records) {
+        if (r.sequenceOffset < next) {
+            return Err(Error::invalid_argument("record batch ranges must be ordered and non-overlapping"));
+        }
+        if (r.sequenceOffset > bytes || r.sequenceBytes > bytes - r.sequenceOffset) {
+            return Err(Error::invalid_argument(exceeds));
+        }
+        if (r.sequenceOffset % Config::symbolWidth != 0 || r.sequenceBytes % Config::symbolWidth != 0) {
+            return Err(Error::invalid_argument(
+                "record batch ranges must align to the configured alphabet symbol width"));
+        }
+        next = r.sequenceOffset + r.sequenceBytes;
+    }
+    return {};
+}
+
+struct QueryLayoutRecord {
+    uint64_t record_index{};
+    uint64_t input_offset{};
+    uint64_t size{};
+    uint64_t valid_kmers{};
+    uint64_t hit_offset{};
+    uint64_t hit_count{};
+};
+
+/// detail/query_layout.cuh:41-72: hit spans per record in the normalised buffer.
+class QueryLayout {
+   public:
+    [[nodiscard]] std::span<const QueryLayoutRecord> records() const noexcept { return records_; }
+    [[nodiscard]] uint64_t total_hit_count() const noexcept { return total_; }
+    [[nodiscard]] std::span<const uint8_t> hits_for_record(std::span<const uint8_t> hits,
+                                                           size_t i) const noexcept {
+        return hits.subspan(records_[i].hit_offset, records_[i].hit_count);
+    }
+
+    template <typename Config>
+    [[nodiscard]] static QueryLayout build(std::span<const NormalizedRecord> records) {
+        QueryLayout layout;
+        layout.records_.reserve(records.size());
+        uint64_t logical = 0;
+        for (const NormalizedRecord& r : records) {
+            if (logical != 0) {
+                const uint64_t rem = logical % Config::symbolWidth;
+                if (rem != 0) logical += Config::symbolWidth - rem;
+                logical += Config::symbolWidth;
+            }
+            layout.records_.push_back(QueryLayoutRecord{r.record_index, r.input_offset, r.size,
+                                                        r.valid_kmers, logical / Config::symbolWidth,
+                                                        record_kmer_count<Config>(r.size)});
+            logical += r.size;
+        }
+        layout.total_ = record_kmer_count<Config>(logical);
+        return layout;
+    }
+
+   private:
+    std::vector<QueryLayoutRecord> records_{};
+    uint64_t total_{};
+};
+
+inline Unexpected status_error(int status) {
+    char buf[1024];
+    cusbf_last_error(buf, sizeof(buf));
+    return Err(Error(static_cast<ErrorCategory>(status - 1), buf));
+}
+
+inline Result<void> check(int status) {
+    if (status == CUSBF_OK) return {};
+    return status_error(status);
+}
+
+template <typename Config>
+cusbf_config c_config() {
+    using A = typename Config::Alphabet;
+    cusbf_config c{};
+    uint32_t code = CUSBF_ALPHABET_CUSTOM;
+    if constexpr (std::is_same_v<A, DnaAlphabet>) {
+        code = CUSBF_ALPHABET_DNA;
+    } else if constexpr (std::is_same_v<A, DnaTripletAlphabet>) {
+        code = CUSBF_ALPHABET_DNA_TRIPLET;
+    } else if constexpr (std::is_same_v<A, ProteinAlphabet>) {
+        code = CUSBF_ALPHABET_PROTEIN;
+    }
+    if (code != CUSBF_ALPHABET_CUSTOM) {
+        cusbf_config_init(&c, Config::k, Config::s, Config::m, Config::hashCount, code);
+        return c;
+    }
+    static_assert(std::is_same_v<A, DnaAlphabet> || std::is_same_v<A, DnaTripletAlphabet> ||
+                      std::is_same_v<A, ProteinAlphabet> || A::symbolWidth == 1,
+                  "custom alphabets must read one byte per symbol");
+    c.k = Config::k;
+    c.s = Config::s;
+    c.m = Config::m;
+    c.hash_count = Config::hashCount;
+    c.alphabet = CUSBF_ALPHABET_CUSTOM;
+    c.symbol_count = static_cast<uint32_t>(A::symbolCount);
+    c.invalid_symbol = A::invalidSymbol;
+    c.separator = A::separator;
+    for (int b = 0; b < 256; ++b) {
+        const char byte = static_cast<char>(b);
+        c.lut[b] = A::encode(&byte);
+    }
+    return c;
+}
+
+}  // namespace detail
+
+/// normalized_record_batch.hpp:288-304: separators between records, per-record metadata.
+template <typename Config>
+[[nodiscard]] Result<void> normalize_record_batch_into(RecordBatchView batch, std::string& sequence_out,
+                                                       std::vector<NormalizedRecord>& records_out) {
+    CUSBF_TRY(detail::validate_record_batch<Config>(batch.sequence.size(), batch.records,
+                                                    "record batch range exceeds the source sequence"));
+    sequence_out.clear();
+    records_out.clear();
+    const char sep = static_cast<char>(Config::Alphabet::separator);
+    for (uint64_t i = 0; i < batch.records.size(); ++i) {
+        const RecordRange& r = batch.records[i];
+        if (!sequence_out.empty()) {
+            const uint64_t rem = sequence_out.size() % Config::symbolWidth;
+            if (rem != 0) sequence_out.append(Config::symbolWidth - rem, sep);
+            sequence_out.append(Config::symbolWidth, sep);
+        }
+        const std::string_view rec = batch.sequence.substr(r.sequenceOffset, r.sequenceBytes);
+        sequence_out.append(rec);
+        records_out.push_back(NormalizedRecord{i, r.sequenceOffset, rec.size(),
+                                               detail::valid_record_kmer_count<Config>(rec)});
+    }
+    return {};
+}
+
+class NormalizedRecordBatch {
+   public:
+    [[nodiscard]] const std::string& sequence() const noexcept { return sequence_; }
+    [[nodiscard]] std::span<const NormalizedRecord> records() const noexcept { return records_; }
+    [[nodiscard]] uint64_t total_valid_kmers() const noexcept {
+        uint64_t t = 0;
+        for (const NormalizedRecord& r : records_) t += r.valid_kmers;
+        return t;
+    }
+
+    std::string sequence_;
+    std::vector<NormalizedRecord> records_;
+};
+
+template <typename Config>
+[[nodiscard]] Result<NormalizedRecordBatch> normalize_record_batch(RecordBatchView batch) {
+    NormalizedRecordBatch out;
+    CUSBF_TRY(normalize_record_batch_into<Config>(batch, out.sequence_, out.records_));
+    return out;
+}
+
+/// dense_packed.cuh:19-28
+template <typename Config>
+[[nodiscard]] constexpr uint64_t dense_packed_word_count(uint64_t num_symbols) {
+    constexpr uint64_t per = 64 / Config::symbolBits;
+    return (num_symbols + per - 1) / per;
+}
+
+template <typename Config>
+[[nodiscard]] constexpr uint64_t dense_packed_kmer_count(uint64_t num_symbols) {
+    return num_symbols < Config::k ? 0 : num_symbols - Config::k + 1;
+}
+
+/// dense_packed.hpp:35-49: LSB-first host packing.
+template <typename Config>
+[[nodiscard]] std::vector<uint64_t> pack_dense_sequence(std::string_view sequence) {
+    const uint64_t n = sequence.size() / Config::symbolWidth;
+    constexpr uint64_t per = 64 / Config::symbolBits;
+    std::vector<uint64_t> words(dense_packed_word_count<Config>(n), 0);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t sym = Config::Alphabet::encode(sequence.data() + i * Config::symbolWidth) &
+                             Config::symbolMask;
+        words[i / per] |= sym << ((i % per) * Config::symbolBits);
+    }
+    return words;
+}
+
+/// dense_packed.hpp:57-73
+template <typename Config>
+[[nodiscard]] Result<void> pack_dense_sequence_device(const char* d_sequence, uint64_t num_symbols,
+                                                      uint64_t* d_words, hipStream_t stream = nullptr) {
+    const cusbf_config c = detail::c_config<Config>();
+    return detail::check(cusbf_pack_dense_sequence_async(
+        &c, d_sequence, num_symbols, d_words, dense_packed_word_count<Config>(num_symbols), stream));
+}
+
+/// Non-owning view of the filter's shard storage (filter_ref.cuh:20-64, host side).
+template <typename Config>
+struct filter_ref {
+    uint64_t* words = nullptr;  // shard i = words[4i .. 4i+4)
+    uint64_t num_blocks = 0;
+    [[nodiscard]] uint64_t* blocks() const noexcept { return words; }
+    [[nodiscard]] static uint64_t shard_index(uint64_t minimizer_hash, uint64_t n) noexcept {
+        return minimizer_hash & (n - 1);
+    }
+};
+
+template <typename Config>
+class filter {
+   public:
+    static_assert(Config::blockWordCount == 4, "Filter only supports the fused 256-bit shard path");
+
+    /// filter.cuh:108-120; throws std::runtime_error when the device allocation fails.
+    explicit filter(uint64_t requestedFilterBits) {
+        const cusbf_config c = detail::c_config<Config>();
+        const int st = cusbf_create(&c, requestedFilterBits, &handle_);
+        if (st != CUSBF_OK) throw std::runtime_error(detail::status_error(st).error.message());
+    }
+    filter(const filter&) = delete;
+    filter& operator=(const filter&) = delete;
+    filter(filter&& o) noexcept : handle_(std::exchange(o.handle_, nullptr)) {}
+    filter& operator=(filter&& o) noexcept {
+        if (this != &o) {
+            cusbf_destroy(handle_);
+            handle_ = std::exchange(o.handle_, nullptr);
+        }
+        return *this;
+    }
+    ~filter() { cusbf_destroy(handle_); }
+
+    [[nodiscard]] filter_ref<Config> ref() const noexcept {
+        uint64_t* w = nullptr;
+        cusbf_words_device_ptr(handle_, &w);
+        return filter_ref<Config>{w, num_shards()};
+    }
+
+    // --- byte sequences (filter.cuh:154-190, 464-533) ---
+    [[nodiscard]] Result<uint64_t> insert_sequence(std::string_view sequence, hipStream_t stream = nullptr) {
+        uint64_t n = 0;
+        CUSBF_TRY(detail::check(cusbf_insert_sequence(handle_, sequence.data(), sequence.size(), stream, &n)));
+        return n;
+    }
+    [[nodiscard]] Result<uint64_t> insert_sequence_async(device_span<const char> d_sequence,
+                                                         hipStream_t stream = nullptr) {
+        uint64_t n = 0;
+        CUSBF_TRY(detail::check(
+            cusbf_insert_sequence_async(handle_, d_sequence.data(), d_sequence.size(), stream, &n)));
+        return n;
+    }
+    [[nodiscard]] Result<void> contains_sequence_async(device_span<const char> d_sequence,
+                                                       device_span<uint8_t> d_output,
+                                                       hipStream_t stream = nullptr) const {
+        return detail::check(cusbf_contains_sequence_async(handle_, d_sequence.data(), d_sequence.size(),
+                                                           d_output.data(), d_output.size(), stream));
+    }
+    [[nodiscard]] Result<void> contains_sequence(std::string_view sequence, device_span<uint8_t> d_output,
+                                                 hipStream_t stream = nullptr) const {
+        return detail::check(cusbf_contains_sequence_host_to_device(
+            handle_, sequence.data(), sequence.size(), d_output.data(), d_output.size(), stream));
+    }
+    [[nodiscard]] Result<std::vector<uint8_t>> contains_sequence(std::string_view sequence,
+                                                                 hipStream_t stream = nullptr) const {
+        std::vector<uint8_t> out(detail::record_kmer_count<Config>(sequence.size()));
+        if (out.empty()) return out;
+        CUSBF_TRY(detail::check(cusbf_contains_sequence(handle_, sequence.data(), sequence.size(),
+                                                        out.data(), out.size(), stream)));
+        return out;
+    }
+
+    // --- dense packed (filter.cuh:207-322) ---
+    [[nodiscard]] Result<uint64_t> insert_dense_packed_async(device_span<const uint64_t> d_words,
+                                                             uint64_t num_symbols,
+                                                             hipStream_t stream = nullptr) {
+        uint64_t n = 0;
+        CUSBF_TRY(detail::check(cusbf_insert_dense_packed_async(handle_, d_words.data(), d_words.size(),
+                                                                num_symbols, stream, &n)));
+        return n;
+    }
+    [[nodiscard]] Result<uint64_t> insert_dense_packed(std::span<const uint64_t> words, uint64_t num_symbols,
+                                                       hipStream_t stream = nullptr) {
+        uint64_t n = 0;
+        CUSBF_TRY(detail::check(
+            cusbf_insert_dense_packed(handle_, words.data(), words.size(), num_symbols, stream, &n)));
+        return n;
+    }
+    [[nodiscard]] Result<void> contains_dense_packed_async(device_span<const uint64_t> d_words,
+                                                           uint64_t num_symbols,
+                                                           device_span<uint8_t> d_output,
+                                                           hipStream_t stream = nullptr) const {
+        return detail::check(cusbf_contains_dense_packed_async(handle_, d_words.data(), d_words.size(),
+                                                               num_symbols, d_output.data(),
+                                                               d_output.size(), stream));
+    }
+    [[nodiscard]] Result<void> contains_dense_packed(std::span<const uint64_t> words, uint64_t num_symbols,
+                                                     device_span<uint8_t> d_output,
+                                                     hipStream_t stream = nullptr) const {
+        return detail::check(cusbf_contains_dense_packed_host_to_device(
+            handle_, words.data(), words.size(), num_symbols, d_output.data(), d_output.size(), stream));
+    }
+    [[nodiscard]] Result<std::vector<uint8_t>> contains_dense_packed(std::span<const uint64_t> words,
+                                                                     uint64_t num_symbols,
+                                                                     hipStream_t stream = nullptr) const {
+        std::vector<uint8_t> out(cusbf::dense_packed_kmer_count<Config>(num_symbols));
+        if (out.empty()) return out;
+        CUSBF_TRY(detail::check(cusbf_contains_dense_packed(handle_, words.data(), words.size(), num_symbols,
+                                                            out.data(), out.size(), stream)));
+        return out;
+    }
+    [[nodiscard]] static constexpr uint64_t dense_packed_word_count(uint64_t num_symbols) {
+        return cusbf::dense_packed_word_count<Config>(num_symbols);
+    }
+    [[nodiscard]] constexpr uint64_t dense_packed_kmer_count(uint64_t num_symbols) const {
+        return cusbf::dense_packed_kmer_count<Config>(num_symbols);
+    }
+
+    // --- record batches (filter.cuh:338-400, 549-618) ---
+    [[nodiscard]] Result<FastxInsertReport> insert_record_batch(RecordBatchView batch,
+                                                                hipStream_t stream = nullptr) {
+        cusbf_insert_report r{};
+        CUSBF_TRY(detail::check(cusbf_insert_record_batch(
+            handle_, batch.sequence.data(), batch.sequence.size(), ranges_of(batch.records),
+            batch.records.size(), stream, &r)));
+        return FastxInsertReport{r.records_indexed, r.indexed_bases, r.inserted_kmers};
+    }
+    [[nodiscard]] Result<uint64_t> insert_record_batch_async(device_span<const char> d_sequence,
+                                                             std::span<const RecordRange> records,
+                                                             hipStream_t stream = nullptr) {
+        uint64_t n = 0;
+        CUSBF_TRY(detail::check(cusbf_insert_record_batch_async(handle_, d_sequence.data(), d_sequence.size(),
+                                                                ranges_of(records), records.size(), stream,
+                                                                &n)));
+        return n;
+    }
+    [[nodiscard]] Result<FastxQueryReport> query_record_batch(RecordBatchView batch,
+                                                              hipStream_t stream = nullptr) const {
+        cusbf_query_report r{};
+        CUSBF_TRY(detail::check(cusbf_query_record_batch(
+            handle_, batch.sequence.data(), batch.sequence.size(), ranges_of(batch.records),
+            batch.records.size(), stream, &r)));
+        return FastxQueryReport{r.records_queried, r.queried_bases, r.queried_kmers, r.positive_kmers};
+    }
+    /// filter.cuh:566-582, 1007-1131: per-record hits delivered to `consume` in source order.
+    template <RecordQueryConsumer Consumer>
+    [[nodiscard]] Result<FastxQueryReport> query_record_batch(RecordBatchView batch, Consumer&& consume,
+                                                              hipStream_t stream = nullptr) const {
+        std::string norm;
+        std::vector<NormalizedRecord> records;
+        CUSBF_TRY(normalize_record_batch_into<Config>(batch, norm, records));
+        const detail::QueryLayout layout = detail::QueryLayout::build<Config>(records);
+        FastxQueryReport report;
+        report.recordsQueried = records.size();
+        for (const NormalizedRecord& r : records) {
+            report.queriedBases += r.size;
+            report.queriedKmers += r.valid_kmers;
+        }
+        if (norm.empty()) return report;
+        std::vector<uint8_t> hits(layout.total_hit_count());
+        if (!hits.empty()) {
+            CUSBF_TRY(detail::check(cusbf_contains_sequence(handle_, norm.data(), norm.size(), hits.data(),
+                                                            hits.size(), stream)));
+        }
+        const std::span<const uint8_t> all(hits);
+        for (size_t i = 0; i < layout.records().size(); ++i) {
+            const detail::QueryLayoutRecord& r = layout.records()[i];
+            const std::string_view seq = batch.sequence.substr(r.input_offset, r.size);
+            std::span<const uint8_t> h{};
+            uint64_t positive = 0;
+            if (r.hit_count != 0) {
+                h = layout.hits_for_record(all, i);
+                for (uint8_t x : h) positive += x;
+            }
+            report.positive_kmers += positive;
+            consume(RecordQueryView{r.record_index, seq, r.size, r.valid_kmers, positive, h});
+        }
+        return report;
+    }
+    [[nodiscard]] Result<void> contains_record_batch_async(device_span<const char> d_sequence,
+                                                           std::span<const RecordRange> records,
+                                                           device_span<uint8_t> d_output,
+                                                           hipStream_t stream = nullptr) const {
+        return detail::check(cusbf_contains_record_batch_async(handle_, d_sequence.data(), d_sequence.size(),
+                                                               ranges_of(records), records.size(),
+                                                               d_output.data(), d_output.size(), stream));
+    }
+
+    // --- state (filter.cuh:793-833) ---
+    [[nodiscard]] Result<void> clear(hipStream_t stream = nullptr) {
+        return detail::check(cusbf_clear(handle_, stream));
+    }
+    [[nodiscard]] float load_factor() const {
+        float lf = 0.0f;
+        if (cusbf_load_factor(handle_, &lf) != CUSBF_OK) {
+            throw std::runtime_error(detail::status_error(CUSBF_ERR_DEVICE).error.message());
+        }
+        return lf;
+    }
+    [[nodiscard]] uint64_t filter_bits() const { return cusbf_filter_bits(handle_); }
+    [[nodiscard]] uint64_t num_shards() const { return cusbf_num_shards(handle_); }
+    /// All filter words on the host (parity checks).
+    [[nodiscard]] Result<std::vector<uint64_t>> words() const {
+        std::vector<uint64_t> w(num_shards() * 4);
+        CUSBF_TRY(detail::check(cusbf_copy_words_to_host(handle_, w.data(), w.size())));
+        return w;
+    }
+    [[nodiscard]] cusbf_filter* native_handle() const noexcept { return handle_; }
+
+   private:
+    static const cusbf_record_range* ranges_of(std::span<const RecordRange> r) {
+        return reinterpret_cast<const cusbf_record_range*>(r.data());
+    }
+
+    cusbf_filter* handle_ = nullptr;
+};
+
+}  // namespace cusbf

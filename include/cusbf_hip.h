@@ -137,6 +137,11 @@ int cusbf_insert_sequence(cusbf_filter* f, const char* h_sequence, uint64_t byte
  * the caller provides at least that many bytes. */
 int cusbf_contains_sequence_async(const cusbf_filter* f, const char* d_sequence, uint64_t bytes,
                                   uint8_t* d_output, uint64_t d_output_len, void* stream);
+/* contains_sequence(std::string_view, device_span<uint8_t>, stream) (filter.cuh:486-501):
+ * host bytes staged to the device, answers written to device memory, synchronises. */
+int cusbf_contains_sequence_host_to_device(const cusbf_filter* f, const char* h_sequence,
+                                           uint64_t bytes, uint8_t* d_output,
+                                           uint64_t d_output_len, void* stream);
 /* contains_sequence(std::string_view, std::vector out) (filter.cuh:514-533):
  * host bytes in, host hits out (h_output must hold the window count), synchronises. */
 int cusbf_contains_sequence(const cusbf_filter* f, const char* h_sequence, uint64_t bytes,
@@ -158,6 +163,20 @@ int cusbf_insert_dense_packed_async(cusbf_filter* f, const uint64_t* d_words, ui
 int cusbf_contains_dense_packed_async(const cusbf_filter* f, const uint64_t* d_words,
                                       uint64_t num_words, uint64_t num_symbols, uint8_t* d_output,
                                       uint64_t d_output_len, void* stream);
+
+/* insert_dense_packed(std::span<const uint64_t>, num_symbols, stream) (filter.cuh:226-240). */
+int cusbf_insert_dense_packed(cusbf_filter* f, const uint64_t* h_words, uint64_t num_words,
+                              uint64_t num_symbols, void* stream, uint64_t* inserted);
+/* contains_dense_packed(std::span, num_symbols, device_span<uint8_t>) (filter.cuh:267-282). */
+int cusbf_contains_dense_packed_host_to_device(const cusbf_filter* f, const uint64_t* h_words,
+                                               uint64_t num_words, uint64_t num_symbols,
+                                               uint8_t* d_output, uint64_t d_output_len,
+                                               void* stream);
+/* contains_dense_packed(std::span, num_symbols) -> std::vector (filter.cuh:289-312):
+ * h_output must hold the window count. */
+int cusbf_contains_dense_packed(const cusbf_filter* f, const uint64_t* h_words, uint64_t num_words,
+                                uint64_t num_symbols, uint8_t* h_output, uint64_t h_output_len,
+                                void* stream);
 
 /* --- device record batches (filter.cuh:385-400, 603-618) ------------------- */
 
@@ -184,6 +203,15 @@ int cusbf_query_record_batch(const cusbf_filter* f, const char* h_sequence, uint
                              const cusbf_record_range* ranges, uint64_t num_ranges, void* stream,
                              cusbf_query_report* report);
 
+/* query_record_batch(RecordBatchView, Consumer) (filter.cuh:566-582, 1007-1131):
+ * like cusbf_query_record_batch, plus one byte per window of the normalised
+ * batch in h_hits (record r's hits start at its normalised offset,
+ * query_layout.cuh:41-72); synchronises. */
+int cusbf_query_record_batch_hits(const cusbf_filter* f, const char* h_sequence, uint64_t bytes,
+                                  const cusbf_record_range* ranges, uint64_t num_ranges,
+                                  void* stream, uint8_t* h_hits, uint64_t hits_len,
+                                  cusbf_query_report* report);
+
 /* --- counting and diagnostics ---------------------------------------------- */
 
 /* thrust::count(hits == 1) (count_positive_kmers.cuh:47-62) as a HIP reduction;
@@ -203,6 +231,54 @@ int cusbf_count_windows_runs_async(const cusbf_filter* f, const char* d_sequence
 
 /* Name of the kernel family the filter dispatches to ("dna_fast" or "generic"). */
 const char* cusbf_kernel_family(const cusbf_filter* f);
+
+/* --- request server (no reference counterpart) ------------------------------- */
+/*
+ * Serving form of the query path: the filter stays resident; a scheduler
+ * thread admits queued requests into batch slots as slots free up
+ * (continuous batching), each slot owning one HIP stream plus device and
+ * pinned host staging carved from `staging_bytes`; batches in different slots
+ * overlap on the device (copies and kernels). Every request is answered
+ * exactly as cusbf_query_record_batch would answer it alone: one byte per
+ * window of its normalised record batch (records joined by the alphabet
+ * separator, filter.cuh:930-990 layout). Not thread-safe for one ticket; any
+ * thread may submit or wait.
+ */
+typedef struct cusbf_server cusbf_server;
+
+typedef struct cusbf_server_options {
+    uint64_t staging_bytes; /* device (and pinned host) staging budget, split over the slots */
+    uint32_t slots;         /* batches in flight */
+    uint32_t max_wait_us;   /* extra coalescing wait for a batch that is not full (0 = none) */
+} cusbf_server_options;
+
+typedef struct cusbf_server_stats {
+    uint64_t requests;
+    uint64_t batches;
+    uint64_t max_requests_per_batch;
+    uint64_t bytes;
+    uint64_t windows;
+} cusbf_server_stats;
+
+int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* options,
+                        cusbf_server** out);
+/* Finishes every queued request, then frees the server. */
+void cusbf_server_destroy(cusbf_server* s);
+/* Queues one request: a host record batch (ranges into h_sequence). h_hits receives
+ * one byte per window of the normalised batch (hits_len must hold them). The
+ * caller keeps h_sequence and h_hits alive until cusbf_server_wait returns.
+ * A request larger than one slot's staging is rejected (CUSBF_ERR_RESOURCE). */
+int cusbf_server_submit(cusbf_server* s, const char* h_sequence, uint64_t bytes,
+                        const cusbf_record_range* ranges, uint64_t num_ranges, uint8_t* h_hits,
+                        uint64_t hits_len, uint64_t* ticket);
+/* Window count of the normalised batch a request will produce. */
+uint64_t cusbf_server_request_windows(const cusbf_server* s, const cusbf_record_range* ranges,
+                                      uint64_t num_ranges);
+/* Blocks until the request is answered; fills its report and its latency
+ * (submit to results in h_hits), then forgets the ticket. */
+int cusbf_server_wait(cusbf_server* s, uint64_t ticket, cusbf_query_report* report,
+                      double* latency_seconds);
+int cusbf_server_stats_get(const cusbf_server* s, cusbf_server_stats* out);
 
 #ifdef __cplusplus
 }
