@@ -43,6 +43,10 @@ def parse_args():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--probes", default="on", choices=["on", "off"],
                     help="measure stream-copy / random-sector ceilings after the timed steps")
+    ap.add_argument("--serve", default="on", choices=["on", "off"],
+                    help="serving scenario: p50/p99 latency under Poisson arrivals")
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-qps", type=float, default=100.0)
     return ap.parse_args()
 
 
@@ -100,6 +104,139 @@ def cpu_baseline(genome, reads, wl, threads: int):
         "insert_kmers_per_s": ins_rate,
         "query_kmers_per_s": q_rate,
         "seconds": (t2 - t0),
+    }
+
+
+def serving_scenario(filt, genome, reads, wl, n_requests: int, qps: float):
+    """Serving form on the resident filter vs plain per-request launches.
+
+    Requests: Poisson arrivals at `qps`; sizes log-uniform over 1..1M reads of
+    150 bp (long-read requests get the same byte sizes, at least one read);
+    half short reads (1% substitutions), half 10-50 kbp long reads (5% mixed
+    substitutions and indels). The server admits requests into 4 batch slots
+    under a 16 GB staging budget. Latency = arrival to answers ready.
+    """
+    import threading
+
+    import numpy as np
+    import torch
+
+    import cusbf_amd as cs
+    from cusbf_amd import workloads as W
+
+    rng = np.random.default_rng(4242)
+    stride = wl.reads.read_len + 1
+    pool_reads = min(wl.reads.n_reads, 8_000_000)
+    short = reads[: pool_reads * stride - 1].cpu().numpy()
+    d_long, long_off = W.generate_long_reads(genome, 20_000, seed=77)
+    long_pool = d_long.cpu().numpy()
+    del d_long
+    long_len = np.diff(long_off) - 1
+    long_len[-1] += 1
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, n_requests))
+    units = np.exp(rng.uniform(0, np.log(1_000_000), n_requests)).astype(np.int64)
+    kinds = rng.random(n_requests) < 0.5
+    reqs = []
+    for i in range(n_requests):
+        if kinds[i]:
+            n = int(units[i])
+            st = int(rng.integers(0, pool_reads - n + 1))
+            seq = short[st * stride:(st + n) * stride - 1]
+            rg = np.stack([np.arange(n, dtype=np.uint64) * stride,
+                           np.full(n, wl.reads.read_len, dtype=np.uint64)], axis=1)
+        else:
+            target = int(units[i]) * wl.reads.read_len
+            st = int(rng.integers(0, 20_000 - 64))
+            c = int(np.searchsorted(np.cumsum(long_len[st:]), target)) + 1
+            c = max(1, min(c, 20_000 - st))
+            base = int(long_off[st])
+            seq = long_pool[base:int(long_off[st + c - 1]) + int(long_len[st + c - 1])]
+            rg = np.stack([(long_off[st:st + c] - base).astype(np.uint64),
+                           long_len[st:st + c].astype(np.uint64)], axis=1)
+        reqs.append((seq, rg))
+    total_bytes = sum(int(r[0].size) for r in reqs)
+    # answers land in one pre-faulted arena reused by every run
+    from cusbf_amd.serve import normalized_size, ranges_array
+
+    win = [filt.record_kmer_count(normalized_size(*ranges_array(rg), 1)) for _, rg in reqs]
+    starts = np.concatenate([[0], np.cumsum(win)]).astype(np.int64)
+    arena = np.ones(int(starts[-1]) + 1, dtype=np.uint8)
+    outs = [arena[starts[i]:starts[i] + max(1, win[i])] for i in range(n_requests)]
+
+    def pct(x, q):
+        return float(np.percentile(np.asarray(x), q)) * 1e3
+
+    def run_server(burst: bool):
+        lat = [0.0] * n_requests
+        kmers = [0] * n_requests
+        positives = [0] * n_requests
+        with cs.Server(filt, staging_bytes=16 << 30, slots=4) as srv:
+            pend = [None] * n_requests
+            lag = [0.0] * n_requests
+            t0 = time.perf_counter()
+
+            def submitter():
+                for i, (seq, rg) in enumerate(reqs):
+                    if not burst:
+                        dt = t0 + arrivals[i] - time.perf_counter()
+                        if dt > 0:
+                            time.sleep(dt)
+                        lag[i] = max(0.0, time.perf_counter() - (t0 + arrivals[i]))
+                    pend[i] = srv.submit(seq, rg, out=outs[i])
+                    ready[i].set()
+
+            ready = [threading.Event() for _ in range(n_requests)]
+            th = threading.Thread(target=submitter)
+            th.start()
+            for i in range(n_requests):
+                ready[i].wait()
+                _, rep, l = srv.wait(pend[i])
+                lat[i] = l + lag[i]
+                kmers[i] = rep.queried_kmers
+                positives[i] = rep.positive_kmers
+                pend[i] = None
+            th.join()
+            span = time.perf_counter() - t0
+            st = srv.stats()
+        return lat, kmers, positives, span, st
+
+    def run_plain(burst: bool):
+        lat = [0.0] * n_requests
+        kmers = [0] * n_requests
+        positives = [0] * n_requests
+        t0 = time.perf_counter()
+        for i, (seq, rg) in enumerate(reqs):
+            if not burst:
+                dt = t0 + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+            _, rep = cs.query_record_batch_hits(filt, seq, rg, out=outs[i])
+            lat[i] = time.perf_counter() - (t0 + (0.0 if burst else arrivals[i]))
+            kmers[i] = rep.queried_kmers
+            positives[i] = rep.positive_kmers
+        return lat, kmers, positives, time.perf_counter() - t0
+
+    log(f"[serve] {n_requests} requests, {total_bytes / 1e9:.2f} GB, Poisson {qps:.0f} QPS")
+    s_lat, s_km, s_pos, s_span, s_st = run_server(False)
+    p_lat, p_km, p_pos, p_span = run_plain(False)
+    log(f"[serve] server p50 {pct(s_lat, 50):.1f} ms p99 {pct(s_lat, 99):.1f} ms; "
+        f"plain p50 {pct(p_lat, 50):.1f} ms p99 {pct(p_lat, 99):.1f} ms")
+    b_lat, b_km, b_pos, b_span, b_st = run_server(True)
+    pb_lat, pb_km, pb_pos, pb_span = run_plain(True)
+    log(f"[serve] burst: server {sum(b_km) / b_span / 1e9:.2f} G k-mers/s, plain "
+        f"{sum(pb_km) / pb_span / 1e9:.2f} G k-mers/s")
+    identical = (s_pos == p_pos == b_pos == pb_pos) and (s_km == p_km)
+    return {
+        "requests": n_requests, "qps": qps, "bytes": total_bytes,
+        "server_p50_ms": pct(s_lat, 50), "server_p99_ms": pct(s_lat, 99),
+        "plain_p50_ms": pct(p_lat, 50), "plain_p99_ms": pct(p_lat, 99),
+        "server_kmers_per_s": sum(s_km) / s_span, "plain_kmers_per_s": sum(p_km) / p_span,
+        "burst_server_kmers_per_s": sum(b_km) / b_span,
+        "burst_plain_kmers_per_s": sum(pb_km) / pb_span,
+        "burst_speedup": (sum(b_km) / b_span) / max(1e-9, sum(pb_km) / pb_span),
+        "server_batches": s_st["batches"], "server_max_requests_per_batch": s_st["max_requests_per_batch"],
+        "answers_identical": bool(identical),
+        "staging_bytes": 16 << 30, "slots": 4,
     }
 
 
@@ -183,6 +320,11 @@ def main() -> int:
     t_dom, b_dom = (tq, bytes_q) if dom == "query" else (ti, bytes_ins)
     achieved = b_dom / t_dom
 
+    # serving runs on the filter as inserted, before the probes overwrite its buffer
+    serving = None
+    if args.serve == "on":
+        serving = serving_scenario(filt, genome, reads, wl, args.serve_requests, args.serve_qps)
+
     ceilings = None
     if args.probes == "on":
         from cusbf_amd import probes
@@ -245,6 +387,10 @@ def main() -> int:
         "serve_p50_ms": None,
         "serve_p99_ms": None,
     }
+    if serving is not None:
+        result["serve_p50_ms"] = serving["server_p50_ms"]
+        result["serve_p99_ms"] = serving["server_p99_ms"]
+        result["serving"] = serving
     if args.cpu_baseline == "on":
         log("[bench] CPU baseline on a bounded sample")
         result["cpu_baseline"] = cpu_baseline(genome, reads, wl, args.cpu_threads)

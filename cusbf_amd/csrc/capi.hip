@@ -224,6 +224,16 @@ uint64_t valid_record_kmers(const cusbf_filter* f, const char* seq, uint64_t byt
     const uint32_t w = f->d.symbol_width;
     uint64_t valid = 0;
     int64_t last_invalid = -1;
+    if (f->cfg.alphabet == CUSBF_ALPHABET_DNA) {
+        // Vectorisable pre-check: records without invalid bytes have every window valid.
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(seq);
+        uint8_t any_bad = 0;
+        for (uint64_t i = 0; i < sym; ++i) {
+            const uint8_t u = p[i] & 0xDFu;
+            any_bad |= static_cast<uint8_t>((u != 'A') & (u != 'C') & (u != 'G') & (u != 'T'));
+        }
+        if (!any_bad) return sym - k + 1;
+    }
     if (w == 1) {
         bool bad[256];
         for (int b = 0; b < 256; ++b) {
@@ -250,17 +260,24 @@ void normalize(const cusbf_filter* f, const char* seq, const cusbf_record_range*
                std::vector<char>& out, std::vector<NormRecord>& records) {
     const uint32_t w = f->d.symbol_width;
     const char sep = static_cast<char>(f->d.separator);
-    out.clear();
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (total) total += (total % w ? w - total % w : 0) + w;
+        total += ranges[i].sequence_bytes;
+    }
+    out.resize(total);
     records.clear();
     records.reserve(n);
+    uint64_t pos = 0;
     for (uint64_t i = 0; i < n; ++i) {
-        if (!out.empty()) {
-            const uint64_t rem = out.size() % w;
-            if (rem != 0) out.insert(out.end(), w - rem, sep);
-            out.insert(out.end(), w, sep);
+        if (pos) {
+            const uint64_t pad = (pos % w ? w - pos % w : 0) + w;
+            std::memset(out.data() + pos, sep, pad);
+            pos += pad;
         }
         const char* rec = seq + ranges[i].sequence_offset;
-        out.insert(out.end(), rec, rec + ranges[i].sequence_bytes);
+        std::memcpy(out.data() + pos, rec, ranges[i].sequence_bytes);
+        pos += ranges[i].sequence_bytes;
         records.push_back(NormRecord{ranges[i].sequence_bytes, valid_record_kmers(f, rec, ranges[i].sequence_bytes)});
     }
 }

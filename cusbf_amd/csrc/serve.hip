@@ -60,6 +60,28 @@ struct Slot {
     std::vector<Request*> batch;
 };
 
+// Large copies run on several threads (host memory bandwidth needs more than one core).
+void parallel_copy(void* dst, const void* src, uint64_t bytes) {
+    constexpr uint64_t kChunk = 32ull << 20;
+    const uint64_t parts = std::min<uint64_t>(8, (bytes + kChunk - 1) / kChunk);
+    if (parts <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const uint64_t per = (bytes + parts - 1) / parts;
+    std::vector<std::thread> th;
+    for (uint64_t t = 1; t < parts; ++t) {
+        const uint64_t off = t * per;
+        if (off >= bytes) break;
+        th.emplace_back([=] {
+            std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off,
+                        std::min(per, bytes - off));
+        });
+    }
+    std::memcpy(dst, src, std::min(per, bytes));
+    for (std::thread& t : th) t.join();
+}
+
 // Separator padding of normalized_record_batch.hpp:208-232.
 uint64_t boundary_bytes(uint64_t current, uint32_t width) {
     if (current == 0) return 0;
@@ -98,26 +120,62 @@ struct cusbf_server {
 
 namespace {
 
+// Writes the slot's batch: requests one after another, each request's records
+// joined by the separator (padded to the symbol width), a separator between
+// requests. Offsets are laid out first; the copies then run on up to eight
+// threads over balanced groups of records.
 void fill_batch(cusbf_server* s, Slot& slot) {
-    char* out = slot.h_seq;
+    struct Piece {
+        uint64_t pad_at;  // separator bytes written at [pad_at, pad_at + pad)
+        uint64_t pad;
+        const char* src;
+        uint64_t len;
+    };
+    std::vector<Piece> pieces;
     uint64_t pos = 0;
     for (Request* r : slot.batch) {
         const uint64_t b = boundary_bytes(pos, s->width);
-        std::memset(out + pos, s->sep, b);
+        pieces.push_back(Piece{pos, b, nullptr, 0});
         pos += b;
         r->batch_offset = pos;
         uint64_t local = 0;
         for (const cusbf_record_range& rr : r->ranges) {
             const uint64_t lb = boundary_bytes(local, s->width);
-            std::memset(out + pos, s->sep, lb);
-            pos += lb;
-            local += lb;
-            std::memcpy(out + pos, r->seq + rr.sequence_offset, rr.sequence_bytes);
-            pos += rr.sequence_bytes;
-            local += rr.sequence_bytes;
+            pieces.push_back(Piece{pos, lb, r->seq + rr.sequence_offset, rr.sequence_bytes});
+            pos += lb + rr.sequence_bytes;
+            local += lb + rr.sequence_bytes;
         }
     }
     slot.batch_bytes = pos;
+    char* out = slot.h_seq;
+    auto run = [&](size_t first, size_t last) {
+        for (size_t i = first; i < last; ++i) {
+            const Piece& p = pieces[i];
+            if (p.pad) std::memset(out + p.pad_at, s->sep, p.pad);
+            if (p.len) std::memcpy(out + p.pad_at + p.pad, p.src, p.len);
+        }
+    };
+    constexpr uint64_t kPerThread = 16ull << 20;
+    const uint64_t threads = std::min<uint64_t>(8, pos / kPerThread + 1);
+    if (threads <= 1 || pieces.size() < 2) {
+        run(0, pieces.size());
+        return;
+    }
+    std::vector<std::thread> th;
+    size_t first = 0;
+    uint64_t done_bytes = 0;
+    for (uint64_t t = 1; t <= threads && first < pieces.size(); ++t) {
+        const uint64_t goal = pos * t / threads;
+        size_t last = first;
+        while (last < pieces.size() && (done_bytes < goal || last == first)) {
+            done_bytes += pieces[last].pad + pieces[last].len;
+            ++last;
+        }
+        if (t == threads) last = pieces.size();
+        th.emplace_back(run, first, last);
+        first = last;
+    }
+    for (std::thread& t : th) t.join();
 }
 
 void launch_batch(cusbf_server* s, Slot& slot) {
@@ -207,7 +265,7 @@ void completion_loop(cusbf_server* s) {
             r->status = slot.status;
             r->message = slot.message;
             if (slot.status == CUSBF_OK && r->windows) {
-                std::memcpy(r->hits, slot.h_hits + r->batch_offset / s->width, r->windows);
+                parallel_copy(r->hits, slot.h_hits + r->batch_offset / s->width, r->windows);
             }
             windows += r->windows;
             r->latency = std::chrono::duration<double>(now - r->submitted).count();

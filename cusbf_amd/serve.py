@@ -92,11 +92,15 @@ class Server:
         arr, n = ranges_array(ranges)
         return int(self._lib.cusbf_server_request_windows(self._h, arr, n))
 
-    def submit(self, sequence, ranges) -> Pending:
+    def submit(self, sequence, ranges, out: np.ndarray | None = None) -> Pending:
+        """Queue one request; `out` (uint8, at least the request's window count)
+        receives the answers, else a new array is allocated."""
         seq = _host_bytes(sequence)
         arr, n = ranges_array(ranges)
         windows = int(self._lib.cusbf_server_request_windows(self._h, arr, n))
-        hits = np.zeros(max(1, windows), dtype=np.uint8)
+        hits = out if out is not None else np.zeros(max(1, windows), dtype=np.uint8)
+        if hits.dtype != np.uint8 or not hits.flags.c_contiguous or hits.size < windows:
+            raise ValueError("out must be a contiguous uint8 array holding every window")
         t = ctypes.c_uint64()
         check(self._lib, self._lib.cusbf_server_submit(self._h, seq.ctypes.data, seq.size, arr, n,
                                                        hits.ctypes.data, hits.size,
@@ -121,7 +125,8 @@ class Server:
                 "windows": st.windows}
 
 
-def query_record_batch_hits(filt: Filter, sequence, ranges, stream=None):
+def query_record_batch_hits(filt: Filter, sequence, ranges, stream=None,
+                            out: np.ndarray | None = None):
     """filter.cuh:566-582, 1007-1131: one request alone -> (hits, QueryReport)."""
     from .filter import stream_handle
 
@@ -129,7 +134,9 @@ def query_record_batch_hits(filt: Filter, sequence, ranges, stream=None):
     seq = _host_bytes(sequence)
     arr, n = ranges_array(ranges)
     windows = filt.record_kmer_count(normalized_size(arr, n, filt.config.symbol_width))
-    hits = np.zeros(max(1, windows), dtype=np.uint8)
+    hits = out if out is not None else np.zeros(max(1, windows), dtype=np.uint8)
+    if hits.dtype != np.uint8 or not hits.flags.c_contiguous or hits.size < windows:
+        raise ValueError("out must be a contiguous uint8 array holding every window")
     rep = _lib.CQueryReport()
     check(lib, lib.cusbf_query_record_batch_hits(filt.handle, seq.ctypes.data, seq.size, arr, n,
                                                   stream_handle(stream), hits.ctypes.data,
