@@ -56,6 +56,11 @@ class CQueryReport(ctypes.Structure):
                 ("positive_kmers", u64)]
 
 
+class CBatchTuning(ctypes.Structure):
+    _fields_ = [("waves_per_simd", u32), ("blocks_per_cu", u32), ("claim", u32),
+                ("reserved", u32)]
+
+
 # (name, restype, argtypes) of every exported entry point of include/cusbf_hip.h.
 CFG = ctypes.POINTER(CConfig)
 RR = ctypes.POINTER(CRecordRange)
@@ -96,6 +101,18 @@ SIGNATURES = [
     ("cusbf_count_positive_per_record_async", i32, [P, P, P, u64, P, P]),
     ("cusbf_count_windows_runs_async", i32, [P, P, u64, P, P]),
     ("cusbf_kernel_family", ctypes.c_char_p, [P]),
+    ("cusbf_batch_bucket", u32, [u64, u64]),
+    ("cusbf_batch_tuning_set", i32, [P, u32, u32, ctypes.POINTER(CBatchTuning)]),
+    ("cusbf_batch_tuning_get", i32, [P, u32, u32, ctypes.POINTER(CBatchTuning)]),
+    ("cusbf_insert_batched_async", i32, [P, P, u64, P, u64, P]),
+    ("cusbf_contains_batched_async", i32, [P, P, u64, P, u64, P, u64, P]),
+    ("cusbf_batched_status", i32, [P, P]),
+    ("cusbf_batch_plan_create", i32, [P, u32, P, u64, P, u64, P, u64,
+                                      ctypes.POINTER(CBatchTuning), ctypes.POINTER(P)]),
+    ("cusbf_batch_plan_launch", i32, [P, P]),
+    ("cusbf_batch_plan_status", i32, [P, P]),
+    ("cusbf_batch_plan_destroy", None, [P]),
+    ("cusbf_count_batched_windows_runs_async", i32, [P, P, u64, P, u64, P, P]),
     ("cusbf_server_create", i32, [P, ctypes.POINTER(CServerOptions), ctypes.POINTER(P)]),
     ("cusbf_server_destroy", None, [P]),
     ("cusbf_server_submit", i32, [P, P, u64, RR, u64, P, u64, ctypes.POINTER(u64)]),

@@ -3,9 +3,9 @@
 // Mirrors the hot-path methods of the reference's cusbf::filter<Config>
 // (include/cusbf/filter.cuh:68-1848): same counting conventions, same argument
 // checks and error strings, Result<T> categories mapped onto status codes.
-// Kernel dispatch: DNA configurations with a compiled fast kernel set use the
-// wave64 lane-strip kernels (kernels_dna.hip), everything else the generic
-// kernels (kernels_generic.hip). There is no CPU fallback.
+// Kernel dispatch: configurations with a compiled lane-strip kernel set use
+// the wave64 lane-strip kernels (kernels_strip.hip), everything else the
+// generic kernels (kernels_generic.hip). There is no CPU fallback.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -108,7 +108,7 @@ struct cusbf_filter {
     cusbf_config cfg{};
     Derived d{};
     sbf::KernelConfig kc{};
-    const sbf::DnaKernelSet* dna = nullptr;
+    const sbf::StripKernelSet* strip = nullptr;
     uint64_t num_shards = 0;
     uint64_t filter_bits = 0;
     uint64_t* d_words = nullptr;
@@ -119,6 +119,23 @@ struct cusbf_filter {
     uint8_t* d_result = nullptr;
     uint64_t result_cap = 0;
     uint64_t* d_scalar = nullptr;  // reductions
+    // batched record form: tuning table, work-queue state, tile -> first record map
+    uint32_t compute_units = 256;
+    cusbf_batch_tuning tuning[CUSBF_BATCH_BUCKETS][2] = {};
+    uint32_t* d_batch_state = nullptr;
+    uint32_t* d_tile_rec = nullptr;
+    uint64_t tile_rec_cap = 0;
+    bool batch_check_pending = false;
+};
+
+struct cusbf_batch_plan {
+    cusbf_filter* f = nullptr;
+    uint32_t op = 0;
+    bool empty = true;
+    uint32_t* d_state = nullptr;
+    uint32_t* d_tile_rec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -163,17 +180,22 @@ void release_staging(cusbf_filter* f) {
 
 sbf::FilterView view(const cusbf_filter* f) { return sbf::FilterView{f->d_words, f->num_shards}; }
 
+// Lane-strip kernels read byte input, and dense packed input for DNA.
+bool strip_reads(const cusbf_filter* f, const sbf::SequenceInput& in) {
+    return f->strip != nullptr && (!in.dense || f->strip->dense);
+}
+
 int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
-    const hipError_t e = f->dna ? f->dna->insert(in, view(f), stream)
-                                : sbf::generic_insert(f->kc, in, view(f), stream);
+    const hipError_t e = strip_reads(f, in) ? f->strip->insert(in, view(f), stream)
+                                            : sbf::generic_insert(f->kc, in, view(f), stream);
     if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
     return CUSBF_OK;
 }
 
 int launch_contains(const cusbf_filter* f, const sbf::SequenceInput& in, uint8_t* out,
                     hipStream_t stream) {
-    const hipError_t e = f->dna ? f->dna->contains(in, view(f), out, stream)
-                                : sbf::generic_contains(f->kc, in, view(f), out, stream);
+    const hipError_t e = strip_reads(f, in) ? f->strip->contains(in, view(f), out, stream)
+                                            : sbf::generic_contains(f->kc, in, view(f), out, stream);
     if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
     return CUSBF_OK;
 }
@@ -282,6 +304,116 @@ void normalize(const cusbf_filter* f, const char* seq, const cusbf_record_range*
     }
 }
 
+// --- batched record form ------------------------------------------------------
+
+// Launch shape used until a tuner sets the table: measured defaults of the
+// lane-strip kernels (the generic kernels ignore it).
+void default_tuning(cusbf_filter* f) {
+    const bool wide = f->strip && f->strip->batched_insert[1] != nullptr;
+    for (auto& bucket : f->tuning) {
+        for (auto& t : bucket) t = cusbf_batch_tuning{wide ? 4u : 3u, wide ? 4u : 3u, 2u, 0u};
+    }
+}
+
+int check_tuning(const cusbf_filter* f, const cusbf_batch_tuning& t) {
+    if (t.waves_per_simd != 3 && t.waves_per_simd != 4) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "waves_per_simd must be 3 or 4");
+    }
+    if (t.waves_per_simd == 4 && f->strip && f->strip->batched_insert[1] == nullptr) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "this configuration's batched kernels are built for 3 waves per SIMD");
+    }
+    if (t.blocks_per_cu < 1 || t.blocks_per_cu > 32) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 32]");
+    }
+    if (t.claim > 64) return fail(CUSBF_ERR_INVALID_ARGUMENT, "claim must be in [0, 64]");
+    return CUSBF_OK;
+}
+
+bool batched_strip(const cusbf_filter* f) { return f->strip && f->strip->batched_insert[0]; }
+
+uint64_t batch_tiles(const cusbf_filter* f, uint64_t num_symbols) {
+    if (!batched_strip(f)) return 0;
+    const uint64_t windows = num_symbols < f->cfg.k ? 0 : num_symbols - f->cfg.k + 1;
+    return (windows + f->strip->wave_windows - 1) / f->strip->wave_windows;
+}
+
+int validate_batched(const cusbf_filter* f, const char* d_symbols, uint64_t num_symbols,
+                     const uint64_t* d_offsets, uint64_t num_records) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (!d_offsets) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device offsets");
+    if (num_symbols && !d_symbols) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device sequence");
+    if (num_records >= 0xFFFFFFFFull) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "a batched call holds fewer than 2^32 - 1 records");
+    }
+    if (batch_tiles(f, num_symbols) >= (1ull << 31)) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "batched sequence is too long for one launch");
+    }
+    return CUSBF_OK;
+}
+
+uint64_t batched_windows(const cusbf_filter* f, uint64_t num_symbols) {
+    return num_symbols < f->cfg.k ? 0 : num_symbols - f->cfg.k + 1;
+}
+
+int ensure_tile_rec(cusbf_filter* f, uint64_t tiles) {
+    if (tiles <= f->tile_rec_cap) return CUSBF_OK;
+    if (f->d_tile_rec) {
+        CUSBF_HIP_TRY(hipDeviceSynchronize());  // an earlier batched launch may still read it
+        CUSBF_HIP_TRY(hipFree(f->d_tile_rec));
+    }
+    f->d_tile_rec = nullptr;
+    f->tile_rec_cap = 0;
+    CUSBF_HIP_TRY(hipMalloc(&f->d_tile_rec, tiles * sizeof(uint32_t)));
+    f->tile_rec_cap = tiles;
+    return CUSBF_OK;
+}
+
+const cusbf_batch_tuning& tuning_for(const cusbf_filter* f, uint32_t op, uint64_t num_symbols,
+                                     uint64_t num_records) {
+    return f->tuning[cusbf_batch_bucket(num_symbols, num_records)][op];
+}
+
+// Reset, prepass and kernel of one batched call, in stream order; allocates
+// nothing, so it can be captured into a graph.
+int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
+                    uint64_t num_symbols, const uint64_t* d_offsets, uint64_t num_records,
+                    uint32_t* d_state, uint32_t* d_tile_rec, const cusbf_batch_tuning& t,
+                    uint8_t* d_output, hipStream_t stream) {
+    sbf::SequenceInput in = byte_input(f, d_symbols, num_symbols * f->d.symbol_width);
+    in.offsets = d_offsets;
+    in.num_records = num_records;
+    in.tile_rec = d_tile_rec;
+    in.state = d_state;
+    const uint64_t tiles = batch_tiles(f, num_symbols);
+    CUSBF_HIP_TRY(sbf::batch_reset(d_state, stream));
+    CUSBF_HIP_TRY(sbf::batch_prepass(in, batched_strip(f) ? f->strip->wave_windows : 0, tiles,
+                                     d_tile_rec, d_state, stream));
+    hipError_t e;
+    if (batched_strip(f)) {
+        const int variant = t.waves_per_simd == 4 && f->strip->batched_insert[1] ? 1 : 0;
+        const sbf::BatchLaunch shape{t.blocks_per_cu, f->compute_units, t.claim};
+        e = op == CUSBF_BATCH_INSERT
+                ? f->strip->batched_insert[variant](in, view(f), shape, stream)
+                : f->strip->batched_contains[variant](in, view(f), shape, d_output, stream);
+    } else {
+        e = op == CUSBF_BATCH_INSERT ? sbf::generic_insert(f->kc, in, view(f), stream)
+                                     : sbf::generic_contains(f->kc, in, view(f), d_output, stream);
+    }
+    if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+    return CUSBF_OK;
+}
+
+int read_batch_flag(const uint32_t* d_state, hipStream_t stream) {
+    CUSBF_HIP_TRY(hipStreamSynchronize(stream));
+    uint32_t flag = 0;
+    CUSBF_HIP_TRY(hipMemcpy(&flag, d_state + 1, sizeof(flag), hipMemcpyDeviceToHost));
+    if (flag != 0) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT,
+                    "batched offsets must be non-decreasing from 0 to num_symbols");
+    }
+    return CUSBF_OK;
+}
+
 }  // namespace
 
 namespace cusbf_internal {
@@ -374,9 +506,10 @@ int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_
     f->kc.invalid_symbol = d.invalid_symbol;
     std::memcpy(f->kc.lut, cfg->lut, 256);
     const char* force_generic = std::getenv("CUSBF_FORCE_GENERIC");
-    if (cfg->alphabet == CUSBF_ALPHABET_DNA && !(force_generic && force_generic[0] == '1')) {
-        f->dna = sbf::find_dna_kernels(cfg->k, cfg->s, cfg->m, cfg->hash_count);
+    if (!(force_generic && force_generic[0] == '1')) {
+        f->strip = sbf::find_strip_kernels(cfg->k, cfg->s, cfg->m, cfg->hash_count, cfg->alphabet);
     }
+    default_tuning(f);
     // filter.cuh:108-117
     uint64_t shards = requested_filter_bits / 256 + (requested_filter_bits % 256 != 0);
     if (shards < 1) shards = 1;
@@ -387,6 +520,14 @@ int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_
     hipError_t e = hipGetDevice(&f->device);
     if (e == hipSuccess) e = hipMalloc(&f->d_words, f->num_shards * 32);
     if (e == hipSuccess) e = hipMalloc(&f->d_scalar, 4 * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&f->d_batch_state, 64);
+    if (e == hipSuccess) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, f->device) == hipSuccess &&
+            cus > 0) {
+            f->compute_units = static_cast<uint32_t>(cus);
+        }
+    }
     if (e != hipSuccess) {
         const int st = fail_hip(e, __FILE__, __LINE__);
         cusbf_destroy(f);
@@ -406,6 +547,8 @@ void cusbf_destroy(cusbf_filter* f) {
     release_staging(f);
     if (f->d_words) (void)hipFree(f->d_words);
     if (f->d_scalar) (void)hipFree(f->d_scalar);
+    if (f->d_batch_state) (void)hipFree(f->d_batch_state);
+    if (f->d_tile_rec) (void)hipFree(f->d_tile_rec);
     delete f;
 }
 
@@ -448,7 +591,7 @@ int cusbf_copy_words_to_host(const cusbf_filter* f, uint64_t* host_words, uint64
 
 const char* cusbf_kernel_family(const cusbf_filter* f) {
     if (!f) return "";
-    return f->dna ? "dna_fast" : "generic";
+    return f->strip ? "lane_strip" : "generic";
 }
 
 // filter.cuh:179-190
@@ -815,6 +958,178 @@ int cusbf_count_windows_runs_async(const cusbf_filter* f, const char* d_sequence
     CUSBF_HIP_TRY(sbf::fill_u64(d_counts, 0, 2, as_stream(stream)));
     if (windows_of(f, bytes) == 0) return CUSBF_OK;
     CUSBF_HIP_TRY(sbf::generic_count(f->kc, byte_input(f, d_sequence, bytes), f->num_shards, d_counts, as_stream(stream)));
+    return CUSBF_OK;
+}
+
+// --- batched record form --------------------------------------------------------
+
+uint32_t cusbf_batch_bucket(uint64_t num_symbols, uint64_t num_records) {
+    const uint64_t mean = num_records ? num_symbols / num_records : num_symbols;
+    if (mean < 256) return 0;
+    if (mean < 4096) return 1;
+    if (mean < 65536) return 2;
+    return 3;
+}
+
+int cusbf_batch_tuning_set(cusbf_filter* f, uint32_t bucket, uint32_t op,
+                           const cusbf_batch_tuning* tuning) {
+    if (!f || !tuning) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    if (bucket >= CUSBF_BATCH_BUCKETS || op > CUSBF_BATCH_CONTAINS) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "batch bucket or op out of range");
+    }
+    const int rc = check_tuning(f, *tuning);
+    if (rc != CUSBF_OK) return rc;
+    f->tuning[bucket][op] = *tuning;
+    return CUSBF_OK;
+}
+
+int cusbf_batch_tuning_get(const cusbf_filter* f, uint32_t bucket, uint32_t op,
+                           cusbf_batch_tuning* out) {
+    if (!f || !out) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    if (bucket >= CUSBF_BATCH_BUCKETS || op > CUSBF_BATCH_CONTAINS) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "batch bucket or op out of range");
+    }
+    *out = f->tuning[bucket][op];
+    return CUSBF_OK;
+}
+
+int cusbf_insert_batched_async(cusbf_filter* f, const char* d_symbols, uint64_t num_symbols,
+                               const uint64_t* d_offsets, uint64_t num_records, void* stream) {
+    int rc = validate_batched(f, d_symbols, num_symbols, d_offsets, num_records);
+    if (rc != CUSBF_OK) return rc;
+    f->batch_check_pending = false;
+    if (batched_windows(f, num_symbols) == 0) return CUSBF_OK;
+    rc = ensure_tile_rec(f, batch_tiles(f, num_symbols) + 1);
+    if (rc != CUSBF_OK) return rc;
+    rc = enqueue_batched(f, CUSBF_BATCH_INSERT, d_symbols, num_symbols, d_offsets, num_records,
+                         f->d_batch_state, f->d_tile_rec,
+                         tuning_for(f, CUSBF_BATCH_INSERT, num_symbols, num_records),
+                         nullptr, as_stream(stream));
+    f->batch_check_pending = rc == CUSBF_OK;
+    return rc;
+}
+
+int cusbf_contains_batched_async(const cusbf_filter* f_const, const char* d_symbols,
+                                 uint64_t num_symbols, const uint64_t* d_offsets,
+                                 uint64_t num_records, uint8_t* d_output, uint64_t d_output_len,
+                                 void* stream) {
+    int rc = validate_batched(f_const, d_symbols, num_symbols, d_offsets, num_records);
+    if (rc != CUSBF_OK) return rc;
+    // the batch scratch is internal state, like the reference's mutable staging
+    cusbf_filter* f = const_cast<cusbf_filter*>(f_const);
+    f->batch_check_pending = false;
+    const uint64_t windows = batched_windows(f, num_symbols);
+    if (windows == 0) return CUSBF_OK;
+    if (!d_output) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device output");
+    if (d_output_len < windows) return fail(CUSBF_ERR_INVALID_ARGUMENT, "batched query output span is too small");
+    rc = ensure_tile_rec(f, batch_tiles(f, num_symbols) + 1);
+    if (rc != CUSBF_OK) return rc;
+    rc = enqueue_batched(f, CUSBF_BATCH_CONTAINS, d_symbols, num_symbols, d_offsets, num_records,
+                         f->d_batch_state, f->d_tile_rec,
+                         tuning_for(f, CUSBF_BATCH_CONTAINS, num_symbols, num_records), d_output,
+                         as_stream(stream));
+    f->batch_check_pending = rc == CUSBF_OK;
+    return rc;
+}
+
+int cusbf_batched_status(const cusbf_filter* f, void* stream) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    if (!f->batch_check_pending) return CUSBF_OK;
+    return read_batch_flag(f->d_batch_state, as_stream(stream));
+}
+
+int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
+                            uint64_t num_symbols, const uint64_t* d_offsets, uint64_t num_records,
+                            uint8_t* d_output, uint64_t d_output_len,
+                            const cusbf_batch_tuning* tuning, cusbf_batch_plan** out) {
+    if (!out) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    int rc = validate_batched(f, d_symbols, num_symbols, d_offsets, num_records);
+    if (rc != CUSBF_OK) return rc;
+    if (op > CUSBF_BATCH_CONTAINS) return fail(CUSBF_ERR_INVALID_ARGUMENT, "batch op out of range");
+    const uint64_t windows = batched_windows(f, num_symbols);
+    if (op == CUSBF_BATCH_CONTAINS && windows != 0) {
+        if (!d_output) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device output");
+        if (d_output_len < windows) return fail(CUSBF_ERR_INVALID_ARGUMENT, "batched query output span is too small");
+    }
+    const cusbf_batch_tuning t = tuning ? *tuning : tuning_for(f, op, num_symbols, num_records);
+    rc = check_tuning(f, t);
+    if (rc != CUSBF_OK) return rc;
+
+    cusbf_batch_plan* p = new cusbf_batch_plan();
+    p->f = f;
+    p->op = op;
+    p->empty = windows == 0;
+    if (p->empty) {
+        *out = p;
+        return CUSBF_OK;
+    }
+    hipStream_t capture = nullptr;
+    hipError_t e = hipMalloc(&p->d_state, 64);
+    if (e == hipSuccess) e = hipMalloc(&p->d_tile_rec, (batch_tiles(f, num_symbols) + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&capture, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(capture, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        rc = fail_hip(e, __FILE__, __LINE__);
+    } else {
+        rc = enqueue_batched(f, op, d_symbols, num_symbols, d_offsets, num_records, p->d_state,
+                             p->d_tile_rec, t, d_output, capture);
+        const hipError_t end = hipStreamEndCapture(capture, &p->graph);
+        if (rc == CUSBF_OK && end != hipSuccess) rc = fail_hip(end, __FILE__, __LINE__);
+        if (rc == CUSBF_OK) {
+            e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = fail_hip(e, __FILE__, __LINE__);
+        }
+    }
+    if (capture) (void)hipStreamDestroy(capture);
+    if (rc != CUSBF_OK) {
+        cusbf_batch_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return CUSBF_OK;
+}
+
+int cusbf_batch_plan_launch(cusbf_batch_plan* p, void* stream) {
+    if (!p) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null plan");
+    if (p->empty) return CUSBF_OK;
+    CUSBF_HIP_TRY(hipGraphLaunch(p->exec, as_stream(stream)));
+    return CUSBF_OK;
+}
+
+int cusbf_batch_plan_status(cusbf_batch_plan* p, void* stream) {
+    if (!p) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null plan");
+    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    if (p->empty) return CUSBF_OK;
+    return read_batch_flag(p->d_state, as_stream(stream));
+}
+
+void cusbf_batch_plan_destroy(cusbf_batch_plan* p) {
+    if (!p) return;
+    if (p->exec || p->d_state) (void)hipDeviceSynchronize();  // a replay may still run
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->d_state) (void)hipFree(p->d_state);
+    if (p->d_tile_rec) (void)hipFree(p->d_tile_rec);
+    delete p;
+}
+
+// No reference counterpart: valid windows and shard runs of a batched input
+// (windows across a record start are not valid), for the byte model.
+int cusbf_count_batched_windows_runs_async(const cusbf_filter* f, const char* d_symbols,
+                                           uint64_t num_symbols, const uint64_t* d_offsets,
+                                           uint64_t num_records, uint64_t* d_counts,
+                                           void* stream) {
+    int rc = validate_batched(f, d_symbols, num_symbols, d_offsets, num_records);
+    if (rc != CUSBF_OK) return rc;
+    if (!d_counts) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    CUSBF_HIP_TRY(sbf::fill_u64(d_counts, 0, 2, as_stream(stream)));
+    if (batched_windows(f, num_symbols) == 0) return CUSBF_OK;
+    sbf::SequenceInput in = byte_input(f, d_symbols, num_symbols * f->d.symbol_width);
+    in.offsets = d_offsets;
+    in.num_records = num_records;
+    CUSBF_HIP_TRY(sbf::generic_count(f->kc, in, f->num_shards, d_counts, as_stream(stream)));
     return CUSBF_OK;
 }
 

@@ -4,9 +4,9 @@
 // One window per thread over an LDS tile of encoded symbols, computed the way
 // the reference computes each window (include/cusbf/detail/kernels.cuh:40-172:
 // pack the k-mer, minimum of the k-m+1 m-mer hashes, k-s+1 s-mer hashes).
-// Configurations without a fast DNA kernel (protein, triplets, custom
-// alphabets, other k/s/m/h) run here; the fast DNA kernels are checked against
-// these and against the CPU oracle.
+// Configurations without lane-strip kernels (triplets, custom alphabets, other
+// k/s/m/h, dense protein input) run here; the lane-strip kernels are checked
+// against these and against the CPU oracle.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -85,11 +85,39 @@ __device__ __forceinline__ void masks(const KernelConfig& c, uint64_t packed, ui
     }
 }
 
+// Batched form: true when a record starts inside window j after its first
+// symbol (the first start after j, found by binary search over the offsets, is
+// closer than k). Malformed offsets give wrong answers, never an access
+// outside the offsets array.
+__device__ __forceinline__ bool crosses_record(const SequenceInput& in, uint64_t j, uint32_t k) {
+    uint64_t lo = 0, hi = in.num_records + 1;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (in.offsets[mid] <= j) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo <= in.num_records && in.offsets[lo] - j < k;
+}
+
+__device__ __forceinline__ bool window_ok(const SequenceInput& in, const Window& w, uint64_t j,
+                                          uint32_t k) {
+    return w.valid && (in.offsets == nullptr || !crosses_record(in, j, k));
+}
+
+// Batched form with offsets the prepass found malformed: no work.
+__device__ __forceinline__ bool batch_rejected(const SequenceInput& in) {
+    return in.state != nullptr && in.state[1] != 0;
+}
+
 // Blocks stride over tiles of kBlock windows: a dispatch must stay below 2^32
 // work-items, so grids are capped and large inputs loop.
 __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, SequenceInput in,
                                                         FilterView f, uint64_t num_windows) {
     __shared__ uint8_t tile[kBlock + kMaxK];
+    if (batch_rejected(in)) return;
     for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
          first += static_cast<uint64_t>(gridDim.x) * kBlock) {
         __syncthreads();
@@ -98,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, Sequence
         const uint64_t j = first + threadIdx.x;
         if (j >= num_windows) continue;
         const Window w = read_window(c, tile + threadIdx.x);
-        if (!w.valid) continue;
+        if (!window_ok(in, w, j, c.k)) continue;
         const uint32_t shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
         uint64_t m[4];
         masks(c, w.packed, m);
@@ -113,6 +141,7 @@ __global__ __launch_bounds__(kBlock) void contains_kernel(KernelConfig c, Sequen
                                                           FilterView f, uint64_t num_windows,
                                                           uint8_t* out) {
     __shared__ uint8_t tile[kBlock + kMaxK];
+    if (batch_rejected(in)) return;
     for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
          first += static_cast<uint64_t>(gridDim.x) * kBlock) {
         __syncthreads();
@@ -121,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void contains_kernel(KernelConfig c, Sequen
         const uint64_t j = first + threadIdx.x;
         if (j >= num_windows) continue;
         const Window w = read_window(c, tile + threadIdx.x);
-        if (!w.valid) {
+        if (!window_ok(in, w, j, c.k)) {
             out[j] = 0;
             continue;
         }
@@ -156,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void count_kernel(KernelConfig c, SequenceI
             uint32_t shard = 0;
             if (j >= 0 && static_cast<uint64_t>(j) < num_windows) {
                 const Window w = read_window(c, tile + t);
-                valid = w.valid;
+                valid = window_ok(in, w, static_cast<uint64_t>(j), c.k);
                 if (valid) shard = static_cast<uint32_t>(minimizer(c, w.packed) & (num_shards - 1));
             }
             shard_of[t] = shard;
@@ -274,6 +303,42 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(uint64_t* dst, uint64_t va
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) dst[i] = value;
 }
 
+__global__ void batch_reset_kernel(uint32_t* state) {
+    if (threadIdx.x < 2) state[threadIdx.x] = 0;
+}
+
+// Batched form prepass: tile_rec[t] = first record whose start is at or after
+// the first symbol of tile t (lower bound over offsets[0..num_records]); sets
+// state[1] when the offsets are not non-decreasing from 0 to num_symbols.
+__global__ __launch_bounds__(kBlock) void batch_prepass_kernel(const uint64_t* offsets,
+                                                               uint64_t num_records,
+                                                               uint64_t num_symbols,
+                                                               uint64_t tile_windows,
+                                                               uint64_t num_tiles,
+                                                               uint32_t* tile_rec,
+                                                               uint32_t* state) {
+    const uint64_t n = num_tiles > num_records + 1 ? num_tiles : num_records + 1;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+        if (i < num_tiles) {
+            const uint64_t first = i * tile_windows;
+            uint64_t lo = 0, hi = num_records + 1;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) / 2;
+                if (offsets[mid] < first) {
+                    lo = mid + 1;
+                } else {
+                    hi = mid;
+                }
+            }
+            tile_rec[i] = static_cast<uint32_t>(lo);
+        }
+        bool bad = i < num_records && offsets[i] > offsets[i + 1];
+        if (i == 0) bad = bad || offsets[0] != 0 || offsets[num_records] != num_symbols;
+        if (bad) atomicOr(&state[1], 1u);
+    }
+}
+
 uint32_t grid_for(uint64_t items, uint64_t per_block, uint64_t cap) {
     uint64_t g = (items + per_block - 1) / per_block;
     if (g < 1) g = 1;
@@ -356,6 +421,22 @@ hipError_t count_ones_per_record(const uint8_t* hits, const uint64_t* offsets,
     hipLaunchKernelGGL(count_ones_per_record_kernel,
                        dim3(grid_for(num_records, kBlock / 64, kMaxBlocks)), dim3(kBlock), 0,
                        stream, hits, offsets, counts, num_records, d_out);
+    return hipGetLastError();
+}
+
+hipError_t batch_reset(uint32_t* state, hipStream_t stream) {
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, state);
+    return hipGetLastError();
+}
+
+hipError_t batch_prepass(const SequenceInput& in, uint64_t tile_windows, uint64_t num_tiles,
+                         uint32_t* tile_rec, uint32_t* state, hipStream_t stream) {
+    const uint64_t n = num_tiles > in.num_records + 1 ? num_tiles : in.num_records + 1;
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(batch_prepass_kernel, dim3(grid_for(n, kBlock, kMaxBlocks)), dim3(kBlock), 0,
+                       stream, in.offsets, in.num_records, in.num_symbols, tile_windows, num_tiles,
+                       tile_rec, state);
     return hipGetLastError();
 }
 

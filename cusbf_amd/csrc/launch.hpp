@@ -12,12 +12,22 @@ namespace sbf {
 // Input of one launch: either a byte sequence (`seq`, symbol_width bytes per
 // symbol) or a dense packed sequence (`words`, LSB-first, 64/symbol_bits
 // symbols per word).
+//
+// Batched form: `offsets` (num_records + 1 non-decreasing symbol offsets,
+// offsets[0] = 0, offsets[num_records] = num_symbols) cut the byte sequence
+// into records concatenated without separators; a window that crosses a
+// record start is invalid. The prepass fills `tile_rec` (lane-strip kernels
+// only) and checks the offsets into `state`.
 struct SequenceInput {
     const char* seq = nullptr;
     const uint64_t* words = nullptr;
     uint64_t num_words = 0;
     uint64_t num_symbols = 0;
     bool dense = false;
+    const uint64_t* offsets = nullptr;
+    uint64_t num_records = 0;
+    const uint32_t* tile_rec = nullptr;  // first record starting at or after each tile's first symbol
+    uint32_t* state = nullptr;           // [0] work-queue counter, [1] malformed-offsets flag
 };
 
 struct FilterView {
@@ -25,17 +35,37 @@ struct FilterView {
     uint64_t num_shards = 0;
 };
 
-// Fast DNA kernels for one compile-time (k, s, m, h).
-struct DnaKernelSet {
-    uint32_t k, s, m, h;
-    hipError_t (*insert)(const SequenceInput&, FilterView, hipStream_t);
-    hipError_t (*contains)(const SequenceInput&, FilterView, uint8_t*, hipStream_t);
+// Launch shape of the persistent batched kernels.
+struct BatchLaunch {
+    uint32_t blocks_per_cu = 4;
+    uint32_t compute_units = 256;
+    uint32_t claim = 1;  // tiles per work-queue claim; 0 = waves stride over the tiles
 };
 
-// Returns the compiled fast kernel set for (k, s, m, h) or nullptr.
-const DnaKernelSet* find_dna_kernels(uint32_t k, uint32_t s, uint32_t m, uint32_t h);
+using InsertFn = hipError_t (*)(const SequenceInput&, FilterView, hipStream_t);
+using ContainsFn = hipError_t (*)(const SequenceInput&, FilterView, uint8_t*, hipStream_t);
+using BatchedInsertFn = hipError_t (*)(const SequenceInput&, FilterView, const BatchLaunch&,
+                                       hipStream_t);
+using BatchedContainsFn = hipError_t (*)(const SequenceInput&, FilterView, const BatchLaunch&,
+                                         uint8_t*, hipStream_t);
 
-// Generic kernels: any alphabet / configuration, one window per thread.
+// Lane-strip kernels for one compile-time (k, s, m, h, alphabet).
+struct StripKernelSet {
+    uint32_t k, s, m, h, alphabet;
+    bool dense;             // plain kernels also read dense packed input
+    uint32_t wave_windows;  // windows per wave tile (batched tile map granularity)
+    InsertFn insert;
+    ContainsFn contains;
+    BatchedInsertFn batched_insert[2];      // [0]: built for 3 waves per SIMD, [1]: for 4 (or null)
+    BatchedContainsFn batched_contains[2];
+};
+
+// Returns the compiled lane-strip kernel set or nullptr.
+const StripKernelSet* find_strip_kernels(uint32_t k, uint32_t s, uint32_t m, uint32_t h,
+                                         uint32_t alphabet);
+
+// Generic kernels: any alphabet / configuration, one window per thread; they
+// honour `offsets` (batched form) by a per-window search.
 hipError_t generic_insert(const KernelConfig& c, const SequenceInput& in, FilterView f,
                           hipStream_t stream);
 hipError_t generic_contains(const KernelConfig& c, const SequenceInput& in, FilterView f,
@@ -45,6 +75,13 @@ hipError_t generic_count(const KernelConfig& c, const SequenceInput& in, uint64_
                          uint64_t* d_counts, hipStream_t stream);
 hipError_t pack_dense(const KernelConfig& c, const char* seq, uint64_t num_symbols,
                       uint64_t* words, hipStream_t stream);
+
+// Batched form set-up, in stream order before the batched kernels: zero the
+// work-queue counter and the flag, then fill tile_rec for tiles of
+// `tile_windows` windows (none when num_tiles is 0) and check the offsets.
+hipError_t batch_reset(uint32_t* state, hipStream_t stream);
+hipError_t batch_prepass(const SequenceInput& in, uint64_t tile_windows, uint64_t num_tiles,
+                         uint32_t* tile_rec, uint32_t* state, hipStream_t stream);
 
 // Reductions.
 hipError_t popcount_words(const uint64_t* words, uint64_t n, uint64_t* d_out, hipStream_t stream);

@@ -229,8 +229,78 @@ int cusbf_count_positive_per_record_async(const uint8_t* d_hits, const uint64_t*
 int cusbf_count_windows_runs_async(const cusbf_filter* f, const char* d_sequence, uint64_t bytes,
                                    uint64_t* d_counts, void* stream);
 
-/* Name of the kernel family the filter dispatches to ("dna_fast" or "generic"). */
+/* Name of the kernel family the filter dispatches to ("lane_strip" or "generic"). */
 const char* cusbf_kernel_family(const cusbf_filter* f);
+
+/* --- batched record form (no single-call counterpart in the reference) -------- */
+/*
+ * Many records of uneven length in ONE launch, without the separator
+ * normalisation of the record-batch path: d_symbols holds the records
+ * concatenated back to back (num_symbols symbols, symbol_width bytes each) and
+ * d_offsets (device memory, num_records + 1 entries) the symbol offset of each
+ * record start, non-decreasing from 0 to num_symbols. Every window that lies
+ * inside one record is inserted / queried exactly as insert_sequence /
+ * contains_sequence would on that record alone (filter.cuh:179-190, 464-474);
+ * a window that crosses a record start is skipped on insert and answered 0 on
+ * query, the answer the reference gives for a window across a separator
+ * (normalized_record_batch.hpp:208-232). Query output: one byte per window
+ * start of the concatenation (num_symbols - k + 1 bytes); record r's answers
+ * are d_output[d_offsets[r] .. d_offsets[r+1] - k].
+ *
+ * Launch sequence (stream order): reset of the work-queue counter (a fill
+ * kernel, so the sequence can be captured in a graph), a prepass that finds
+ * each tile's first record and checks the offsets, and one persistent kernel
+ * whose waves take tiles from the work queue. Malformed offsets make the
+ * kernel do nothing; cusbf_batched_status / cusbf_batch_plan_status report it.
+ * Launch shape per record-length bucket comes from the filter's tuning table.
+ * Calls on one filter share its batch scratch: issue them on one stream.
+ */
+#define CUSBF_BATCH_BUCKETS 4 /* mean record length < 256, < 4096, < 65536, >= 65536 symbols */
+typedef enum cusbf_batch_op { CUSBF_BATCH_INSERT = 0, CUSBF_BATCH_CONTAINS = 1 } cusbf_batch_op;
+
+typedef struct cusbf_batch_tuning {
+    uint32_t waves_per_simd; /* register budget of the kernel variant: 3 or 4 (4: DNA only) */
+    uint32_t blocks_per_cu;  /* persistent blocks (4 waves each) per compute unit, 1..32 */
+    uint32_t claim;          /* tiles per work-queue claim, 1..64; 0 = static striding */
+    uint32_t reserved;
+} cusbf_batch_tuning;
+
+/* Bucket of a batch: by mean record length num_symbols / num_records. */
+uint32_t cusbf_batch_bucket(uint64_t num_symbols, uint64_t num_records);
+int cusbf_batch_tuning_set(cusbf_filter* f, uint32_t bucket, uint32_t op,
+                           const cusbf_batch_tuning* tuning);
+int cusbf_batch_tuning_get(const cusbf_filter* f, uint32_t bucket, uint32_t op,
+                           cusbf_batch_tuning* out);
+int cusbf_insert_batched_async(cusbf_filter* f, const char* d_symbols, uint64_t num_symbols,
+                               const uint64_t* d_offsets, uint64_t num_records, void* stream);
+int cusbf_contains_batched_async(const cusbf_filter* f, const char* d_symbols,
+                                 uint64_t num_symbols, const uint64_t* d_offsets,
+                                 uint64_t num_records, uint8_t* d_output, uint64_t d_output_len,
+                                 void* stream);
+/* Synchronises the stream; CUSBF_ERR_INVALID_ARGUMENT if the last batched call's
+ * offsets were malformed. */
+int cusbf_batched_status(const cusbf_filter* f, void* stream);
+
+/* A batched insert or query captured once into a hipGraph (reset, prepass,
+ * persistent kernel) over fixed buffers and replayed with one launch. The plan
+ * owns its scratch; `tuning` NULL = the filter's table entry for the batch's
+ * bucket at creation. */
+typedef struct cusbf_batch_plan cusbf_batch_plan;
+int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
+                            uint64_t num_symbols, const uint64_t* d_offsets, uint64_t num_records,
+                            uint8_t* d_output, uint64_t d_output_len,
+                            const cusbf_batch_tuning* tuning, cusbf_batch_plan** out);
+int cusbf_batch_plan_launch(cusbf_batch_plan* p, void* stream);
+/* Synchronises the stream; CUSBF_ERR_INVALID_ARGUMENT if the offsets were malformed. */
+int cusbf_batch_plan_status(cusbf_batch_plan* p, void* stream);
+void cusbf_batch_plan_destroy(cusbf_batch_plan* p);
+/* No reference counterpart: valid windows and shard runs of a batched input
+ * (windows across a record start are not valid), for the byte model.
+ * d_counts receives two uint64 (device memory). */
+int cusbf_count_batched_windows_runs_async(const cusbf_filter* f, const char* d_symbols,
+                                           uint64_t num_symbols, const uint64_t* d_offsets,
+                                           uint64_t num_records, uint64_t* d_counts,
+                                           void* stream);
 
 /* --- request server (no reference counterpart) ------------------------------- */
 /*

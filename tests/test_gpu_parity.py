@@ -125,10 +125,11 @@ def test_host_and_device_query_agree(gpu):
 
 # --- seeded random parity, bit-exact words and answers ------------------------------------------
 
-FAST = [(31, 28, 16, 4), (31, 30, 16, 4), (31, 31, 16, 4), (31, 28, 21, 4), (31, 28, 16, 8),
-        (31, 28, 16, 12), (31, 28, 16, 16), (5, 4, 3, 4), (3, 2, 2, 4), (21, 19, 11, 4)]
-GENERIC = [("dna", 27, 25, 13, 4), ("dna", 32, 20, 32, 8), ("protein", 10, 8, 6, 4),
-           ("protein", 12, 8, 5, 16), ("triplet", 7, 5, 4, 4), ("triplet", 10, 9, 3, 12),
+FAST = [("dna", 31, 28, 16, 4), ("dna", 31, 30, 16, 4), ("dna", 31, 31, 16, 4),
+        ("dna", 31, 28, 21, 4), ("dna", 31, 28, 16, 8), ("dna", 31, 28, 16, 12),
+        ("dna", 31, 28, 16, 16), ("dna", 5, 4, 3, 4), ("dna", 3, 2, 2, 4), ("dna", 21, 19, 11, 4),
+        ("protein", 10, 8, 6, 4), ("protein", 5, 4, 3, 4)]
+GENERIC = [("dna", 27, 25, 13, 4), ("dna", 32, 20, 32, 8), ("protein", 12, 8, 5, 16), ("triplet", 7, 5, 4, 4), ("triplet", 10, 9, 3, 12),
            ("custom_xyz", 12, 10, 6, 4), ("custom_xyz", 20, 14, 9, 8)]
 
 
@@ -163,10 +164,10 @@ def _parity(alphabet, k, s, m, h, n, seed, invalid_rate, bits_per_kmer=8.0, offs
     return f
 
 
-@pytest.mark.parametrize("k,s,m,h", FAST)
-def test_fast_dna_parity(gpu, k, s, m, h):
-    f = _parity("dna", k, s, m, h, 300_000, seed=k * 131 + s * 7 + m + h, invalid_rate=0.003)
-    assert f.kernel_family() == "dna_fast"
+@pytest.mark.parametrize("alphabet,k,s,m,h", FAST)
+def test_lane_strip_parity(gpu, alphabet, k, s, m, h):
+    f = _parity(alphabet, k, s, m, h, 300_000, seed=k * 131 + s * 7 + m + h, invalid_rate=0.003)
+    assert f.kernel_family() == "lane_strip"
 
 
 @pytest.mark.parametrize("alphabet,k,s,m,h", GENERIC)
@@ -176,14 +177,21 @@ def test_generic_parity(gpu, alphabet, k, s, m, h):
 
 
 @pytest.mark.parametrize("offset", [1, 3, 8, 13])
-def test_unaligned_device_buffers(gpu, offset):
-    _parity("dna", 31, 28, 16, 4, 50_001, seed=offset, invalid_rate=0.002, offset=offset)
+@pytest.mark.parametrize("alphabet,k,s,m", [("dna", 31, 28, 16), ("protein", 10, 8, 6)])
+def test_unaligned_device_buffers(gpu, offset, alphabet, k, s, m):
+    _parity(alphabet, k, s, m, 4, 50_001, seed=offset, invalid_rate=0.002, offset=offset)
 
 
 @pytest.mark.parametrize("n", [31, 32, 46, 47, 991 + 30, 992 + 30, 993 + 30, 3968 + 30,
                                3969 + 30, 4096, 65_535, 65_536 + 29])
 def test_tile_seams(gpu, n):
     _parity("dna", 31, 28, 16, 4, n, seed=n, invalid_rate=0.0, bits_per_kmer=64)
+
+
+@pytest.mark.parametrize("n", [10, 11, 25, 26, 1008 + 9, 1009 + 9, 4032 + 9, 4096, 65_536 + 9])
+def test_tile_seams_protein(gpu, n):
+    # protein k=10: one halo lane, 63 window lanes, 1008 windows per wave tile
+    _parity("protein", 10, 8, 6, 4, n, seed=n, invalid_rate=0.0, bits_per_kmer=64)
 
 
 def test_invalid_heavy_input(gpu):
@@ -209,7 +217,7 @@ def test_fast_matches_generic_kernels(gpu, monkeypatch):
     fast = cs.Filter(cfg, 1 << 23)
     monkeypatch.setenv("CUSBF_FORCE_GENERIC", "1")
     slow = cs.Filter(cfg, 1 << 23)
-    assert fast.kernel_family() == "dna_fast" and slow.kernel_family() == "generic"
+    assert fast.kernel_family() == "lane_strip" and slow.kernel_family() == "generic"
     d = dev(seq)
     fast.insert_sequence_async(d)
     slow.insert_sequence_async(d)
