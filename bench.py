@@ -47,6 +47,12 @@ def parse_args():
                     help="serving scenario: p50/p99 latency under Poisson arrivals")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-qps", type=float, default=100.0)
+    ap.add_argument("--batched", default="on", choices=["on", "off"],
+                    help="batched uneven-length form (DNA and protein) vs plain launches")
+    ap.add_argument("--batched-records", type=int, default=1_000_000)
+    ap.add_argument("--batched-reps", type=int, default=5)
+    ap.add_argument("--tuning-out", default="",
+                    help="write the autotuner's full sweep (JSON) to this path")
     return ap.parse_args()
 
 
@@ -240,6 +246,221 @@ def serving_scenario(filt, genome, reads, wl, n_requests: int, qps: float):
     }
 
 
+def _events(n):
+    import torch
+
+    return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+
+
+def _equal_chunked(a, b, step=1 << 28) -> bool:
+    import torch
+
+    if a.numel() != b.numel():
+        return False
+    return all(torch.equal(a[i:i + step], b[i:i + step]) for i in range(0, a.numel(), step))
+
+
+def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str = "") -> dict:
+    """Batched uneven-length form: 1M log-normal records (median 1 kbp, sigma 1.5,
+    clipped to [31 bp, 2 Mbp]) inserted and queried by graph-captured persistent
+    launches, against (a) one plain launch per record, (b) the plain host record
+    batch path (normalise, stage, one launch) and (c) one plain launch over the
+    records joined by separators on the device. Answers and filter words are
+    compared in full."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import cusbf_amd as cs
+    from cusbf_amd import _lib, batched as bt, tune
+    from cusbf_amd import workloads as W
+    from cusbf_amd.serve import ranges_array
+
+    if alphabet == "dna":
+        cfg = cs.Config(31, 28, 16, 4)
+        letters, sep = W.DNA_LETTERS, ord("N")
+    else:
+        cfg = cs.Config(10, 8, 6, 4, alphabet=cs.ProteinAlphabet)
+        letters, sep = W.PROTEIN_LETTERS, ord("*")
+    k = cfg.k
+    spec = W.BatchedSpec(records=n_records, letters=letters, seed=31 if alphabet == "dna" else 37)
+    data, d_off, lens = W.generate_batched(spec)
+    query = W.mutate_odd_records(data, lens, letters, spec.seed + 1)
+    total = data.numel()
+    record_kmers = int(np.maximum(lens - k + 1, 0).sum())
+    log(f"[batched:{alphabet}] {lens.size} records, {total / 1e9:.2f} G symbols, "
+        f"{record_kmers / 1e9:.2f} G in-record k-mers")
+    filt = cs.Filter(cfg, int(21.4 * record_kmers))
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    t0 = time.perf_counter()
+    table = tune.tune(filt, log=log)
+    tune_s = time.perf_counter() - t0
+    if tuning_out:
+        tune.save(table, tuning_out)
+    bucket = bt.batch_bucket(total, lens.size)
+    v_ins, s_ins = bt.count_batched_windows_runs(filt, data, d_off)
+    v_q, s_q = bt.count_batched_windows_runs(filt, query, d_off)
+
+    windows = total - k + 1
+    out_b = torch.empty(windows, dtype=torch.uint8, device="cuda")
+    ins = bt.BatchPlan(filt, bt.INSERT, data, d_off)
+    qry = bt.BatchPlan(filt, bt.CONTAINS, query, d_off, out_b)
+    ti, tq = [], []
+    for r in range(reps + 1):
+        filt.clear(sh)
+        ev = _events(3)
+        ev[0].record(stream)
+        ins.launch(sh)
+        ev[1].record(stream)
+        qry.launch(sh)
+        ev[2].record(stream)
+        ev[2].synchronize()
+        if r:  # first pass is warm-up
+            ti.append(ev[0].elapsed_time(ev[1]) / 1e3)
+            tq.append(ev[1].elapsed_time(ev[2]) / 1e3)
+    ins.status(sh)
+    qry.status(sh)
+    bi, bq = statistics.median(ti), statistics.median(tq)
+    positives = int(cs.count_positive(out_b))
+    words_b = filt.words_tensor()
+    log(f"[batched:{alphabet}] graph launches: insert {bi * 1e3:.2f} ms, query {bq * 1e3:.2f} ms "
+        f"({v_ins / bi / 1e9:.1f} / {v_q / bq / 1e9:.1f} G k-mers/s)")
+
+    # (c) plain single launch over the records joined by separators on the device
+    rec_idx = W.record_index(lens)
+    ev = _events(2)
+    ev[0].record(stream)
+    norm_a = W.join_records(data, lens, sep, rec_idx)
+    norm_q = W.join_records(query, lens, sep, rec_idx)
+    ev[1].record(stream)
+    ev[1].synchronize()
+    join_s = ev[0].elapsed_time(ev[1]) / 1e3 / 2
+    f2 = cs.Filter(cfg, filt.filter_bits())
+    out_n = torch.empty(norm_q.numel() - k + 1, dtype=torch.uint8, device="cuda")
+    ni, nq = [], []
+    for r in range(reps + 1):
+        f2.clear(sh)
+        ev = _events(3)
+        ev[0].record(stream)
+        f2.insert_sequence_async(norm_a, stream=sh)
+        ev[1].record(stream)
+        f2.contains_sequence_async(norm_q, out_n, stream=sh)
+        ev[2].record(stream)
+        ev[2].synchronize()
+        if r:
+            ni.append(ev[0].elapsed_time(ev[1]) / 1e3)
+            nq.append(ev[1].elapsed_time(ev[2]) / 1e3)
+    pi_, pq_ = statistics.median(ni), statistics.median(nq)
+    words_equal_norm = torch.equal(f2.words_tensor(), words_b)
+    same_norm = True
+    step = 1 << 28
+    for a in range(0, windows, step):
+        b = min(windows, a + step)
+        idx = torch.arange(a, b, device="cuda", dtype=torch.int64) + rec_idx[a:b]
+        same_norm = same_norm and torch.equal(out_n[idx], out_b[a:b])
+    del norm_a, norm_q, out_n, rec_idx
+    f2.close()
+    torch.cuda.empty_cache()
+
+    # (a) one plain launch per record, issued from native code
+    lib = _lib.load()
+    synth = W.synth_lib()
+    f3 = cs.Filter(cfg, filt.filter_bits())
+    out_r = torch.zeros(windows, dtype=torch.uint8, device="cuda")
+    h_off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64))
+    fn_ins = ctypes.cast(lib.cusbf_insert_sequence_async, ctypes.c_void_p).value
+    fn_q = ctypes.cast(lib.cusbf_contains_sequence_async, ctypes.c_void_p).value
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc = synth.cusbf_bench_each_record(fn_ins, 0, f3.handle, data.data_ptr(), cfg.symbol_width,
+                                       h_off.ctypes.data, lens.size, k, None, sh)
+    torch.cuda.synchronize()
+    ri = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rc |= synth.cusbf_bench_each_record(fn_q, 1, f3.handle, query.data_ptr(), cfg.symbol_width,
+                                        h_off.ctypes.data, lens.size, k, out_r.data_ptr(), sh)
+    torch.cuda.synchronize()
+    rq = time.perf_counter() - t0
+    if rc:
+        raise RuntimeError(f"per-record launches failed with status {rc}")
+    words_equal_rec = torch.equal(f3.words_tensor(), words_b)
+    same_rec = _equal_chunked(out_r, out_b)
+    del out_r
+    f3.close()
+    log(f"[batched:{alphabet}] per-record launches: insert {ri:.2f} s, query {rq:.2f} s")
+
+    # (b) plain host record batch: host records -> normalise -> stage -> one launch
+    h_data = data.cpu().numpy()
+    h_query = query.cpu().numpy()
+    rr = np.stack([h_off[:-1] * cfg.symbol_width, lens.astype(np.uint64) * cfg.symbol_width], 1)
+    arr, n = ranges_array(rr)
+    f4 = cs.Filter(cfg, filt.filter_bits())
+    rep_i = _lib.CInsertReport()
+    rep_q = _lib.CQueryReport()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = lib.cusbf_insert_record_batch(f4.handle, h_data.ctypes.data, h_data.size, arr, n, sh,
+                                       ctypes.byref(rep_i))
+    hi = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st |= lib.cusbf_query_record_batch(f4.handle, h_query.ctypes.data, h_query.size, arr, n, sh,
+                                       ctypes.byref(rep_q))
+    hq = time.perf_counter() - t0
+    if st:
+        raise RuntimeError(f"host record batch failed with status {st}")
+    words_equal_host = torch.equal(f4.words_tensor(), words_b)
+    f4.close()
+    log(f"[batched:{alphabet}] host record batch: insert {hi:.2f} s, query {hq:.2f} s")
+
+    kmers = v_ins + v_q
+    batched_rate = kmers / (bi + bq)
+    return {
+        "workload": {"records": int(lens.size), "symbols": int(total), "median": spec.median,
+                     "sigma": spec.sigma, "clip": [spec.min_len, spec.max_len],
+                     "alphabet": alphabet, "k": cfg.k, "s": cfg.s, "m": cfg.m,
+                     "hash_count": cfg.hash_count, "filter_bits": filt.filter_bits(),
+                     "bits_per_kmer_requested": 21.4, "length_bucket": bucket,
+                     "insert_valid_kmers": v_ins, "insert_shard_runs": s_ins,
+                     "query_valid_kmers": v_q, "query_shard_runs": s_q},
+        "kernels": filt.kernel_family(),
+        "tuning": {key: {op: r["best"] | {"ms": r["best_ms"]} for op, r in ops.items()}
+                   for key, ops in table.items()},
+        "tuning_seconds": tune_s,
+        "batched_insert_ms": bi * 1e3,
+        "batched_query_ms": bq * 1e3,
+        "batched_insert_kmers_per_s": v_ins / bi,
+        "batched_query_kmers_per_s": v_q / bq,
+        "batched_kmers_per_s": batched_rate,
+        "batched_insert_roofline_frac": (total * cfg.symbol_width + 64 * s_ins) / bi / HBM_PEAK,
+        "batched_query_roofline_frac": (total * cfg.symbol_width + windows + 32 * s_q) / bq / HBM_PEAK,
+        "per_record_insert_s": ri,
+        "per_record_query_s": rq,
+        "per_record_kmers_per_s": kmers / (ri + rq),
+        "host_record_batch_insert_s": hi,
+        "host_record_batch_query_s": hq,
+        "host_record_batch_kmers_per_s": kmers / (hi + hq),
+        "joined_plain_insert_ms": pi_ * 1e3,
+        "joined_plain_query_ms": pq_ * 1e3,
+        "joined_plain_kmers_per_s": kmers / (pi_ + pq_),
+        "join_ms_per_input": join_s * 1e3,
+        "speedup_vs_per_record": batched_rate / (kmers / (ri + rq)),
+        "speedup_vs_host_record_batch": batched_rate / (kmers / (hi + hq)),
+        "speedup_vs_joined_plain_kernels": batched_rate / (kmers / (pi_ + pq_)),
+        "speedup_vs_joined_plain_incl_join": batched_rate / (kmers / (pi_ + pq_ + 2 * join_s)),
+        "query_positive_kmers": positives,
+        "host_record_batch_positive_kmers": int(rep_q.positive_kmers),
+        "identical": {"words_per_record": bool(words_equal_rec),
+                      "answers_per_record": bool(same_rec),
+                      "words_joined_plain": bool(words_equal_norm),
+                      "answers_joined_plain": bool(same_norm),
+                      "words_host_record_batch": bool(words_equal_host),
+                      "positives_host_record_batch": int(rep_q.positive_kmers) == positives},
+    }
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -391,6 +612,14 @@ def main() -> int:
         result["serve_p50_ms"] = serving["server_p50_ms"]
         result["serve_p99_ms"] = serving["server_p99_ms"]
         result["serving"] = serving
+    if args.batched == "on":
+        del out
+        filt.close()
+        torch.cuda.empty_cache()
+        result["batched"] = {
+            a: batched_scenario(a, args.batched_records, args.batched_reps,
+                                args.tuning_out.replace(".json", f"_{a}.json") if args.tuning_out else "")
+            for a in ("dna", "protein")}
     if args.cpu_baseline == "on":
         log("[bench] CPU baseline on a bounded sample")
         result["cpu_baseline"] = cpu_baseline(genome, reads, wl, args.cpu_threads)

@@ -101,6 +101,7 @@ SIGNATURES = [
     ("cusbf_count_positive_per_record_async", i32, [P, P, P, u64, P, P]),
     ("cusbf_count_windows_runs_async", i32, [P, P, u64, P, P]),
     ("cusbf_kernel_family", ctypes.c_char_p, [P]),
+    ("cusbf_batch_variants", u32, [P]),
     ("cusbf_batch_bucket", u32, [u64, u64]),
     ("cusbf_batch_tuning_set", i32, [P, u32, u32, ctypes.POINTER(CBatchTuning)]),
     ("cusbf_batch_tuning_get", i32, [P, u32, u32, ctypes.POINTER(CBatchTuning)]),

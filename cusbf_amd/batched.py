@@ -43,6 +43,12 @@ class BatchTuning:
         return BatchTuning(int(t.waves_per_simd), int(t.blocks_per_cu), int(t.claim))
 
 
+def batch_variants(filt: Filter) -> tuple:
+    """Waves-per-SIMD variants of the filter's batched lane-strip kernels (empty: generic)."""
+    v = int(_lib.load().cusbf_batch_variants(filt.handle))
+    return tuple(w for bit, w in ((1, 3), (2, 4)) if v & bit)
+
+
 def batch_bucket(num_symbols: int, num_records: int) -> int:
     return int(_lib.load().cusbf_batch_bucket(int(num_symbols), int(num_records)))
 

@@ -971,6 +971,11 @@ uint32_t cusbf_batch_bucket(uint64_t num_symbols, uint64_t num_records) {
     return 3;
 }
 
+uint32_t cusbf_batch_variants(const cusbf_filter* f) {
+    if (!f || !batched_strip(f)) return 0;
+    return 1u | (f->strip->batched_insert[1] ? 2u : 0u);
+}
+
 int cusbf_batch_tuning_set(cusbf_filter* f, uint32_t bucket, uint32_t op,
                            const cusbf_batch_tuning* tuning) {
     if (!f || !tuning) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");

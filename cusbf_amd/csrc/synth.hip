@@ -176,9 +176,38 @@ __global__ void long_reads_kernel(uint64_t seed, const uint8_t* genome, uint64_t
     if (!last) dst[len] = 'N';
 }
 
+// Uniform letters: byte i is letters[(32 random bits of (seed, i)) * n >> 32].
+struct Letters {
+    uint8_t c[64];
+};
+
+__global__ void letters_kernel(uint64_t seed, Letters letters, uint32_t n_letters, uint8_t* out,
+                               uint64_t n) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t r = static_cast<uint32_t>(rnd(seed, 11, i));
+        out[i] = letters.c[(static_cast<uint64_t>(r) * n_letters) >> 32];
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+// n bytes drawn uniformly from `letters` (host string of 1..64 letters).
+int cusbf_synth_letters(uint64_t seed, const char* letters, uint32_t n_letters, uint8_t* d_out,
+                        uint64_t n, void* stream) {
+    if (n_letters == 0 || n_letters > 64) return static_cast<int>(hipErrorInvalidValue);
+    if (n == 0) return 0;
+    Letters l{};
+    for (uint32_t i = 0; i < n_letters; ++i) l.c[i] = static_cast<uint8_t>(letters[i]);
+    uint64_t blocks = (n + 256 * 16 - 1) / (256 * 16);
+    if (blocks > 65536) blocks = 65536;
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(letters_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), seed, l, n_letters, d_out, n);
+    return static_cast<int>(hipGetLastError());
+}
 
 int cusbf_synth_long_reads(uint64_t seed, const uint8_t* d_genome, uint64_t genome_len,
                            uint64_t num_reads, const uint64_t* d_offsets, uint32_t err_ppm,

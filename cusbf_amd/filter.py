@@ -91,6 +91,14 @@ def _ranges(ranges: Iterable) -> Tuple[ctypes.Array, int]:
     return arr, len(items)
 
 
+class _DeviceArray:
+    """__cuda_array_interface__ description of library-owned device memory."""
+
+    def __init__(self, ptr: int, n: int, typestr: str):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False),
+                                         "version": 3, "strides": None}
+
+
 class Filter:
     """GPU-resident Super Bloom filter (filter.cuh:68-1848)."""
 
@@ -146,6 +154,14 @@ class Filter:
         p = ctypes.c_void_p()
         check(self._lib, self._lib.cusbf_words_device_ptr(self._h, ctypes.byref(p)))
         return int(p.value or 0)
+
+    def words_tensor(self):
+        """The filter words as an int64 torch tensor viewing the library's device
+        memory (no copy; valid while the filter lives)."""
+        import torch
+
+        view = _DeviceArray(self.words_device_ptr(), self.num_shards() * 4, "<i8")
+        return torch.as_tensor(view, device="cuda")
 
     def words(self) -> np.ndarray:
         """All num_shards*4 filter words (synchronous copy to the host)."""

@@ -49,6 +49,10 @@ def synth_lib():
         L.cusbf_synth_repeat_copy_size.restype = u64
         L.cusbf_synth_long_reads.argtypes = [u64, P, u64, u64, P, u32, P, P]
         L.cusbf_synth_long_reads.restype = ctypes.c_int
+        L.cusbf_synth_letters.argtypes = [u64, ctypes.c_char_p, u32, P, u64, P]
+        L.cusbf_synth_letters.restype = ctypes.c_int
+        L.cusbf_bench_each_record.argtypes = [P, ctypes.c_int, P, P, u32, P, u64, u32, P, P]
+        L.cusbf_bench_each_record.restype = ctypes.c_int
         assert L.cusbf_synth_repeat_copy_size() == REPEAT_DTYPE.itemsize
         _synth = L
     return _synth
@@ -237,3 +241,88 @@ def generate_long_reads(genome, num_reads: int, min_len: int = 10_000, max_len: 
         raise RuntimeError(f"long-read generation launch failed (hip error {rc})")
     torch.cuda.synchronize()
     return out, offsets
+
+
+# --- batched uneven-length records -------------------------------------------------
+
+DNA_LETTERS = b"ACGT"
+PROTEIN_LETTERS = b"ACDEFGHIKLMNPQRSTVWY"  # the 20 standard residues
+
+
+@dataclass
+class BatchedSpec:
+    """Many records of log-normal length, back to back (the batched record form)."""
+
+    name: str = "lognormal-1M"
+    records: int = 1_000_000
+    median: float = 1000.0
+    sigma: float = 1.5
+    min_len: int = 31
+    max_len: int = 2_000_000
+    letters: bytes = DNA_LETTERS
+    seed: int = 31
+
+
+def batched_lengths(spec: BatchedSpec) -> np.ndarray:
+    rng = np.random.default_rng(spec.seed)
+    lens = np.round(rng.lognormal(np.log(spec.median), spec.sigma, spec.records))
+    return np.clip(lens, spec.min_len, spec.max_len).astype(np.int64)
+
+
+def uniform_letters(n: int, letters: bytes, seed: int, device="cuda"):
+    """n bytes drawn uniformly from `letters`, generated on the device."""
+    import torch
+
+    out = torch.empty(max(1, n), dtype=torch.uint8, device=device)
+    rc = synth_lib().cusbf_synth_letters(seed, letters, len(letters), out.data_ptr(), n, 0)
+    if rc != 0:
+        raise RuntimeError(f"letters generator failed: hip error {rc}")
+    return out[:n]
+
+
+def generate_batched(spec: BatchedSpec, device="cuda"):
+    """(device symbols, device int64 offsets, host lengths) of the spec's records."""
+    import torch
+
+    lens = batched_lengths(spec)
+    off = np.zeros(lens.size + 1, dtype=np.int64)
+    off[1:] = np.cumsum(lens)
+    data = uniform_letters(int(off[-1]), spec.letters, spec.seed, device)
+    return data, torch.from_numpy(off).to(device), lens
+
+
+def record_index(lens: np.ndarray, device="cuda"):
+    """Device int64 array: the record of every symbol."""
+    import torch
+
+    d_lens = torch.from_numpy(lens).to(device)
+    ids = torch.arange(lens.size, device=device, dtype=torch.int64)
+    return torch.repeat_interleave(ids, d_lens, output_size=int(lens.sum()))
+
+
+def mutate_odd_records(data, lens: np.ndarray, letters: bytes, seed: int):
+    """Same lengths; every odd record replaced by fresh random letters."""
+    import torch
+
+    fresh = uniform_letters(data.numel(), letters, seed, data.device)
+    parity = torch.from_numpy((np.arange(lens.size) & 1).astype(np.uint8)).to(data.device)
+    odd = torch.repeat_interleave(parity, torch.from_numpy(lens).to(data.device),
+                                  output_size=data.numel()).bool()
+    return torch.where(odd, fresh, data)
+
+
+def join_records(data, lens: np.ndarray, separator: int, rec_idx=None):
+    """Device copy of the records joined by one separator byte (the normalised
+    record batch of a width-1 alphabet)."""
+    import torch
+
+    n = data.numel()
+    if rec_idx is None:
+        rec_idx = record_index(lens, data.device)
+    out = torch.full((n + max(0, lens.size - 1),), separator, dtype=torch.uint8, device=data.device)
+    step = 1 << 28
+    for a in range(0, n, step):
+        b = min(n, a + step)
+        pos = torch.arange(a, b, device=data.device, dtype=torch.int64) + rec_idx[a:b]
+        out[pos] = data[a:b]
+    return out

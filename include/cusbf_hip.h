@@ -265,6 +265,10 @@ typedef struct cusbf_batch_tuning {
     uint32_t reserved;
 } cusbf_batch_tuning;
 
+/* Batched kernels the filter's configuration has: bit 0 = lane-strip kernels
+ * built for 3 waves per SIMD, bit 1 = for 4; 0 = the generic kernels (the
+ * tuning table is then not used). */
+uint32_t cusbf_batch_variants(const cusbf_filter* f);
 /* Bucket of a batch: by mean record length num_symbols / num_records. */
 uint32_t cusbf_batch_bucket(uint64_t num_symbols, uint64_t num_records);
 int cusbf_batch_tuning_set(cusbf_filter* f, uint32_t bucket, uint32_t op,

@@ -346,3 +346,66 @@ def test_batched_malformed_offsets(gpu):
         bt.contains_batched_async(f, data, device_offsets(np.array([0, 5000], np.uint64)), out,
                                   output_size=100)
     assert "too small" in str(e.value)
+
+
+def test_words_tensor_views_device_words(gpu):
+    import cusbf_amd as cs
+
+    f = cs.Filter(make_config(31, 28, 16, 4), 1 << 20)
+    f.insert_sequence(random_dna(20000, 4))
+    t = f.words_tensor()
+    assert t.numel() == f.num_shards() * 4
+    assert np.array_equal(t.cpu().numpy().view(np.uint64), f.words())
+
+
+def test_tuner_picks_a_measured_candidate(gpu):
+    import cusbf_amd as cs
+    from cusbf_amd import batched as bt, tune
+
+    f = cs.Filter(make_config(31, 28, 16, 4), 1 << 24)
+    res = tune.tune_bucket(f, 0, sample_symbols=1 << 21, reps=1)
+    for op in ("insert", "contains"):
+        r = res[op]
+        assert len(r.timings) == len(tune.candidates(f)) == 40
+        assert r.best_ms == min(t[3] for t in r.timings)
+    table = tune.tune(f, buckets=(1,), sample_symbols=1 << 21, reps=1)
+    best = table["bucket1"]["contains"]["best"]
+    assert bt.get_tuning(f, 1, bt.CONTAINS) == bt.BatchTuning(**best)
+    assert f.load_factor() == 0.0  # tuning never writes the target filter
+    g = cs.Filter(make_config(17, 15, 9, 4), 1 << 20)
+    assert tune.candidates(g) == [] and tune.tune(g) == {}
+
+
+def test_per_record_launch_loop_matches_batched(gpu):
+    """The plain per-record baseline loop answers exactly like the batched form."""
+    import ctypes
+
+    import cusbf_amd as cs
+    from cusbf_amd import _lib, batched as bt
+    from cusbf_amd import workloads as W
+
+    torch = _torch()
+    lens = lognormal_lengths(3000, 200, 1.5, seed=41, lo=0)
+    off = offsets_of(lens)
+    data = symbols_for("protein", int(off[-1]), seed=42)
+    cfg = make_config(10, 8, 6, 4, "protein")
+    d_data, d_off = device(data), device_offsets(off)
+    n_win = int(off[-1]) - 9
+    a = cs.Filter(cfg, 1 << 24)
+    b = cs.Filter(cfg, 1 << 24)
+    bt.insert_batched_async(a, d_data, d_off)
+    out_a = torch.zeros(n_win, dtype=torch.uint8, device="cuda")
+    bt.contains_batched_async(a, d_data, d_off, out_a)
+    bt.batched_status(a)
+    lib, synth = _lib.load(), W.synth_lib()
+    fn_i = ctypes.cast(lib.cusbf_insert_sequence_async, ctypes.c_void_p).value
+    fn_q = ctypes.cast(lib.cusbf_contains_sequence_async, ctypes.c_void_p).value
+    h_off = np.ascontiguousarray(off)
+    out_b = torch.zeros(n_win, dtype=torch.uint8, device="cuda")
+    assert synth.cusbf_bench_each_record(fn_i, 0, b.handle, d_data.data_ptr(), 1, h_off.ctypes.data,
+                                         len(lens), 10, None, 0) == 0
+    assert synth.cusbf_bench_each_record(fn_q, 1, b.handle, d_data.data_ptr(), 1, h_off.ctypes.data,
+                                         len(lens), 10, out_b.data_ptr(), 0) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(a.words_tensor(), b.words_tensor())
+    assert torch.equal(out_a, out_b)
