@@ -51,6 +51,10 @@ def parse_args():
                     help="batched uneven-length form (DNA and protein) vs plain launches")
     ap.add_argument("--batched-records", type=int, default=1_000_000)
     ap.add_argument("--batched-reps", type=int, default=5)
+    ap.add_argument("--batched-baselines", default="on", choices=["on", "off"],
+                    help="time the per-record and host record-batch baselines (1M launches)")
+    ap.add_argument("--tuning-in", default="",
+                    help="apply a saved sweep (written by --tuning-out) instead of tuning")
     ap.add_argument("--tuning-out", default="",
                     help="write the autotuner's full sweep (JSON) to this path")
     return ap.parse_args()
@@ -260,7 +264,8 @@ def _equal_chunked(a, b, step=1 << 28) -> bool:
     return all(torch.equal(a[i:i + step], b[i:i + step]) for i in range(0, a.numel(), step))
 
 
-def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str = "") -> dict:
+def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str = "",
+                     baselines: bool = True, tuning_in: str = "") -> dict:
     """Batched uneven-length form: 1M log-normal records (median 1 kbp, sigma 1.5,
     clipped to [31 bp, 2 Mbp]) inserted and queried by graph-captured persistent
     launches, against (a) one plain launch per record, (b) the plain host record
@@ -296,7 +301,11 @@ def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str =
     sh = stream.cuda_stream
 
     t0 = time.perf_counter()
-    table = tune.tune(filt, log=log)
+    if tuning_in:
+        table = tune.load(tuning_in)
+        tune.apply(filt, table)
+    else:
+        table = tune.tune(filt, log=log)
     tune_s = time.perf_counter() - t0
     if tuning_out:
         tune.save(table, tuning_out)
@@ -365,8 +374,42 @@ def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str =
     f2.close()
     torch.cuda.empty_cache()
 
-    # (a) one plain launch per record, issued from native code
     lib = _lib.load()
+    kmers = v_ins + v_q
+    batched_rate = kmers / (bi + bq)
+    result = {
+        "workload": {"records": int(lens.size), "symbols": int(total), "median": spec.median,
+                     "sigma": spec.sigma, "clip": [spec.min_len, spec.max_len],
+                     "alphabet": alphabet, "k": cfg.k, "s": cfg.s, "m": cfg.m,
+                     "hash_count": cfg.hash_count, "filter_bits": filt.filter_bits(),
+                     "bits_per_kmer_requested": 21.4, "length_bucket": bucket,
+                     "insert_valid_kmers": v_ins, "insert_shard_runs": s_ins,
+                     "query_valid_kmers": v_q, "query_shard_runs": s_q},
+        "kernels": filt.kernel_family(),
+        "tuning": {key: {op: r["best"] | {"ms": r["best_ms"]} for op, r in ops.items()}
+                   for key, ops in table.items()},
+        "tuning_seconds": tune_s,
+        "batched_insert_ms": bi * 1e3,
+        "batched_query_ms": bq * 1e3,
+        "batched_insert_kmers_per_s": v_ins / bi,
+        "batched_query_kmers_per_s": v_q / bq,
+        "batched_kmers_per_s": batched_rate,
+        "batched_insert_roofline_frac": (total * cfg.symbol_width + 64 * s_ins) / bi / HBM_PEAK,
+        "batched_query_roofline_frac": (total * cfg.symbol_width + windows + 32 * s_q) / bq / HBM_PEAK,
+        "joined_plain_insert_ms": pi_ * 1e3,
+        "joined_plain_query_ms": pq_ * 1e3,
+        "joined_plain_kmers_per_s": kmers / (pi_ + pq_),
+        "join_ms_per_input": join_s * 1e3,
+        "speedup_vs_joined_plain_kernels": batched_rate / (kmers / (pi_ + pq_)),
+        "speedup_vs_joined_plain_incl_join": batched_rate / (kmers / (pi_ + pq_ + 2 * join_s)),
+        "query_positive_kmers": positives,
+        "identical": {"words_joined_plain": bool(words_equal_norm),
+                      "answers_joined_plain": bool(same_norm)},
+    }
+    if not baselines:
+        return result
+
+    # (a) one plain launch per record, issued from native code
     synth = W.synth_lib()
     f3 = cs.Filter(cfg, filt.filter_bits())
     out_r = torch.zeros(windows, dtype=torch.uint8, device="cuda")
@@ -415,50 +458,23 @@ def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str =
     f4.close()
     log(f"[batched:{alphabet}] host record batch: insert {hi:.2f} s, query {hq:.2f} s")
 
-    kmers = v_ins + v_q
-    batched_rate = kmers / (bi + bq)
-    return {
-        "workload": {"records": int(lens.size), "symbols": int(total), "median": spec.median,
-                     "sigma": spec.sigma, "clip": [spec.min_len, spec.max_len],
-                     "alphabet": alphabet, "k": cfg.k, "s": cfg.s, "m": cfg.m,
-                     "hash_count": cfg.hash_count, "filter_bits": filt.filter_bits(),
-                     "bits_per_kmer_requested": 21.4, "length_bucket": bucket,
-                     "insert_valid_kmers": v_ins, "insert_shard_runs": s_ins,
-                     "query_valid_kmers": v_q, "query_shard_runs": s_q},
-        "kernels": filt.kernel_family(),
-        "tuning": {key: {op: r["best"] | {"ms": r["best_ms"]} for op, r in ops.items()}
-                   for key, ops in table.items()},
-        "tuning_seconds": tune_s,
-        "batched_insert_ms": bi * 1e3,
-        "batched_query_ms": bq * 1e3,
-        "batched_insert_kmers_per_s": v_ins / bi,
-        "batched_query_kmers_per_s": v_q / bq,
-        "batched_kmers_per_s": batched_rate,
-        "batched_insert_roofline_frac": (total * cfg.symbol_width + 64 * s_ins) / bi / HBM_PEAK,
-        "batched_query_roofline_frac": (total * cfg.symbol_width + windows + 32 * s_q) / bq / HBM_PEAK,
+    result.update({
         "per_record_insert_s": ri,
         "per_record_query_s": rq,
         "per_record_kmers_per_s": kmers / (ri + rq),
         "host_record_batch_insert_s": hi,
         "host_record_batch_query_s": hq,
         "host_record_batch_kmers_per_s": kmers / (hi + hq),
-        "joined_plain_insert_ms": pi_ * 1e3,
-        "joined_plain_query_ms": pq_ * 1e3,
-        "joined_plain_kmers_per_s": kmers / (pi_ + pq_),
-        "join_ms_per_input": join_s * 1e3,
         "speedup_vs_per_record": batched_rate / (kmers / (ri + rq)),
         "speedup_vs_host_record_batch": batched_rate / (kmers / (hi + hq)),
-        "speedup_vs_joined_plain_kernels": batched_rate / (kmers / (pi_ + pq_)),
-        "speedup_vs_joined_plain_incl_join": batched_rate / (kmers / (pi_ + pq_ + 2 * join_s)),
-        "query_positive_kmers": positives,
         "host_record_batch_positive_kmers": int(rep_q.positive_kmers),
-        "identical": {"words_per_record": bool(words_equal_rec),
-                      "answers_per_record": bool(same_rec),
-                      "words_joined_plain": bool(words_equal_norm),
-                      "answers_joined_plain": bool(same_norm),
-                      "words_host_record_batch": bool(words_equal_host),
-                      "positives_host_record_batch": int(rep_q.positive_kmers) == positives},
-    }
+    })
+    result["identical"].update({
+        "words_per_record": bool(words_equal_rec),
+        "answers_per_record": bool(same_rec),
+        "words_host_record_batch": bool(words_equal_host),
+        "positives_host_record_batch": int(rep_q.positive_kmers) == positives})
+    return result
 
 
 def main() -> int:
@@ -616,9 +632,13 @@ def main() -> int:
         del out
         filt.close()
         torch.cuda.empty_cache()
+        def per_alphabet(path, a):
+            return path.replace(".json", f"_{a}.json") if path else ""
+
         result["batched"] = {
             a: batched_scenario(a, args.batched_records, args.batched_reps,
-                                args.tuning_out.replace(".json", f"_{a}.json") if args.tuning_out else "")
+                                per_alphabet(args.tuning_out, a), args.batched_baselines == "on",
+                                per_alphabet(args.tuning_in, a))
             for a in ("dna", "protein")}
     if args.cpu_baseline == "on":
         log("[bench] CPU baseline on a bounded sample")

@@ -119,12 +119,12 @@ struct cusbf_filter {
     uint8_t* d_result = nullptr;
     uint64_t result_cap = 0;
     uint64_t* d_scalar = nullptr;  // reductions
-    // batched record form: tuning table, work-queue state, tile -> first record map
+    // batched record form: tuning table, work-queue state, record-start bitmap
     uint32_t compute_units = 256;
     cusbf_batch_tuning tuning[CUSBF_BATCH_BUCKETS][2] = {};
     uint32_t* d_batch_state = nullptr;
-    uint32_t* d_tile_rec = nullptr;
-    uint64_t tile_rec_cap = 0;
+    uint64_t* d_starts = nullptr;
+    uint64_t starts_cap = 0;  // words
     bool batch_check_pending = false;
 };
 
@@ -133,7 +133,7 @@ struct cusbf_batch_plan {
     uint32_t op = 0;
     bool empty = true;
     uint32_t* d_state = nullptr;
-    uint32_t* d_tile_rec = nullptr;
+    uint64_t* d_starts = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -355,16 +355,17 @@ uint64_t batched_windows(const cusbf_filter* f, uint64_t num_symbols) {
     return num_symbols < f->cfg.k ? 0 : num_symbols - f->cfg.k + 1;
 }
 
-int ensure_tile_rec(cusbf_filter* f, uint64_t tiles) {
-    if (tiles <= f->tile_rec_cap) return CUSBF_OK;
-    if (f->d_tile_rec) {
+int ensure_starts(cusbf_filter* f, uint64_t num_symbols) {
+    const uint64_t words = sbf::start_words(num_symbols);
+    if (words <= f->starts_cap) return CUSBF_OK;
+    if (f->d_starts) {
         CUSBF_HIP_TRY(hipDeviceSynchronize());  // an earlier batched launch may still read it
-        CUSBF_HIP_TRY(hipFree(f->d_tile_rec));
+        CUSBF_HIP_TRY(hipFree(f->d_starts));
     }
-    f->d_tile_rec = nullptr;
-    f->tile_rec_cap = 0;
-    CUSBF_HIP_TRY(hipMalloc(&f->d_tile_rec, tiles * sizeof(uint32_t)));
-    f->tile_rec_cap = tiles;
+    f->d_starts = nullptr;
+    f->starts_cap = 0;
+    CUSBF_HIP_TRY(hipMalloc(&f->d_starts, words * sizeof(uint64_t)));
+    f->starts_cap = words;
     return CUSBF_OK;
 }
 
@@ -377,17 +378,14 @@ const cusbf_batch_tuning& tuning_for(const cusbf_filter* f, uint32_t op, uint64_
 // nothing, so it can be captured into a graph.
 int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
                     uint64_t num_symbols, const uint64_t* d_offsets, uint64_t num_records,
-                    uint32_t* d_state, uint32_t* d_tile_rec, const cusbf_batch_tuning& t,
+                    uint32_t* d_state, uint64_t* d_starts, const cusbf_batch_tuning& t,
                     uint8_t* d_output, hipStream_t stream) {
     sbf::SequenceInput in = byte_input(f, d_symbols, num_symbols * f->d.symbol_width);
     in.offsets = d_offsets;
     in.num_records = num_records;
-    in.tile_rec = d_tile_rec;
+    in.starts = d_starts;
     in.state = d_state;
-    const uint64_t tiles = batch_tiles(f, num_symbols);
-    CUSBF_HIP_TRY(sbf::batch_reset(d_state, stream));
-    CUSBF_HIP_TRY(sbf::batch_prepass(in, batched_strip(f) ? f->strip->wave_windows : 0, tiles,
-                                     d_tile_rec, d_state, stream));
+    CUSBF_HIP_TRY(sbf::batch_prepass(in, d_starts, d_state, stream));
     hipError_t e;
     if (batched_strip(f)) {
         const int variant = t.waves_per_simd == 4 && f->strip->batched_insert[1] ? 1 : 0;
@@ -548,7 +546,7 @@ void cusbf_destroy(cusbf_filter* f) {
     if (f->d_words) (void)hipFree(f->d_words);
     if (f->d_scalar) (void)hipFree(f->d_scalar);
     if (f->d_batch_state) (void)hipFree(f->d_batch_state);
-    if (f->d_tile_rec) (void)hipFree(f->d_tile_rec);
+    if (f->d_starts) (void)hipFree(f->d_starts);
     delete f;
 }
 
@@ -1004,10 +1002,10 @@ int cusbf_insert_batched_async(cusbf_filter* f, const char* d_symbols, uint64_t 
     if (rc != CUSBF_OK) return rc;
     f->batch_check_pending = false;
     if (batched_windows(f, num_symbols) == 0) return CUSBF_OK;
-    rc = ensure_tile_rec(f, batch_tiles(f, num_symbols) + 1);
+    rc = ensure_starts(f, num_symbols);
     if (rc != CUSBF_OK) return rc;
     rc = enqueue_batched(f, CUSBF_BATCH_INSERT, d_symbols, num_symbols, d_offsets, num_records,
-                         f->d_batch_state, f->d_tile_rec,
+                         f->d_batch_state, f->d_starts,
                          tuning_for(f, CUSBF_BATCH_INSERT, num_symbols, num_records),
                          nullptr, as_stream(stream));
     f->batch_check_pending = rc == CUSBF_OK;
@@ -1027,10 +1025,10 @@ int cusbf_contains_batched_async(const cusbf_filter* f_const, const char* d_symb
     if (windows == 0) return CUSBF_OK;
     if (!d_output) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device output");
     if (d_output_len < windows) return fail(CUSBF_ERR_INVALID_ARGUMENT, "batched query output span is too small");
-    rc = ensure_tile_rec(f, batch_tiles(f, num_symbols) + 1);
+    rc = ensure_starts(f, num_symbols);
     if (rc != CUSBF_OK) return rc;
     rc = enqueue_batched(f, CUSBF_BATCH_CONTAINS, d_symbols, num_symbols, d_offsets, num_records,
-                         f->d_batch_state, f->d_tile_rec,
+                         f->d_batch_state, f->d_starts,
                          tuning_for(f, CUSBF_BATCH_CONTAINS, num_symbols, num_records), d_output,
                          as_stream(stream));
     f->batch_check_pending = rc == CUSBF_OK;
@@ -1072,14 +1070,14 @@ int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
     }
     hipStream_t capture = nullptr;
     hipError_t e = hipMalloc(&p->d_state, 64);
-    if (e == hipSuccess) e = hipMalloc(&p->d_tile_rec, (batch_tiles(f, num_symbols) + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&p->d_starts, sbf::start_words(num_symbols) * sizeof(uint64_t));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&capture, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(capture, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
         rc = fail_hip(e, __FILE__, __LINE__);
     } else {
         rc = enqueue_batched(f, op, d_symbols, num_symbols, d_offsets, num_records, p->d_state,
-                             p->d_tile_rec, t, d_output, capture);
+                             p->d_starts, t, d_output, capture);
         const hipError_t end = hipStreamEndCapture(capture, &p->graph);
         if (rc == CUSBF_OK && end != hipSuccess) rc = fail_hip(end, __FILE__, __LINE__);
         if (rc == CUSBF_OK) {
@@ -1116,7 +1114,7 @@ void cusbf_batch_plan_destroy(cusbf_batch_plan* p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->d_state) (void)hipFree(p->d_state);
-    if (p->d_tile_rec) (void)hipFree(p->d_tile_rec);
+    if (p->d_starts) (void)hipFree(p->d_starts);
     delete p;
 }
 

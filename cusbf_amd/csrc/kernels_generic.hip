@@ -86,10 +86,23 @@ __device__ __forceinline__ void masks(const KernelConfig& c, uint64_t packed, ui
 }
 
 // Batched form: true when a record starts inside window j after its first
-// symbol (the first start after j, found by binary search over the offsets, is
-// closer than k). Malformed offsets give wrong answers, never an access
-// outside the offsets array.
+// symbol: a set bit among symbols j+1 .. j+k-1 of the record-start bitmap, or,
+// without the bitmap, a first start after j (binary search over the offsets)
+// closer than k. Malformed offsets give wrong answers, never an access outside
+// the offsets array or the bitmap.
 __device__ __forceinline__ bool crosses_record(const SequenceInput& in, uint64_t j, uint32_t k) {
+    if (in.starts != nullptr) {
+        if (k < 2) return false;
+        const uint64_t first = j + 1;
+        const uint64_t last = j + k - 1;  // < num_symbols for a window in range
+        for (uint64_t w = first >> 6; w <= (last >> 6); ++w) {
+            uint64_t bits = in.starts[w];
+            if (w == (first >> 6)) bits &= ~0ull << (first & 63);
+            if (w == (last >> 6) && (last & 63) != 63) bits &= (2ull << (last & 63)) - 1ull;
+            if (bits) return true;
+        }
+        return false;
+    }
     uint64_t lo = 0, hi = in.num_records + 1;
     while (lo < hi) {
         const uint64_t mid = (lo + hi) / 2;
@@ -303,38 +316,21 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(uint64_t* dst, uint64_t va
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) dst[i] = value;
 }
 
-__global__ void batch_reset_kernel(uint32_t* state) {
-    if (threadIdx.x < 2) state[threadIdx.x] = 0;
-}
-
-// Batched form prepass: tile_rec[t] = first record whose start is at or after
-// the first symbol of tile t (lower bound over offsets[0..num_records]); sets
-// state[1] when the offsets are not non-decreasing from 0 to num_symbols.
+// Batched form prepass (after the state and the bitmap are zeroed): one bit
+// per record start below num_symbols; state[1] is set when the offsets are not
+// non-decreasing from 0 to num_symbols.
 __global__ __launch_bounds__(kBlock) void batch_prepass_kernel(const uint64_t* offsets,
                                                                uint64_t num_records,
                                                                uint64_t num_symbols,
-                                                               uint64_t tile_windows,
-                                                               uint64_t num_tiles,
-                                                               uint32_t* tile_rec,
+                                                               unsigned long long* starts,
                                                                uint32_t* state) {
-    const uint64_t n = num_tiles > num_records + 1 ? num_tiles : num_records + 1;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
-        if (i < num_tiles) {
-            const uint64_t first = i * tile_windows;
-            uint64_t lo = 0, hi = num_records + 1;
-            while (lo < hi) {
-                const uint64_t mid = (lo + hi) / 2;
-                if (offsets[mid] < first) {
-                    lo = mid + 1;
-                } else {
-                    hi = mid;
-                }
-            }
-            tile_rec[i] = static_cast<uint32_t>(lo);
-        }
-        bool bad = i < num_records && offsets[i] > offsets[i + 1];
-        if (i == 0) bad = bad || offsets[0] != 0 || offsets[num_records] != num_symbols;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i <= num_records;
+         i += stride) {
+        const uint64_t v = offsets[i];
+        if (i < num_records && v < num_symbols) atomicOr(&starts[v >> 6], 1ull << (v & 63));
+        bool bad = i < num_records && v > offsets[i + 1];
+        if (i == 0) bad = bad || v != 0 || offsets[num_records] != num_symbols;
         if (bad) atomicOr(&state[1], 1u);
     }
 }
@@ -424,19 +420,15 @@ hipError_t count_ones_per_record(const uint8_t* hits, const uint64_t* offsets,
     return hipGetLastError();
 }
 
-hipError_t batch_reset(uint32_t* state, hipStream_t stream) {
+hipError_t batch_prepass(const SequenceInput& in, uint64_t* starts, uint32_t* state,
+                         hipStream_t stream) {
+    hipError_t e = fill_u64(reinterpret_cast<uint64_t*>(state), 0, 1, stream);
+    if (e == hipSuccess) e = fill_u64(starts, 0, start_words(in.num_symbols), stream);
+    if (e != hipSuccess) return e;
     (void)hipGetLastError();  // report only this launch
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, state);
-    return hipGetLastError();
-}
-
-hipError_t batch_prepass(const SequenceInput& in, uint64_t tile_windows, uint64_t num_tiles,
-                         uint32_t* tile_rec, uint32_t* state, hipStream_t stream) {
-    const uint64_t n = num_tiles > in.num_records + 1 ? num_tiles : in.num_records + 1;
-    (void)hipGetLastError();  // report only this launch
-    hipLaunchKernelGGL(batch_prepass_kernel, dim3(grid_for(n, kBlock, kMaxBlocks)), dim3(kBlock), 0,
-                       stream, in.offsets, in.num_records, in.num_symbols, tile_windows, num_tiles,
-                       tile_rec, state);
+    hipLaunchKernelGGL(batch_prepass_kernel, dim3(grid_for(in.num_records + 1, kBlock, kMaxBlocks)),
+                       dim3(kBlock), 0, stream, in.offsets, in.num_records, in.num_symbols,
+                       reinterpret_cast<unsigned long long*>(starts), state);
     return hipGetLastError();
 }
 

@@ -16,8 +16,9 @@ namespace sbf {
 // Batched form: `offsets` (num_records + 1 non-decreasing symbol offsets,
 // offsets[0] = 0, offsets[num_records] = num_symbols) cut the byte sequence
 // into records concatenated without separators; a window that crosses a
-// record start is invalid. The prepass fills `tile_rec` (lane-strip kernels
-// only) and checks the offsets into `state`.
+// record start is invalid. The prepass turns the offsets into `starts`, a
+// bitmap with bit i set when symbol i starts a record, and checks the offsets
+// into `state`.
 struct SequenceInput {
     const char* seq = nullptr;
     const uint64_t* words = nullptr;
@@ -26,8 +27,8 @@ struct SequenceInput {
     bool dense = false;
     const uint64_t* offsets = nullptr;
     uint64_t num_records = 0;
-    const uint32_t* tile_rec = nullptr;  // first record starting at or after each tile's first symbol
-    uint32_t* state = nullptr;           // [0] work-queue counter, [1] malformed-offsets flag
+    const uint64_t* starts = nullptr;  // record-start bitmap, start_words(num_symbols) words
+    uint32_t* state = nullptr;         // [0] work-queue counter, [1] malformed-offsets flag
 };
 
 struct FilterView {
@@ -53,7 +54,7 @@ using BatchedContainsFn = hipError_t (*)(const SequenceInput&, FilterView, const
 struct StripKernelSet {
     uint32_t k, s, m, h, alphabet;
     bool dense;             // plain kernels also read dense packed input
-    uint32_t wave_windows;  // windows per wave tile (batched tile map granularity)
+    uint32_t wave_windows;  // windows per wave tile
     InsertFn insert;
     ContainsFn contains;
     BatchedInsertFn batched_insert[2];      // [0]: built for 3 waves per SIMD, [1]: for 4 (or null)
@@ -64,8 +65,9 @@ struct StripKernelSet {
 const StripKernelSet* find_strip_kernels(uint32_t k, uint32_t s, uint32_t m, uint32_t h,
                                          uint32_t alphabet);
 
-// Generic kernels: any alphabet / configuration, one window per thread; they
-// honour `offsets` (batched form) by a per-window search.
+// Generic kernels: any alphabet / configuration, one window per thread; in the
+// batched form they test the record-start bitmap (or, without it, search the
+// offsets) for each window.
 hipError_t generic_insert(const KernelConfig& c, const SequenceInput& in, FilterView f,
                           hipStream_t stream);
 hipError_t generic_contains(const KernelConfig& c, const SequenceInput& in, FilterView f,
@@ -76,12 +78,15 @@ hipError_t generic_count(const KernelConfig& c, const SequenceInput& in, uint64_
 hipError_t pack_dense(const KernelConfig& c, const char* seq, uint64_t num_symbols,
                       uint64_t* words, hipStream_t stream);
 
-// Batched form set-up, in stream order before the batched kernels: zero the
-// work-queue counter and the flag, then fill tile_rec for tiles of
-// `tile_windows` windows (none when num_tiles is 0) and check the offsets.
-hipError_t batch_reset(uint32_t* state, hipStream_t stream);
-hipError_t batch_prepass(const SequenceInput& in, uint64_t tile_windows, uint64_t num_tiles,
-                         uint32_t* tile_rec, uint32_t* state, hipStream_t stream);
+// Words of the record-start bitmap for num_symbols symbols (one spare word, so
+// a 16-bit lane chunk just past the end still reads inside the bitmap).
+__host__ __device__ inline uint64_t start_words(uint64_t num_symbols) { return num_symbols / 64 + 2; }
+
+// Batched form set-up, in stream order before the batched kernels (fill
+// kernels only, so the sequence can be captured in a graph): zero the state
+// and the bitmap, then set one bit per record start and check the offsets.
+hipError_t batch_prepass(const SequenceInput& in, uint64_t* starts, uint32_t* state,
+                         hipStream_t stream);
 
 // Reductions.
 hipError_t popcount_words(const uint64_t* words, uint64_t n, uint64_t* d_out, hipStream_t stream);

@@ -511,6 +511,46 @@ class filter {
                                                                d_output.data(), d_output.size(), stream));
     }
 
+    // --- batched record form (no single-call counterpart in the reference) ---
+    // Records back to back in d_symbols, d_offsets = num_records + 1 symbol
+    // offsets (device memory, non-decreasing from 0 to d_symbols' symbol
+    // count). Windows inside a record behave as insert_sequence /
+    // contains_sequence on that record alone; windows across a record start
+    // are skipped / answered 0. Query output: one byte per window start.
+    [[nodiscard]] Result<void> insert_batched_async(device_span<const char> d_symbols,
+                                                    device_span<const uint64_t> d_offsets,
+                                                    hipStream_t stream = nullptr) {
+        return detail::check(cusbf_insert_batched_async(handle_, d_symbols.data(), symbols_of(d_symbols),
+                                                        d_offsets.data(), records_of(d_offsets), stream));
+    }
+    [[nodiscard]] Result<void> contains_batched_async(device_span<const char> d_symbols,
+                                                      device_span<const uint64_t> d_offsets,
+                                                      device_span<uint8_t> d_output,
+                                                      hipStream_t stream = nullptr) const {
+        return detail::check(cusbf_contains_batched_async(handle_, d_symbols.data(), symbols_of(d_symbols),
+                                                          d_offsets.data(), records_of(d_offsets),
+                                                          d_output.data(), d_output.size(), stream));
+    }
+    /// Synchronises; invalid_argument when the last batched call's offsets were malformed.
+    [[nodiscard]] Result<void> batched_status(hipStream_t stream = nullptr) const {
+        return detail::check(cusbf_batched_status(handle_, stream));
+    }
+    [[nodiscard]] Result<void> set_batch_tuning(uint32_t bucket, cusbf_batch_op op,
+                                                const cusbf_batch_tuning& tuning) {
+        return detail::check(cusbf_batch_tuning_set(handle_, bucket, op, &tuning));
+    }
+    [[nodiscard]] Result<cusbf_batch_tuning> batch_tuning(uint32_t bucket, cusbf_batch_op op) const {
+        cusbf_batch_tuning t{};
+        CUSBF_TRY(detail::check(cusbf_batch_tuning_get(handle_, bucket, op, &t)));
+        return t;
+    }
+    [[nodiscard]] static uint64_t symbols_of(device_span<const char> d) {
+        return d.size() / Config::Alphabet::symbolWidth;
+    }
+    [[nodiscard]] static uint64_t records_of(device_span<const uint64_t> off) {
+        return off.empty() ? 0 : off.size() - 1;
+    }
+
     // --- state (filter.cuh:793-833) ---
     [[nodiscard]] Result<void> clear(hipStream_t stream = nullptr) {
         return detail::check(cusbf_clear(handle_, stream));
@@ -538,6 +578,49 @@ class filter {
     }
 
     cusbf_filter* handle_ = nullptr;
+};
+
+/// A batched insert or query captured once into a hipGraph over fixed buffers
+/// and replayed with one launch (cusbf_batch_plan_*). Move-only; the buffers
+/// and the filter must outlive it.
+class batch_plan {
+   public:
+    template <typename Config>
+    [[nodiscard]] static Result<batch_plan> create(filter<Config>& f, cusbf_batch_op op,
+                                                   device_span<const char> d_symbols,
+                                                   device_span<const uint64_t> d_offsets,
+                                                   device_span<uint8_t> d_output = {},
+                                                   const cusbf_batch_tuning* tuning = nullptr) {
+        batch_plan p;
+        CUSBF_TRY(detail::check(cusbf_batch_plan_create(
+            f.native_handle(), op, d_symbols.data(), filter<Config>::symbols_of(d_symbols),
+            d_offsets.data(), filter<Config>::records_of(d_offsets), d_output.data(), d_output.size(),
+            tuning, &p.handle_)));
+        return p;
+    }
+    batch_plan(const batch_plan&) = delete;
+    batch_plan& operator=(const batch_plan&) = delete;
+    batch_plan(batch_plan&& o) noexcept : handle_(std::exchange(o.handle_, nullptr)) {}
+    batch_plan& operator=(batch_plan&& o) noexcept {
+        if (this != &o) {
+            cusbf_batch_plan_destroy(handle_);
+            handle_ = std::exchange(o.handle_, nullptr);
+        }
+        return *this;
+    }
+    ~batch_plan() { cusbf_batch_plan_destroy(handle_); }
+
+    [[nodiscard]] Result<void> launch(hipStream_t stream = nullptr) {
+        return detail::check(cusbf_batch_plan_launch(handle_, stream));
+    }
+    /// Synchronises; invalid_argument when the offsets were malformed.
+    [[nodiscard]] Result<void> status(hipStream_t stream = nullptr) {
+        return detail::check(cusbf_batch_plan_status(handle_, stream));
+    }
+
+   private:
+    batch_plan() = default;
+    cusbf_batch_plan* handle_ = nullptr;
 };
 
 }  // namespace cusbf

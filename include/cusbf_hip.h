@@ -247,11 +247,13 @@ const char* cusbf_kernel_family(const cusbf_filter* f);
  * start of the concatenation (num_symbols - k + 1 bytes); record r's answers
  * are d_output[d_offsets[r] .. d_offsets[r+1] - k].
  *
- * Launch sequence (stream order): reset of the work-queue counter (a fill
- * kernel, so the sequence can be captured in a graph), a prepass that finds
- * each tile's first record and checks the offsets, and one persistent kernel
- * whose waves take tiles from the work queue. Malformed offsets make the
- * kernel do nothing; cusbf_batched_status / cusbf_batch_plan_status report it.
+ * Launch sequence (stream order): fill kernels that zero the work-queue state
+ * and a record-start bitmap (num_symbols / 8 bytes of library scratch; fill
+ * kernels rather than memsets, so the sequence can be captured in a graph), a
+ * prepass that sets one bit per record start and checks the offsets, and one
+ * persistent kernel whose waves take tiles from the work queue (or stride over
+ * them). Malformed offsets make the kernel do nothing; cusbf_batched_status /
+ * cusbf_batch_plan_status report it.
  * Launch shape per record-length bucket comes from the filter's tuning table.
  * Calls on one filter share its batch scratch: issue them on one stream.
  */

@@ -328,6 +328,73 @@ TEST_GPU(DeviceRecordBatchQueryMatchesHostContains) {
     EXPECT_EQ(r.error().message(), std::string("record batch query output span is too small"));
 }
 
+// Batched record form: the two records of the device record-batch test above,
+// back to back without the separator. Windows inside a record answer as in the
+// record batch; the 4 windows across the record start answer 0.
+TEST_GPU(BatchedRecordsMatchRecordBatchAnswers) {
+    cusbf::filter<TestConfig> filter(1 << 14);
+    const std::string r1 = "ACGTACGTACGT", r2 = "TTGGAACCTTGG";
+    const std::string all = r1 + r2;
+    DeviceBuffer<char> d_seq(all.data(), all.size());
+    const std::array<uint64_t, 3> off{0, 12, 24};
+    DeviceBuffer<uint64_t> d_off(off.data(), off.size());
+    const cusbf::device_span<const char> syms{d_seq.ptr, d_seq.n};
+    const cusbf::device_span<const uint64_t> offs{d_off.ptr, d_off.n};
+    cusbf::require_void(filter.insert_batched_async(syms, offs));
+    cusbf::require_void(filter.batched_status());
+    EXPECT_TRUE(all_ones(CUSBF_UNWRAP(filter.contains_sequence(r1))));
+    EXPECT_TRUE(all_ones(CUSBF_UNWRAP(filter.contains_sequence(r2))));
+    DeviceBuffer<uint8_t> d_out(20);
+    cusbf::require_void(filter.contains_batched_async(syms, offs, cusbf::device_span<uint8_t>{d_out.ptr, 20}));
+    cusbf::require_void(filter.batched_status());
+    const auto hits = d_out.host();
+    for (int i = 0; i < 8; ++i) EXPECT_EQ(hits[i], 1);
+    for (int i = 8; i < 12; ++i) EXPECT_EQ(hits[i], 0);
+    for (int i = 12; i < 20; ++i) EXPECT_EQ(hits[i], 1);
+    const auto small = filter.contains_batched_async(syms, offs, cusbf::device_span<uint8_t>{d_out.ptr, 19});
+    EXPECT_TRUE(!small && small.error().category() == cusbf::ErrorCategory::invalid_argument);
+    // malformed offsets are reported by batched_status
+    const std::array<uint64_t, 3> bad{0, 13, 12};
+    DeviceBuffer<uint64_t> d_bad(bad.data(), bad.size());
+    cusbf::require_void(filter.insert_batched_async(syms, cusbf::device_span<const uint64_t>{d_bad.ptr, 3}));
+    const auto st = filter.batched_status();
+    EXPECT_TRUE(!st && st.error().category() == cusbf::ErrorCategory::invalid_argument);
+}
+
+TEST_GPU(BatchPlanReplaysMatchEagerCall) {
+    cusbf::filter<TestConfig> filter(1 << 14);
+    const std::string all = "ACGTACGTACGTTTGGAACCTTGGACNTTGACCA";
+    DeviceBuffer<char> d_seq(all.data(), all.size());
+    const std::array<uint64_t, 4> off{0, 12, 24, 34};
+    DeviceBuffer<uint64_t> d_off(off.data(), off.size());
+    const cusbf::device_span<const char> syms{d_seq.ptr, d_seq.n};
+    const cusbf::device_span<const uint64_t> offs{d_off.ptr, d_off.n};
+    cusbf::require_void(filter.insert_batched_async(syms, offs));
+    cusbf::require_void(filter.batched_status());
+    const auto words = CUSBF_UNWRAP(filter.words());
+    cusbf::require_void(filter.clear());
+    auto ins = CUSBF_UNWRAP(cusbf::batch_plan::create(filter, CUSBF_BATCH_INSERT, syms, offs));
+    DeviceBuffer<uint8_t> d_out(30);
+    auto qry = CUSBF_UNWRAP(cusbf::batch_plan::create(filter, CUSBF_BATCH_CONTAINS, syms, offs,
+                                                      cusbf::device_span<uint8_t>{d_out.ptr, 30}));
+    for (int rep = 0; rep < 3; ++rep) {
+        cusbf::require_void(ins.launch());
+        cusbf::require_void(ins.status());
+        EXPECT_TRUE(CUSBF_UNWRAP(filter.words()) == words);
+        cusbf::require_void(qry.launch());
+        cusbf::require_void(qry.status());
+        const auto hits = d_out.host();
+        uint64_t ones = 0;
+        for (uint8_t h : hits) ones += h;
+        // records of 12, 12 and 10 symbols; the N at position 2 of the third
+        // leaves its windows 3..5: 8 + 8 + 3 valid windows, all inserted
+        EXPECT_EQ(ones, 8u + 8u + 3u);
+    }
+    cusbf_batch_tuning t{3, 2, 1, 0};
+    cusbf::require_void(filter.set_batch_tuning(1, CUSBF_BATCH_CONTAINS, t));
+    EXPECT_EQ(CUSBF_UNWRAP(filter.batch_tuning(1, CUSBF_BATCH_CONTAINS)).blocks_per_cu, 2u);
+}
+
 TEST_GPU(HostRecordBatchReportsAndPerRecordHits) {
     cusbf::filter<TestConfig> filter(1 << 14);
     const std::string seq = "ACGTACGTACGTTTGGAACCTTGGACNTT";
