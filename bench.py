@@ -62,8 +62,8 @@ def parse_args():
 
 def committed_traffic(workload: str, kernel: str):
     """Calibrated HBM bytes per launch of `kernel` from the committed counter summary."""
-    name = {"insert": "insert_kernel<31, 28, 16, 4, false>",
-            "query": "contains_kernel<31, 28, 16, 4, false>"}[kernel]
+    name = {"insert": "insert_kernel<31, 28, 16, 4, Dna2, false, 4>",
+            "query": "contains_kernel<31, 28, 16, 4, Dna2, false, 4>"}[kernel]
     path = os.path.join(ROOT, "profiles", f"dna_insert_query_{workload}_pmc_traffic.md")
     if not os.path.exists(path):
         return None
@@ -317,6 +317,7 @@ def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str =
     out_b = torch.empty(windows, dtype=torch.uint8, device="cuda")
     ins = bt.BatchPlan(filt, bt.INSERT, data, d_off)
     qry = bt.BatchPlan(filt, bt.CONTAINS, query, d_off, out_b)
+    reps = max(1, reps)
     ti, tq = [], []
     for r in range(reps + 1):
         filt.clear(sh)

@@ -26,4 +26,11 @@ int validate_host_ranges(const cusbf_filter* f, uint64_t bytes, const cusbf_reco
 int launch_contains_bytes(const cusbf_filter* f, const char* d_seq, uint64_t bytes, uint8_t* d_out,
                           hipStream_t stream);
 
+// Report counts of a batch's requests on the device (zeroes d_counts first):
+// request r owns windows [d_req[2r], d_req[2r] + d_req[2r+1]); d_counts[2r] =
+// valid windows, d_counts[2r+1] = positive windows.
+int launch_request_counts(const cusbf_filter* f, const char* d_seq, uint64_t bytes,
+                          const uint8_t* d_hits, const uint64_t* d_req, uint64_t n_req,
+                          uint64_t* d_counts, hipStream_t stream);
+
 }  // namespace cusbf_internal

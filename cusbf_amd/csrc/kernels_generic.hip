@@ -217,6 +217,147 @@ __global__ __launch_bounds__(kBlock) void count_kernel(KernelConfig c, SequenceI
     if (threadIdx.x < 2 && totals[threadIdx.x] != 0) atomicAdd(&counts[threadIdx.x], totals[threadIdx.x]);
 }
 
+// Report counts of a server batch (filter.cuh:951-990 FastxQueryReport, done on
+// the device): request r owns windows [req[2r], req[2r] + req[2r+1]) of the
+// batch; counts[2r] += its valid windows (no invalid symbol among the k),
+// counts[2r+1] += its hit bytes equal to 1. A thread takes 16 consecutive
+// windows: the invalid-symbol mask of their 16 + k - 1 symbols (128-bit loads;
+// SWAR test for DNA), the hit bytes with one 128-bit load, then the counts go
+// to the owning request -- summed over the wave first when the whole wave falls
+// in one request, which is the common case.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Bit i set when byte i of x is not A, C, G or T in either case
+// (DnaAlphabet::encode, Alphabet.cuh:132-141).
+__device__ __forceinline__ uint32_t dna_invalid4(uint32_t x) {
+    const uint32_t sym = (x >> 1) & 0x03030303u;
+    const uint32_t expect = __builtin_amdgcn_perm(0u, 0x47544341u, sym);  // 'A','C','T','G' by code
+    const uint32_t diff = (x & 0xDFDFDFDFu) ^ expect;
+    const uint32_t nz = ((((diff & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | diff) & 0x80808080u) >> 7;
+    uint32_t inv = nz | (nz >> 7);
+    inv |= inv >> 14;
+    return inv & 0xFu;
+}
+
+// Bit i set when symbol first+i is invalid or past the end (16 symbols).
+__device__ __forceinline__ uint32_t invalid16(const KernelConfig& c, const char* seq, uint64_t n,
+                                              uint64_t first) {
+    if (c.symbol_width == 1 && first + 16 <= n &&
+        (reinterpret_cast<uintptr_t>(seq + first) & 15u) == 0) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(seq + first);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        uint32_t m = 0;
+        if (c.alphabet == kDna) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) m |= dna_invalid4(w[q]) << (4 * q);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const char b = static_cast<char>((w[i >> 2] >> (8 * (i & 3))) & 0xFFu);
+                m |= (encode_symbol(c, &b) == static_cast<uint8_t>(c.invalid_symbol) ? 1u : 0u) << i;
+            }
+        }
+        return m;
+    }
+    uint32_t m = 0;
+    for (int i = 0; i < 16; ++i) {
+        const uint64_t q = first + static_cast<uint64_t>(i);
+        const bool bad = q >= n || encode_symbol(c, seq + q * c.symbol_width) ==
+                                       static_cast<uint8_t>(c.invalid_symbol);
+        m |= (bad ? 1u : 0u) << i;
+    }
+    return m;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kBlock) void request_counts_kernel(KernelConfig c, const char* seq,
+                                                                uint64_t num_symbols,
+                                                                const uint8_t* hits,
+                                                                const uint64_t* req, uint64_t n_req,
+                                                                unsigned long long* counts) {
+    const uint64_t num_windows = num_symbols < c.k ? 0 : num_symbols - c.k + 1;
+    const uint64_t groups = (num_windows + 15) / 16;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    const uint64_t kmask = c.k >= 64 ? ~0ull : ((1ull << c.k) - 1ull);
+    // every lane runs every iteration (wave sums below)
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlock; base < groups; base += stride) {
+        const uint64_t g = base + threadIdx.x;
+        const bool active = g < groups;
+        uint32_t valid = 0, positive = 0;
+        uint64_t r = 0;
+        bool whole = true;  // the lane's windows lie in request r (or the lane has none)
+        if (active) {
+            const uint64_t p0 = 16 * g;
+            // symbols p0 .. p0 + 79 cover 16 windows of k <= 64
+            const uint64_t m0 = static_cast<uint64_t>(invalid16(c, seq, num_symbols, p0)) |
+                                (static_cast<uint64_t>(invalid16(c, seq, num_symbols, p0 + 16)) << 16) |
+                                (static_cast<uint64_t>(invalid16(c, seq, num_symbols, p0 + 32)) << 32) |
+                                (static_cast<uint64_t>(invalid16(c, seq, num_symbols, p0 + 48)) << 48);
+            const uint64_t m1 = invalid16(c, seq, num_symbols, p0 + 64);
+            const uint64_t n = num_windows - p0 < 16 ? num_windows - p0 : 16;
+            for (uint32_t o = 0; o < n; ++o) {
+                const uint64_t w = o == 0 ? m0 : ((m0 >> o) | (m1 << (64 - o)));
+                valid |= ((w & kmask) == 0 ? 1u : 0u) << o;
+            }
+            if (n == 16 && (reinterpret_cast<uintptr_t>(hits + p0) & 15u) == 0) {
+                const u32x4 h = *reinterpret_cast<const u32x4*>(hits + p0);
+                const uint32_t hw[4] = {h.x, h.y, h.z, h.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t x = hw[q] ^ 0x01010101u;  // zero byte <=> hit byte == 1
+                    const uint32_t nz = ((((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u) >> 7;
+                    const uint32_t bits = ~(nz | (nz >> 7) | (nz >> 14) | (nz >> 21)) & 0xFu;
+                    positive |= bits << (4 * q);
+                }
+            } else {
+                for (uint32_t o = 0; o < n; ++o) positive |= (hits[p0 + o] == 1 ? 1u : 0u) << o;
+            }
+            // last request starting at or before p0
+            uint64_t lo = 0, hi = n_req;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) / 2;
+                if (req[2 * mid] <= p0) {
+                    lo = mid + 1;
+                } else {
+                    hi = mid;
+                }
+            }
+            r = lo == 0 ? 0 : lo - 1;
+            whole = lo != 0 && p0 + n <= req[2 * r] + req[2 * r + 1];
+            if (!whole) {
+                // windows split over requests (or outside them): per window
+                uint64_t rr = lo == 0 ? 0 : lo - 1;
+                for (uint32_t o = 0; o < n; ++o) {
+                    const uint64_t p = p0 + o;
+                    while (rr < n_req && p >= req[2 * rr] + req[2 * rr + 1]) ++rr;
+                    if (rr < n_req && p >= req[2 * rr]) {
+                        if ((valid >> o) & 1u) atomicAdd(&counts[2 * rr], 1ull);
+                        if ((positive >> o) & 1u) atomicAdd(&counts[2 * rr + 1], 1ull);
+                    }
+                }
+                valid = positive = 0;
+            }
+        }
+        const uint64_t r0 = __shfl(r, 0, 64);
+        const bool uniform = __all(!active || !whole || r == r0);
+        if (uniform) {
+            const unsigned long long sv = wave_sum64(static_cast<unsigned long long>(__popc(valid)));
+            const unsigned long long sp = wave_sum64(static_cast<unsigned long long>(__popc(positive)));
+            if ((threadIdx.x & 63) == 0 && (sv | sp)) {
+                atomicAdd(&counts[2 * r0], sv);
+                atomicAdd(&counts[2 * r0 + 1], sp);
+            }
+        } else if (valid | positive) {
+            atomicAdd(&counts[2 * r], static_cast<unsigned long long>(__popc(valid)));
+            atomicAdd(&counts[2 * r + 1], static_cast<unsigned long long>(__popc(positive)));
+        }
+    }
+}
+
 // dense_packed.cuh:126-149, staged through LDS: the block reads its input
 // bytes with coalesced loads, then each thread packs one output word.
 constexpr int kPackMaxBytes = kBlock * 32;
@@ -429,6 +570,18 @@ hipError_t batch_prepass(const SequenceInput& in, uint64_t* starts, uint32_t* st
     hipLaunchKernelGGL(batch_prepass_kernel, dim3(grid_for(in.num_records + 1, kBlock, kMaxBlocks)),
                        dim3(kBlock), 0, stream, in.offsets, in.num_records, in.num_symbols,
                        reinterpret_cast<unsigned long long*>(starts), state);
+    return hipGetLastError();
+}
+
+hipError_t request_counts(const KernelConfig& c, const char* seq, uint64_t num_symbols,
+                          const uint8_t* hits, const uint64_t* req, uint64_t n_req,
+                          uint64_t* counts, hipStream_t stream) {
+    const uint64_t n = num_symbols < c.k ? 0 : num_symbols - c.k + 1;
+    if (n == 0 || n_req == 0) return hipSuccess;
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(request_counts_kernel, dim3(grid_for((n + 15) / 16, kBlock, kMaxBlocks)),
+                       dim3(kBlock), 0, stream, c, seq, num_symbols, hits, req, n_req,
+                       reinterpret_cast<unsigned long long*>(counts));
     return hipGetLastError();
 }
 

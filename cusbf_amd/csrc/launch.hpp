@@ -88,6 +88,13 @@ __host__ __device__ inline uint64_t start_words(uint64_t num_symbols) { return n
 hipError_t batch_prepass(const SequenceInput& in, uint64_t* starts, uint32_t* state,
                          hipStream_t stream);
 
+// Server report counts: request r owns windows [req[2r], req[2r] + req[2r+1])
+// of a batch (sorted, disjoint); counts[2r] += valid windows, counts[2r+1] +=
+// hits equal to 1 (caller zeroes counts).
+hipError_t request_counts(const KernelConfig& c, const char* seq, uint64_t num_symbols,
+                          const uint8_t* hits, const uint64_t* req, uint64_t n_req,
+                          uint64_t* counts, hipStream_t stream);
+
 // Reductions.
 hipError_t popcount_words(const uint64_t* words, uint64_t n, uint64_t* d_out, hipStream_t stream);
 hipError_t count_ones(const uint8_t* hits, uint64_t n, uint64_t* d_out, hipStream_t stream);

@@ -7,7 +7,9 @@
 // slot's stream. A completion thread waits for each slot's event, hands every
 // request its windows and report, and returns the slot; the scheduler admits
 // new requests as slots come back. With several slots the copies and kernels
-// of different batches overlap on the device. The filter is read-only here.
+// of different batches overlap on the device. Each request's report (valid and
+// positive windows) is counted on the device in the same batch, so no host
+// thread scans request bytes or answers. The filter is read-only here.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,6 +49,9 @@ struct Request {
     std::string message;
 };
 
+// Requests per batch (bounds the per-slot request descriptors and counters).
+constexpr uint64_t kMaxBatchRequests = 4096;
+
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -54,6 +59,10 @@ struct Slot {
     uint8_t* d_hits = nullptr;
     char* h_seq = nullptr;
     uint8_t* h_hits = nullptr;
+    uint64_t* d_req = nullptr;     // per request: first window in the batch, window count
+    uint64_t* h_req = nullptr;
+    uint64_t* d_counts = nullptr;  // per request: valid windows, positive windows
+    uint64_t* h_counts = nullptr;
     uint64_t batch_bytes = 0;
     int status = CUSBF_OK;
     std::string message;
@@ -181,9 +190,22 @@ void fill_batch(cusbf_server* s, Slot& slot) {
 void launch_batch(cusbf_server* s, Slot& slot) {
     slot.status = CUSBF_OK;
     const uint64_t windows = ci::window_count(s->filter, slot.batch_bytes);
+    const uint64_t n_req = slot.batch.size();
+    for (uint64_t i = 0; i < n_req; ++i) {
+        slot.h_req[2 * i] = slot.batch[i]->batch_offset / s->width;
+        slot.h_req[2 * i + 1] = slot.batch[i]->windows;
+    }
     hipError_t e = hipMemcpyAsync(slot.d_seq, slot.h_seq, slot.batch_bytes, hipMemcpyHostToDevice, slot.stream);
+    if (e == hipSuccess) {
+        e = hipMemcpyAsync(slot.d_req, slot.h_req, 2 * n_req * sizeof(uint64_t), hipMemcpyHostToDevice,
+                           slot.stream);
+    }
     if (e == hipSuccess && windows) {
-        const int rc = ci::launch_contains_bytes(s->filter, slot.d_seq, slot.batch_bytes, slot.d_hits, slot.stream);
+        int rc = ci::launch_contains_bytes(s->filter, slot.d_seq, slot.batch_bytes, slot.d_hits, slot.stream);
+        if (rc == CUSBF_OK) {
+            rc = ci::launch_request_counts(s->filter, slot.d_seq, slot.batch_bytes, slot.d_hits, slot.d_req,
+                                           n_req, slot.d_counts, slot.stream);
+        }
         if (rc != CUSBF_OK) {
             char buf[512];
             cusbf_last_error(buf, sizeof(buf));
@@ -191,6 +213,10 @@ void launch_batch(cusbf_server* s, Slot& slot) {
             slot.message = buf;
         }
         e = hipMemcpyAsync(slot.h_hits, slot.d_hits, windows, hipMemcpyDeviceToHost, slot.stream);
+        if (e == hipSuccess) {
+            e = hipMemcpyAsync(slot.h_counts, slot.d_counts, 2 * n_req * sizeof(uint64_t),
+                               hipMemcpyDeviceToHost, slot.stream);
+        }
     }
     if (e == hipSuccess) e = hipEventRecord(slot.done, slot.stream);
     if (e != hipSuccess) {
@@ -219,7 +245,7 @@ void scheduler_loop(cusbf_server* s) {
         Slot& slot = s->slots[idx];
         slot.batch.clear();
         uint64_t bytes = 0;
-        while (!s->queue.empty()) {
+        while (!s->queue.empty() && slot.batch.size() < kMaxBatchRequests) {
             Request* r = s->queue.front();
             const uint64_t need = boundary_bytes(bytes, s->width) + r->norm_bytes;
             if (bytes + need > s->seq_cap && !slot.batch.empty()) break;
@@ -261,11 +287,14 @@ void completion_loop(cusbf_server* s) {
         }
         const auto now = Clock::now();
         uint64_t windows = 0;
-        for (Request* r : slot.batch) {
+        for (size_t i = 0; i < slot.batch.size(); ++i) {
+            Request* r = slot.batch[i];
             r->status = slot.status;
             r->message = slot.message;
             if (slot.status == CUSBF_OK && r->windows) {
                 parallel_copy(r->hits, slot.h_hits + r->batch_offset / s->width, r->windows);
+                r->report.queried_kmers = slot.h_counts[2 * i];
+                r->report.positive_kmers = slot.h_counts[2 * i + 1];
             }
             windows += r->windows;
             r->latency = std::chrono::duration<double>(now - r->submitted).count();
@@ -288,6 +317,10 @@ void free_slots(cusbf_server* s) {
         if (slot.d_hits) (void)hipFree(slot.d_hits);
         if (slot.h_seq) (void)hipHostFree(slot.h_seq);
         if (slot.h_hits) (void)hipHostFree(slot.h_hits);
+        if (slot.d_req) (void)hipFree(slot.d_req);
+        if (slot.h_req) (void)hipHostFree(slot.h_req);
+        if (slot.d_counts) (void)hipFree(slot.d_counts);
+        if (slot.h_counts) (void)hipHostFree(slot.h_counts);
     }
 }
 
@@ -317,6 +350,11 @@ int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* optio
         if (e == hipSuccess) e = hipMalloc(&slot.d_hits, s->seq_cap);
         if (e == hipSuccess) e = hipHostMalloc(&slot.h_seq, s->seq_cap, hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc(&slot.h_hits, s->seq_cap, hipHostMallocDefault);
+        const uint64_t desc = 2 * kMaxBatchRequests * sizeof(uint64_t);
+        if (e == hipSuccess) e = hipMalloc(&slot.d_req, desc);
+        if (e == hipSuccess) e = hipHostMalloc(&slot.h_req, desc, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc(&slot.d_counts, desc);
+        if (e == hipSuccess) e = hipHostMalloc(&slot.h_counts, desc, hipHostMallocDefault);
         if (e != hipSuccess) {
             free_slots(s.get());
             return ci::set_hip_error(e, __FILE__, __LINE__);
@@ -360,6 +398,8 @@ int cusbf_server_submit(cusbf_server* s, const char* h_sequence, uint64_t bytes,
     r->ranges.assign(ranges, ranges + num_ranges);
     r->hits = h_hits;
     r->norm_bytes = normalized_size(ranges, num_ranges, s->width);
+    r->report.records_queried = num_ranges;
+    for (uint64_t i = 0; i < num_ranges; ++i) r->report.queried_bases += ranges[i].sequence_bytes;
     r->windows = ci::window_count(s->filter, r->norm_bytes);
     if (hits_len < r->windows) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "request hit buffer is too small");
     if (r->norm_bytes > s->seq_cap) {
@@ -396,17 +436,9 @@ int cusbf_server_wait(cusbf_server* s, uint64_t ticket, cusbf_query_report* repo
     }
     if (latency_seconds) *latency_seconds = r->latency;
     if (r->status != CUSBF_OK) return ci::set_error(r->status, r->message.c_str());
-    if (report) {
-        // FastxQueryReport of the request (filter.cuh:951-990), derived after the
-        // answer is ready so the scans stay off the serving threads.
-        cusbf_query_report rep{r->ranges.size(), 0, 0, 0};
-        for (const cusbf_record_range& rr : r->ranges) {
-            rep.queried_bases += rr.sequence_bytes;
-            rep.queried_kmers += ci::valid_kmers(s->filter, r->seq + rr.sequence_offset, rr.sequence_bytes);
-        }
-        for (uint64_t i = 0; i < r->windows; ++i) rep.positive_kmers += r->hits[i] == 1;
-        *report = rep;
-    }
+    // FastxQueryReport of the request (filter.cuh:951-990): records and bases
+    // from the submit, valid and positive windows counted on the device.
+    if (report) *report = r->report;
     return CUSBF_OK;
 }
 
