@@ -61,6 +61,16 @@ class CBatchTuning(ctypes.Structure):
                 ("reserved", u32)]
 
 
+class CFastxRecordView(ctypes.Structure):
+    _fields_ = [("record_index", u64), ("header", ctypes.c_void_p), ("header_len", u64),
+                ("sequence", ctypes.c_void_p), ("sequence_len", u64), ("queried_bases", u64),
+                ("queried_kmers", u64), ("positive_kmers", u64), ("hits", ctypes.c_void_p),
+                ("hits_len", u64)]
+
+
+FASTX_CONSUMER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(CFastxRecordView), ctypes.c_void_p)
+READ_FN = ctypes.CFUNCTYPE(ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64)
+
 # (name, restype, argtypes) of every exported entry point of include/cusbf_hip.h.
 CFG = ctypes.POINTER(CConfig)
 RR = ctypes.POINTER(CRecordRange)
@@ -114,6 +124,23 @@ SIGNATURES = [
     ("cusbf_batch_plan_status", i32, [P, P]),
     ("cusbf_batch_plan_destroy", None, [P]),
     ("cusbf_count_batched_windows_runs_async", i32, [P, P, u64, P, u64, P, P]),
+    ("cusbf_insert_fastx_buffer", i32, [P, P, u64, ctypes.c_char_p, ctypes.c_double, P,
+                                        ctypes.POINTER(CInsertReport)]),
+    ("cusbf_insert_fastx_reader", i32, [P, READ_FN, P, ctypes.c_char_p, ctypes.c_double, P,
+                                        ctypes.POINTER(CInsertReport)]),
+    ("cusbf_insert_fastx_file", i32, [P, ctypes.c_char_p, ctypes.c_double, P,
+                                      ctypes.POINTER(CInsertReport)]),
+    ("cusbf_query_fastx_buffer", i32, [P, P, u64, ctypes.c_char_p, ctypes.c_double, P,
+                                       FASTX_CONSUMER, P, ctypes.POINTER(CQueryReport)]),
+    ("cusbf_query_fastx_reader", i32, [P, READ_FN, P, ctypes.c_char_p, ctypes.c_double, P,
+                                       FASTX_CONSUMER, P, ctypes.POINTER(CQueryReport)]),
+    ("cusbf_query_fastx_file", i32, [P, ctypes.c_char_p, ctypes.c_double, P, FASTX_CONSUMER, P,
+                                     ctypes.POINTER(CQueryReport)]),
+    ("cusbf_parse_fastx_buffer", i32, [P, u64, ctypes.c_char_p, FASTX_CONSUMER, P,
+                                       ctypes.POINTER(u64)]),
+    ("cusbf_fastx_set_chunk_bytes", i32, [P, u64]),
+    ("cusbf_last_error_location", i32, [ctypes.c_char_p, sz, ctypes.POINTER(u32),
+                                        ctypes.POINTER(u32)]),
     ("cusbf_server_create", i32, [P, ctypes.POINTER(CServerOptions), ctypes.POINTER(P)]),
     ("cusbf_server_destroy", None, [P]),
     ("cusbf_server_submit", i32, [P, P, u64, RR, u64, P, u64, ctypes.POINTER(u64)]),

@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_PATH = os.path.join(HERE, "libcusbf_hip.so")
 SYNTH_LIB_PATH = os.path.join(HERE, "libcusbf_synth.so")
 
-HIP_SOURCES = ["kernels_strip.hip", "kernels_strip_batched.hip", "kernels_generic.hip", "capi.hip", "serve.hip"]
+HIP_SOURCES = ["kernels_strip.hip", "kernels_strip_batched.hip", "kernels_generic.hip", "capi.hip",
+               "serve.hip", "fastx.hip"]
 SYNTH_SOURCES = ["synth.hip", "probe.hip", "baseline.hip"]
 
 # -packed-fp32-ops: the kernels are integer code, and packed FP32 forms stay off.
@@ -42,9 +43,9 @@ def _compile(src: str, obj: str) -> None:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr[-6000:]}")
 
 
-def _link(objs, path: str) -> None:
+def _link(objs, path: str, libs=()) -> None:
     tmp = path + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, *libs]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr[-4000:]}")
@@ -66,7 +67,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for fut in [ex.submit(_compile, s, o) for s, o in work]:
             fut.result()
-    _link(objs_of(HIP_SOURCES), LIB_PATH)
+    _link(objs_of(HIP_SOURCES), LIB_PATH, ["-lz"])  # zlib: gzip FASTX input
     _link(objs_of(SYNTH_SOURCES), SYNTH_LIB_PATH)
     build_cpp_tests()
     if verbose:

@@ -24,10 +24,15 @@ namespace {
 
 thread_local int g_last_status = CUSBF_OK;
 thread_local std::string g_last_message;
+thread_local std::string g_last_file;  // FASTX parse errors: source, 1-based line and column
+thread_local uint32_t g_last_line = 0;
+thread_local uint32_t g_last_column = 0;
 
 int fail(int status, std::string message) {
     g_last_status = status;
     g_last_message = std::move(message);
+    g_last_file.clear();
+    g_last_line = g_last_column = 0;
     return status;
 }
 
@@ -126,6 +131,7 @@ struct cusbf_filter {
     uint64_t* d_starts = nullptr;
     uint64_t starts_cap = 0;  // words
     bool batch_check_pending = false;
+    uint64_t fastx_chunk_bytes = 0;  // FASTX chunk capacity override (0 = from fill_fraction)
 };
 
 struct cusbf_batch_plan {
@@ -437,6 +443,17 @@ int launch_contains_bytes(const cusbf_filter* f, const char* d_seq, uint64_t byt
                           hipStream_t stream) {
     return launch_contains(f, byte_input(f, d_seq, bytes), d_out, stream);
 }
+uint32_t kmer_length(const cusbf_filter* f) { return f->cfg.k; }
+uint64_t fastx_chunk_bytes(const cusbf_filter* f) { return f->fastx_chunk_bytes; }
+int set_fastx_error(const char* source, uint32_t line, uint32_t column, const char* detail) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), ":%u:%u: ", line, column);
+    const int st = fail(CUSBF_ERR_FASTX_PARSE, std::string(source) + buf + detail);
+    g_last_file = source;
+    g_last_line = line;
+    g_last_column = column;
+    return st;
+}
 int launch_request_counts(const cusbf_filter* f, const char* d_seq, uint64_t bytes,
                           const uint8_t* d_hits, const uint64_t* d_req, uint64_t n_req,
                           uint64_t* d_counts, hipStream_t stream) {
@@ -451,6 +468,23 @@ int launch_request_counts(const cusbf_filter* f, const char* d_seq, uint64_t byt
 extern "C" {
 
 int cusbf_abi_version(void) { return CUSBF_ABI_VERSION; }
+
+int cusbf_last_error_location(char* file, size_t file_len, uint32_t* line, uint32_t* column) {
+    if (file && file_len) {
+        const size_t n = std::min(file_len - 1, g_last_file.size());
+        std::memcpy(file, g_last_file.data(), n);
+        file[n] = '\0';
+    }
+    if (line) *line = g_last_line;
+    if (column) *column = g_last_column;
+    return g_last_status;
+}
+
+int cusbf_fastx_set_chunk_bytes(cusbf_filter* f, uint64_t bytes) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    f->fastx_chunk_bytes = bytes;
+    return CUSBF_OK;
+}
 
 int cusbf_last_error(char* buf, size_t buf_len) {
     if (buf && buf_len > 0) {

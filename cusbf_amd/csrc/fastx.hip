@@ -1,0 +1,846 @@
+// fastx.hip -- FASTA/FASTQ ingest for the filter (host side of libcusbf_hip.so).
+//
+// Text -> records -> device, in chunks:
+//  * byte sources: a caller buffer, an mmap'd file, a zlib stream (gzip files,
+//    detected by their magic bytes like gzstreambuf.hpp:101-110) or a caller
+//    read function;
+//  * a line reader with std::getline semantics (lines end at '\n', one
+//    trailing '\r' is dropped, text after the last '\n' is a line when it is
+//    not empty) and a parser with the semantics and error texts of the
+//    reference's streaming reader (Fastx.hpp:240-460): the first header fixes
+//    FASTA ('>') or FASTQ ('@'); FASTA sequence lines run to the next '>'
+//    line; FASTQ sequence lines run to a '+' line, then quality lines until
+//    their total length reaches the sequence length; errors carry
+//    "source:line:column";
+//  * record sequences are appended back to back straight into pinned staging
+//    (a record is padded to the symbol width with the alphabet separator), with
+//    the record's first symbol in an offsets array -- the batched form's input,
+//    so no separator normalisation pass runs;
+//  * a chunk is flushed when the next record does not fit (or the record table
+//    is full): H2D, one batched insert / query launch, the device-side report
+//    counts per record (valid windows, positive windows), D2H of the counts and
+//    hits. Two staging sets alternate, so the host parses the next chunk while
+//    the device works on the previous one; per-record results are delivered in
+//    source order once a chunk's event completes.
+// Chunk size follows the reference's budget rule (filter.cuh:402-425,
+// fastx_chunk.cuh:94-152): fill_fraction of free device memory minus a 64 MiB
+// slack, divided over the chunk's device buffers, is the flush threshold; a
+// zero budget puts every record in a chunk of its own, and a record larger
+// than the staging grows it (the reference processes such a record alone).
+// A consumer returning non-zero stops the query (the filter stays usable).
+#include <hip/hip_runtime.h>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <zlib.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <string_view>
+#include <vector>
+
+#include "../../include/cusbf_hip.h"
+#include "filter_internal.hpp"
+
+namespace {
+
+namespace ci = cusbf_internal;
+
+// --- byte sources -----------------------------------------------------------------
+
+class Source {
+   public:
+    virtual ~Source() = default;
+    // Next bytes into dst (at most cap): the count, 0 at the end, -1 on a read error.
+    virtual int64_t read(char* dst, uint64_t cap) = 0;
+    // The whole input when it is already in memory (data() == nullptr otherwise).
+    virtual std::string_view whole() const { return {}; }
+};
+
+class MemorySource final : public Source {
+   public:
+    MemorySource(const char* data, uint64_t bytes) : data_(data, bytes) {}
+    int64_t read(char*, uint64_t) override { return 0; }
+    std::string_view whole() const override {
+        return data_.data() ? data_ : std::string_view("", 0);
+    }
+
+   private:
+    std::string_view data_;
+};
+
+class MappedFile final : public Source {
+   public:
+    ~MappedFile() override {
+        if (map_ && map_ != MAP_FAILED) munmap(map_, size_);
+    }
+    // 0 ok, else the failing step ("open", "stat", "mmap").
+    const char* open(const char* path) {
+        const int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return "open";
+        struct stat st {};
+        if (fstat(fd, &st) != 0) {
+            close(fd);
+            return "stat";
+        }
+        size_ = static_cast<uint64_t>(st.st_size);
+        if (size_ > 0) {
+            map_ = mmap(nullptr, size_, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (map_ == MAP_FAILED) {
+                close(fd);
+                return "mmap";
+            }
+            madvise(map_, size_, MADV_SEQUENTIAL);
+        }
+        close(fd);
+        return nullptr;
+    }
+    int64_t read(char*, uint64_t) override { return 0; }
+    std::string_view whole() const override {
+        return size_ ? std::string_view(static_cast<const char*>(map_), size_) : std::string_view("", 0);
+    }
+
+   private:
+    void* map_ = nullptr;
+    uint64_t size_ = 0;
+};
+
+class GzipSource final : public Source {
+   public:
+    ~GzipSource() override {
+        if (gz_) gzclose(gz_);
+    }
+    bool open(const char* path) {
+        gz_ = gzopen(path, "rb");
+        if (gz_) gzbuffer(gz_, 1u << 20);
+        return gz_ != nullptr;
+    }
+    int64_t read(char* dst, uint64_t cap) override {
+        const unsigned n = static_cast<unsigned>(std::min<uint64_t>(cap, 1u << 30));
+        const int got = gzread(gz_, dst, n);
+        return got < 0 ? -1 : got;
+    }
+
+   private:
+    gzFile gz_ = nullptr;
+};
+
+class CallbackSource final : public Source {
+   public:
+    CallbackSource(cusbf_read_fn fn, void* user) : fn_(fn), user_(user) {}
+    int64_t read(char* dst, uint64_t cap) override { return fn_(user_, dst, cap); }
+
+   private:
+    cusbf_read_fn fn_;
+    void* user_;
+};
+
+bool is_gzip(const char* path) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    unsigned char magic[2] = {0, 0};
+    const size_t n = std::fread(magic, 1, 2, f);
+    std::fclose(f);
+    return n == 2 && magic[0] == 0x1F && magic[1] == 0x8B;
+}
+
+// --- lines ----------------------------------------------------------------------------
+
+// std::getline over a source: a line is the text up to '\n' (not included) with
+// one trailing '\r' removed; the text after the last '\n' is a line when it is
+// not empty. `unterminated()` tells whether the last line ended at the end of
+// the input (the only case in which a failing getline leaves it in its buffer).
+class LineReader {
+   public:
+    explicit LineReader(Source& src) : src_(src), mem_(src.whole()) {
+        if (!mem_.data()) buf_.resize(1u << 22);
+    }
+    // 1: a line, 0: end of input, -1: read error.
+    int next(std::string_view& line) {
+        if (mem_.data()) {
+            if (pos_ >= mem_.size()) return 0;
+            const char* start = mem_.data() + pos_;
+            const uint64_t left = mem_.size() - pos_;
+            const void* nl = std::memchr(start, '\n', left);
+            uint64_t len;
+            if (nl) {
+                len = static_cast<uint64_t>(static_cast<const char*>(nl) - start);
+                pos_ += len + 1;
+                unterminated_ = false;
+            } else {
+                len = left;
+                pos_ = mem_.size();
+                unterminated_ = true;
+            }
+            line = trim(std::string_view(start, len));
+            return 1;
+        }
+        for (;;) {
+            if (beg_ < end_) {
+                const void* nl = std::memchr(buf_.data() + beg_, '\n', end_ - beg_);
+                if (nl) {
+                    const uint64_t len = static_cast<uint64_t>(static_cast<const char*>(nl) - (buf_.data() + beg_));
+                    line = trim(std::string_view(buf_.data() + beg_, len));
+                    beg_ += len + 1;
+                    unterminated_ = false;
+                    return 1;
+                }
+            }
+            if (eof_) {
+                if (beg_ >= end_) return 0;
+                line = trim(std::string_view(buf_.data() + beg_, end_ - beg_));
+                beg_ = end_;
+                unterminated_ = true;
+                return 1;
+            }
+            // compact, grow when one line fills the buffer, read more
+            if (beg_ > 0) {
+                std::memmove(buf_.data(), buf_.data() + beg_, end_ - beg_);
+                end_ -= beg_;
+                beg_ = 0;
+            }
+            if (end_ == buf_.size()) buf_.resize(buf_.size() * 2);
+            const int64_t got = src_.read(buf_.data() + end_, buf_.size() - end_);
+            if (got < 0) return -1;
+            if (got == 0) eof_ = true;
+            end_ += static_cast<uint64_t>(got);
+        }
+    }
+    bool unterminated() const { return unterminated_; }
+
+   private:
+    static std::string_view trim(std::string_view s) {
+        if (!s.empty() && s.back() == '\r') s.remove_suffix(1);
+        return s;
+    }
+    Source& src_;
+    std::string_view mem_;
+    uint64_t pos_ = 0;
+    std::vector<char> buf_;
+    uint64_t beg_ = 0, end_ = 0;
+    bool eof_ = false;
+    bool unterminated_ = false;
+};
+
+// --- parser -----------------------------------------------------------------------------
+
+// fastx_column_at (Fastx.hpp:261-268): 1-based column of byte_index, clamped to the line.
+uint32_t column_at(std::string_view line, uint64_t byte_index) {
+    if (line.empty()) return 1;
+    return static_cast<uint32_t>(std::min<uint64_t>(byte_index, line.size() - 1) + 1);
+}
+
+struct ParseStatus {
+    int code = 0;  // 0 ok, CUSBF_ERR_FASTX_PARSE, CUSBF_ERR_IO
+    uint32_t line = 0, column = 0;
+    const char* detail = nullptr;
+};
+
+// Receives a record's sequence piece by piece.
+class SequenceSink {
+   public:
+    virtual ~SequenceSink() = default;
+    virtual void append(std::string_view piece) = 0;
+};
+
+class Parser {
+   public:
+    explicit Parser(Source& src) : lines_(src) {}
+
+    // 1: a record (header into `header`, sequence into `sink`), 0: end, -1: error.
+    int next(std::string& header, SequenceSink& sink, uint64_t& seq_len, ParseStatus& st) {
+        std::string_view h;
+        const int got = header_line(h, st);
+        if (got <= 0) return got;
+        const char tag = h.front();
+        if (format_ == 0) {
+            if (tag == '>') {
+                format_ = 1;
+            } else if (tag == '@') {
+                format_ = 2;
+            } else {
+                return fail(st, "expected FASTA or FASTQ header", column_at(h, 0));
+            }
+        }
+        if (tag != (format_ == 1 ? '>' : '@')) {
+            return fail(st, "mixed FASTA and FASTQ records are not supported", column_at(h, 0));
+        }
+        header.assign(h.data() + 1, h.size() - 1);
+        seq_len = 0;
+        return format_ == 1 ? fasta_sequence(sink, seq_len, st) : fastq_sequence(sink, seq_len, st);
+    }
+
+   private:
+    int fail(ParseStatus& st, const char* detail, uint32_t column) {
+        st.code = CUSBF_ERR_FASTX_PARSE;
+        st.line = static_cast<uint32_t>(line_no_);
+        st.column = column;
+        st.detail = detail;
+        return -1;
+    }
+    int io_fail(ParseStatus& st) {
+        st.code = CUSBF_ERR_IO;
+        return -1;
+    }
+    // 1 line, 0 end, -1 io error; counts lines like the reference's getline loop
+    int line(std::string_view& out, ParseStatus& st) {
+        const int r = lines_.next(out);
+        if (r < 0) return io_fail(st);
+        if (r > 0) {
+            ++line_no_;
+            last_ = out;
+        }
+        return r;
+    }
+    int header_line(std::string_view& h, ParseStatus& st) {
+        if (has_pending_) {
+            has_pending_ = false;
+            header_buf_.swap(pending_);
+            h = header_buf_;
+            return 1;
+        }
+        std::string_view l;
+        for (;;) {
+            const int r = line(l, st);
+            if (r <= 0) return r;
+            if (!l.empty()) {
+                header_buf_.assign(l.data(), l.size());
+                h = header_buf_;
+                return 1;
+            }
+        }
+    }
+    int fasta_sequence(SequenceSink& sink, uint64_t& len, ParseStatus& st) {
+        std::string_view l;
+        for (;;) {
+            const int r = line(l, st);
+            if (r < 0) return -1;
+            if (r == 0) return 1;
+            if (!l.empty() && l.front() == '>') {
+                pending_.assign(l.data(), l.size());
+                has_pending_ = true;
+                return 1;
+            }
+            sink.append(l);
+            len += l.size();
+        }
+    }
+    int fastq_sequence(SequenceSink& sink, uint64_t& len, ParseStatus& st) {
+        std::string_view l;
+        for (;;) {
+            const int r = line(l, st);
+            if (r < 0) return -1;
+            if (r == 0) {
+                // what getline leaves in its buffer: the last line only when the
+                // input ended without a newline
+                const std::string_view buf = lines_.unterminated() ? last_ : std::string_view();
+                return fail(st, "unterminated FASTQ record: missing '+' separator",
+                            column_at(buf, buf.size() > 0 ? buf.size() - 1 : 0));
+            }
+            if (!l.empty() && l.front() == '+') return fastq_qualities(len, st);
+            sink.append(l);
+            len += l.size();
+        }
+    }
+    // Fastx.hpp:436-460 with the columns of :270-287
+    int fastq_qualities(uint64_t expected, ParseStatus& st) {
+        uint64_t got = 0;
+        uint32_t short_column = 1;
+        std::string_view l;
+        while (got < expected) {
+            const int r = line(l, st);
+            if (r < 0) return -1;
+            if (r == 0) break;
+            short_column = static_cast<uint32_t>(l.empty() ? 1 : l.size() + 1);
+            got += l.size();
+            if (got > expected) {
+                const uint64_t before = got - l.size();
+                const uint64_t idx = expected > before ? expected - before : l.size();
+                return fail(st, "FASTQ quality length exceeds sequence length", column_at(l, idx));
+            }
+        }
+        if (got == expected) return 1;
+        return fail(st, "FASTQ quality length does not match sequence length", short_column);
+    }
+
+    LineReader lines_;
+    int format_ = 0;  // 0 unknown, 1 FASTA, 2 FASTQ
+    uint64_t line_no_ = 0;
+    std::string pending_;
+    bool has_pending_ = false;
+    std::string header_buf_;
+    std::string_view last_;
+};
+
+// --- chunk pipeline -------------------------------------------------------------------
+
+struct RecordMeta {
+    uint64_t index;
+    uint64_t header_at, header_len;
+    uint64_t byte_at, bytes;  // in the chunk's staging
+};
+
+struct Chunk {
+    char* h_seq = nullptr;
+    uint64_t used = 0;  // bytes
+    uint64_t* h_off = nullptr;     // records + 1 symbol offsets
+    uint64_t* h_req = nullptr;     // per record: first window, window count
+    uint64_t* h_counts = nullptr;  // per record: valid windows, positive windows
+    uint8_t* h_hits = nullptr;
+    hipEvent_t done = nullptr;
+    bool in_flight = false;
+    std::vector<RecordMeta> records;
+    std::string headers;
+};
+
+constexpr uint64_t kSlackBytes = 64ull << 20;
+constexpr uint64_t kDefaultMaxChunk = 512ull << 20;
+constexpr uint64_t kMinStaging = 1ull << 20;
+constexpr uint64_t kBytesPerRecordSlot = 32;  // staging bytes per record-table entry
+
+class Pipeline final : public SequenceSink {
+   public:
+    Pipeline(cusbf_filter* f, bool query, hipStream_t stream, cusbf_fastx_consumer consumer,
+             void* user, std::string source)
+        : f_(f), query_(query), stream_(stream), consumer_(consumer), user_(user),
+          source_(std::move(source)), width_(ci::symbol_width(f)), k_(ci::kmer_length(f)),
+          sep_(static_cast<char>(ci::separator(f))) {}
+
+    ~Pipeline() override {
+        (void)hipStreamSynchronize(stream_);
+        free_buffers();
+        for (Chunk& c : chunks_) {
+            if (c.done) (void)hipEventDestroy(c.done);
+        }
+    }
+
+    // Flush threshold from fill_fraction (or the filter's chunk override), then
+    // both staging sets and the device buffers.
+    int init(double fill_fraction) {
+        uint64_t target = ci::fastx_chunk_bytes(f_);
+        if (target == 0) {
+            size_t free_b = 0, total_b = 0;
+            const hipError_t e = hipMemGetInfo(&free_b, &total_b);
+            if (e != hipSuccess) return ci::set_hip_error(e, __FILE__, __LINE__);
+            const double avail = free_b > kSlackBytes ? static_cast<double>(free_b - kSlackBytes) : 0.0;
+            const uint64_t budget = fill_fraction > 0.0 ? static_cast<uint64_t>(avail * fill_fraction) : 0;
+            // per chunk byte: sequence + start bitmap + record table (+ hits)
+            const double per_byte = 1.0 + 0.125 + 40.0 / kBytesPerRecordSlot + (query_ ? 1.0 : 0.0);
+            target = std::min(static_cast<uint64_t>(static_cast<double>(budget) / per_byte), kDefaultMaxChunk);
+        }
+        target_ = target;
+        for (Chunk& c : chunks_) {
+            const hipError_t e = hipEventCreateWithFlags(&c.done, hipEventDisableTiming);
+            if (e != hipSuccess) return ci::set_hip_error(e, __FILE__, __LINE__);
+        }
+        return allocate(std::max(target_, kMinStaging));
+    }
+
+    int run(Source& src) {
+        Parser parser(src);
+        std::string header;
+        ParseStatus st;
+        for (uint64_t index = 0;; ++index) {
+            Chunk& c = chunks_[cur_];
+            rec_at_ = c.used;
+            spilled_ = false;
+            spill_.clear();
+            uint64_t len = 0;
+            const int got = parser.next(header, *this, len, st);
+            if (got < 0) {
+                const int rc = drain();
+                if (rc != CUSBF_OK) return rc;
+                if (st.code == CUSBF_ERR_IO) {
+                    return ci::set_error(CUSBF_ERR_IO,
+                                         ("Failed to read FASTA/FASTQ input from " + source_).c_str());
+                }
+                return ci::set_fastx_error(source_.c_str(), st.line, st.column, st.detail);
+            }
+            if (got == 0) break;
+            const int rc = close_record(index, header, len);
+            if (rc != CUSBF_OK) return rc;
+            if (stopped_) break;
+        }
+        int rc = stopped_ ? CUSBF_OK : flush();
+        if (rc == CUSBF_OK) rc = drain();
+        if (rc == CUSBF_OK && stopped_) {
+            return ci::set_error(CUSBF_ERR_IO, "FASTX record consumer stopped the query");
+        }
+        return rc;
+    }
+
+    // Sequence bytes of the record being parsed go to the current chunk while
+    // they fit, otherwise to the spill string (moved to the next chunk when the
+    // record ends).
+    void append(std::string_view piece) override {
+        Chunk& c = chunks_[cur_];
+        if (!spilled_ && c.used + piece.size() <= alloc_) {
+            std::memcpy(c.h_seq + c.used, piece.data(), piece.size());
+            c.used += piece.size();
+            return;
+        }
+        if (!spilled_) {
+            spill_.assign(c.h_seq + rec_at_, c.used - rec_at_);
+            c.used = rec_at_;
+            spilled_ = true;
+        }
+        spill_.append(piece.data(), piece.size());
+    }
+
+    cusbf_insert_report insert_report() const {
+        return cusbf_insert_report{records_, bases_, valid_};
+    }
+    cusbf_query_report query_report() const {
+        return cusbf_query_report{records_, bases_, valid_, positive_};
+    }
+
+   private:
+    uint64_t padded(uint64_t bytes) const {
+        const uint64_t rem = bytes % width_;
+        return rem ? bytes + (width_ - rem) : bytes;
+    }
+
+    int close_record(uint64_t index, const std::string& header, uint64_t len) {
+        const uint64_t need = padded(len);
+        Chunk* c = &chunks_[cur_];
+        const bool fits = !spilled_ && rec_at_ + need <= alloc_ && c->records.size() < max_records_;
+        const bool full = !c->records.empty() && rec_at_ + need > target_;
+        if (!fits || full) {
+            // the record starts the next chunk
+            if (!spilled_) {
+                spill_.assign(c->h_seq + rec_at_, c->used - rec_at_);
+                c->used = rec_at_;
+            }
+            int rc = flush();
+            if (rc == CUSBF_OK) rc = take_next();
+            if (rc == CUSBF_OK && need > alloc_) {
+                // a record larger than the staging: finish the other chunk, then grow
+                rc = finish(chunks_[cur_ ^ 1]);
+                if (rc == CUSBF_OK) rc = allocate(need);
+            }
+            if (rc != CUSBF_OK) return rc;
+            c = &chunks_[cur_];
+            std::memcpy(c->h_seq, spill_.data(), spill_.size());
+            c->used = spill_.size();
+            rec_at_ = 0;
+        }
+        if (need > len) {
+            std::memset(c->h_seq + c->used, sep_, need - len);
+            c->used += need - len;
+        }
+        RecordMeta m{index, 0, 0, rec_at_, len};
+        if (query_ && consumer_) {
+            m.header_at = c->headers.size();
+            m.header_len = header.size();
+            c->headers.append(header);
+        }
+        c->records.push_back(m);
+        ++records_;
+        bases_ += len;
+        return CUSBF_OK;
+    }
+
+    // (Re)allocates both staging sets and the device buffers for `bytes` of
+    // sequence per chunk; both chunks must be idle.
+    int allocate(uint64_t bytes) {
+        (void)hipStreamSynchronize(stream_);
+        free_buffers();
+        alloc_ = (bytes + 63) / 64 * 64;
+        max_records_ = std::max<uint64_t>(1024, alloc_ / kBytesPerRecordSlot);
+        hipError_t e = hipSuccess;
+        for (Chunk& c : chunks_) {
+            if (e == hipSuccess) e = hipHostMalloc(&c.h_seq, alloc_, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipHostMalloc(&c.h_off, (max_records_ + 1) * 8, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipHostMalloc(&c.h_req, 2 * max_records_ * 8, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipHostMalloc(&c.h_counts, 2 * max_records_ * 8, hipHostMallocDefault);
+            if (e == hipSuccess && query_) e = hipHostMalloc(&c.h_hits, alloc_, hipHostMallocDefault);
+        }
+        if (e == hipSuccess) e = hipMalloc(&d_seq_, alloc_);
+        if (e == hipSuccess) e = hipMalloc(&d_off_, (max_records_ + 1) * 8);
+        if (e == hipSuccess) e = hipMalloc(&d_req_, 2 * max_records_ * 8);
+        if (e == hipSuccess) e = hipMalloc(&d_counts_, 2 * max_records_ * 8);
+        if (e == hipSuccess && query_) e = hipMalloc(&d_hits_, alloc_);
+        if (e != hipSuccess) return ci::set_hip_error(e, __FILE__, __LINE__);
+        return CUSBF_OK;
+    }
+
+    void free_buffers() {
+        for (Chunk& c : chunks_) {
+            if (c.h_seq) (void)hipHostFree(c.h_seq);
+            if (c.h_off) (void)hipHostFree(c.h_off);
+            if (c.h_req) (void)hipHostFree(c.h_req);
+            if (c.h_counts) (void)hipHostFree(c.h_counts);
+            if (c.h_hits) (void)hipHostFree(c.h_hits);
+            c.h_seq = nullptr;
+            c.h_off = c.h_req = c.h_counts = nullptr;
+            c.h_hits = nullptr;
+        }
+        if (d_seq_) (void)hipFree(d_seq_);
+        if (d_off_) (void)hipFree(d_off_);
+        if (d_req_) (void)hipFree(d_req_);
+        if (d_counts_) (void)hipFree(d_counts_);
+        if (d_hits_) (void)hipFree(d_hits_);
+        d_seq_ = nullptr;
+        d_off_ = d_req_ = d_counts_ = nullptr;
+        d_hits_ = nullptr;
+    }
+
+    // Enqueues the current chunk's device work (nothing when it holds no record).
+    int flush() {
+        Chunk& c = chunks_[cur_];
+        const uint64_t n = c.records.size();
+        if (n == 0) return CUSBF_OK;
+        const uint64_t symbols = c.used / width_;
+        for (uint64_t r = 0; r < n; ++r) {
+            const RecordMeta& m = c.records[r];
+            const uint64_t first = m.byte_at / width_;
+            const uint64_t syms = m.bytes / width_;
+            c.h_off[r] = first;
+            c.h_req[2 * r] = first;
+            c.h_req[2 * r + 1] = syms >= k_ ? syms - k_ + 1 : 0;
+        }
+        c.h_off[n] = symbols;
+        hipError_t e = hipMemcpyAsync(d_seq_, c.h_seq, c.used, hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_off_, c.h_off, (n + 1) * 8, hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_req_, c.h_req, 2 * n * 8, hipMemcpyHostToDevice, stream_);
+        if (e != hipSuccess) return ci::set_hip_error(e, __FILE__, __LINE__);
+        const uint64_t windows = symbols >= k_ ? symbols - k_ + 1 : 0;
+        int rc = query_ ? cusbf_contains_batched_async(f_, d_seq_, symbols, d_off_, n, d_hits_, alloc_, stream_)
+                        : cusbf_insert_batched_async(f_, d_seq_, symbols, d_off_, n, stream_);
+        if (rc != CUSBF_OK) return rc;
+        rc = ci::launch_request_counts(f_, d_seq_, c.used, query_ ? d_hits_ : nullptr, d_req_, n,
+                                       d_counts_, stream_);
+        if (rc != CUSBF_OK) return rc;
+        e = hipMemcpyAsync(c.h_counts, d_counts_, 2 * n * 8, hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess && query_ && windows) {
+            e = hipMemcpyAsync(c.h_hits, d_hits_, windows, hipMemcpyDeviceToHost, stream_);
+        }
+        if (e == hipSuccess) e = hipEventRecord(c.done, stream_);
+        if (e != hipSuccess) return ci::set_hip_error(e, __FILE__, __LINE__);
+        c.in_flight = true;
+        return CUSBF_OK;
+    }
+
+    // Switches to the other staging set, finishing the chunk that used it.
+    int take_next() {
+        cur_ ^= 1;
+        return finish(chunks_[cur_]);
+    }
+
+    // Waits for a chunk's device work, adds its counts to the report and hands
+    // its records to the consumer in source order; leaves the chunk empty.
+    int finish(Chunk& c) {
+        if (c.in_flight) {
+            const hipError_t e = hipEventSynchronize(c.done);
+            if (e != hipSuccess) return ci::set_hip_error(e, __FILE__, __LINE__);
+            c.in_flight = false;
+            for (size_t r = 0; r < c.records.size(); ++r) {
+                const RecordMeta& m = c.records[r];
+                valid_ += c.h_counts[2 * r];
+                positive_ += c.h_counts[2 * r + 1];
+                if (query_ && consumer_) {
+                    const uint64_t syms = m.bytes / width_;
+                    const uint64_t hits = syms >= k_ ? syms - k_ + 1 : 0;
+                    cusbf_fastx_record_view v{};
+                    v.record_index = m.index;
+                    v.header = c.headers.data() + m.header_at;
+                    v.header_len = m.header_len;
+                    v.sequence = c.h_seq + m.byte_at;
+                    v.sequence_len = m.bytes;
+                    v.queried_bases = m.bytes;
+                    v.queried_kmers = c.h_counts[2 * r];
+                    v.positive_kmers = c.h_counts[2 * r + 1];
+                    v.hits = hits ? c.h_hits + m.byte_at / width_ : nullptr;
+                    v.hits_len = hits;
+                    if (!stopped_ && consumer_(&v, user_) != 0) stopped_ = true;
+                }
+            }
+        }
+        c.records.clear();
+        c.headers.clear();
+        c.used = 0;
+        return CUSBF_OK;
+    }
+
+    // Finishes both chunks, oldest first.
+    int drain() {
+        int rc = finish(chunks_[cur_ ^ 1]);
+        if (rc == CUSBF_OK) rc = finish(chunks_[cur_]);
+        return rc;
+    }
+
+    cusbf_filter* f_;
+    bool query_;
+    hipStream_t stream_;
+    cusbf_fastx_consumer consumer_;
+    void* user_;
+    std::string source_;
+    uint32_t width_;
+    uint32_t k_;
+    char sep_;
+    uint64_t target_ = 0, alloc_ = 0, max_records_ = 0;
+    bool stopped_ = false;
+    Chunk chunks_[2];
+    char* d_seq_ = nullptr;  // device side of the chunk in flight (one stream orders them)
+    uint64_t* d_off_ = nullptr;
+    uint64_t* d_req_ = nullptr;
+    uint64_t* d_counts_ = nullptr;
+    uint8_t* d_hits_ = nullptr;
+    int cur_ = 0;
+    uint64_t rec_at_ = 0;
+    bool spilled_ = false;
+    std::string spill_;
+    uint64_t records_ = 0, bases_ = 0, valid_ = 0, positive_ = 0;
+
+   public:
+    bool stopped() const { return stopped_; }
+};
+
+int run_pipeline(cusbf_filter* f, bool query, Source& src, const char* source_name,
+                 double fill_fraction, void* stream, cusbf_fastx_consumer consumer, void* user,
+                 cusbf_insert_report* ins, cusbf_query_report* qry) {
+    Pipeline p(f, query, reinterpret_cast<hipStream_t>(stream), consumer, user, source_name);
+    int rc = p.init(fill_fraction);
+    if (rc == CUSBF_OK) rc = p.run(src);
+    if (rc == CUSBF_OK) {
+        if (ins) *ins = p.insert_report();
+        if (qry) *qry = p.query_report();
+    }
+    return rc;
+}
+
+// Opens `path` as a source: gzip stream or mmap'd file.
+int open_file(const char* path, std::unique_ptr<Source>& out) {
+    if (!path) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null path");
+    if (is_gzip(path)) {
+        auto gz = std::make_unique<GzipSource>();
+        if (!gz->open(path)) {
+            return ci::set_error(CUSBF_ERR_IO, (std::string("Failed to open FASTA/FASTQ file: ") + path).c_str());
+        }
+        out = std::move(gz);
+        return CUSBF_OK;
+    }
+    auto m = std::make_unique<MappedFile>();
+    if (const char* step = m->open(path)) {
+        return ci::set_error(CUSBF_ERR_IO,
+                             (std::string("Failed to ") + step + " FASTA/FASTQ file: " + path).c_str());
+    }
+    out = std::move(m);
+    return CUSBF_OK;
+}
+
+class ParseOnly final : public SequenceSink {
+   public:
+    void append(std::string_view piece) override { seq.append(piece.data(), piece.size()); }
+    std::string seq;
+};
+
+}  // namespace
+
+extern "C" {
+
+int cusbf_insert_fastx_buffer(cusbf_filter* f, const char* text, uint64_t bytes,
+                              const char* source_name, double fill_fraction, void* stream,
+                              cusbf_insert_report* report) {
+    if (!f || (!text && bytes)) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    MemorySource src(text, bytes);
+    return run_pipeline(f, false, src, source_name ? source_name : "<stream>", fill_fraction, stream,
+                        nullptr, nullptr, report, nullptr);
+}
+
+int cusbf_insert_fastx_reader(cusbf_filter* f, cusbf_read_fn read, void* user,
+                              const char* source_name, double fill_fraction, void* stream,
+                              cusbf_insert_report* report) {
+    if (!f || !read) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    CallbackSource src(read, user);
+    return run_pipeline(f, false, src, source_name ? source_name : "<stream>", fill_fraction, stream,
+                        nullptr, nullptr, report, nullptr);
+}
+
+int cusbf_insert_fastx_file(cusbf_filter* f, const char* path, double fill_fraction, void* stream,
+                            cusbf_insert_report* report) {
+    if (!f) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    std::unique_ptr<Source> src;
+    const int rc = open_file(path, src);
+    if (rc != CUSBF_OK) return rc;
+    return run_pipeline(f, false, *src, path, fill_fraction, stream, nullptr, nullptr, report, nullptr);
+}
+
+int cusbf_query_fastx_buffer(const cusbf_filter* f, const char* text, uint64_t bytes,
+                             const char* source_name, double fill_fraction, void* stream,
+                             cusbf_fastx_consumer consumer, void* user, cusbf_query_report* report) {
+    if (!f || (!text && bytes)) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    MemorySource src(text, bytes);
+    return run_pipeline(const_cast<cusbf_filter*>(f), true, src, source_name ? source_name : "<stream>",
+                        fill_fraction, stream, consumer, user, nullptr, report);
+}
+
+int cusbf_query_fastx_reader(const cusbf_filter* f, cusbf_read_fn read, void* user_read,
+                             const char* source_name, double fill_fraction, void* stream,
+                             cusbf_fastx_consumer consumer, void* user, cusbf_query_report* report) {
+    if (!f || !read) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    CallbackSource src(read, user_read);
+    return run_pipeline(const_cast<cusbf_filter*>(f), true, src, source_name ? source_name : "<stream>",
+                        fill_fraction, stream, consumer, user, nullptr, report);
+}
+
+int cusbf_query_fastx_file(const cusbf_filter* f, const char* path, double fill_fraction,
+                           void* stream, cusbf_fastx_consumer consumer, void* user,
+                           cusbf_query_report* report) {
+    if (!f) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    std::unique_ptr<Source> src;
+    const int rc = open_file(path, src);
+    if (rc != CUSBF_OK) return rc;
+    return run_pipeline(const_cast<cusbf_filter*>(f), true, *src, path, fill_fraction, stream, consumer,
+                        user, nullptr, report);
+}
+
+int cusbf_parse_fastx_buffer(const char* text, uint64_t bytes, const char* source_name,
+                             cusbf_fastx_consumer consumer, void* user, uint64_t* records) {
+    if (!text && bytes) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    MemorySource src(text, bytes);
+    Parser parser(src);
+    const std::string name = source_name ? source_name : "<stream>";
+    std::string header;
+    ParseOnly sink;
+    ParseStatus st;
+    uint64_t n = 0;
+    for (;;) {
+        sink.seq.clear();
+        uint64_t len = 0;
+        const int got = parser.next(header, sink, len, st);
+        if (got < 0) {
+            if (records) *records = n;
+            if (st.code == CUSBF_ERR_IO) {
+                return ci::set_error(CUSBF_ERR_IO, ("Failed to read FASTA/FASTQ input from " + name).c_str());
+            }
+            return ci::set_fastx_error(name.c_str(), st.line, st.column, st.detail);
+        }
+        if (got == 0) break;
+        if (consumer) {
+            cusbf_fastx_record_view v{};
+            v.record_index = n;
+            v.header = header.data();
+            v.header_len = header.size();
+            v.sequence = sink.seq.data();
+            v.sequence_len = sink.seq.size();
+            v.queried_bases = sink.seq.size();
+            if (consumer(&v, user) != 0) {
+                if (records) *records = n + 1;
+                return ci::set_error(CUSBF_ERR_IO, "FASTX record consumer stopped the query");
+            }
+        }
+        ++n;
+    }
+    if (records) *records = n;
+    return CUSBF_OK;
+}
+
+}  // extern "C"

@@ -33,4 +33,10 @@ int launch_request_counts(const cusbf_filter* f, const char* d_seq, uint64_t byt
                           const uint8_t* d_hits, const uint64_t* d_req, uint64_t n_req,
                           uint64_t* d_counts, hipStream_t stream);
 
+uint32_t kmer_length(const cusbf_filter* f);
+// FASTX chunk capacity override in bytes (0 = from fill_fraction).
+uint64_t fastx_chunk_bytes(const cusbf_filter* f);
+// Error::fastx_parse (error.hpp:123-129): "source:line:column: detail", location kept.
+int set_fastx_error(const char* source, uint32_t line, uint32_t column, const char* detail);
+
 }  // namespace cusbf_internal

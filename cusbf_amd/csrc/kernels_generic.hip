@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void count_kernel(KernelConfig c, SequenceI
 // Report counts of a server batch (filter.cuh:951-990 FastxQueryReport, done on
 // the device): request r owns windows [req[2r], req[2r] + req[2r+1]) of the
 // batch; counts[2r] += its valid windows (no invalid symbol among the k),
-// counts[2r+1] += its hit bytes equal to 1. A thread takes 16 consecutive
+// counts[2r+1] += its hit bytes equal to 1 (none without hits). A thread takes 16 consecutive
 // windows: the invalid-symbol mask of their 16 + k - 1 symbols (128-bit loads;
 // SWAR test for DNA), the hit bytes with one 128-bit load, then the counts go
 // to the owning request -- summed over the wave first when the whole wave falls
@@ -303,7 +303,9 @@ __global__ __launch_bounds__(kBlock) void request_counts_kernel(KernelConfig c, 
                 const uint64_t w = o == 0 ? m0 : ((m0 >> o) | (m1 << (64 - o)));
                 valid |= ((w & kmask) == 0 ? 1u : 0u) << o;
             }
-            if (n == 16 && (reinterpret_cast<uintptr_t>(hits + p0) & 15u) == 0) {
+            if (hits == nullptr) {
+                // insert reports: valid windows only
+            } else if (n == 16 && (reinterpret_cast<uintptr_t>(hits + p0) & 15u) == 0) {
                 const u32x4 h = *reinterpret_cast<const u32x4*>(hits + p0);
                 const uint32_t hw[4] = {h.x, h.y, h.z, h.w};
 #pragma unroll

@@ -23,11 +23,25 @@ class ErrorCategory(enum.IntEnum):
     resource = 4
 
 
+@dataclass(frozen=True)
+class SourceLocation:
+    """error.hpp:20-41: file, 1-based line and column of a FASTX parse error."""
+
+    file: str
+    line: int
+    column: int
+
+    def to_string(self) -> str:
+        return f"{self.file}:{self.line}:{self.column}"
+
+
 class CusbfError(RuntimeError):
-    def __init__(self, category: ErrorCategory, message: str):
+    def __init__(self, category: ErrorCategory, message: str,
+                 location: Optional[SourceLocation] = None):
         super().__init__(message)
         self.category = ErrorCategory(category)
         self.message = message
+        self.location = location
 
     def __repr__(self) -> str:
         return f"CusbfError({self.category.name}, {self.message!r})"
@@ -70,6 +84,13 @@ def check(lib, status: int) -> None:
         return
     import ctypes
 
-    buf = ctypes.create_string_buffer(1024)
+    buf = ctypes.create_string_buffer(4096)
     lib.cusbf_last_error(buf, len(buf))
-    raise CusbfError(ErrorCategory(status - 1), buf.value.decode(errors="replace"))
+    category = ErrorCategory(status - 1)
+    location = None
+    if category == ErrorCategory.fastx_parse:
+        fbuf = ctypes.create_string_buffer(4096)
+        line, col = ctypes.c_uint32(), ctypes.c_uint32()
+        lib.cusbf_last_error_location(fbuf, len(fbuf), ctypes.byref(line), ctypes.byref(col))
+        location = SourceLocation(fbuf.value.decode(errors="replace"), line.value, col.value)
+    raise CusbfError(category, buf.value.decode(errors="replace"), location)

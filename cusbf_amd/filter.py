@@ -311,6 +311,39 @@ class Filter:
             self._h, dev_ptr(d_sequence), n, arr, nr, dev_ptr(d_output), on,
             stream_handle(stream)))
 
+    # FASTA / FASTQ (filter.cuh:402-453, 620-800); see cusbf_amd.fastx ------------
+    def insert_fastx(self, source, fill_fraction: float = 0.7, stream=None):
+        from . import fastx
+        return fastx.insert_fastx(self, source, fill_fraction, stream)
+
+    def insert_fastx_file(self, path, fill_fraction: float = 0.7, stream=None):
+        from . import fastx
+        return fastx.insert_fastx_file(self, path, fill_fraction, stream)
+
+    def query_fastx(self, source, fill_fraction: float = 0.7, stream=None):
+        from . import fastx
+        return fastx.query_fastx(self, source, fill_fraction, stream)
+
+    def query_fastx_file(self, path, fill_fraction: float = 0.7, stream=None):
+        from . import fastx
+        return fastx.query_fastx_file(self, path, fill_fraction, stream)
+
+    def query_fastx_records(self, source, consumer, fill_fraction: float = 0.7, stream=None):
+        from . import fastx
+        return fastx.query_fastx_records(self, source, consumer, fill_fraction, stream)
+
+    def query_fastx_file_records(self, path, consumer, fill_fraction: float = 0.7, stream=None):
+        from . import fastx
+        return fastx.query_fastx_file_records(self, path, consumer, fill_fraction, stream)
+
+    def query_fastx_detailed(self, source, fill_fraction: float = 0.7, stream=None):
+        from . import fastx
+        return fastx.query_fastx_detailed(self, source, fill_fraction, stream)
+
+    def query_fastx_file_detailed(self, path, fill_fraction: float = 0.7, stream=None):
+        from . import fastx
+        return fastx.query_fastx_file_detailed(self, path, fill_fraction, stream)
+
     # diagnostics -----------------------------------------------------------------
     def count_windows_runs(self, d_sequence, size: int | None = None) -> Tuple[int, int]:
         """(valid windows, shard runs) of a device sequence (byte model of the roofline)."""

@@ -308,6 +308,74 @@ int cusbf_count_batched_windows_runs_async(const cusbf_filter* f, const char* d_
                                            uint64_t num_records, uint64_t* d_counts,
                                            void* stream);
 
+/* --- FASTA / FASTQ ingest (filter.cuh:402-470, 620-800; Fastx.hpp) ------------ */
+/*
+ * Parses FASTA or FASTQ text (the first header decides; CRLF line ends and
+ * wrapped sequence / quality lines accepted; Fastx.hpp:240-460) and inserts or
+ * queries every record, chunk by chunk: a chunk is flushed before it would pass
+ * fill_fraction of free device memory (minus 64 MiB) spread over its buffers,
+ * or the filter's chunk override (0 = one record per chunk); a record larger
+ * than that is processed in a chunk of its own. Parse errors are CUSBF_ERR_FASTX_PARSE with the message
+ * "source:line:column: detail" (cusbf_last_error_location gives the parts);
+ * chunks before the error have been applied. Reports follow FastxInsertReport /
+ * FastxQueryReport: records, sequence bytes, valid windows (no invalid symbol,
+ * not across records), positive windows. Windows never span two records. The
+ * sources: a host buffer ("<stream>" when source_name is NULL), a read callback,
+ * or a file path (gzip when it starts with the gzip magic bytes, else mmap'd).
+ */
+typedef struct cusbf_fastx_record_view {
+    uint64_t record_index;  /* in the input */
+    const char* header;     /* without the leading '>' or '@' */
+    uint64_t header_len;
+    const char* sequence;
+    uint64_t sequence_len;
+    uint64_t queried_bases;
+    uint64_t queried_kmers;  /* valid windows */
+    uint64_t positive_kmers;
+    const uint8_t* hits;     /* one byte per window (1 present), valid during the call */
+    uint64_t hits_len;
+} cusbf_fastx_record_view;
+
+/* FastxRecordConsumer (Fastx.hpp:230-238): called once per record, in input
+ * order; returning non-zero stops the call (it then fails with CUSBF_ERR_IO,
+ * "FASTX record consumer stopped the query"; the filter stays usable). */
+typedef int (*cusbf_fastx_consumer)(const cusbf_fastx_record_view* record, void* user);
+/* Reads up to `capacity` bytes into dst: the count, 0 at the end, -1 on error. */
+typedef int64_t (*cusbf_read_fn)(void* user, char* dst, uint64_t capacity);
+
+/* insert_fastx(std::istream&) (filter.cuh:416-422) over a host buffer / a reader. */
+int cusbf_insert_fastx_buffer(cusbf_filter* f, const char* text, uint64_t bytes,
+                              const char* source_name, double fill_fraction, void* stream,
+                              cusbf_insert_report* report);
+int cusbf_insert_fastx_reader(cusbf_filter* f, cusbf_read_fn read, void* user,
+                              const char* source_name, double fill_fraction, void* stream,
+                              cusbf_insert_report* report);
+/* insert_fastx_file (filter.cuh:437-453). */
+int cusbf_insert_fastx_file(cusbf_filter* f, const char* path, double fill_fraction, void* stream,
+                            cusbf_insert_report* report);
+/* query_fastx / query_fastx_records / query_fastx_detailed (filter.cuh:632-760):
+ * consumer may be NULL (aggregate report only). */
+int cusbf_query_fastx_buffer(const cusbf_filter* f, const char* text, uint64_t bytes,
+                             const char* source_name, double fill_fraction, void* stream,
+                             cusbf_fastx_consumer consumer, void* user, cusbf_query_report* report);
+int cusbf_query_fastx_reader(const cusbf_filter* f, cusbf_read_fn read, void* user_read,
+                             const char* source_name, double fill_fraction, void* stream,
+                             cusbf_fastx_consumer consumer, void* user, cusbf_query_report* report);
+/* query_fastx_file / _records / _detailed (filter.cuh:652-800). */
+int cusbf_query_fastx_file(const cusbf_filter* f, const char* path, double fill_fraction,
+                           void* stream, cusbf_fastx_consumer consumer, void* user,
+                           cusbf_query_report* report);
+/* FastxReader alone (Fastx.hpp:302-460): records to the consumer (counts 0, no
+ * hits); no device work. */
+int cusbf_parse_fastx_buffer(const char* text, uint64_t bytes, const char* source_name,
+                             cusbf_fastx_consumer consumer, void* user, uint64_t* records);
+/* No reference counterpart: fixed FASTX chunk capacity in bytes (0 = from fill_fraction). */
+int cusbf_fastx_set_chunk_bytes(cusbf_filter* f, uint64_t bytes);
+/* SourceLocation of the last error on this thread when it is a FASTX parse error
+ * (error.hpp:20-41, 123-129): file, 1-based line and column (empty / 0 otherwise).
+ * Returns the status of that error. */
+int cusbf_last_error_location(char* file, size_t file_len, uint32_t* line, uint32_t* column);
+
 /* --- request server (no reference counterpart) ------------------------------- */
 /*
  * Serving form of the query path: the filter stays resident; a scheduler

@@ -1,0 +1,147 @@
+"""FASTX parsing on the host: the library's native parser against the oracle.
+
+The oracle (oracle/fastx_oracle.py, a line-by-line restatement of the
+reference's streaming reader) is pinned by the reference's own FASTX test
+inputs (tests/golden/reference_fastx_cases.json, from tests/test_fastx.cu) and
+hand-derived cases for every error path; the library's parser must then agree
+with it record for record, or error for error (line, column, detail), on those
+cases and on seeded random texts with wrapping, CRLF, blank lines and defects.
+No device work: cusbf_parse_fastx_buffer only parses.
+"""
+from __future__ import annotations
+
+import json
+import os
+import random
+
+import pytest
+
+from oracle import fastx_oracle as fo
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = json.load(open(os.path.join(HERE, "golden", "reference_fastx_cases.json")))["cases"]
+
+
+def lib_parse(text: bytes):
+    """(records, None) or (None, (line, column, detail, message)) from the library."""
+    from cusbf_amd import fastx
+    from cusbf_amd.errors import CusbfError, ErrorCategory
+
+    try:
+        recs = fastx.parse_fastx(text)
+        return [(h.encode("latin-1"), s) for h, s in recs], None
+    except CusbfError as e:
+        assert e.category == ErrorCategory.fastx_parse, e
+        detail = e.message.split(": ", 1)[1]
+        return None, (e.location.line, e.location.column, detail, e.message, e.location.file)
+
+
+def oracle_parse(text: bytes):
+    recs, err = fo.parse_or_error(text)
+    if err is not None:
+        return None, (err.line, err.column, err.detail, str(err), err.source)
+    return [(r.header, r.sequence) for r in recs], None
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c["name"])
+def test_reference_cases(case):
+    text = case["text"].encode()
+    o_recs, o_err = oracle_parse(text)
+    l_recs, l_err = lib_parse(text)
+    if "error" in case:
+        want = case["error"]
+        assert o_err[:3] == (want["line"], want["column"], want["detail"])
+        assert l_err == o_err
+        assert l_err[3] == f"<stream>:{want['line']}:{want['column']}: {want['detail']}"
+    else:
+        want = [(h.encode(), s.encode()) for h, s in case["records"]]
+        assert o_recs == want
+        assert l_recs == want
+
+
+# Hand-derived cases for the paths the reference's tests leave open (each
+# expectation follows from Fastx.hpp's rules, worked out by hand).
+MANUAL = [
+    (b"", [], None),
+    (b"\n\n\r\n", [], None),
+    (b">a\n", [(b"a", b"")], None),                                   # header only
+    (b">a", [(b"a", b"")], None),                                      # no newline at all
+    (b">a\nAC\n\n>b\r\nGG\r\r\n", [(b"a", b"AC"), (b"b", b"GG\r")], None),  # one '\r' trimmed
+    (b"\n\n>x\nA\nC", [(b"x", b"AC")], None),
+    (b"@q\nACGT\n+\n@@++\n@r\nA\n+x\nI\n", [(b"q", b"ACGT"), (b"r", b"A")], None),  # '@'/'+' qualities
+    (b"@q\n\n+\n\n@r\n\n+\n", [(b"q", b""), (b"r", b"")], None),      # empty FASTQ records
+    (b"@q\nAC\nGT\n+\nII\nII\n", [(b"q", b"ACGT")], None),
+    (b"ACGT\n", None, (1, 1, "expected FASTA or FASTQ header")),
+    (b"\n\n  >a\n", None, (3, 1, "expected FASTA or FASTQ header")),
+    (b">a\nAC\n@b\nGG\n", [(b"a", b"AC@bGG")], None),                 # '@' inside FASTA is sequence
+    (b"@a\nAC\n+\nII\n>b\nGG\n", None, (5, 1, "mixed FASTA and FASTQ records are not supported")),
+    (b"@a\nACGT\n", None, (2, 1, "unterminated FASTQ record: missing '+' separator")),
+    (b"@a\nACGT", None, (2, 4, "unterminated FASTQ record: missing '+' separator")),
+    (b"@a\nACGT\r\n", None, (2, 1, "unterminated FASTQ record: missing '+' separator")),
+    (b"@a\nACGT\nAC", None, (3, 2, "unterminated FASTQ record: missing '+' separator")),
+    (b"@a\nACGT\n+\nIIIII\n", None, (4, 5, "FASTQ quality length exceeds sequence length")),
+    (b"@a\nACGT\n+\nII\nIIII\n", None, (5, 3, "FASTQ quality length exceeds sequence length")),
+    (b"@a\nACGT\n+\nII\n", None, (4, 3, "FASTQ quality length does not match sequence length")),
+    (b"@a\nACGT\n+\n", None, (3, 1, "FASTQ quality length does not match sequence length")),
+    (b"@a\nACGT\n+\nII\n\n", None, (5, 1, "FASTQ quality length does not match sequence length")),
+]
+
+
+@pytest.mark.parametrize("text,records,error", MANUAL, ids=lambda x: repr(x)[:40])
+def test_manual_cases(text, records, error):
+    o_recs, o_err = oracle_parse(text)
+    l_recs, l_err = lib_parse(text)
+    if error is None:
+        assert o_err is None and o_recs == records
+    else:
+        assert o_recs is None and o_err[:3] == error
+    assert (l_recs, l_err) == (o_recs, o_err)
+
+
+def random_text(rng: random.Random) -> bytes:
+    fastq = rng.random() < 0.5
+    eol = [b"\n", b"\r\n"]
+    out = bytearray()
+    for r in range(rng.randint(0, 6)):
+        for _ in range(rng.choice([0, 0, 0, 1, 2])):
+            out += rng.choice(eol)  # blank lines
+        seq = bytes(rng.choice(b"ACGTNacgt@+>") for _ in range(rng.randint(0, 40)))
+        if not fastq:
+            seq = seq.replace(b">", b"A")
+        header = (b"@" if fastq else b">") + bytes(rng.choice(b"abc r1:") for _ in range(rng.randint(0, 6)))
+        if rng.random() < 0.03:
+            header = (b">" if fastq else b"@") + header[1:]  # mixed
+        if rng.random() < 0.02:
+            header = b"x" + header[1:]
+        out += header + rng.choice(eol)
+        wrap = rng.choice([1, 3, 7, 80])
+        lines = [seq[i:i + wrap] for i in range(0, len(seq), wrap)] or [b""]
+        for line in lines:
+            if fastq:
+                line = line.replace(b"+", b"A")
+            out += line + rng.choice(eol)
+        if fastq:
+            if rng.random() < 0.97:
+                out += b"+" + rng.choice([b"", b"r"]) + rng.choice(eol)
+            qlen = len(seq)
+            if rng.random() < 0.05:
+                qlen += rng.choice([-2, -1, 1, 3])
+            qual = bytes(rng.choice(b"I#@+5") for _ in range(max(0, qlen)))
+            qw = rng.choice([1, 4, 100])
+            for i in range(0, len(qual), qw):
+                out += qual[i:i + qw] + rng.choice(eol)
+    if out and rng.random() < 0.3:
+        out = out.rstrip(b"\r\n")
+    return bytes(out)
+
+
+def test_parser_matches_oracle_on_random_texts():
+    rng = random.Random(20261203)
+    errors = 0
+    for i in range(3000):
+        text = random_text(rng)
+        o = oracle_parse(text)
+        l = lib_parse(text)
+        assert l == o, (i, text)
+        errors += o[1] is not None
+    assert 100 < errors < 2000  # both kinds of outcome are exercised
