@@ -47,6 +47,10 @@ def parse_args():
                     help="serving scenario: p50/p99 latency under Poisson arrivals")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-qps", type=float, default=100.0)
+    ap.add_argument("--fastx", default="on", choices=["on", "off"],
+                    help="FASTA insert / FASTQ query through files (text ingest throughput)")
+    ap.add_argument("--fastx-bases", type=int, default=400_000_000)
+    ap.add_argument("--fastx-reads", type=int, default=4_000_000)
     ap.add_argument("--batched", default="on", choices=["on", "off"],
                     help="batched uneven-length form (DNA and protein) vs plain launches")
     ap.add_argument("--batched-records", type=int, default=1_000_000)
@@ -262,6 +266,89 @@ def _equal_chunked(a, b, step=1 << 28) -> bool:
     if a.numel() != b.numel():
         return False
     return all(torch.equal(a[i:i + step], b[i:i + step]) for i in range(0, a.numel(), step))
+
+
+def fastx_scenario(genome, reads, wl, n_bases: int, n_reads: int) -> dict:
+    """FASTX ingest: a FASTA of the genome's first `n_bases` (records of 50 Mbp,
+    60 bases per line) inserted from a file, then a FASTQ of the first `n_reads`
+    150 bp reads (CRLF lines) queried from a file, on a fresh filter. Times the
+    whole call (mmap, parse, stage, kernels, reports); beside it the parse-only
+    time of the same text on the host, and a check that the filter words equal
+    the record-batch path's."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+
+    import cusbf_amd as cs
+    from cusbf_amd import fastx
+
+    n_bases = min(n_bases, genome.numel())
+    g = genome[:n_bases].cpu().numpy()
+    rec = 50_000_000
+    parts = []
+    for i, a in enumerate(range(0, n_bases, rec)):
+        chunk = g[a:a + rec]
+        pad = (-chunk.size) % 60
+        rows = np.concatenate([chunk, np.full(pad, 0, np.uint8)]).reshape(-1, 60)
+        body = np.concatenate([rows, np.full((rows.shape[0], 1), 10, np.uint8)], axis=1).tobytes()
+        if pad:
+            body = body[:-(pad + 1)] + b"\n"
+        parts.append(b">chunk%d synthetic\n" % i + body)
+    fasta = b"".join(parts)
+    stride = wl.reads.read_len + 1
+    n_reads = min(n_reads, (reads.numel() + 1) // stride)
+    r = reads[: n_reads * stride - 1].cpu().numpy()
+    r = np.concatenate([r, np.zeros(1, np.uint8)]).reshape(n_reads, stride)[:, :-1]
+    qual = np.full_like(r, ord("I"))
+
+    def cols(text: bytes):
+        return np.tile(np.frombuffer(text, np.uint8), (n_reads, 1))
+
+    idx = np.arange(n_reads, dtype=np.int64)
+    digits = np.stack([(idx // 10 ** j) % 10 for j in range(9, -1, -1)], axis=1).astype(np.uint8) + 48
+    fastq = np.concatenate([cols(b"@r"), digits, cols(b"\r\n"), r, cols(b"\r\n+\r\n"), qual,
+                            cols(b"\r\n")], axis=1).tobytes()
+    tmp = tempfile.mkdtemp(prefix="cusbf_fastx_")
+    try:
+        fa_path, fq_path = os.path.join(tmp, "genome.fa"), os.path.join(tmp, "reads.fq")
+        with open(fa_path, "wb") as fh:
+            fh.write(fasta)
+        with open(fq_path, "wb") as fh:
+            fh.write(fastq)
+        cfg = cs.Config(wl.k, wl.s, wl.m, wl.h)
+        filt = cs.Filter(cfg, int(wl.bits_per_kmer * n_bases))
+        t0 = time.perf_counter()
+        ins = filt.insert_fastx_file(fa_path)
+        t_ins = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        q = filt.query_fastx_file(fq_path)
+        t_q = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        n_fa = len(fastx.parse_fastx(fasta[: 64 << 20]))
+        t_parse_fa = (time.perf_counter() - t0) * len(fasta) / min(len(fasta), 64 << 20)
+        # the record-batch path on the same records gives the same words
+        ref = cs.Filter(cfg, filt.filter_bits())
+        seqs = [p.split(b"\n", 1)[1].replace(b"\n", b"") for p in parts]
+        off = np.cumsum([0] + [len(x) for x in seqs])
+        ref.insert_record_batch(b"".join(seqs), [(int(off[i]), len(x)) for i, x in enumerate(seqs)])
+        same = bool(np.array_equal(ref.words(), filt.words()))
+        ref.close()
+        filt.close()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    log(f"[fastx] FASTA {len(fasta) / 1e9:.2f} GB insert {t_ins:.2f} s, "
+        f"FASTQ {len(fastq) / 1e9:.2f} GB query {t_q:.2f} s")
+    return {
+        "fasta_bytes": len(fasta), "fastq_bytes": len(fastq),
+        "insert_seconds": t_ins, "query_seconds": t_q,
+        "insert_text_GBps": len(fasta) / t_ins / 1e9, "query_text_GBps": len(fastq) / t_q / 1e9,
+        "insert_kmers_per_s": ins.inserted_kmers / t_ins, "query_kmers_per_s": q.queried_kmers / t_q,
+        "insert_report": [ins.records_indexed, ins.indexed_bases, ins.inserted_kmers],
+        "query_report": [q.records_queried, q.queried_bases, q.queried_kmers, q.positive_kmers],
+        "host_parse_only_fasta_GBps": len(fasta) / t_parse_fa / 1e9,
+        "words_equal_record_batch_path": same,
+    }
 
 
 def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str = "",
@@ -629,6 +716,8 @@ def main() -> int:
         result["serve_p50_ms"] = serving["server_p50_ms"]
         result["serve_p99_ms"] = serving["server_p99_ms"]
         result["serving"] = serving
+    if args.fastx == "on":
+        result["fastx"] = fastx_scenario(genome, reads, wl, args.fastx_bases, args.fastx_reads)
     if args.batched == "on":
         del out
         filt.close()

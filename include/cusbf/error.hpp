@@ -31,6 +31,23 @@ static_assert(static_cast<int>(ErrorCategory::fastx_parse) == 2);
 static_assert(static_cast<int>(ErrorCategory::invalid_argument) == 3);
 static_assert(static_cast<int>(ErrorCategory::resource) == 4);
 
+/// error.hpp:20-41: file, line and column of a FASTX input position.
+struct SourceLocation {
+    std::string file;
+    uint32_t line{};
+    uint32_t column{};
+
+    [[nodiscard]] std::string to_string() const {
+        return file + ":" + std::to_string(line) + ":" + std::to_string(column);
+    }
+};
+
+/// error.hpp:50-53: a FASTX parse error and where it happened.
+struct FastxParseError {
+    SourceLocation location;
+    std::string message;
+};
+
 class Error {
    public:
     Error(ErrorCategory category, std::string message)
@@ -39,14 +56,23 @@ class Error {
     [[nodiscard]] ErrorCategory category() const noexcept { return category_; }
     [[nodiscard]] const std::string& message() const noexcept { return message_; }
 
+    /// error.hpp:97-99: the parse error payload (nullptr for other categories).
+    [[nodiscard]] const FastxParseError* as_fastx_parse() const noexcept {
+        return category_ == ErrorCategory::fastx_parse ? &parse_ : nullptr;
+    }
+
+    /// error.hpp:123-129: message "file:line:column: detail".
+    [[nodiscard]] static Error fastx_parse(SourceLocation site, std::string message) {
+        Error e(ErrorCategory::fastx_parse, message);
+        e.parse_ = FastxParseError{std::move(site), std::move(message)};
+        return e;
+    }
+
     [[nodiscard]] static Error cuda(std::string message) {
         return Error(ErrorCategory::cuda, std::move(message));
     }
     [[nodiscard]] static Error io(std::string message) {
         return Error(ErrorCategory::io, std::move(message));
-    }
-    [[nodiscard]] static Error fastx_parse(std::string message) {
-        return Error(ErrorCategory::fastx_parse, std::move(message));
     }
     [[nodiscard]] static Error invalid_argument(std::string message) {
         return Error(ErrorCategory::invalid_argument, std::move(message));
@@ -58,6 +84,7 @@ class Error {
    private:
     ErrorCategory category_;
     std::string message_;
+    FastxParseError parse_;
 };
 
 struct Unexpected {

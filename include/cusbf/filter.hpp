@@ -15,6 +15,9 @@
 
 #include <cstdint>
 #include <cstring>
+#include <exception>
+#include <filesystem>
+#include <istream>
 #include <span>
 #include <string>
 #include <string_view>
@@ -74,6 +77,40 @@ struct FastxQueryReport {
     uint64_t queriedBases{};
     uint64_t queriedKmers{};
     uint64_t positive_kmers{};
+};
+
+/// Fastx.hpp:172-190: one record of a FASTX query (hits valid during the call only).
+struct FastxRecordView {
+    uint64_t record_index{};
+    std::string_view header{};
+    std::string_view sequence{};
+    uint64_t queriedBases{};
+    uint64_t queriedKmers{};
+    uint64_t positive_kmers{};
+    std::span<const uint8_t> hits{};
+};
+
+/// Fastx.hpp:193-208
+struct FastxDetailedQueryRecord {
+    uint64_t record_index{};
+    std::string header;
+    std::string sequence;
+    uint64_t queriedBases{};
+    uint64_t queriedKmers{};
+    uint64_t positive_kmers{};
+    std::vector<uint8_t> hits;
+};
+
+/// Fastx.hpp:211-216
+struct FastxDetailedQueryReport {
+    FastxQueryReport summary{};
+    std::vector<FastxDetailedQueryRecord> records;
+};
+
+/// Fastx.hpp:230-238
+template <typename Functor>
+concept FastxRecordConsumer = requires(Functor f, const FastxRecordView& record) {
+    { f(record) } -> std::same_as<void>;
 };
 
 struct RecordQueryView {
@@ -196,14 +233,60 @@ class QueryLayout {
 };
 
 inline Unexpected status_error(int status) {
-    char buf[1024];
+    char buf[4096];
     cusbf_last_error(buf, sizeof(buf));
-    return Err(Error(static_cast<ErrorCategory>(status - 1), buf));
+    const auto category = static_cast<ErrorCategory>(status - 1);
+    if (category == ErrorCategory::fastx_parse) {
+        char file[4096];
+        uint32_t line = 0, column = 0;
+        cusbf_last_error_location(file, sizeof(file), &line, &column);
+        return Err(Error::fastx_parse(SourceLocation{file, line, column}, buf));
+    }
+    return Err(Error(category, buf));
 }
 
 inline Result<void> check(int status) {
     if (status == CUSBF_OK) return {};
     return status_error(status);
+}
+
+// Reader over a std::istream for cusbf_*_fastx_reader.
+inline int64_t istream_read(void* user, char* dst, uint64_t capacity) {
+    auto* in = static_cast<std::istream*>(user);
+    in->read(dst, static_cast<std::streamsize>(capacity));
+    const std::streamsize got = in->gcount();
+    if (got > 0) return got;
+    return in->bad() ? -1 : 0;
+}
+
+// A C++ consumer behind the C callback: an exception it throws stops the
+// query and is rethrown to the caller once the library has returned.
+template <typename Consumer>
+struct ConsumerBox {
+    Consumer* consume;
+    std::exception_ptr error;
+
+    static int call(const cusbf_fastx_record_view* v, void* user) {
+        auto* box = static_cast<ConsumerBox*>(user);
+        try {
+            (*box->consume)(FastxRecordView{
+                v->record_index, std::string_view(v->header, v->header_len),
+                std::string_view(v->sequence, v->sequence_len), v->queried_bases, v->queried_kmers,
+                v->positive_kmers, std::span<const uint8_t>(v->hits, v->hits_len)});
+            return 0;
+        } catch (...) {
+            box->error = std::current_exception();
+            return 1;
+        }
+    }
+};
+
+inline FastxInsertReport insert_report(const cusbf_insert_report& r) {
+    return FastxInsertReport{r.records_indexed, r.indexed_bases, r.inserted_kmers};
+}
+
+inline FastxQueryReport query_report(const cusbf_query_report& r) {
+    return FastxQueryReport{r.records_queried, r.queried_bases, r.queried_kmers, r.positive_kmers};
 }
 
 template <typename Config>
@@ -511,6 +594,80 @@ class filter {
                                                                d_output.data(), d_output.size(), stream));
     }
 
+    // --- FASTA / FASTQ (filter.cuh:402-453, 620-800) ---
+    [[nodiscard]] Result<FastxInsertReport> insert_fastx(std::istream& input, double fill_fraction = 0.7,
+                                                         hipStream_t stream = nullptr) {
+        cusbf_insert_report r{};
+        CUSBF_TRY(detail::check(cusbf_insert_fastx_reader(handle_, &detail::istream_read, &input,
+                                                          "<stream>", fill_fraction, stream, &r)));
+        return detail::insert_report(r);
+    }
+    [[nodiscard]] Result<FastxInsertReport> insert_fastx_file(const std::filesystem::path& path,
+                                                              double fill_fraction = 0.7,
+                                                              hipStream_t stream = nullptr) {
+        cusbf_insert_report r{};
+        CUSBF_TRY(detail::check(cusbf_insert_fastx_file(handle_, path.c_str(), fill_fraction, stream, &r)));
+        return detail::insert_report(r);
+    }
+    [[nodiscard]] Result<FastxQueryReport> query_fastx(std::istream& input, double fill_fraction = 0.7,
+                                                       hipStream_t stream = nullptr) const {
+        cusbf_query_report r{};
+        CUSBF_TRY(detail::check(cusbf_query_fastx_reader(handle_, &detail::istream_read, &input, "<stream>",
+                                                         fill_fraction, stream, nullptr, nullptr, &r)));
+        return detail::query_report(r);
+    }
+    [[nodiscard]] Result<FastxQueryReport> query_fastx_file(const std::filesystem::path& path,
+                                                            double fill_fraction = 0.7,
+                                                            hipStream_t stream = nullptr) const {
+        cusbf_query_report r{};
+        CUSBF_TRY(detail::check(cusbf_query_fastx_file(handle_, path.c_str(), fill_fraction, stream, nullptr,
+                                                       nullptr, &r)));
+        return detail::query_report(r);
+    }
+    template <FastxRecordConsumer Consumer>
+    [[nodiscard]] Result<FastxQueryReport> query_fastx_records(std::istream& input, Consumer&& consume,
+                                                               double fill_fraction = 0.7,
+                                                               hipStream_t stream = nullptr) const {
+        using C = std::remove_reference_t<Consumer>;
+        detail::ConsumerBox<C> box{&consume, nullptr};
+        cusbf_query_report r{};
+        const int st = cusbf_query_fastx_reader(handle_, &detail::istream_read, &input, "<stream>",
+                                                fill_fraction, stream, &detail::ConsumerBox<C>::call, &box, &r);
+        if (box.error) std::rethrow_exception(box.error);
+        CUSBF_TRY(detail::check(st));
+        return detail::query_report(r);
+    }
+    template <FastxRecordConsumer Consumer>
+    [[nodiscard]] Result<FastxQueryReport> query_fastx_file_records(const std::filesystem::path& path,
+                                                                    Consumer&& consume,
+                                                                    double fill_fraction = 0.7,
+                                                                    hipStream_t stream = nullptr) const {
+        using C = std::remove_reference_t<Consumer>;
+        detail::ConsumerBox<C> box{&consume, nullptr};
+        cusbf_query_report r{};
+        const int st = cusbf_query_fastx_file(handle_, path.c_str(), fill_fraction, stream,
+                                              &detail::ConsumerBox<C>::call, &box, &r);
+        if (box.error) std::rethrow_exception(box.error);
+        CUSBF_TRY(detail::check(st));
+        return detail::query_report(r);
+    }
+    [[nodiscard]] Result<FastxDetailedQueryReport> query_fastx_detailed(std::istream& input,
+                                                                        double fill_fraction = 0.7,
+                                                                        hipStream_t stream = nullptr) const {
+        FastxDetailedQueryReport out;
+        out.summary = CUSBF_TRY(query_fastx_records(
+            input, [&](const FastxRecordView& v) { out.records.push_back(detailed(v)); }, fill_fraction, stream));
+        return out;
+    }
+    [[nodiscard]] Result<FastxDetailedQueryReport> query_fastx_file_detailed(const std::filesystem::path& path,
+                                                                             double fill_fraction = 0.7,
+                                                                             hipStream_t stream = nullptr) const {
+        FastxDetailedQueryReport out;
+        out.summary = CUSBF_TRY(query_fastx_file_records(
+            path, [&](const FastxRecordView& v) { out.records.push_back(detailed(v)); }, fill_fraction, stream));
+        return out;
+    }
+
     // --- batched record form (no single-call counterpart in the reference) ---
     // Records back to back in d_symbols, d_offsets = num_records + 1 symbol
     // offsets (device memory, non-decreasing from 0 to d_symbols' symbol
@@ -573,6 +730,11 @@ class filter {
     [[nodiscard]] cusbf_filter* native_handle() const noexcept { return handle_; }
 
    private:
+    static FastxDetailedQueryRecord detailed(const FastxRecordView& v) {
+        return FastxDetailedQueryRecord{v.record_index, std::string(v.header), std::string(v.sequence),
+                                        v.queriedBases, v.queriedKmers, v.positive_kmers,
+                                        std::vector<uint8_t>(v.hits.begin(), v.hits.end())};
+    }
     static const cusbf_record_range* ranges_of(std::span<const RecordRange> r) {
         return reinterpret_cast<const cusbf_record_range*>(r.data());
     }

@@ -11,7 +11,10 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <sstream>
+#include <stdexcept>
 #include <string>
+#include <unistd.h>
 #include <vector>
 
 #include "cusbf/filter.hpp"
@@ -423,6 +426,150 @@ TEST_GPU(DenseSpanTooSmallIsInvalidArgument) {
 }
 
 // ---- host-only: tests/test_query_layout.cpp, tests/test_normalized_record_batch.cpp -----
+
+// ---- tests/test_fastx.cu ------------------------------------------------------------
+
+// A temporary file holding `text`, removed when it goes out of scope.
+struct TempFile {
+    std::string path;
+    explicit TempFile(const std::string& text) {
+        char name[] = "/tmp/cusbf_fastx_XXXXXX";
+        const int fd = mkstemp(name);
+        if (fd < 0) throw std::runtime_error("mkstemp");
+        if (write(fd, text.data(), text.size()) != static_cast<ssize_t>(text.size())) {
+            close(fd);
+            throw std::runtime_error("write");
+        }
+        close(fd);
+        path = name;
+    }
+    ~TempFile() { unlink(path.c_str()); }
+};
+
+TEST_GPU(InsertFastxFileParsesWrappedFastaRecords) {
+    cusbf::filter<TestConfig> filter(1 << 12);
+    const TempFile file(">wrapped\nACGT\nACGT\nACGT\n");
+    const auto report = CUSBF_UNWRAP(filter.insert_fastx_file(file.path));
+    EXPECT_EQ(report.recordsIndexed, 1u);
+    EXPECT_EQ(report.indexedBases, 12u);
+    EXPECT_EQ(report.insertedKmers, 12u - TestConfig::k + 1);
+    EXPECT_TRUE(all_ones(CUSBF_UNWRAP(filter.contains_sequence("ACGTACGTACGT"))));
+}
+
+TEST_GPU(QueryFastxFileRecordsParsesWrappedFastqWithCrLf) {
+    cusbf::filter<TestConfig> filter(1 << 12);
+    (void)CUSBF_UNWRAP(filter.insert_sequence("ACGTACGTACGT"));
+    const TempFile file("@wrapped\r\nACGTAC\r\nGTACGT\r\n+\r\nIIIIII\r\nIIIIII\r\n");
+    const auto report = CUSBF_UNWRAP(filter.query_fastx_file(file.path));
+    EXPECT_EQ(report.recordsQueried, 1u);
+    EXPECT_EQ(report.queriedBases, 12u);
+    EXPECT_EQ(report.queriedKmers, 8u);
+    EXPECT_EQ(report.positive_kmers, 8u);
+    std::vector<cusbf::FastxDetailedQueryRecord> seen;
+    const auto summary = CUSBF_UNWRAP(filter.query_fastx_file_records(
+        file.path, [&](const cusbf::FastxRecordView& r) {
+            seen.push_back({r.record_index, std::string(r.header), std::string(r.sequence), r.queriedBases,
+                            r.queriedKmers, r.positive_kmers, std::vector<uint8_t>(r.hits.begin(), r.hits.end())});
+        }));
+    EXPECT_EQ(summary.queriedKmers, 8u);
+    EXPECT_EQ(seen.size(), 1u);
+    EXPECT_EQ(seen[0].header, std::string("wrapped"));
+    EXPECT_EQ(seen[0].sequence, std::string("ACGTACGTACGT"));
+    EXPECT_EQ(seen[0].hits.size(), 8u);
+    EXPECT_TRUE(all_ones(seen[0].hits));
+}
+
+TEST_GPU(QueryFastxDoesNotCreateCrossRecordKmers) {
+    cusbf::filter<TestConfig> filter(1 << 12);
+    (void)CUSBF_UNWRAP(filter.insert_sequence("ACGTACGT"));
+    (void)CUSBF_UNWRAP(filter.insert_sequence("TGCATGCA"));
+    std::istringstream input(">first\nACGT\nACGT\n>second\nTGCA\nTGCA\n");
+    const auto report = CUSBF_UNWRAP(filter.query_fastx_detailed(input));
+    EXPECT_EQ(report.summary.recordsQueried, 2u);
+    EXPECT_EQ(report.summary.queriedBases, 16u);
+    EXPECT_EQ(report.summary.queriedKmers, 8u);
+    EXPECT_EQ(report.summary.positive_kmers, 8u);
+    EXPECT_EQ(report.records.size(), 2u);
+    EXPECT_EQ(report.records[1].header, std::string("second"));
+    EXPECT_EQ(report.records[1].record_index, 1u);
+}
+
+TEST_GPU(TripletQueryFastxRecordsDoNotCreateCrossRecordKmers) {
+    cusbf::filter<TripletTestConfig> filter(1 << 12);
+    (void)CUSBF_UNWRAP(filter.insert_sequence("ACGTACGTT"));
+    (void)CUSBF_UNWRAP(filter.insert_sequence("GGGTTTAAA"));
+    const TempFile file(">first\nACGTACGTT\n>second\nGGGTTTAAA\n");
+    const auto report = CUSBF_UNWRAP(filter.query_fastx_file_detailed(file.path));
+    EXPECT_EQ(report.summary.queriedKmers, 2u);
+    EXPECT_EQ(report.summary.positive_kmers, 2u);
+    EXPECT_EQ(report.records.size(), 2u);
+    EXPECT_TRUE(report.records[0].hits == (std::vector<uint8_t>{1}));
+    EXPECT_TRUE(report.records[1].hits == (std::vector<uint8_t>{1}));
+}
+
+TEST_GPU(FastxDetailedQueryReportsInvalidWindowsAsMisses) {
+    cusbf::filter<TestConfig> filter(1 << 12);
+    const TempFile file(">with-invalid\nACGTNACGTACGTA\n");
+    const auto ins = CUSBF_UNWRAP(filter.insert_fastx_file(file.path));
+    EXPECT_EQ(ins.insertedKmers, 5u);
+    const auto report = CUSBF_UNWRAP(filter.query_fastx_file_detailed(file.path));
+    EXPECT_EQ(report.summary.queriedKmers, 5u);
+    EXPECT_EQ(report.summary.positive_kmers, 5u);
+    EXPECT_TRUE(report.records[0].hits == (std::vector<uint8_t>{0, 0, 0, 0, 0, 1, 1, 1, 1, 1}));
+}
+
+TEST_GPU(ForcedFastxChunkFlushPreservesRecordOrderAndCounts) {
+    cusbf::filter<TestConfig> filter(1 << 12);
+    const TempFile file(">first\nACGTACGT\n>second\nACGTNACGTACGTA\n>third\nTGCATGCA\n");
+    const auto ins = CUSBF_UNWRAP(filter.insert_fastx_file(file.path, 0.0));
+    EXPECT_EQ(ins.recordsIndexed, 3u);
+    EXPECT_EQ(ins.indexedBases, 30u);
+    EXPECT_EQ(ins.insertedKmers, 13u);
+    const auto report = CUSBF_UNWRAP(filter.query_fastx_file_detailed(file.path, 0.0));
+    EXPECT_EQ(report.records.size(), 3u);
+    EXPECT_EQ(report.records[2].header, std::string("third"));
+    EXPECT_TRUE(report.records[1].hits == (std::vector<uint8_t>{0, 0, 0, 0, 0, 1, 1, 1, 1, 1}));
+    EXPECT_EQ(report.summary.positive_kmers, 13u);
+}
+
+TEST_GPU(MalformedFastqReportsParseLocation) {
+    cusbf::filter<TestConfig> filter(1 << 12);
+    const TempFile file("@broken\nACGTACGT\n+\nIIIIIII\n");
+    const auto result = filter.query_fastx_file(file.path);
+    EXPECT_TRUE(!result);
+    EXPECT_TRUE(result.error().category() == cusbf::ErrorCategory::fastx_parse);
+    const cusbf::FastxParseError* e = result.error().as_fastx_parse();
+    EXPECT_TRUE(e != nullptr);
+    EXPECT_EQ(e->location.line, 4u);
+    EXPECT_EQ(e->location.column, 8u);
+    EXPECT_EQ(e->location.file, file.path);
+    EXPECT_EQ(result.error().message(),
+              file.path + ":4:8: FASTQ quality length does not match sequence length");
+}
+
+TEST_GPU(CallbackAbortStopsProcessingPreservingFilterReuse) {
+    cusbf::filter<TestConfig> filter(1 << 12);
+    (void)CUSBF_UNWRAP(filter.insert_sequence("ACGTACGTACGT"));
+    const std::string text = ">seqA\nACGTACGTACGT\n>seqB\nTGCATGCA\n>seqC\nGGGAAAACCC\n";
+    size_t calls = 0;
+    bool thrown = false;
+    try {
+        std::istringstream in(text);
+        (void)filter.query_fastx_records(in, [&](const cusbf::FastxRecordView&) {
+            ++calls;
+            throw std::runtime_error("abort");
+        });
+    } catch (const std::runtime_error& e) {
+        thrown = std::string(e.what()) == "abort";
+    }
+    EXPECT_TRUE(thrown);
+    EXPECT_EQ(calls, 1u);
+    std::istringstream again(text);
+    size_t n = 0;
+    const auto summary = CUSBF_UNWRAP(filter.query_fastx_records(again, [&](const cusbf::FastxRecordView&) { ++n; }));
+    EXPECT_EQ(n, 3u);
+    EXPECT_EQ(summary.recordsQueried, 3u);
+}
 
 TEST_HOST(QueryLayoutDerivesPerRecordHitSpans) {
     const std::array<cusbf::NormalizedRecord, 2> records{{{0, 0, 6, 1}, {1, 6, 8, 2}}};
