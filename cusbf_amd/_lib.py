@@ -141,6 +141,9 @@ SIGNATURES = [
     ("cusbf_fastx_set_chunk_bytes", i32, [P, u64]),
     ("cusbf_last_error_location", i32, [ctypes.c_char_p, sz, ctypes.POINTER(u32),
                                         ctypes.POINTER(u32)]),
+    ("cusbf_save", i32, [P, ctypes.c_char_p]),
+    ("cusbf_image_config", i32, [ctypes.c_char_p, CFG, ctypes.POINTER(u64)]),
+    ("cusbf_load", i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
     ("cusbf_server_create", i32, [P, ctypes.POINTER(CServerOptions), ctypes.POINTER(P)]),
     ("cusbf_server_destroy", None, [P]),
     ("cusbf_server_submit", i32, [P, P, u64, RR, u64, P, u64, ctypes.POINTER(u64)]),

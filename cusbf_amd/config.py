@@ -99,5 +99,18 @@ class Config:
     def c_config(self) -> _lib.CConfig:
         return self._c
 
+    @staticmethod
+    def from_c(c: _lib.CConfig) -> "Config":
+        """The Config a cusbf_config describes (filter images store one)."""
+        by_code = {ALPHABET_DNA: DnaAlphabet, ALPHABET_DNA_TRIPLET: DnaTripletAlphabet,
+                   ALPHABET_PROTEIN: ProteinAlphabet}
+        if c.alphabet in by_code:
+            alphabet = by_code[c.alphabet]
+        else:
+            lut = bytes(c.lut)
+            alphabet = Alphabet.custom("image", int(c.symbol_count), int(c.invalid_symbol),
+                                       int(c.separator), b"", lambda b: lut[b])
+        return Config(int(c.k), int(c.s), int(c.m), int(c.hash_count), alphabet=alphabet)
+
     def __repr__(self) -> str:
         return (f"Config<{self.k},{self.s},{self.m},{self.hash_count},{self.alphabet.name}>")

@@ -444,6 +444,7 @@ int launch_contains_bytes(const cusbf_filter* f, const char* d_seq, uint64_t byt
     return launch_contains(f, byte_input(f, d_seq, bytes), d_out, stream);
 }
 uint32_t kmer_length(const cusbf_filter* f) { return f->cfg.k; }
+const cusbf_config& config(const cusbf_filter* f) { return f->cfg; }
 uint64_t fastx_chunk_bytes(const cusbf_filter* f) { return f->fastx_chunk_bytes; }
 int set_fastx_error(const char* source, uint32_t line, uint32_t column, const char* detail) {
     char buf[64];

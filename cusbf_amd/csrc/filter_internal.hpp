@@ -34,6 +34,7 @@ int launch_request_counts(const cusbf_filter* f, const char* d_seq, uint64_t byt
                           uint64_t* d_counts, hipStream_t stream);
 
 uint32_t kmer_length(const cusbf_filter* f);
+const cusbf_config& config(const cusbf_filter* f);
 // FASTX chunk capacity override in bytes (0 = from fill_fraction).
 uint64_t fastx_chunk_bytes(const cusbf_filter* f);
 // Error::fastx_parse (error.hpp:123-129): "source:line:column: detail", location kept.

@@ -407,6 +407,26 @@ __global__ __launch_bounds__(kBlock) void popcount_kernel(const uint64_t* words,
     if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
 }
 
+// Filter image checksum: sum over i of mix(words[i] ^ (i * golden)) modulo 2^64
+// (order-free, so waves add their sums atomically); mix is the splitmix64
+// finaliser.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void checksum_kernel(const uint64_t* words, uint64_t n,
+                                                          unsigned long long* out) {
+    unsigned long long acc = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+        acc += mix64(words[i] ^ (i * 0x9E3779B97F4A7C15ull));
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
+}
+
 // count_positive_kmers.cuh:47-62: number of bytes equal to 1.
 __global__ __launch_bounds__(kBlock) void count_ones_kernel(const uint8_t* hits, uint64_t n,
                                                             unsigned long long* out) {
@@ -541,6 +561,15 @@ hipError_t popcount_words(const uint64_t* words, uint64_t n, uint64_t* d_out, hi
     (void)hipGetLastError();  // report only this launch
     hipLaunchKernelGGL(popcount_kernel, dim3(grid_for(n, kBlock * 8, 4096)), dim3(kBlock), 0,
                        stream, words, n, reinterpret_cast<unsigned long long*>(d_out));
+    return hipGetLastError();
+}
+
+hipError_t words_checksum(const uint64_t* words, uint64_t n, uint64_t* d_out, hipStream_t stream) {
+    hipError_t e = fill_u64(d_out, 0, 1, stream);
+    if (e != hipSuccess || n == 0) return e;
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(checksum_kernel, dim3(grid_for(n, kBlock * 8, 4096)), dim3(kBlock), 0, stream,
+                       words, n, reinterpret_cast<unsigned long long*>(d_out));
     return hipGetLastError();
 }
 

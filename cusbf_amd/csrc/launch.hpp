@@ -102,5 +102,7 @@ hipError_t count_ones_per_record(const uint8_t* hits, const uint64_t* offsets,
                                  const uint64_t* counts, uint64_t num_records, uint64_t* d_out,
                                  hipStream_t stream);
 hipError_t fill_u64(uint64_t* dst, uint64_t value, uint64_t n, hipStream_t stream);
+// *d_out = sum over i of mix(words[i] ^ i * golden) mod 2^64 (filter image checksum).
+hipError_t words_checksum(const uint64_t* words, uint64_t n, uint64_t* d_out, hipStream_t stream);
 
 }  // namespace sbf

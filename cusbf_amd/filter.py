@@ -132,6 +132,29 @@ class Filter:
     def handle(self) -> ctypes.c_void_p:
         return self._h
 
+    # filter images (no reference counterpart) -------------------------------------
+    def save(self, path) -> None:
+        """Writes the filter image (header + words) to `path`."""
+        import os
+
+        check(self._lib, self._lib.cusbf_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path) -> "Filter":
+        """A filter recreated from an image on the current device (checksum verified)."""
+        import os
+
+        lib = _lib.load()
+        c = _lib.CConfig()
+        bits = ctypes.c_uint64()
+        check(lib, lib.cusbf_image_config(os.fsencode(path), ctypes.byref(c), ctypes.byref(bits)))
+        config = Config.from_c(c)
+        h = ctypes.c_void_p()
+        check(lib, lib.cusbf_load(os.fsencode(path), ctypes.byref(h)))
+        f = cls.__new__(cls)
+        f.config, f._lib, f._h = config, lib, h
+        return f
+
     # properties (filter.cuh:391-414) ---------------------------------------
     def filter_bits(self) -> int:
         return int(self._lib.cusbf_filter_bits(self._h))

@@ -729,7 +729,31 @@ class filter {
     }
     [[nodiscard]] cusbf_filter* native_handle() const noexcept { return handle_; }
 
+    // --- filter images (no reference counterpart) ---
+    /// Writes header + words to `path` (cusbf_save).
+    [[nodiscard]] Result<void> save(const std::filesystem::path& path) const {
+        return detail::check(cusbf_save(handle_, path.c_str()));
+    }
+    /// Recreates a filter from an image; invalid_argument when the image's
+    /// configuration is not Config's.
+    [[nodiscard]] static Result<filter> load(const std::filesystem::path& path) {
+        cusbf_config stored{};
+        uint64_t bits = 0;
+        CUSBF_TRY(detail::check(cusbf_image_config(path.c_str(), &stored, &bits)));
+        const cusbf_config want = detail::c_config<Config>();
+        if (stored.k != want.k || stored.s != want.s || stored.m != want.m ||
+            stored.hash_count != want.hash_count || stored.alphabet != want.alphabet ||
+            std::memcmp(stored.lut, want.lut, sizeof(want.lut)) != 0) {
+            return Err(Error::invalid_argument("filter image configuration does not match Config"));
+        }
+        cusbf_filter* h = nullptr;
+        CUSBF_TRY(detail::check(cusbf_load(path.c_str(), &h)));
+        return filter(h);
+    }
+
    private:
+    explicit filter(cusbf_filter* adopted) noexcept : handle_(adopted) {}
+
     static FastxDetailedQueryRecord detailed(const FastxRecordView& v) {
         return FastxDetailedQueryRecord{v.record_index, std::string(v.header), std::string(v.sequence),
                                         v.queriedBases, v.queriedKmers, v.positive_kmers,

@@ -376,6 +376,20 @@ int cusbf_fastx_set_chunk_bytes(cusbf_filter* f, uint64_t bytes);
  * Returns the status of that error. */
 int cusbf_last_error_location(char* file, size_t file_len, uint32_t* line, uint32_t* column);
 
+/* --- filter images (no reference counterpart: filters live in device memory only) */
+/*
+ * A filter image is a 336-byte header (magic "CUSBFIMG", version 1, the
+ * cusbf_config, shard count, filter bits, word count, checksum of the words)
+ * followed by the num_shards * 4 words. cusbf_load recreates the filter on the
+ * current device and checks the checksum on the device (CUSBF_ERR_IO "filter
+ * image checksum mismatch: <path>"); a file that is not an image is
+ * CUSBF_ERR_INVALID_ARGUMENT.
+ */
+int cusbf_save(const cusbf_filter* f, const char* path);
+/* Header only: the image's configuration and filter bits. */
+int cusbf_image_config(const char* path, cusbf_config* config, uint64_t* filter_bits);
+int cusbf_load(const char* path, cusbf_filter** out);
+
 /* --- request server (no reference counterpart) ------------------------------- */
 /*
  * Serving form of the query path: the filter stays resident; a scheduler

@@ -571,6 +571,23 @@ TEST_GPU(CallbackAbortStopsProcessingPreservingFilterReuse) {
     EXPECT_EQ(summary.recordsQueried, 3u);
 }
 
+// ---- filter images (no reference counterpart) -------------------------------------------
+
+TEST_GPU(FilterImageRoundTripAndConfigCheck) {
+    cusbf::filter<TestConfig> filter(1 << 14);
+    (void)CUSBF_UNWRAP(filter.insert_sequence("ACGTACGTACGTTTGGAACC"));
+    char name[] = "/tmp/cusbf_image_XXXXXX";
+    const int fd = mkstemp(name);
+    close(fd);
+    cusbf::require_void(filter.save(name));
+    auto loaded = CUSBF_UNWRAP(cusbf::filter<TestConfig>::load(name));
+    EXPECT_TRUE(CUSBF_UNWRAP(loaded.words()) == CUSBF_UNWRAP(filter.words()));
+    EXPECT_TRUE(all_ones(CUSBF_UNWRAP(loaded.contains_sequence("ACGTACGTACGTTTGGAACC"))));
+    const auto other = cusbf::filter<ProteinTestConfig>::load(name);
+    EXPECT_TRUE(!other && other.error().category() == cusbf::ErrorCategory::invalid_argument);
+    unlink(name);
+}
+
 TEST_HOST(QueryLayoutDerivesPerRecordHitSpans) {
     const std::array<cusbf::NormalizedRecord, 2> records{{{0, 0, 6, 1}, {1, 6, 8, 2}}};
     const auto layout = cusbf::detail::QueryLayout::build<LayoutConfig>(records);
