@@ -26,6 +26,7 @@ if ROOT not in sys.path:
 
 METRIC = "insert & query k-mers/s at 1 GPU (% HBM roofline) + p50/p99 serve latency"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+INSERT_PATHS = {"auto": 0, "atomic": 1, "sorted": 2}  # CUSBF_INSERT_*
 
 
 def log(msg: str) -> None:
@@ -39,6 +40,8 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="human", choices=["human", "nematode"])
     ap.add_argument("--reads", type=int, default=0, help="override the number of reads")
+    ap.add_argument("--insert-path", default="auto", choices=["auto", "atomic", "sorted"],
+                    help="insert path of the lane-strip kernels (auto: sorted when it pays)")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--probes", default="on", choices=["on", "off"],
@@ -594,6 +597,7 @@ def main() -> int:
     cfg = cs.Config(wl.k, wl.s, wl.m, wl.h)
     genome_windows = genome.numel() - wl.k + 1
     filt = cs.Filter(cfg, int(wl.bits_per_kmer * genome_windows))
+    filt.set_insert_path(INSERT_PATHS[args.insert_path])
     out = torch.empty(reads.numel() - wl.k + 1, dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -690,6 +694,7 @@ def main() -> int:
                    "bits_per_kmer_requested": wl.bits_per_kmer,
                    "filter_bits": filt.filter_bits(),
                    "bits_per_kmer_effective": filt.filter_bits() / max(1, v_ins)},
+        "insert_path": {1: "atomic", 2: "sorted"}[filt.insert_path().last_path],
         "insert_kmers_per_s": ins_rate,
         "query_kmers_per_s": q_rate,
         "insert_ms": ti * 1e3,

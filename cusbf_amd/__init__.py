@@ -12,7 +12,11 @@ from .config import (  # noqa: F401
 )
 from .errors import CusbfError, ErrorCategory, Result  # noqa: F401
 from .filter import (  # noqa: F401
+    INSERT_ATOMIC,
+    INSERT_AUTO,
+    INSERT_SORTED,
     Filter,
+    InsertPathInfo,
     InsertReport,
     QueryReport,
     RecordRange,
@@ -29,5 +33,5 @@ __all__ = [
     "ErrorCategory", "Result", "Filter", "InsertReport", "QueryReport", "RecordRange",
     "count_positive", "encode_symbols", "pack_dense_sequence", "pack_dense_sequence_device",
     "NormalizedRecord", "normalize_record_batch", "query_layout", "Server",
-    "query_record_batch_hits",
+    "query_record_batch_hits", "INSERT_AUTO", "INSERT_ATOMIC", "INSERT_SORTED", "InsertPathInfo",
 ]

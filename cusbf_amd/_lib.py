@@ -111,6 +111,9 @@ SIGNATURES = [
     ("cusbf_count_positive_per_record_async", i32, [P, P, P, u64, P, P]),
     ("cusbf_count_windows_runs_async", i32, [P, P, u64, P, P]),
     ("cusbf_kernel_family", ctypes.c_char_p, [P]),
+    ("cusbf_set_insert_path", i32, [P, u32, u32, u32]),
+    ("cusbf_get_insert_path", i32, [P, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                    ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     ("cusbf_batch_variants", u32, [P]),
     ("cusbf_batch_bucket", u32, [u64, u64]),
     ("cusbf_batch_tuning_set", i32, [P, u32, u32, ctypes.POINTER(CBatchTuning)]),

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -132,6 +133,15 @@ struct cusbf_filter {
     uint64_t starts_cap = 0;  // words
     bool batch_check_pending = false;
     uint64_t fastx_chunk_bytes = 0;  // FASTX chunk capacity override (0 = from fill_fraction)
+    // insert path (atomic or sorted) and the sorted insert's workspace
+    uint32_t insert_path = CUSBF_INSERT_AUTO;
+    uint32_t slice_bits = 12;
+    uint32_t reserve_per_tile = 0;  // 0 = from the minimizer span
+    uint32_t last_path = CUSBF_INSERT_ATOMIC;
+    void* d_sorted = nullptr;
+    uint64_t sorted_bytes = 0;
+    hipEvent_t sorted_done = nullptr;  // recorded after the last sorted insert's kernels
+    bool sorted_used = false;
 };
 
 struct cusbf_batch_plan {
@@ -140,6 +150,7 @@ struct cusbf_batch_plan {
     bool empty = true;
     uint32_t* d_state = nullptr;
     uint64_t* d_starts = nullptr;
+    void* d_sorted = nullptr;  // sorted insert workspace of an insert plan (or null)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -191,11 +202,146 @@ bool strip_reads(const cusbf_filter* f, const sbf::SequenceInput& in) {
     return f->strip != nullptr && (!in.dense || f->strip->dense);
 }
 
-int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
-    const hipError_t e = strip_reads(f, in) ? f->strip->insert(in, view(f), stream)
-                                            : sbf::generic_insert(f->kc, in, view(f), stream);
-    if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+// --- sorted insert -------------------------------------------------------------
+
+// Records per wave tile the sorted insert reserves: 2.5 / (w + 1) of the
+// tile's windows for a minimizer span of w = k - m + 1 m-mers (a random
+// sequence has 2 / (w + 1) shard runs per window), at most the tile's 256 LDS
+// run slots (runs past them use atomics anyway).
+uint32_t reserve_per_tile(const cusbf_filter* f) {
+    if (f->reserve_per_tile) return f->reserve_per_tile;
+    const double w = static_cast<double>(f->cfg.k - f->cfg.m + 1);
+    const double r = std::ceil(f->strip->wave_windows * 2.5 / (w + 1.0));
+    return static_cast<uint32_t>(std::min(256.0, std::max(1.0, r)));
+}
+
+// Cost model of AUTO, from MI355X measurements: the atomic insert does four
+// memory-side atomic ORs per run at ~25.6 G/s; the sorted insert moves ~45 ps
+// of record traffic per run and loads and stores each slice it touches, plus
+// about a dozen launches.
+bool sorted_pays(const cusbf_filter* f, uint64_t windows, const sbf::SortedLayout& l) {
+    const double w = static_cast<double>(f->cfg.k - f->cfg.m + 1);
+    const double runs = static_cast<double>(windows) * 2.0 / (w + 1.0);
+    const double atomic_s = runs * 4.0 / 25.6e9;
+    const double slices = static_cast<double>(l.num_slices);
+    const double touched = slices * (1.0 - std::exp(-runs / slices));
+    const double slice_bytes = 32.0 * static_cast<double>(1u << l.slice_bits);
+    const double sorted_s = runs * 45e-12 + touched * slice_bytes * 2.0 / 4.0e12 + 80e-6;
+    return sorted_s < 0.8 * atomic_s;
+}
+
+bool capturing(hipStream_t stream) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+// Decides the path of a lane-strip insert over `tiles` wave tiles run by
+// `blocks` blocks (4 regions each; `slack` scales the tiles a region
+// reserves for); *layout is filled when the sorted path is taken.
+int choose_sorted(const cusbf_filter* f, uint64_t windows, uint64_t tiles, uint32_t blocks,
+                  double slack, hipStream_t stream, sbf::SortedLayout* layout, bool* use) {
+    *use = false;
+    if (f->insert_path == CUSBF_INSERT_ATOMIC || windows == 0 || capturing(stream)) return CUSBF_OK;
+    const bool forced = f->insert_path == CUSBF_INSERT_SORTED;
+    const uint64_t regions = 4ull * blocks;
+    const uint64_t per_region = static_cast<uint64_t>(
+        std::ceil(static_cast<double>((tiles + regions - 1) / regions) * slack));
+    const uint64_t cap = std::max<uint64_t>(1, per_region * reserve_per_tile(f));
+    if (regions * cap >= (1ull << 31)) {
+        if (forced) return fail(CUSBF_ERR_RESOURCE, "insert is too large for one sorted insert");
+        return CUSBF_OK;
+    }
+    const sbf::SortedLayout l = sbf::sorted::sorted_layout(static_cast<uint32_t>(regions),
+                                                           static_cast<uint32_t>(cap), f->num_shards,
+                                                           f->slice_bits);
+    if (!forced && !sorted_pays(f, windows, l)) return CUSBF_OK;
+    if (l.bytes > f->sorted_bytes) {
+        size_t free_b = 0, total_b = 0;
+        const bool fits = hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+                          l.bytes <= (static_cast<uint64_t>(free_b) + f->sorted_bytes) / 2;
+        if (!fits) {
+            if (forced) return fail(CUSBF_ERR_RESOURCE, "sorted insert workspace does not fit in device memory");
+            return CUSBF_OK;
+        }
+    }
+    *layout = l;
+    *use = true;
     return CUSBF_OK;
+}
+
+// Makes the filter's workspace hold `bytes` and orders this stream after the
+// previous sorted insert (which may have run on another stream).
+int begin_sorted(cusbf_filter* f, uint64_t bytes, hipStream_t stream) {
+    if (!f->sorted_done) CUSBF_HIP_TRY(hipEventCreateWithFlags(&f->sorted_done, hipEventDisableTiming));
+    if (bytes > f->sorted_bytes) {
+        if (f->d_sorted) {
+            CUSBF_HIP_TRY(hipEventSynchronize(f->sorted_done));
+            CUSBF_HIP_TRY(hipFree(f->d_sorted));
+        }
+        f->d_sorted = nullptr;
+        f->sorted_bytes = 0;
+        CUSBF_HIP_TRY(hipMalloc(&f->d_sorted, bytes));
+        f->sorted_bytes = bytes;
+    } else if (f->sorted_used) {
+        CUSBF_HIP_TRY(hipStreamWaitEvent(stream, f->sorted_done, 0));
+    }
+    CUSBF_HIP_TRY(sbf::sorted::prepare());
+    return CUSBF_OK;
+}
+
+int end_sorted(cusbf_filter* f, hipStream_t stream) {
+    CUSBF_HIP_TRY(hipEventRecord(f->sorted_done, stream));
+    f->sorted_used = true;
+    return CUSBF_OK;
+}
+
+// Zeroes the region counts, then runs `emit(sink)` (the insert kernel writing
+// records) and the partition and apply kernels, in stream order.
+template <class Emit>
+hipError_t enqueue_sorted(const cusbf_filter* f, const sbf::SortedLayout& l, void* base,
+                          hipStream_t stream, Emit&& emit) {
+    const sbf::RecordSink sink = sbf::sorted::record_sink(l, base);
+    hipError_t e = sbf::fill_u64(reinterpret_cast<uint64_t*>(sink.counts), 0, (l.regions + 1ull) / 2, stream);
+    if (e == hipSuccess) e = emit(sink);
+    if (e == hipSuccess) e = sbf::sorted::sorted_apply(l, base, view(f), stream);
+    return e;
+}
+
+// Blocks of a sorted plain insert: 16 per compute unit at most, so that every
+// region reserves for a few hundred tiles.
+uint32_t plain_sorted_blocks(const cusbf_filter* f, uint64_t tiles) {
+    uint64_t blocks = (tiles + sbf::kStripWavesPerBlock - 1) / sbf::kStripWavesPerBlock;
+    blocks = std::min<uint64_t>(blocks, 16ull * f->compute_units);
+    blocks = std::min<uint64_t>(blocks, sbf::kStripMaxBlocks);
+    return static_cast<uint32_t>(std::max<uint64_t>(blocks, 1));
+}
+
+int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
+    f->last_path = CUSBF_INSERT_ATOMIC;
+    if (!strip_reads(f, in)) {
+        const hipError_t e = sbf::generic_insert(f->kc, in, view(f), stream);
+        if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+        return CUSBF_OK;
+    }
+    const uint64_t windows = in.num_symbols < f->cfg.k ? 0 : in.num_symbols - f->cfg.k + 1;
+    const uint64_t tiles = (windows + f->strip->wave_windows - 1) / f->strip->wave_windows;
+    sbf::SortedLayout l;
+    bool sorted = false;
+    int rc = choose_sorted(f, windows, tiles, plain_sorted_blocks(f, tiles), 1.0, stream, &l, &sorted);
+    if (rc != CUSBF_OK) return rc;
+    if (!sorted) {
+        const hipError_t e = f->strip->insert(in, view(f), nullptr, stream);
+        if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+        return CUSBF_OK;
+    }
+    rc = begin_sorted(f, l.bytes, stream);
+    if (rc != CUSBF_OK) return rc;
+    const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, [&](const sbf::RecordSink& sink) {
+        return f->strip->insert(in, view(f), &sink, stream);
+    });
+    if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+    f->last_path = CUSBF_INSERT_SORTED;
+    return end_sorted(f, stream);
 }
 
 int launch_contains(const cusbf_filter* f, const sbf::SequenceInput& in, uint8_t* out,
@@ -385,11 +531,13 @@ const cusbf_batch_tuning& tuning_for(const cusbf_filter* f, uint32_t op, uint64_
 }
 
 // Reset, prepass and kernel of one batched call, in stream order; allocates
-// nothing, so it can be captured into a graph.
+// nothing, so it can be captured into a graph. `sorted` (insert only, or
+// null): the layout of the sorted insert's workspace at `sorted_base`.
 int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
                     uint64_t num_symbols, const uint64_t* d_offsets, uint64_t num_records,
                     uint32_t* d_state, uint64_t* d_starts, const cusbf_batch_tuning& t,
-                    uint8_t* d_output, hipStream_t stream) {
+                    uint8_t* d_output, hipStream_t stream,
+                    const sbf::SortedLayout* sorted = nullptr, void* sorted_base = nullptr) {
     sbf::SequenceInput in = byte_input(f, d_symbols, num_symbols * f->d.symbol_width);
     in.offsets = d_offsets;
     in.num_records = num_records;
@@ -400,15 +548,33 @@ int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
     if (batched_strip(f)) {
         const int variant = t.waves_per_simd == 4 && f->strip->batched_insert[1] ? 1 : 0;
         const sbf::BatchLaunch shape{t.blocks_per_cu, f->compute_units, t.claim};
-        e = op == CUSBF_BATCH_INSERT
-                ? f->strip->batched_insert[variant](in, view(f), shape, stream)
-                : f->strip->batched_contains[variant](in, view(f), shape, d_output, stream);
+        if (op == CUSBF_BATCH_INSERT && sorted) {
+            e = enqueue_sorted(f, *sorted, sorted_base, stream, [&](const sbf::RecordSink& sink) {
+                return f->strip->batched_insert[variant](in, view(f), shape, &sink, stream);
+            });
+        } else {
+            e = op == CUSBF_BATCH_INSERT
+                    ? f->strip->batched_insert[variant](in, view(f), shape, nullptr, stream)
+                    : f->strip->batched_contains[variant](in, view(f), shape, d_output, stream);
+        }
     } else {
         e = op == CUSBF_BATCH_INSERT ? sbf::generic_insert(f->kc, in, view(f), stream)
                                      : sbf::generic_contains(f->kc, in, view(f), d_output, stream);
     }
     if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
     return CUSBF_OK;
+}
+
+// Sorted path of a batched insert: the regions are the persistent grid's
+// waves; a work queue (claim > 0) hands a wave a varying share of the tiles,
+// so its regions reserve for half as many tiles again.
+int choose_batched_sorted(const cusbf_filter* f, uint64_t num_symbols, const cusbf_batch_tuning& t,
+                          hipStream_t stream, sbf::SortedLayout* layout, bool* use) {
+    const uint64_t tiles = batch_tiles(f, num_symbols);
+    const sbf::BatchLaunch shape{t.blocks_per_cu, f->compute_units, t.claim};
+    const uint32_t blocks = sbf::batched_grid(shape, static_cast<uint32_t>(tiles));
+    return choose_sorted(f, batched_windows(f, num_symbols), tiles, blocks, t.claim ? 1.5 : 1.0,
+                         stream, layout, use);
 }
 
 int read_batch_flag(const uint32_t* d_state, hipStream_t stream) {
@@ -594,6 +760,11 @@ void cusbf_destroy(cusbf_filter* f) {
     if (f->d_scalar) (void)hipFree(f->d_scalar);
     if (f->d_batch_state) (void)hipFree(f->d_batch_state);
     if (f->d_starts) (void)hipFree(f->d_starts);
+    if (f->d_sorted) {
+        (void)hipEventSynchronize(f->sorted_done);
+        (void)hipFree(f->d_sorted);
+    }
+    if (f->sorted_done) (void)hipEventDestroy(f->sorted_done);
     delete f;
 }
 
@@ -637,6 +808,30 @@ int cusbf_copy_words_to_host(const cusbf_filter* f, uint64_t* host_words, uint64
 const char* cusbf_kernel_family(const cusbf_filter* f) {
     if (!f) return "";
     return f->strip ? "lane_strip" : "generic";
+}
+
+int cusbf_set_insert_path(cusbf_filter* f, uint32_t path, uint32_t slice_bits,
+                          uint32_t reserve_per_tile) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (path > CUSBF_INSERT_SORTED) return fail(CUSBF_ERR_INVALID_ARGUMENT, "unknown insert path");
+    if (slice_bits > 12) return fail(CUSBF_ERR_INVALID_ARGUMENT, "slice_bits must be in [1, 12]");
+    if (reserve_per_tile > 256) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "reserve_per_tile must be in [1, 256]");
+    }
+    f->insert_path = path;
+    f->slice_bits = slice_bits ? slice_bits : 12;
+    f->reserve_per_tile = reserve_per_tile;
+    return CUSBF_OK;
+}
+
+int cusbf_get_insert_path(const cusbf_filter* f, uint32_t* path, uint32_t* slice_bits,
+                          uint32_t* reserve_per_tile, uint32_t* last_path) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (path) *path = f->insert_path;
+    if (slice_bits) *slice_bits = f->slice_bits;
+    if (reserve_per_tile) *reserve_per_tile = f->reserve_per_tile;
+    if (last_path) *last_path = f->last_path;
+    return CUSBF_OK;
 }
 
 // filter.cuh:179-190
@@ -1051,10 +1246,22 @@ int cusbf_insert_batched_async(cusbf_filter* f, const char* d_symbols, uint64_t 
     if (batched_windows(f, num_symbols) == 0) return CUSBF_OK;
     rc = ensure_starts(f, num_symbols);
     if (rc != CUSBF_OK) return rc;
+    const cusbf_batch_tuning& t = tuning_for(f, CUSBF_BATCH_INSERT, num_symbols, num_records);
+    sbf::SortedLayout l;
+    bool sorted = false;
+    f->last_path = CUSBF_INSERT_ATOMIC;
+    if (batched_strip(f)) {
+        rc = choose_batched_sorted(f, num_symbols, t, as_stream(stream), &l, &sorted);
+        if (rc == CUSBF_OK && sorted) rc = begin_sorted(f, l.bytes, as_stream(stream));
+        if (rc != CUSBF_OK) return rc;
+    }
     rc = enqueue_batched(f, CUSBF_BATCH_INSERT, d_symbols, num_symbols, d_offsets, num_records,
-                         f->d_batch_state, f->d_starts,
-                         tuning_for(f, CUSBF_BATCH_INSERT, num_symbols, num_records),
-                         nullptr, as_stream(stream));
+                         f->d_batch_state, f->d_starts, t, nullptr, as_stream(stream),
+                         sorted ? &l : nullptr, f->d_sorted);
+    if (rc == CUSBF_OK && sorted) {
+        f->last_path = CUSBF_INSERT_SORTED;
+        rc = end_sorted(f, as_stream(stream));
+    }
     f->batch_check_pending = rc == CUSBF_OK;
     return rc;
 }
@@ -1116,15 +1323,26 @@ int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
         return CUSBF_OK;
     }
     hipStream_t capture = nullptr;
+    sbf::SortedLayout l;
+    bool sorted = false;
+    if (op == CUSBF_BATCH_INSERT && batched_strip(f)) {
+        rc = choose_batched_sorted(f, num_symbols, t, nullptr, &l, &sorted);
+        if (rc != CUSBF_OK) {
+            cusbf_batch_plan_destroy(p);
+            return rc;
+        }
+    }
     hipError_t e = hipMalloc(&p->d_state, 64);
     if (e == hipSuccess) e = hipMalloc(&p->d_starts, sbf::start_words(num_symbols) * sizeof(uint64_t));
+    if (e == hipSuccess && sorted) e = hipMalloc(&p->d_sorted, l.bytes);
+    if (e == hipSuccess && sorted) e = sbf::sorted::prepare();
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&capture, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(capture, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
         rc = fail_hip(e, __FILE__, __LINE__);
     } else {
         rc = enqueue_batched(f, op, d_symbols, num_symbols, d_offsets, num_records, p->d_state,
-                             p->d_starts, t, d_output, capture);
+                             p->d_starts, t, d_output, capture, sorted ? &l : nullptr, p->d_sorted);
         const hipError_t end = hipStreamEndCapture(capture, &p->graph);
         if (rc == CUSBF_OK && end != hipSuccess) rc = fail_hip(end, __FILE__, __LINE__);
         if (rc == CUSBF_OK) {
@@ -1162,6 +1380,7 @@ void cusbf_batch_plan_destroy(cusbf_batch_plan* p) {
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->d_state) (void)hipFree(p->d_state);
     if (p->d_starts) (void)hipFree(p->d_starts);
+    if (p->d_sorted) (void)hipFree(p->d_sorted);
     delete p;
 }
 

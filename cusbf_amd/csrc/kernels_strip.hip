@@ -7,7 +7,7 @@ namespace sbf::strip {
 
 #define SBF_STRIP_EXTERN(FE, K, S, M, H, W)                                                   \
     extern template hipError_t launch_batched_insert<K, S, M, H, FE, W>(                      \
-        const SequenceInput&, FilterView, const BatchLaunch&, hipStream_t);                  \
+        const SequenceInput&, FilterView, const BatchLaunch&, const RecordSink*, hipStream_t); \
     extern template hipError_t launch_batched_contains<K, S, M, H, FE, W>(                    \
         const SequenceInput&, FilterView, const BatchLaunch&, uint8_t*, hipStream_t);
 SBF_STRIP_BATCHED_INSTANCES(SBF_STRIP_EXTERN)

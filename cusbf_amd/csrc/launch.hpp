@@ -43,10 +43,24 @@ struct BatchLaunch {
     uint32_t claim = 1;  // tiles per work-queue claim; 0 = waves stride over the tiles
 };
 
-using InsertFn = hipError_t (*)(const SequenceInput&, FilterView, hipStream_t);
+// Sorted insert: where the lane-strip insert kernels write their runs instead
+// of applying them with atomics. Wave w of the grid (block * 4 + wave) owns
+// region w, slots [w * cap, (w + 1) * cap): slot i holds the pair
+// (shard << 32) | i and the run's OR mask (4 words at masks + 4 i); the wave
+// writes its record count to counts[w]. Runs past the region's capacity use
+// atomics. The launch uses at most regions / 4 blocks.
+struct RecordSink {
+    uint64_t* pairs = nullptr;
+    uint64_t* masks = nullptr;
+    uint32_t* counts = nullptr;
+    uint32_t cap = 0;
+    uint32_t regions = 0;
+};
+
+using InsertFn = hipError_t (*)(const SequenceInput&, FilterView, const RecordSink*, hipStream_t);
 using ContainsFn = hipError_t (*)(const SequenceInput&, FilterView, uint8_t*, hipStream_t);
 using BatchedInsertFn = hipError_t (*)(const SequenceInput&, FilterView, const BatchLaunch&,
-                                       hipStream_t);
+                                       const RecordSink*, hipStream_t);
 using BatchedContainsFn = hipError_t (*)(const SequenceInput&, FilterView, const BatchLaunch&,
                                          uint8_t*, hipStream_t);
 
@@ -64,6 +78,51 @@ struct StripKernelSet {
 // Returns the compiled lane-strip kernel set or nullptr.
 const StripKernelSet* find_strip_kernels(uint32_t k, uint32_t s, uint32_t m, uint32_t h,
                                          uint32_t alphabet);
+
+// Blocks of the plain lane-strip kernels (a dispatch stays below 2^32
+// work-items) and of the persistent batched kernels.
+constexpr uint64_t kStripMaxBlocks = 1ull << 16;
+constexpr uint32_t kStripWavesPerBlock = 4;
+inline uint32_t batched_grid(const BatchLaunch& b, uint32_t num_tiles) {
+    uint64_t blocks = static_cast<uint64_t>(b.blocks_per_cu) * b.compute_units;
+    const uint64_t need = (static_cast<uint64_t>(num_tiles) + kStripWavesPerBlock - 1) / kStripWavesPerBlock;
+    if (blocks > need) blocks = need;
+    if (blocks > kStripMaxBlocks) blocks = kStripMaxBlocks;
+    if (blocks == 0) blocks = 1;
+    return static_cast<uint32_t>(blocks);
+}
+
+// Sorted insert (sorted_insert.hip): one device allocation holding the record
+// buffer of `regions` x `cap` slots (pairs twice, for the radix passes, and
+// masks), the region counts, the radix histograms and scan partials, and the
+// slice bounds. The partition groups the pairs by shard >> slice_bits, in
+// ceil((log2(num_shards) - slice_bits) / 8) passes (at least one).
+struct SortedLayout {
+    uint32_t regions = 0;
+    uint32_t cap = 0;
+    uint64_t slots = 0;
+    uint32_t slice_bits = 0;        // shards per slice = 2^slice_bits (32 B each in LDS)
+    uint32_t slice_index_bits = 0;  // log2(num_shards) - slice_bits
+    uint32_t num_slices = 0;
+    uint32_t passes = 0;
+    uint64_t hist_len = 0;
+    uint32_t partials_len = 0;
+    uint64_t off_pairs[2] = {0, 0};
+    uint64_t off_masks = 0, off_counts = 0, off_hist = 0, off_partials = 0, off_totals = 0,
+             off_bounds = 0;
+    uint64_t bytes = 0;
+};
+
+namespace sorted {
+// regions * cap must stay below 2^31; slice_bits in [1, 12].
+SortedLayout sorted_layout(uint32_t regions, uint32_t cap, uint64_t num_shards, uint32_t slice_bits);
+RecordSink record_sink(const SortedLayout& l, void* base);
+// Allows the apply kernel its LDS slice; call before launching (outside any capture).
+hipError_t prepare();
+// Partition the records the insert kernels wrote and OR them into the filter,
+// in stream order after those kernels.
+hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStream_t stream);
+}  // namespace sorted
 
 // Generic kernels: any alphabet / configuration, one window per thread; in the
 // batched form they test the record-start bitmap (or, without it, search the

@@ -1,0 +1,424 @@
+// sorted_insert.hip -- the sorted form of the lane-strip insert (partition and apply).
+//
+// The atomic insert issues one 64-bit device-scope atomic OR per non-zero word
+// of every shard run (filter_ref.cuh:94-108 issues them per warp run). On this
+// GPU each such atomic is performed at the memory side as a read-modify-write
+// of DRAM, which caps the insert at about 6.4 G runs/s whatever the kernel
+// does. The sorted form produces the same filter words with ordinary,
+// coalesced loads and stores:
+//   1. emit (strip_kernels.hpp, insert_tile): each wave writes its tile's
+//      slotted runs as records into its own region of a record buffer -- an
+//      8-byte pair (shard << 32 | record index) and the run's 32-byte OR mask;
+//      runs beyond the region's capacity (and a tile's runs beyond its LDS
+//      slots) still take the atomic path;
+//   2. partition (here): stable LSD radix passes, 8 bits each, over the pairs,
+//      keyed on the slice index shard >> slice_bits, until the pairs are
+//      grouped by slice;
+//   3. apply (here): one block per non-empty slice loads the slice's words
+//      into LDS, ORs the masks of its records into them with LDS atomics and
+//      stores the slice back.
+// OR is commutative and idempotent, so the words equal the atomic insert's;
+// the atomics of step 1 land before step 3 reads the slice (stream order).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "launch.hpp"
+
+namespace sbf::sorted {
+
+namespace {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRankBlock = 256;                       // threads of histogram / scatter blocks
+constexpr int kRankItems = 16;                        // pairs per thread
+constexpr int kBlockItems = kRankBlock * kRankItems;  // pairs per block
+constexpr int kBins = 256;                            // 8-bit digits
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = kScanBlock * kScanItems;
+constexpr int kTopBlock = 1024;
+constexpr int kApplyBlock = 1024;
+constexpr uint32_t kApplyGrid = 1u << 16;
+
+// Input of one pass: slot i of `pairs` holds a record when it lies below its
+// region's count (the emit buffer has `regions` regions of `cap` slots; a
+// compacted buffer is one region whose count is the record total).
+struct PassInput {
+    const uint64_t* pairs;
+    uint32_t slots;
+    uint32_t cap;
+    const uint32_t* counts;
+};
+
+__device__ __forceinline__ bool slot_valid(const PassInput& in, uint32_t i) {
+    return i < in.slots && i % in.cap < in.counts[i / in.cap];
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t pair, uint32_t shift, uint32_t mask) {
+    return static_cast<uint32_t>(pair >> shift) & mask;
+}
+
+// Exclusive prefix of v over the block (kThreads threads, a multiple of 64)
+// and the block total. `tmp` holds kThreads / 64 words; every thread must call.
+template <int kThreads>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* tmp, uint32_t& total) {
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) tmp[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        const uint32_t s = tmp[w];
+        before += w < wave ? s : 0u;
+        total += s;
+    }
+    __syncthreads();  // tmp may be reused by the caller
+    return before + x - v;
+}
+
+// hist[d * nblocks + b] = records of block b with digit d.
+__global__ __launch_bounds__(kRankBlock) void hist_kernel(PassInput in, uint32_t shift,
+                                                          uint32_t digit_mask, uint32_t nblocks,
+                                                          uint32_t* hist) {
+    __shared__ uint32_t h[kBins];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kBlockItems;
+#pragma unroll 4
+    for (int j = 0; j < kRankItems; ++j) {
+        const uint32_t i = base + static_cast<uint32_t>(j * kRankBlock) + threadIdx.x;
+        if (slot_valid(in, i)) atomicAdd(&h[digit_of(in.pairs[i], shift, digit_mask)], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// partials[c] = sum of chunk c of v (kScanChunk values per chunk).
+__global__ __launch_bounds__(kScanBlock) void scan_partials_kernel(const uint32_t* v, uint32_t n,
+                                                                   uint32_t* partials) {
+    __shared__ uint32_t tmp[kScanBlock / 64];
+    const uint32_t first = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (first + j < n) s += v[first + j];
+    }
+    uint32_t total;
+    (void)block_exclusive_scan<kScanBlock>(s, tmp, total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Exclusive scan of partials[0..n) in place (one block); *total = their sum.
+__global__ __launch_bounds__(kTopBlock) void scan_top_kernel(uint32_t* partials, uint32_t n,
+                                                             uint32_t* total) {
+    __shared__ uint32_t tmp[kTopBlock / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += kTopBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t x = i < n ? partials[i] : 0u;
+        uint32_t sum;
+        const uint32_t pre = block_exclusive_scan<kTopBlock>(x, tmp, sum);
+        if (i < n) partials[i] = carry + pre;
+        carry += sum;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+// Exclusive scan of v in place, chunk c offset by partials[c].
+__global__ __launch_bounds__(kScanBlock) void scan_finish_kernel(uint32_t* v, uint32_t n,
+                                                                 const uint32_t* partials) {
+    __shared__ uint32_t tmp[kScanBlock / 64];
+    const uint32_t first = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+    uint32_t x[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        x[j] = first + j < n ? v[first + j] : 0u;
+        s += x[j];
+    }
+    uint32_t total;
+    uint32_t run = partials[blockIdx.x] + block_exclusive_scan<kScanBlock>(s, tmp, total);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (first + j < n) v[first + j] = run;
+        run += x[j];
+    }
+}
+
+// Stable scatter of one LSD pass. Records of block b keep their input order
+// within each digit: round j ranks pairs base + j*256 + t in thread order, so
+// the ranks follow the slot order. The block's records are first placed in
+// digit order in LDS, then written out in order, so each digit's records of
+// the block go out as one contiguous run.
+__global__ __launch_bounds__(kRankBlock) void scatter_kernel(PassInput in, uint32_t shift,
+                                                             uint32_t digit_bits, uint32_t nblocks,
+                                                             const uint32_t* offsets, uint64_t* out) {
+    __shared__ uint32_t running[kBins];
+    __shared__ uint32_t wcount[kRankBlock / 64][kBins];
+    __shared__ uint32_t start[kBins];
+    __shared__ uint32_t dest[kBins];
+    __shared__ uint32_t tmp[kRankBlock / 64];
+    __shared__ uint64_t sorted[kBlockItems];
+    const int t = static_cast<int>(threadIdx.x);
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    const uint32_t mask = (1u << digit_bits) - 1u;
+    running[t] = 0;
+#pragma unroll
+    for (int w = 0; w < kRankBlock / 64; ++w) wcount[w][t] = 0;
+    dest[t] = offsets[static_cast<uint32_t>(t) * nblocks + blockIdx.x];
+    __syncthreads();
+
+    const uint32_t base = blockIdx.x * kBlockItems;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t item[kRankItems];
+    uint32_t rank[kRankItems];
+#pragma unroll
+    for (int j = 0; j < kRankItems; ++j) {
+        const uint32_t i = base + static_cast<uint32_t>(j * kRankBlock + t);
+        const bool v = slot_valid(in, i);
+        const uint64_t p = v ? in.pairs[i] : 0ull;
+        const uint32_t d = digit_of(p, shift, mask);
+        uint64_t peers = __ballot(v);
+        for (uint32_t b = 0; b < digit_bits; ++b) {
+            const bool bit = ((d >> b) & 1u) != 0u;
+            const uint64_t set = __ballot(bit);
+            peers &= bit ? set : ~set;
+        }
+        const uint32_t r = static_cast<uint32_t>(__popcll(peers & below));
+        if (v && r == 0) wcount[wave][d] = static_cast<uint32_t>(__popcll(peers));
+        __syncthreads();
+        uint32_t pre = running[d];
+#pragma unroll
+        for (int w = 0; w < kRankBlock / 64; ++w) pre += w < wave ? wcount[w][d] : 0u;
+        item[j] = p;
+        rank[j] = v ? pre + r : 0xFFFFFFFFu;
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < kRankBlock / 64; ++w) {
+            add += wcount[w][t];
+            wcount[w][t] = 0;
+        }
+        running[t] += add;
+        __syncthreads();
+    }
+
+    uint32_t total;
+    start[t] = block_exclusive_scan<kRankBlock>(running[t], tmp, total);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRankItems; ++j) {
+        if (rank[j] != 0xFFFFFFFFu) sorted[start[digit_of(item[j], shift, mask)] + rank[j]] = item[j];
+    }
+    __syncthreads();
+    for (uint32_t x = static_cast<uint32_t>(t); x < total; x += kRankBlock) {
+        const uint64_t p = sorted[x];
+        const uint32_t d = digit_of(p, shift, mask);
+        out[dest[d] + (x - start[d])] = p;
+    }
+}
+
+// bounds[b] = first record of slice b in the grouped pairs, bounds[num_slices]
+// = record total. Record i writes the starts of the slices after its
+// predecessor's slice up to its own; slot `total` closes the table.
+__global__ __launch_bounds__(256) void bounds_kernel(const uint64_t* pairs, const uint32_t* total,
+                                                     uint32_t shift, uint32_t num_slices,
+                                                     uint32_t* bounds) {
+    const uint32_t n = *total;
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    uint32_t cur = num_slices;
+    if (i < n) {
+        const uint64_t s = pairs[i] >> shift;
+        cur = s < num_slices ? static_cast<uint32_t>(s) : num_slices;
+    }
+    uint32_t from = 0;
+    if (i > 0) {
+        const uint64_t s = pairs[i - 1] >> shift;
+        from = (s < num_slices ? static_cast<uint32_t>(s) : num_slices) + 1;
+    }
+    for (uint32_t b = from; b <= cur; ++b) bounds[b] = static_cast<uint32_t>(i);
+}
+
+// One block per non-empty slice at a time (blocks stride over the slices):
+// slice words -> LDS, OR of the slice's records, LDS -> slice words.
+__global__ __launch_bounds__(kApplyBlock) void apply_kernel(const uint64_t* pairs,
+                                                            const uint32_t* bounds,
+                                                            const u32x4* masks, uint64_t* words,
+                                                            uint32_t slice_bits,
+                                                            uint32_t num_slices) {
+    extern __shared__ uint64_t slice[];  // 4 << slice_bits words
+    u32x4* lds4 = reinterpret_cast<u32x4*>(slice);
+    const uint32_t vecs = 2u << slice_bits;  // 16-byte vectors of one slice
+    const uint32_t local_mask = (1u << slice_bits) - 1u;
+    constexpr int kUnroll = 4;
+    for (uint32_t b = blockIdx.x; b < num_slices; b += gridDim.x) {
+        const uint32_t lo = bounds[b];
+        const uint32_t hi = bounds[b + 1];
+        if (lo >= hi) continue;  // uniform: every thread reads the same bounds
+        u32x4* g = reinterpret_cast<u32x4*>(words + (static_cast<uint64_t>(b) << slice_bits) * 4ull);
+        for (uint32_t x = threadIdx.x; x < vecs; x += kApplyBlock) lds4[x] = g[x];
+        __syncthreads();
+        for (uint32_t i0 = lo; i0 < hi; i0 += kUnroll * kApplyBlock) {
+            uint64_t p[kUnroll];
+            u32x4 a[kUnroll], c[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const uint32_t i = i0 + static_cast<uint32_t>(u * kApplyBlock) + threadIdx.x;
+                p[u] = i < hi ? pairs[i] : ~0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (p[u] != ~0ull) {
+                    const uint64_t idx = static_cast<uint32_t>(p[u]);
+                    a[u] = masks[2 * idx];
+                    c[u] = masks[2 * idx + 1];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (p[u] != ~0ull) {
+                    uint64_t* s = slice + 4u * (static_cast<uint32_t>(p[u] >> 32) & local_mask);
+                    const uint64_t m[4] = {a[u].x | (static_cast<uint64_t>(a[u].y) << 32),
+                                           a[u].z | (static_cast<uint64_t>(a[u].w) << 32),
+                                           c[u].x | (static_cast<uint64_t>(c[u].y) << 32),
+                                           c[u].z | (static_cast<uint64_t>(c[u].w) << 32)};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (m[q] != 0) {
+                            __hip_atomic_fetch_or(s + q, m[q], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t x = threadIdx.x; x < vecs; x += kApplyBlock) g[x] = lds4[x];
+        __syncthreads();  // the next slice reuses the LDS
+    }
+}
+
+uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
+
+uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
+uint32_t log2_u64(uint64_t x) {
+    uint32_t r = 0;
+    while ((1ull << r) < x) ++r;
+    return r;
+}
+
+}  // namespace
+
+SortedLayout sorted_layout(uint32_t regions, uint32_t cap, uint64_t num_shards, uint32_t slice_bits) {
+    SortedLayout l{};
+    l.regions = regions;
+    l.cap = cap;
+    l.slots = static_cast<uint64_t>(regions) * cap;
+    const uint32_t shard_bits = log2_u64(num_shards);
+    l.slice_bits = slice_bits < shard_bits ? slice_bits : shard_bits;
+    l.slice_index_bits = shard_bits - l.slice_bits;
+    l.num_slices = 1u << l.slice_index_bits;
+    l.passes = l.slice_index_bits == 0 ? 1u : (l.slice_index_bits + 7) / 8;
+    const uint32_t nblocks = div_up(l.slots, kBlockItems);
+    l.hist_len = static_cast<uint64_t>(kBins) * nblocks;
+    l.partials_len = div_up(l.hist_len, kScanChunk);
+    uint64_t at = 0;
+    l.off_pairs[0] = at;
+    at += align256(l.slots * 8);
+    l.off_pairs[1] = at;
+    at += align256(l.slots * 8);
+    l.off_masks = at;
+    at += align256(l.slots * 32);
+    l.off_counts = at;
+    at += align256(static_cast<uint64_t>(regions) * 4);
+    l.off_hist = at;
+    at += align256(l.hist_len * 4);
+    l.off_partials = at;
+    at += align256(static_cast<uint64_t>(l.partials_len) * 4);
+    l.off_totals = at;
+    at += 256;
+    l.off_bounds = at;
+    at += align256((static_cast<uint64_t>(l.num_slices) + 1) * 4);
+    l.bytes = at;
+    return l;
+}
+
+RecordSink record_sink(const SortedLayout& l, void* base) {
+    char* p = static_cast<char*>(base);
+    RecordSink s;
+    s.pairs = reinterpret_cast<uint64_t*>(p + l.off_pairs[0]);
+    s.masks = reinterpret_cast<uint64_t*>(p + l.off_masks);
+    s.counts = reinterpret_cast<uint32_t*>(p + l.off_counts);
+    s.cap = l.cap;
+    s.regions = l.regions;
+    return s;
+}
+
+hipError_t prepare() {
+    // the apply kernel's LDS slice: up to 2^12 shards of 32 bytes
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&apply_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 32 << 12);
+}
+
+hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStream_t stream) {
+    char* p = static_cast<char*>(base);
+    uint64_t* pairs[2] = {reinterpret_cast<uint64_t*>(p + l.off_pairs[0]),
+                          reinterpret_cast<uint64_t*>(p + l.off_pairs[1])};
+    const u32x4* masks = reinterpret_cast<const u32x4*>(p + l.off_masks);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(p + l.off_hist);
+    uint32_t* partials = reinterpret_cast<uint32_t*>(p + l.off_partials);
+    uint32_t* totals = reinterpret_cast<uint32_t*>(p + l.off_totals);
+    uint32_t* bounds = reinterpret_cast<uint32_t*>(p + l.off_bounds);
+
+    (void)hipGetLastError();  // report only these launches
+    PassInput in{pairs[0], static_cast<uint32_t>(l.slots), l.cap,
+                 reinterpret_cast<const uint32_t*>(p + l.off_counts)};
+    const uint32_t nblocks = div_up(l.slots, kBlockItems);
+    const uint32_t hist_len = static_cast<uint32_t>(l.hist_len);
+    int cur = 0;
+    for (uint32_t pass = 0; pass < l.passes; ++pass) {
+        const uint32_t done = 8 * pass;
+        const uint32_t bits = l.slice_index_bits > done
+                                  ? (l.slice_index_bits - done < 8 ? l.slice_index_bits - done : 8u)
+                                  : 0u;
+        const uint32_t shift = 32 + l.slice_bits + done;
+        const uint32_t mask = (1u << bits) - 1u;
+        uint32_t* total = totals + (pass & 1u);
+        hipLaunchKernelGGL(hist_kernel, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift, mask,
+                           nblocks, hist);
+        hipLaunchKernelGGL(scan_partials_kernel, dim3(l.partials_len), dim3(kScanBlock), 0, stream,
+                           hist, hist_len, partials);
+        hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, partials,
+                           l.partials_len, total);
+        hipLaunchKernelGGL(scan_finish_kernel, dim3(l.partials_len), dim3(kScanBlock), 0, stream,
+                           hist, hist_len, partials);
+        hipLaunchKernelGGL(scatter_kernel, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift, bits,
+                           nblocks, hist, pairs[cur ^ 1]);
+        cur ^= 1;
+        // the next pass reads the compacted pairs: one region holding `total` records
+        in = PassInput{pairs[cur], static_cast<uint32_t>(l.slots), static_cast<uint32_t>(l.slots),
+                       total};
+    }
+    const uint32_t* total = in.counts;
+    hipLaunchKernelGGL(bounds_kernel, dim3(div_up(l.slots + 1, 256)), dim3(256), 0, stream, pairs[cur],
+                       total, 32 + l.slice_bits, l.num_slices, bounds);
+    const uint32_t grid = l.num_slices < kApplyGrid ? l.num_slices : kApplyGrid;
+    hipLaunchKernelGGL(apply_kernel, dim3(grid), dim3(kApplyBlock), (32u << l.slice_bits), stream,
+                       pairs[cur], bounds, masks, f.words, l.slice_bits, l.num_slices);
+    return hipGetLastError();
+}
+
+}  // namespace sbf::sorted

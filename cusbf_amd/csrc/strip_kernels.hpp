@@ -675,10 +675,14 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
     }
 }
 
+// Sorted insert: `sink.pairs` non-null makes the wave write each slotted run
+// as a record into its region (`region`, `emitted` records so far) instead of
+// issuing the run's atomics; runs past the region's capacity keep the atomics.
 template <int K, int S, int M, int H, class FE, bool Dense, bool Batched>
 __device__ __forceinline__ void insert_tile(const SequenceInput& in, const FilterView& filter,
                                             uint64_t num_windows, uint64_t tile, int lane,
-                                            WaveLds<FE, Batched>& L) {
+                                            WaveLds<FE, Batched>& L, const RecordSink& sink,
+                                            uint32_t region, uint32_t& emitted) {
     using Sh = Shape<K, S, M, H, FE>;
     constexpr int F = Sh::kFindereSpan;
     const uint64_t tile_first = tile * Sh::kWaveWindows;
@@ -748,20 +752,32 @@ __device__ __forceinline__ void insert_tile(const SequenceInput& in, const Filte
     }
     wave_lds_sync();
 
-    // dense global atomics: one 64-bit OR per non-zero word of each slotted run
+    // dense global atomics: one 64-bit OR per non-zero word of each slotted
+    // run; or, sorted insert, one record per slotted run with coalesced stores
     const uint32_t n_slots = total < kSlots ? total : kSlots;
+    const uint32_t room = sink.pairs != nullptr ? sink.cap - emitted : 0u;
 #pragma unroll
     for (int j = 0; j < kSlots / kWave; ++j) {
         const uint32_t q = static_cast<uint32_t>(lane + j * kWave);
         if (q < n_slots) {
-            uint64_t* dst = filter.words + 4ull * L.slot_shard[q];
+            if (q < room) {
+                const uint64_t at = static_cast<uint64_t>(region) * sink.cap + emitted + q;
+                sink.pairs[at] = (static_cast<uint64_t>(L.slot_shard[q]) << 32) | at;
+                const u32x4* src = reinterpret_cast<const u32x4*>(L.slot_w[q]);
+                u32x4* dst = reinterpret_cast<u32x4*>(sink.masks + 4 * at);
+                dst[0] = src[0];
+                dst[1] = src[1];
+            } else {
+                uint64_t* dst = filter.words + 4ull * L.slot_shard[q];
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                const uint64_t m = L.slot_w[q][w];
-                if (m != 0) __hip_atomic_fetch_or(dst + w, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int w = 0; w < 4; ++w) {
+                    const uint64_t m = L.slot_w[q][w];
+                    if (m != 0) __hip_atomic_fetch_or(dst + w, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
         }
     }
+    emitted += n_slots < room ? n_slots : room;
     wave_lds_sync();  // the next tile of this wave reuses the LDS
 }
 
@@ -784,15 +800,18 @@ __global__ __launch_bounds__(kBlock, WPE) void contains_kernel(SequenceInput in,
 template <int K, int S, int M, int H, class FE, bool Dense, int WPE>
 __global__ __launch_bounds__(kBlock, WPE) void insert_kernel(SequenceInput in, FilterView filter,
                                                              uint64_t num_windows,
-                                                             uint64_t num_tiles) {
+                                                             uint64_t num_tiles, RecordSink sink) {
     __shared__ WaveLds<FE, false> lds[kWavesPerBlock];
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const uint32_t region = blockIdx.x * kWavesPerBlock + static_cast<uint32_t>(wave);
+    uint32_t emitted = 0;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
-    for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wave; tile < num_tiles;
-         tile += stride) {
-        insert_tile<K, S, M, H, FE, Dense, false>(in, filter, num_windows, tile, lane, lds[wave]);
+    for (uint64_t tile = region; tile < num_tiles; tile += stride) {
+        insert_tile<K, S, M, H, FE, Dense, false>(in, filter, num_windows, tile, lane, lds[wave], sink,
+                                                  region, emitted);
     }
+    if (sink.pairs != nullptr && lane == 0) sink.counts[region] = emitted;
 }
 
 // --- batched kernels: persistent waves over the tiles ---------------------------------
@@ -851,30 +870,42 @@ __global__ __launch_bounds__(kBlock, WPE) void batched_insert_kernel(SequenceInp
                                                                      FilterView filter,
                                                                      uint64_t num_windows,
                                                                      uint32_t num_tiles,
-                                                                     uint32_t claim) {
+                                                                     uint32_t claim, RecordSink sink) {
     __shared__ WaveLds<FE, true> lds[kWavesPerBlock];
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = static_cast<int>(threadIdx.x) / kWave;
-    if (__builtin_amdgcn_readfirstlane(in.state[1]) != 0) return;  // malformed offsets
-    init_wave_lds(lds[wave], lane);
-    for_each_tile<Queue>(in.state, num_tiles, claim, lane, wave, [&](uint32_t tile) {
-        insert_tile<K, S, M, H, FE, false, true>(in, filter, num_windows, tile, lane, lds[wave]);
-    });
+    const uint32_t region = blockIdx.x * kWavesPerBlock + static_cast<uint32_t>(wave);
+    uint32_t emitted = 0;
+    if (__builtin_amdgcn_readfirstlane(in.state[1]) == 0) {  // else malformed offsets: no work
+        init_wave_lds(lds[wave], lane);
+        for_each_tile<Queue>(in.state, num_tiles, claim, lane, wave, [&](uint32_t tile) {
+            insert_tile<K, S, M, H, FE, false, true>(in, filter, num_windows, tile, lane, lds[wave],
+                                                     sink, region, emitted);
+        });
+    }
+    if (sink.pairs != nullptr && lane == 0) sink.counts[region] = emitted;
 }
 
 // --- host launchers ----------------------------------------------------------
 
-// Grid of at most kMaxBlocks blocks (a dispatch must stay below 2^32 work-items);
-// waves stride over the tiles.
-constexpr uint64_t kMaxBlocks = 1ull << 16;
+// Grid of at most kStripMaxBlocks blocks (a dispatch must stay below 2^32
+// work-items); waves stride over the tiles.
+static_assert(kWavesPerBlock == kStripWavesPerBlock, "launch.hpp block shape");
 
 template <int K, int S, int M, int H, class FE>
 uint32_t tile_grid(uint64_t num_windows, uint64_t& num_tiles) {
     using Sh = Shape<K, S, M, H, FE>;
     num_tiles = (num_windows + Sh::kWaveWindows - 1) / Sh::kWaveWindows;
     uint64_t blocks = (num_tiles + kWavesPerBlock - 1) / kWavesPerBlock;
-    if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+    if (blocks > kStripMaxBlocks) blocks = kStripMaxBlocks;
     return static_cast<uint32_t>(blocks);
+}
+
+// Sorted insert: never more blocks than the sink has regions for.
+inline uint32_t sink_grid(uint32_t grid, const RecordSink* sink) {
+    if (sink == nullptr) return grid;
+    const uint32_t most = sink->regions / kWavesPerBlock;
+    return grid < most ? grid : most;
 }
 
 template <int K>
@@ -890,22 +921,24 @@ constexpr int plain_waves() {
 }
 
 template <int K, int S, int M, int H, class FE>
-hipError_t launch_insert(const SequenceInput& in, FilterView f, hipStream_t stream) {
+hipError_t launch_insert(const SequenceInput& in, FilterView f, const RecordSink* sink,
+                         hipStream_t stream) {
     const uint64_t n = windows_of<K>(in.num_symbols);
-    if (n == 0) return hipSuccess;
-    uint64_t tiles;
-    const uint32_t grid = tile_grid<K, S, M, H, FE>(n, tiles);
+    const RecordSink s = sink ? *sink : RecordSink{};
+    uint64_t tiles = 0;
+    const uint32_t grid = n == 0 ? 0u : sink_grid(tile_grid<K, S, M, H, FE>(n, tiles), sink);
+    if (grid == 0) return hipSuccess;
     constexpr int WPE = plain_waves<FE>();
     (void)hipGetLastError();  // report only this launch
     if constexpr (FE::kBits == 2) {
         if (in.dense) {
             hipLaunchKernelGGL((insert_kernel<K, S, M, H, FE, true, WPE>), dim3(grid), dim3(kBlock), 0,
-                               stream, in, f, n, tiles);
+                               stream, in, f, n, tiles, s);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((insert_kernel<K, S, M, H, FE, false, WPE>), dim3(grid), dim3(kBlock), 0,
-                       stream, in, f, n, tiles);
+                       stream, in, f, n, tiles, s);
     return hipGetLastError();
 }
 
@@ -930,32 +963,27 @@ hipError_t launch_contains(const SequenceInput& in, FilterView f, uint8_t* out,
     return hipGetLastError();
 }
 
-// Persistent grid: blocks_per_cu blocks on each compute unit, never more
-// blocks than the tiles need.
-inline uint32_t batched_grid(const BatchLaunch& b, uint32_t num_tiles) {
-    uint64_t blocks = static_cast<uint64_t>(b.blocks_per_cu) * b.compute_units;
-    const uint64_t need = (static_cast<uint64_t>(num_tiles) + kWavesPerBlock - 1) / kWavesPerBlock;
-    if (blocks > need) blocks = need;
-    if (blocks > kMaxBlocks) blocks = kMaxBlocks;
-    if (blocks == 0) blocks = 1;
-    return static_cast<uint32_t>(blocks);
-}
-
+// Persistent grid (batched_grid in launch.hpp): blocks_per_cu blocks on each
+// compute unit, never more blocks than the tiles need. A sorted insert with no
+// windows still launches, so that every region count is written.
 template <int K, int S, int M, int H, class FE, int WPE>
 hipError_t launch_batched_insert(const SequenceInput& in, FilterView f, const BatchLaunch& b,
-                                 hipStream_t stream) {
+                                 const RecordSink* sink, hipStream_t stream) {
     using Sh = Shape<K, S, M, H, FE>;
     const uint64_t n = windows_of<K>(in.num_symbols);
-    if (n == 0) return hipSuccess;
+    if (n == 0 && sink == nullptr) return hipSuccess;
+    const RecordSink s = sink ? *sink : RecordSink{};
     const uint32_t tiles = static_cast<uint32_t>((n + Sh::kWaveWindows - 1) / Sh::kWaveWindows);
-    const dim3 grid(batched_grid(b, tiles));
+    const uint32_t blocks = sink_grid(batched_grid(b, tiles), sink);
+    if (blocks == 0) return hipSuccess;
+    const dim3 grid(blocks);
     (void)hipGetLastError();  // report only this launch
     if (b.claim == 0) {
         hipLaunchKernelGGL((batched_insert_kernel<K, S, M, H, FE, WPE, false>), grid, dim3(kBlock), 0,
-                           stream, in, f, n, tiles, 0u);
+                           stream, in, f, n, tiles, 0u, s);
     } else {
         hipLaunchKernelGGL((batched_insert_kernel<K, S, M, H, FE, WPE, true>), grid, dim3(kBlock), 0,
-                           stream, in, f, n, tiles, b.claim);
+                           stream, in, f, n, tiles, b.claim, s);
     }
     return hipGetLastError();
 }

@@ -99,6 +99,20 @@ class _DeviceArray:
                                          "version": 3, "strides": None}
 
 
+# Insert paths of the lane-strip kernels (include/cusbf_hip.h, CUSBF_INSERT_*).
+INSERT_AUTO = 0
+INSERT_ATOMIC = 1
+INSERT_SORTED = 2
+
+
+@dataclass
+class InsertPathInfo:
+    path: int
+    slice_bits: int
+    reserve_per_tile: int
+    last_path: int  # INSERT_ATOMIC or INSERT_SORTED: the path the last insert call took
+
+
 class Filter:
     """GPU-resident Super Bloom filter (filter.cuh:68-1848)."""
 
@@ -172,6 +186,19 @@ class Filter:
 
     def kernel_family(self) -> str:
         return self._lib.cusbf_kernel_family(self._h).decode()
+
+    # insert path (no reference counterpart) ------------------------------------
+    def set_insert_path(self, path: int, slice_bits: int = 0, reserve_per_tile: int = 0) -> None:
+        """INSERT_AUTO / INSERT_ATOMIC / INSERT_SORTED for the lane-strip inserts;
+        slice_bits (1..12, 0 = 12) and records reserved per wave tile (1..256, 0 =
+        from the minimizer span) shape the sorted path."""
+        check(self._lib, self._lib.cusbf_set_insert_path(self._h, int(path), int(slice_bits),
+                                                         int(reserve_per_tile)))
+
+    def insert_path(self) -> InsertPathInfo:
+        v = [ctypes.c_uint32() for _ in range(4)]
+        check(self._lib, self._lib.cusbf_get_insert_path(self._h, *[ctypes.byref(x) for x in v]))
+        return InsertPathInfo(*(int(x.value) for x in v))
 
     def words_device_ptr(self) -> int:
         p = ctypes.c_void_p()

@@ -729,6 +729,27 @@ class filter {
     }
     [[nodiscard]] cusbf_filter* native_handle() const noexcept { return handle_; }
 
+    // --- insert path (no reference counterpart; cusbf_set_insert_path) ---
+    enum class InsertPath : uint32_t {
+        automatic = CUSBF_INSERT_AUTO,
+        atomic = CUSBF_INSERT_ATOMIC,
+        sorted = CUSBF_INSERT_SORTED
+    };
+    /// Atomic ORs per shard word, or records partitioned by filter slice and
+    /// applied in LDS; both give identical words. slice_bits 1..12 (0 = 12),
+    /// reserve_per_tile 1..256 (0 = from the minimizer span).
+    [[nodiscard]] Result<void> set_insert_path(InsertPath path, uint32_t slice_bits = 0,
+                                               uint32_t reserve_per_tile = 0) {
+        return detail::check(cusbf_set_insert_path(handle_, static_cast<uint32_t>(path), slice_bits,
+                                                   reserve_per_tile));
+    }
+    /// The path the last insert call took (atomic or sorted).
+    [[nodiscard]] InsertPath last_insert_path() const {
+        uint32_t last = CUSBF_INSERT_ATOMIC;
+        (void)cusbf_get_insert_path(handle_, nullptr, nullptr, nullptr, &last);
+        return static_cast<InsertPath>(last);
+    }
+
     // --- filter images (no reference counterpart) ---
     /// Writes header + words to `path` (cusbf_save).
     [[nodiscard]] Result<void> save(const std::filesystem::path& path) const {

@@ -232,6 +232,35 @@ int cusbf_count_windows_runs_async(const cusbf_filter* f, const char* d_sequence
 /* Name of the kernel family the filter dispatches to ("lane_strip" or "generic"). */
 const char* cusbf_kernel_family(const cusbf_filter* f);
 
+/* --- insert path (no reference counterpart) ---------------------------------- */
+/*
+ * The lane-strip inserts apply a shard run either with one 64-bit device-scope
+ * atomic OR per non-zero shard word (ATOMIC: the reference's method,
+ * filter_ref.cuh:94-108; on MI355X each atomic is a memory-side
+ * read-modify-write), or SORTED: the insert kernel writes each run as a
+ * record (shard, 32-byte OR mask), radix passes group the records by filter
+ * slice (2^slice_bits shards), and one block per touched slice ORs its records
+ * into the slice in LDS and writes it back. Both give identical filter words.
+ * SORTED needs a workspace of about (48 bytes x reserved records) that the
+ * filter keeps (grow-only; freed by cusbf_destroy); runs beyond a wave's
+ * reservation, and beyond the 256 LDS run slots of a wave tile, still use
+ * atomics. AUTO (the default) takes SORTED when a cost model says it is faster
+ * and the workspace fits in half the free device memory, else ATOMIC. Inserts
+ * recorded into a stream capture and the generic kernels always use ATOMIC.
+ * Sorted inserts on one filter share its workspace: one on another stream
+ * waits for the previous one (stream-ordered, no host wait).
+ */
+#define CUSBF_INSERT_AUTO 0u
+#define CUSBF_INSERT_ATOMIC 1u
+#define CUSBF_INSERT_SORTED 2u
+/* slice_bits: 1..12 (0 = 12); reserve_per_tile: records reserved per wave tile,
+ * 1..256 (0 = 2.5 / (k - m + 2) of the tile's windows, at most 256). */
+int cusbf_set_insert_path(cusbf_filter* f, uint32_t path, uint32_t slice_bits,
+                          uint32_t reserve_per_tile);
+/* Current settings and the path the last insert call took (ATOMIC or SORTED). */
+int cusbf_get_insert_path(const cusbf_filter* f, uint32_t* path, uint32_t* slice_bits,
+                          uint32_t* reserve_per_tile, uint32_t* last_path);
+
 /* --- batched record form (no single-call counterpart in the reference) -------- */
 /*
  * Many records of uneven length in ONE launch, without the separator

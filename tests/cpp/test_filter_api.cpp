@@ -588,6 +588,28 @@ TEST_GPU(FilterImageRoundTripAndConfigCheck) {
     unlink(name);
 }
 
+TEST_GPU(SortedInsertPathMatchesAtomicPath) {
+    std::string sequence(300000, 'A');
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    for (char& c : sequence) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        c = "ACGTN"[(x >> 20) % 64 == 0 ? 4 : (x >> 30) & 3];
+    }
+    using Filter = cusbf::filter<TestConfig>;
+    Filter sorted(1 << 20);
+    Filter atomic(1 << 20);
+    cusbf::require_void(sorted.set_insert_path(Filter::InsertPath::sorted, 3));
+    cusbf::require_void(atomic.set_insert_path(Filter::InsertPath::atomic));
+    EXPECT_EQ(CUSBF_UNWRAP(sorted.insert_sequence(sequence)), CUSBF_UNWRAP(atomic.insert_sequence(sequence)));
+    EXPECT_TRUE(sorted.last_insert_path() == Filter::InsertPath::sorted);
+    EXPECT_TRUE(atomic.last_insert_path() == Filter::InsertPath::atomic);
+    EXPECT_TRUE(CUSBF_UNWRAP(sorted.words()) == CUSBF_UNWRAP(atomic.words()));
+    const auto bad = sorted.set_insert_path(Filter::InsertPath::sorted, 13);
+    EXPECT_TRUE(!bad && bad.error().category() == cusbf::ErrorCategory::invalid_argument);
+}
+
 TEST_HOST(QueryLayoutDerivesPerRecordHitSpans) {
     const std::array<cusbf::NormalizedRecord, 2> records{{{0, 0, 6, 1}, {1, 6, 8, 2}}};
     const auto layout = cusbf::detail::QueryLayout::build<LayoutConfig>(records);

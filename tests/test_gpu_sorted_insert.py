@@ -1,0 +1,265 @@
+"""GPU parity of the sorted insert path (records, radix partition, LDS apply).
+
+The sorted path must leave filter words bit-identical to the CPU oracle's and
+to the atomic path's on the same input: plain sequences (DNA and protein front
+ends, several k/s/m/h), dense packed input, the batched record form (direct
+calls and graph-captured plans), one to three radix passes (slice sizes from
+2 to 4096 shards), records that overflow a wave's reservation (atomics), words
+already set before the insert, inserts on two streams, empty and tiny inputs,
+and the AUTO path choice.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.helpers import make_config, make_oracle, random_dna, random_protein
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def seq_for(alphabet, n, seed):
+    if alphabet == "protein":
+        return random_protein(n, seed, invalid_rate=0.002)
+    return random_dna(n, seed, n_rate=0.001, lower_rate=0.01)
+
+
+def sorted_filter(cfg, bits, slice_bits=0, reserve=0):
+    import cusbf_amd as cs
+
+    f = cs.Filter(cfg, bits)
+    f.set_insert_path(cs.INSERT_SORTED, slice_bits, reserve)
+    return f
+
+
+def atomic_filter(cfg, bits):
+    import cusbf_amd as cs
+
+    f = cs.Filter(cfg, bits)
+    f.set_insert_path(cs.INSERT_ATOMIC)
+    return f
+
+
+def oracle_words(k, s, m, h, alphabet, seqs, num_shards):
+    o = make_oracle(k, s, m, h, alphabet)
+    w = o.new_filter(num_shards)
+    for q in seqs:
+        if len(q) >= k:
+            o.insert(q, w)
+    return w
+
+
+CONFIGS = [
+    (31, 28, 16, 4, "dna"),
+    (31, 30, 16, 4, "dna"),
+    (31, 31, 16, 4, "dna"),
+    (31, 28, 21, 4, "dna"),
+    (31, 28, 16, 8, "dna"),
+    (31, 28, 16, 16, "dna"),
+    (21, 19, 11, 4, "dna"),
+    (5, 4, 3, 4, "dna"),
+    (10, 8, 6, 4, "protein"),
+]
+
+
+@pytest.mark.parametrize("cfg_t", CONFIGS, ids=lambda c: f"{c[4]}-k{c[0]}s{c[1]}m{c[2]}h{c[3]}")
+@pytest.mark.parametrize("slice_bits", [12, 5, 1])
+def test_sorted_matches_oracle_and_atomic(gpu, cfg_t, slice_bits):
+    import cusbf_amd as cs
+
+    k, s, m, h, alphabet = cfg_t
+    seq = seq_for(alphabet, 400_003, seed=k * 7 + s + m + h)
+    cfg = make_config(k, s, m, h, alphabet)
+    bits = 1 << 22  # 2^14 shards: 2 .. 13 slice-index bits, one or two radix passes
+    f = sorted_filter(cfg, bits, slice_bits)
+    assert f.insert_sequence(seq) == len(seq) - k + 1
+    assert f.insert_path().last_path == cs.INSERT_SORTED
+    a = atomic_filter(cfg, bits)
+    a.insert_sequence(seq)
+    assert a.insert_path().last_path == cs.INSERT_ATOMIC
+    w = oracle_words(k, s, m, h, alphabet, [seq], f.num_shards())
+    assert (a.words() == w).all()
+    assert (f.words() == w).all()
+
+
+def test_sorted_three_passes_and_existing_words(gpu):
+    """2^19 shards in slices of 2 -> 18 index bits (8 + 8 + 2); a second insert ORs into
+    the words the first left."""
+    k, s, m, h = 31, 28, 16, 4
+    first = random_dna(600_000, 1, n_rate=0.001)
+    second = random_dna(300_000, 2)
+    cfg = make_config(k, s, m, h)
+    f = sorted_filter(cfg, 1 << 27, slice_bits=1)
+    f.insert_sequence(first)
+    f.insert_sequence(second)
+    w = oracle_words(k, s, m, h, "dna", [first, second], f.num_shards())
+    assert (f.words() == w).all()
+
+
+@pytest.mark.parametrize("reserve", [1, 8])
+def test_sorted_region_overflow_falls_back_to_atomics(gpu, reserve):
+    """A reservation far below the runs of a tile: most runs take the atomic path."""
+    k, s, m, h = 31, 28, 16, 4
+    seq = random_dna(500_000, 3, n_rate=0.002)
+    cfg = make_config(k, s, m, h)
+    f = sorted_filter(cfg, 1 << 24, slice_bits=8, reserve=reserve)
+    f.insert_sequence(seq)
+    w = oracle_words(k, s, m, h, "dna", [seq], f.num_shards())
+    assert (f.words() == w).all()
+
+
+def test_sorted_dense_packed(gpu):
+    import cusbf_amd as cs
+
+    k, s, m, h = 31, 28, 16, 4
+    seq = random_dna(300_001, 4)
+    cfg = make_config(k, s, m, h)
+    words_in = cs.pack_dense_sequence(cfg, seq)
+    f = sorted_filter(cfg, 1 << 22, slice_bits=4)
+    f.insert_dense_packed(words_in, len(seq))
+    assert f.insert_path().last_path == cs.INSERT_SORTED
+    w = oracle_words(k, s, m, h, "dna", [seq], f.num_shards())
+    assert (f.words() == w).all()
+
+
+def test_sorted_record_batch(gpu):
+    """Host record batch (records joined by separators, one insert launch)."""
+    k, s, m, h = 31, 28, 16, 4
+    rng = np.random.default_rng(5)
+    recs = [random_dna(int(n), 100 + i) for i, n in enumerate(rng.integers(0, 3000, 400))]
+    data = b"".join(recs)
+    off = np.cumsum([0] + [len(r) for r in recs])
+    ranges = [(int(off[i]), len(r)) for i, r in enumerate(recs)]
+    cfg = make_config(k, s, m, h)
+    f = sorted_filter(cfg, 1 << 23, slice_bits=6)
+    f.insert_record_batch(data, ranges)
+    w = oracle_words(k, s, m, h, "dna", recs, f.num_shards())
+    assert (f.words() == w).all()
+
+
+def _batched_inputs(alphabet, n, median, seed):
+    from tests.test_gpu_batched import (device, device_offsets, lognormal_lengths, offsets_of,
+                                        symbols_for)
+
+    lens = lognormal_lengths(n, median, 1.5, seed=seed)
+    lens[:4] = [0, 1, 30, 31]
+    off = offsets_of(lens)
+    data = symbols_for(alphabet, int(off[-1]), seed=seed + 1)
+    return data, off, device(data), device_offsets(off)
+
+
+@pytest.mark.parametrize("alphabet,k,s,m,claim", [("dna", 31, 28, 16, 0), ("dna", 31, 28, 16, 4),
+                                                  ("protein", 10, 8, 6, 0)])
+def test_sorted_batched_and_plan(gpu, alphabet, k, s, m, claim):
+    import cusbf_amd as cs
+    from cusbf_amd import batched as bt
+    from tests.test_gpu_batched import oracle_words as batched_oracle_words
+
+    data, off, d_data, d_off = _batched_inputs(alphabet, 3000, 400, seed=k + claim)
+    cfg = make_config(k, s, m, 4, alphabet)
+    bits = max(1 << 20, int(off[-1]) * 16)
+    w = batched_oracle_words(make_oracle(k, s, m, 4, alphabet), data, off, alphabet,
+                             cs.Filter(cfg, bits).num_shards())
+    f = sorted_filter(cfg, bits, slice_bits=3)
+    bt.set_tuning(f, bt.batch_bucket(int(off[-1]), len(off) - 1), bt.INSERT,
+                  bt.BatchTuning(3, 4, claim))
+    bt.insert_batched_async(f, d_data, d_off)
+    bt.batched_status(f)
+    assert f.insert_path().last_path == cs.INSERT_SORTED
+    assert (f.words() == w).all()
+    with bt.BatchPlan(f, bt.INSERT, d_data, d_off) as plan:
+        for rep in range(3):
+            f.clear()
+            plan.launch()
+            plan.status()
+            assert (f.words() == w).all(), rep
+
+
+def test_sorted_batched_malformed_offsets_leave_filter_clear(gpu):
+    from cusbf_amd import batched as bt
+    from cusbf_amd.errors import CusbfError
+    from tests.test_gpu_batched import device, device_offsets
+
+    f = sorted_filter(make_config(31, 28, 16, 4), 1 << 20, slice_bits=4)
+    data = device(random_dna(5000, 9))
+    with pytest.raises(CusbfError):
+        bt.insert_batched_async(f, data, device_offsets(np.array([0, 3000, 2000, 5000], np.uint64)))
+        bt.batched_status(f)
+    assert f.load_factor() == 0.0
+
+
+def test_sorted_two_streams(gpu):
+    """Sorted inserts on two streams share the workspace; the second waits for the first."""
+    torch = _torch()
+    k, s, m, h = 31, 28, 16, 4
+    a_seq, b_seq = random_dna(700_000, 11), random_dna(500_000, 12)
+    f = sorted_filter(make_config(k, s, m, h), 1 << 24, slice_bits=7)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    d_a = torch.frombuffer(bytearray(a_seq), dtype=torch.uint8).cuda()
+    d_b = torch.frombuffer(bytearray(b_seq), dtype=torch.uint8).cuda()
+    torch.cuda.synchronize()
+    f.insert_sequence_async(d_a, stream=s1)
+    f.insert_sequence_async(d_b, stream=s2)
+    torch.cuda.synchronize()
+    w = oracle_words(k, s, m, h, "dna", [a_seq, b_seq], f.num_shards())
+    assert (f.words() == w).all()
+
+
+def test_sorted_tiny_and_empty(gpu):
+    import cusbf_amd as cs
+
+    cfg = make_config(31, 28, 16, 4)
+    f = sorted_filter(cfg, 1 << 20, slice_bits=2)
+    assert f.insert_sequence(b"") == 0
+    assert f.insert_sequence(b"ACGT" * 7) == 0  # 28 symbols < k
+    assert f.load_factor() == 0.0
+    one = random_dna(31, 13)
+    assert f.insert_sequence(one) == 1
+    assert f.insert_path().last_path == cs.INSERT_SORTED
+    w = oracle_words(31, 28, 16, 4, "dna", [one], f.num_shards())
+    assert (f.words() == w).all()
+    allbad = b"N" * 5000
+    f.insert_sequence(allbad)
+    assert (f.words() == w).all()
+
+
+def test_auto_path_choice(gpu):
+    """AUTO keeps small inserts on the atomic path and moves large ones to the sorted path;
+    both give the same words."""
+    import cusbf_amd as cs
+
+    torch = _torch()
+    cfg = make_config(31, 28, 16, 4)
+    f = cs.Filter(cfg, 1 << 24)
+    assert f.insert_path().path == cs.INSERT_AUTO
+    f.insert_sequence(random_dna(20_000, 14))
+    assert f.insert_path().last_path == cs.INSERT_ATOMIC
+    big = torch.randint(0, 4, (48_000_000,), dtype=torch.uint8, device="cuda")
+    big = torch.tensor(list(b"ACGT"), dtype=torch.uint8, device="cuda")[big.long()]
+    g = cs.Filter(cfg, 1 << 24)
+    g.insert_sequence_async(big)
+    torch.cuda.synchronize()
+    assert g.insert_path().last_path == cs.INSERT_SORTED
+    a = atomic_filter(cfg, 1 << 24)
+    a.insert_sequence_async(big)
+    torch.cuda.synchronize()
+    assert (g.words() == a.words()).all()
+
+
+def test_insert_path_arguments(gpu):
+    import cusbf_amd as cs
+    from cusbf_amd.errors import CusbfError
+
+    f = cs.Filter(make_config(31, 28, 16, 4), 1 << 20)
+    for bad in [(3, 0, 0), (cs.INSERT_SORTED, 13, 0), (cs.INSERT_SORTED, 0, 257)]:
+        with pytest.raises(CusbfError):
+            f.set_insert_path(*bad)
+    f.set_insert_path(cs.INSERT_SORTED, 0, 0)
+    info = f.insert_path()
+    assert (info.path, info.slice_bits, info.reserve_per_tile) == (cs.INSERT_SORTED, 12, 0)
