@@ -23,6 +23,7 @@ from .filter import Filter, dev_ptr, nbytes, stream_handle
 
 INSERT = 0
 CONTAINS = 1
+INSERT_SORTED = 2  # table entry of a batched insert on the sorted path (not a plan op)
 BUCKETS = 4
 BUCKET_BOUNDS = (256, 4096, 65536)  # mean record length (symbols) bounds of buckets 0..2
 

@@ -127,7 +127,7 @@ struct cusbf_filter {
     uint64_t* d_scalar = nullptr;  // reductions
     // batched record form: tuning table, work-queue state, record-start bitmap
     uint32_t compute_units = 256;
-    cusbf_batch_tuning tuning[CUSBF_BATCH_BUCKETS][2] = {};
+    cusbf_batch_tuning tuning[CUSBF_BATCH_BUCKETS][3] = {};
     uint32_t* d_batch_state = nullptr;
     uint64_t* d_starts = nullptr;
     uint64_t starts_cap = 0;  // words
@@ -460,14 +460,17 @@ void normalize(const cusbf_filter* f, const char* seq, const cusbf_record_range*
 
 // Launch shape used until a tuner sets the table (the generic kernels ignore
 // it): the shapes the tuner picked on MI355X for 1M log-normal records, DNA
-// k=31 and protein k=10 -- insert: three waves per SIMD, two persistent
-// blocks per compute unit, strided tiles; query: 32 blocks per compute unit,
-// strided tiles, four waves per SIMD where that variant exists.
+// k=31 and protein k=10 -- atomic insert: three waves per SIMD, two
+// persistent blocks per compute unit, strided tiles; query: 32 blocks per
+// compute unit, strided tiles, four waves per SIMD where that variant exists;
+// sorted insert: as many blocks as stay resident, strided tiles.
 void default_tuning(cusbf_filter* f) {
     const bool wide = f->strip && f->strip->batched_insert[1] != nullptr;
     for (auto& bucket : f->tuning) {
         bucket[CUSBF_BATCH_INSERT] = cusbf_batch_tuning{3u, 2u, 0u, 0u};
         bucket[CUSBF_BATCH_CONTAINS] = cusbf_batch_tuning{wide ? 4u : 3u, 32u, 0u, 0u};
+        // sorted path: the insert kernel only computes and stores, so fill the SIMDs
+        bucket[CUSBF_BATCH_INSERT_SORTED] = cusbf_batch_tuning{wide ? 4u : 3u, wide ? 4u : 3u, 0u, 0u};
     }
 }
 
@@ -1219,7 +1222,7 @@ uint32_t cusbf_batch_variants(const cusbf_filter* f) {
 int cusbf_batch_tuning_set(cusbf_filter* f, uint32_t bucket, uint32_t op,
                            const cusbf_batch_tuning* tuning) {
     if (!f || !tuning) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
-    if (bucket >= CUSBF_BATCH_BUCKETS || op > CUSBF_BATCH_CONTAINS) {
+    if (bucket >= CUSBF_BATCH_BUCKETS || op > CUSBF_BATCH_INSERT_SORTED) {
         return fail(CUSBF_ERR_INVALID_ARGUMENT, "batch bucket or op out of range");
     }
     const int rc = check_tuning(f, *tuning);
@@ -1231,7 +1234,7 @@ int cusbf_batch_tuning_set(cusbf_filter* f, uint32_t bucket, uint32_t op,
 int cusbf_batch_tuning_get(const cusbf_filter* f, uint32_t bucket, uint32_t op,
                            cusbf_batch_tuning* out) {
     if (!f || !out) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
-    if (bucket >= CUSBF_BATCH_BUCKETS || op > CUSBF_BATCH_CONTAINS) {
+    if (bucket >= CUSBF_BATCH_BUCKETS || op > CUSBF_BATCH_INSERT_SORTED) {
         return fail(CUSBF_ERR_INVALID_ARGUMENT, "batch bucket or op out of range");
     }
     *out = f->tuning[bucket][op];
@@ -1246,15 +1249,17 @@ int cusbf_insert_batched_async(cusbf_filter* f, const char* d_symbols, uint64_t 
     if (batched_windows(f, num_symbols) == 0) return CUSBF_OK;
     rc = ensure_starts(f, num_symbols);
     if (rc != CUSBF_OK) return rc;
-    const cusbf_batch_tuning& t = tuning_for(f, CUSBF_BATCH_INSERT, num_symbols, num_records);
+    const cusbf_batch_tuning& ts = tuning_for(f, CUSBF_BATCH_INSERT_SORTED, num_symbols, num_records);
     sbf::SortedLayout l;
     bool sorted = false;
     f->last_path = CUSBF_INSERT_ATOMIC;
     if (batched_strip(f)) {
-        rc = choose_batched_sorted(f, num_symbols, t, as_stream(stream), &l, &sorted);
+        rc = choose_batched_sorted(f, num_symbols, ts, as_stream(stream), &l, &sorted);
         if (rc == CUSBF_OK && sorted) rc = begin_sorted(f, l.bytes, as_stream(stream));
         if (rc != CUSBF_OK) return rc;
     }
+    const cusbf_batch_tuning& t =
+        sorted ? ts : tuning_for(f, CUSBF_BATCH_INSERT, num_symbols, num_records);
     rc = enqueue_batched(f, CUSBF_BATCH_INSERT, d_symbols, num_symbols, d_offsets, num_records,
                          f->d_batch_state, f->d_starts, t, nullptr, as_stream(stream),
                          sorted ? &l : nullptr, f->d_sorted);
@@ -1310,8 +1315,12 @@ int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
         if (!d_output) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device output");
         if (d_output_len < windows) return fail(CUSBF_ERR_INVALID_ARGUMENT, "batched query output span is too small");
     }
-    const cusbf_batch_tuning t = tuning ? *tuning : tuning_for(f, op, num_symbols, num_records);
+    // an explicit tuning is used on whichever insert path the plan takes
+    cusbf_batch_tuning t = tuning ? *tuning : tuning_for(f, op, num_symbols, num_records);
+    const cusbf_batch_tuning ts =
+        tuning ? *tuning : tuning_for(f, CUSBF_BATCH_INSERT_SORTED, num_symbols, num_records);
     rc = check_tuning(f, t);
+    if (rc == CUSBF_OK) rc = check_tuning(f, ts);
     if (rc != CUSBF_OK) return rc;
 
     cusbf_batch_plan* p = new cusbf_batch_plan();
@@ -1326,11 +1335,12 @@ int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
     sbf::SortedLayout l;
     bool sorted = false;
     if (op == CUSBF_BATCH_INSERT && batched_strip(f)) {
-        rc = choose_batched_sorted(f, num_symbols, t, nullptr, &l, &sorted);
+        rc = choose_batched_sorted(f, num_symbols, ts, nullptr, &l, &sorted);
         if (rc != CUSBF_OK) {
             cusbf_batch_plan_destroy(p);
             return rc;
         }
+        if (sorted) t = ts;
     }
     hipError_t e = hipMalloc(&p->d_state, 64);
     if (e == hipSuccess) e = hipMalloc(&p->d_starts, sbf::start_words(num_symbols) * sizeof(uint64_t));

@@ -52,9 +52,39 @@ struct PassInput {
     const uint32_t* counts;
 };
 
-__device__ __forceinline__ bool slot_valid(const PassInput& in, uint32_t i) {
-    return i < in.slots && i % in.cap < in.counts[i / in.cap];
-}
+// Walks slots first, first + step, ... of a pass input, tracking the region
+// and the position inside it without a division per slot. Near: the wave's
+// slots span at most two regions (cap >= kWaveItems, or one region), whose
+// counts are loaded once; otherwise each slot loads its region's count.
+template <bool Near>
+struct SlotCursor {
+    uint32_t i, region, rem, first_region, cnt0, cnt1;
+    __device__ __forceinline__ SlotCursor(const PassInput& in, uint32_t first)
+        : i(first), region(first / in.cap), rem(first % in.cap), first_region(region), cnt0(0),
+          cnt1(0) {
+        if constexpr (Near) {
+            const uint32_t regions = in.slots / in.cap;
+            if (region < regions) cnt0 = in.counts[region];
+            if (region + 1 < regions) cnt1 = in.counts[region + 1];
+        }
+    }
+    __device__ __forceinline__ bool valid(const PassInput& in) const {
+        if (i >= in.slots) return false;
+        if constexpr (Near) {
+            return rem < (region == first_region ? cnt0 : cnt1);
+        } else {
+            return rem < in.counts[region];
+        }
+    }
+    __device__ __forceinline__ void advance(const PassInput& in, uint32_t step) {
+        i += step;
+        rem += step;
+        while (rem >= in.cap) {
+            rem -= in.cap;
+            ++region;
+        }
+    }
+};
 
 __device__ __forceinline__ uint32_t digit_of(uint64_t pair, uint32_t shift, uint32_t mask) {
     return static_cast<uint32_t>(pair >> shift) & mask;
@@ -86,18 +116,36 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     return before + x - v;
 }
 
+// Block b covers slots [b * kBlockItems, (b + 1) * kBlockItems); wave w of
+// the block takes the contiguous quarter w of it, 64 slots per round.
+constexpr int kWaveItems = kBlockItems / (kRankBlock / 64);
+
+__device__ __forceinline__ uint32_t wave_first_slot() {
+    return blockIdx.x * kBlockItems + (threadIdx.x >> 6) * kWaveItems + (threadIdx.x & 63);
+}
+
 // hist[d * nblocks + b] = records of block b with digit d.
+template <bool Near>
 __global__ __launch_bounds__(kRankBlock) void hist_kernel(PassInput in, uint32_t shift,
                                                           uint32_t digit_mask, uint32_t nblocks,
                                                           uint32_t* hist) {
     __shared__ uint32_t h[kBins];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * kBlockItems;
-#pragma unroll 4
+    // all loads first (16 in flight per lane), then the LDS counts
+    uint64_t p[kRankItems];
+    uint32_t valid = 0;
+    SlotCursor<Near> c(in, wave_first_slot());
+#pragma unroll
     for (int j = 0; j < kRankItems; ++j) {
-        const uint32_t i = base + static_cast<uint32_t>(j * kRankBlock) + threadIdx.x;
-        if (slot_valid(in, i)) atomicAdd(&h[digit_of(in.pairs[i], shift, digit_mask)], 1u);
+        const bool v = c.valid(in);
+        p[j] = v ? in.pairs[c.i] : 0ull;
+        valid |= (v ? 1u : 0u) << j;
+        c.advance(in, 64);
+    }
+#pragma unroll
+    for (int j = 0; j < kRankItems; ++j) {
+        if ((valid >> j) & 1u) atomicAdd(&h[digit_of(p[j], shift, digit_mask)], 1u);
     }
     __syncthreads();
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
@@ -155,39 +203,48 @@ __global__ __launch_bounds__(kScanBlock) void scan_finish_kernel(uint32_t* v, ui
     }
 }
 
-// Stable scatter of one LSD pass. Records of block b keep their input order
-// within each digit: round j ranks pairs base + j*256 + t in thread order, so
-// the ranks follow the slot order. The block's records are first placed in
-// digit order in LDS, then written out in order, so each digit's records of
-// the block go out as one contiguous run.
+// Stable scatter of one LSD pass. Each wave ranks its contiguous quarter of
+// the block in slot order, 64 slots per round, with running per-digit counts
+// of its own (no block barrier between rounds); the block then orders its
+// records by (digit, wave, rank) -- slot order within each digit -- in LDS and
+// writes them out in that order, so each digit's records of the block go out
+// as one contiguous run.
+template <bool Near>
 __global__ __launch_bounds__(kRankBlock) void scatter_kernel(PassInput in, uint32_t shift,
                                                              uint32_t digit_bits, uint32_t nblocks,
                                                              const uint32_t* offsets, uint64_t* out) {
-    __shared__ uint32_t running[kBins];
-    __shared__ uint32_t wcount[kRankBlock / 64][kBins];
+    constexpr int kWaves = kRankBlock / 64;
+    __shared__ uint32_t wrun[kWaves][kBins];  // per-wave running counts, then per-wave bases
     __shared__ uint32_t start[kBins];
     __shared__ uint32_t dest[kBins];
-    __shared__ uint32_t tmp[kRankBlock / 64];
+    __shared__ uint32_t tmp[kWaves];
     __shared__ uint64_t sorted[kBlockItems];
     const int t = static_cast<int>(threadIdx.x);
     const int lane = t & 63;
     const int wave = t >> 6;
     const uint32_t mask = (1u << digit_bits) - 1u;
-    running[t] = 0;
 #pragma unroll
-    for (int w = 0; w < kRankBlock / 64; ++w) wcount[w][t] = 0;
+    for (int w = 0; w < kWaves; ++w) wrun[w][t] = 0;
     dest[t] = offsets[static_cast<uint32_t>(t) * nblocks + blockIdx.x];
     __syncthreads();
 
-    const uint32_t base = blockIdx.x * kBlockItems;
     const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t* run = wrun[wave];
     uint64_t item[kRankItems];
     uint32_t rank[kRankItems];
+    uint32_t valid = 0;
+    SlotCursor<Near> c(in, wave_first_slot());
+#pragma unroll
+    for (int j = 0; j < kRankItems; ++j) {  // all loads first (16 in flight per lane)
+        const bool v = c.valid(in);
+        item[j] = v ? in.pairs[c.i] : 0ull;
+        valid |= (v ? 1u : 0u) << j;
+        c.advance(in, 64);
+    }
 #pragma unroll
     for (int j = 0; j < kRankItems; ++j) {
-        const uint32_t i = base + static_cast<uint32_t>(j * kRankBlock + t);
-        const bool v = slot_valid(in, i);
-        const uint64_t p = v ? in.pairs[i] : 0ull;
+        const bool v = ((valid >> j) & 1u) != 0u;
+        const uint64_t p = item[j];
         const uint32_t d = digit_of(p, shift, mask);
         uint64_t peers = __ballot(v);
         for (uint32_t b = 0; b < digit_bits; ++b) {
@@ -196,30 +253,31 @@ __global__ __launch_bounds__(kRankBlock) void scatter_kernel(PassInput in, uint3
             peers &= bit ? set : ~set;
         }
         const uint32_t r = static_cast<uint32_t>(__popcll(peers & below));
-        if (v && r == 0) wcount[wave][d] = static_cast<uint32_t>(__popcll(peers));
-        __syncthreads();
-        uint32_t pre = running[d];
-#pragma unroll
-        for (int w = 0; w < kRankBlock / 64; ++w) pre += w < wave ? wcount[w][d] : 0u;
-        item[j] = p;
+        // LDS operations of one wave take effect in program order: every lane
+        // reads the count before the lowest lane of its digit writes it back.
+        const uint32_t pre = run[d];
+        if (v && r == 0) run[d] = pre + static_cast<uint32_t>(__popcll(peers));
         rank[j] = v ? pre + r : 0xFFFFFFFFu;
-        __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int w = 0; w < kRankBlock / 64; ++w) {
-            add += wcount[w][t];
-            wcount[w][t] = 0;
-        }
-        running[t] += add;
-        __syncthreads();
     }
+    __syncthreads();
 
+    // digit t: the waves' counts -> block start of the digit and each wave's base
+    uint32_t wsum[kWaves];
+    uint32_t count = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        wsum[w] = count;
+        count += wrun[w][t];
+    }
     uint32_t total;
-    start[t] = block_exclusive_scan<kRankBlock>(running[t], tmp, total);
+    const uint32_t st = block_exclusive_scan<kRankBlock>(count, tmp, total);
+    start[t] = st;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wrun[w][t] = st + wsum[w];
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kRankItems; ++j) {
-        if (rank[j] != 0xFFFFFFFFu) sorted[start[digit_of(item[j], shift, mask)] + rank[j]] = item[j];
+        if (rank[j] != 0xFFFFFFFFu) sorted[run[digit_of(item[j], shift, mask)] + rank[j]] = item[j];
     }
     __syncthreads();
     for (uint32_t x = static_cast<uint32_t>(t); x < total; x += kRankBlock) {
@@ -397,16 +455,27 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStre
         const uint32_t shift = 32 + l.slice_bits + done;
         const uint32_t mask = (1u << bits) - 1u;
         uint32_t* total = totals + (pass & 1u);
-        hipLaunchKernelGGL(hist_kernel, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift, mask,
-                           nblocks, hist);
+        const bool near = in.cap >= static_cast<uint32_t>(kWaveItems) || in.cap == in.slots;
+        if (near) {
+            hipLaunchKernelGGL(hist_kernel<true>, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift,
+                               mask, nblocks, hist);
+        } else {
+            hipLaunchKernelGGL(hist_kernel<false>, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift,
+                               mask, nblocks, hist);
+        }
         hipLaunchKernelGGL(scan_partials_kernel, dim3(l.partials_len), dim3(kScanBlock), 0, stream,
                            hist, hist_len, partials);
         hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, partials,
                            l.partials_len, total);
         hipLaunchKernelGGL(scan_finish_kernel, dim3(l.partials_len), dim3(kScanBlock), 0, stream,
                            hist, hist_len, partials);
-        hipLaunchKernelGGL(scatter_kernel, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift, bits,
-                           nblocks, hist, pairs[cur ^ 1]);
+        if (near) {
+            hipLaunchKernelGGL(scatter_kernel<true>, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift,
+                               bits, nblocks, hist, pairs[cur ^ 1]);
+        } else {
+            hipLaunchKernelGGL(scatter_kernel<false>, dim3(nblocks), dim3(kRankBlock), 0, stream, in,
+                               shift, bits, nblocks, hist, pairs[cur ^ 1]);
+        }
         cur ^= 1;
         // the next pass reads the compacted pairs: one region holding `total` records
         in = PassInput{pairs[cur], static_cast<uint32_t>(l.slots), static_cast<uint32_t>(l.slots),

@@ -5,10 +5,13 @@ the tuner generates a sample batch whose record lengths follow a log-normal
 centred in the bucket, captures a graph plan for every candidate launch shape
 (kernel variant built for 3 or 4 waves per SIMD, persistent blocks per compute
 unit, tiles per work-queue claim or static striding) and times plan replays
-with HIP events. Insert is timed into a scratch filter of the same
-configuration and size as the target (so the target filter is never written);
-queries are timed against that scratch filter after the insert. The fastest
-shape per (bucket, op) goes into the target filter's tuning table.
+with HIP events. Inserts are timed into a scratch filter of the same
+configuration and size as the target (so the target filter is never written),
+once on the atomic path and once on the sorted path (each path has its own
+table entry: the insert kernel is bound by memory-side atomics on one and by
+its own arithmetic on the other); queries are timed against that scratch
+filter after the insert. The fastest shape per (bucket, op) goes into the
+target filter's tuning table.
 
 Usage:
     table = tune(filter)                  # all buckets
@@ -24,7 +27,7 @@ import numpy as np
 
 from . import batched as bt
 from .config import ALPHABET_PROTEIN
-from .filter import Filter
+from .filter import INSERT_ATOMIC, INSERT_AUTO, INSERT_SORTED, Filter
 from .workloads import PROTEIN_LETTERS, uniform_letters
 
 # (median, sigma, min, max) of the sample's record lengths per bucket
@@ -37,6 +40,7 @@ BUCKET_SAMPLES = {
 
 BLOCKS_PER_CU = (2, 4, 8, 16, 32)
 CLAIMS = (0, 1, 4, 16)
+OPS = {"insert": bt.INSERT, "insert_sorted": bt.INSERT_SORTED, "contains": bt.CONTAINS}
 
 
 @dataclass
@@ -109,28 +113,34 @@ def tune_bucket(filt: Filter, bucket: int, sample_symbols: int = 1 << 28, reps: 
     n_win = max(1, data.numel() // filt.config.symbol_width - k + 1)
     out = torch.empty(n_win, dtype=torch.uint8, device="cuda")
     results = {}
+    ops = ((bt.INSERT, "insert", INSERT_ATOMIC), (bt.INSERT_SORTED, "insert_sorted", INSERT_SORTED),
+           (bt.CONTAINS, "contains", None))
     try:
-        for op, name in ((bt.INSERT, "insert"), (bt.CONTAINS, "contains")):
+        for op, name, path in ops:
             timings = []
+            if path is not None:
+                scratch.set_insert_path(path)
             for t in candidates(filt):
-                if op == bt.INSERT:
-                    plan = bt.BatchPlan(scratch, op, data, off, tuning=t)
+                if path is not None:
+                    plan = bt.BatchPlan(scratch, bt.INSERT, data, off, tuning=t)
                     ms = _time_plan(plan, reps, before=lambda: scratch.clear())
                 else:
                     plan = bt.BatchPlan(scratch, op, data, off, out, tuning=t)
                     ms = _time_plan(plan, reps)
                 plan.close()
                 timings.append((t.waves_per_simd, t.blocks_per_cu, t.claim, round(ms, 4)))
-                if op == bt.INSERT:
+                if path is not None:
                     scratch.clear()
-            if op == bt.INSERT:  # leave the sample inserted for the query timings
-                with bt.BatchPlan(scratch, op, data, off) as plan:
+            scratch.set_insert_path(INSERT_AUTO)
+            if op == bt.INSERT_SORTED:  # leave the sample inserted for the query timings
+                with bt.BatchPlan(scratch, bt.INSERT, data, off) as plan:
                     plan.launch()
                     plan.status()
             best = min(timings, key=lambda r: r[3])
             results[name] = BucketResult(bucket, name, asdict(bt.BatchTuning(*best[:3])), best[3],
                                          timings)
     finally:
+        scratch.set_insert_path(INSERT_AUTO)
         if own_scratch:
             scratch.close()
     return results
@@ -149,8 +159,7 @@ def tune(filt: Filter, buckets=(0, 1, 2, 3), sample_symbols: int = 1 << 28, reps
             res = tune_bucket(filt, b, sample_symbols, reps, scratch=scratch)
             table[f"bucket{b}"] = {op: asdict(r) for op, r in res.items()}
             for op_name, r in res.items():
-                op = bt.INSERT if op_name == "insert" else bt.CONTAINS
-                bt.set_tuning(filt, b, op, bt.BatchTuning(**r.best))
+                bt.set_tuning(filt, b, OPS[op_name], bt.BatchTuning(**r.best))
                 if log:
                     log(f"[tune] bucket {b} {op_name}: {r.best} {r.best_ms:.3f} ms")
             scratch.clear()
@@ -163,8 +172,7 @@ def apply(filt: Filter, table: dict) -> None:
     for key, ops in table.items():
         b = int(key.replace("bucket", ""))
         for op_name, r in ops.items():
-            op = bt.INSERT if op_name == "insert" else bt.CONTAINS
-            bt.set_tuning(filt, b, op, bt.BatchTuning(**r["best"]))
+            bt.set_tuning(filt, b, OPS[op_name], bt.BatchTuning(**r["best"]))
 
 
 def save(table: dict, path: str) -> None:

@@ -287,7 +287,14 @@ int cusbf_get_insert_path(const cusbf_filter* f, uint32_t* path, uint32_t* slice
  * Calls on one filter share its batch scratch: issue them on one stream.
  */
 #define CUSBF_BATCH_BUCKETS 4 /* mean record length < 256, < 4096, < 65536, >= 65536 symbols */
-typedef enum cusbf_batch_op { CUSBF_BATCH_INSERT = 0, CUSBF_BATCH_CONTAINS = 1 } cusbf_batch_op;
+/* Table entries: CUSBF_BATCH_INSERT is the shape of a batched insert on the
+ * atomic path, CUSBF_BATCH_INSERT_SORTED of one on the sorted path (see
+ * cusbf_set_insert_path); plans take INSERT or CONTAINS as their op. */
+typedef enum cusbf_batch_op {
+    CUSBF_BATCH_INSERT = 0,
+    CUSBF_BATCH_CONTAINS = 1,
+    CUSBF_BATCH_INSERT_SORTED = 2
+} cusbf_batch_op;
 
 typedef struct cusbf_batch_tuning {
     uint32_t waves_per_simd; /* register budget of the kernel variant: 3 or 4 (4: DNA only) */

@@ -364,14 +364,16 @@ def test_tuner_picks_a_measured_candidate(gpu):
 
     f = cs.Filter(make_config(31, 28, 16, 4), 1 << 24)
     res = tune.tune_bucket(f, 0, sample_symbols=1 << 21, reps=1)
-    for op in ("insert", "contains"):
+    for op in ("insert", "insert_sorted", "contains"):
         r = res[op]
         assert len(r.timings) == len(tune.candidates(f)) == 40
         assert r.best_ms == min(t[3] for t in r.timings)
     table = tune.tune(f, buckets=(1,), sample_symbols=1 << 21, reps=1)
-    best = table["bucket1"]["contains"]["best"]
-    assert bt.get_tuning(f, 1, bt.CONTAINS) == bt.BatchTuning(**best)
+    for name, op in (("insert", bt.INSERT), ("insert_sorted", bt.INSERT_SORTED),
+                     ("contains", bt.CONTAINS)):
+        assert bt.get_tuning(f, 1, op) == bt.BatchTuning(**table["bucket1"][name]["best"])
     assert f.load_factor() == 0.0  # tuning never writes the target filter
+    assert f.insert_path().path == cs.INSERT_AUTO
     g = cs.Filter(make_config(17, 15, 9, 4), 1 << 20)
     assert tune.candidates(g) == [] and tune.tune(g) == {}
 

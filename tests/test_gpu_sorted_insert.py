@@ -167,7 +167,7 @@ def test_sorted_batched_and_plan(gpu, alphabet, k, s, m, claim):
     w = batched_oracle_words(make_oracle(k, s, m, 4, alphabet), data, off, alphabet,
                              cs.Filter(cfg, bits).num_shards())
     f = sorted_filter(cfg, bits, slice_bits=3)
-    bt.set_tuning(f, bt.batch_bucket(int(off[-1]), len(off) - 1), bt.INSERT,
+    bt.set_tuning(f, bt.batch_bucket(int(off[-1]), len(off) - 1), bt.INSERT_SORTED,
                   bt.BatchTuning(3, 4, claim))
     bt.insert_batched_async(f, d_data, d_off)
     bt.batched_status(f)
