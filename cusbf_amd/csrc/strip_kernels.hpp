@@ -50,7 +50,7 @@
 
 // Minimum waves per SIMD the register allocation of the plain DNA kernels must allow.
 #ifndef SBF_DNA_WAVES_PER_EU
-#define SBF_DNA_WAVES_PER_EU 4
+#define SBF_DNA_WAVES_PER_EU 3
 #endif
 
 namespace sbf::strip {
@@ -215,10 +215,27 @@ __device__ __forceinline__ Dna2::Ctx lane_context(const Dna2::Lds& st, int lane)
 // counted from the top.
 template <int L>
 __device__ __forceinline__ void substring_values(const Dna2::Ctx& c, uint64_t (&v)[kLaneWindows]) {
+    // top:w2 as one 96-bit value B = t1:t0:w; the substring at p is
+    // (B >> r) masked to 2L bits, r = 96 - 2p - 2L, taken with funnel shifts
+    const uint32_t t1 = static_cast<uint32_t>(c.top >> 32);
+    const uint32_t t0 = static_cast<uint32_t>(c.top);
+    const uint32_t w = c.w2;
 #pragma unroll
     for (int p = 0; p < kLaneWindows; ++p) {
-        const uint64_t x = p == 0 ? c.top : ((c.top << (2 * p)) | (c.w2 >> (32 - 2 * p)));
-        v[p] = L == 32 ? x : (x >> (64 - 2 * L));
+        const int r = 96 - 2 * p - 2 * L;
+        uint32_t lo, hi;
+        if (r >= 64) {
+            lo = t1 >> (r - 64);
+            hi = 0;
+        } else if (r >= 32) {
+            lo = __builtin_amdgcn_alignbit(t1, t0, static_cast<uint32_t>(r - 32));
+            hi = t1 >> (r - 32);
+        } else {
+            lo = __builtin_amdgcn_alignbit(t0, w, static_cast<uint32_t>(r));
+            hi = __builtin_amdgcn_alignbit(t1, t0, static_cast<uint32_t>(r));
+        }
+        const uint64_t x = (static_cast<uint64_t>(hi) << 32) | lo;
+        v[p] = L == 32 ? x : (x & ((1ull << (2 * L)) - 1ull));
     }
 }
 
@@ -579,6 +596,106 @@ __device__ __forceinline__ void init_wave_lds(WaveLds<FE, Batched>& L, int lane)
 
 // --- tile bodies -----------------------------------------------------------------
 
+__device__ __forceinline__ uint32_t lo32(uint64_t x) { return static_cast<uint32_t>(x); }
+__device__ __forceinline__ uint32_t hi32(uint64_t x) { return static_cast<uint32_t>(x >> 32); }
+
+// (a | c) & ~b in one v_bitop3_b32 (truth table 0x32: symmetric in a and c,
+// so it does not depend on which operand indexes the table's high bit).
+__device__ __forceinline__ uint32_t or_and_not(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x32);
+}
+
+// Shard words of window o's run: its LDS slot, or (a run past the slots) the
+// filter itself.
+template <int K, int S, int M, int H, class FE, bool Batched, bool AllSlotted>
+__device__ __forceinline__ void run_words(const LaneWindows<K, S, M, H, FE>& lw, int o,
+                                          uint32_t heads, uint32_t base,
+                                          const WaveLds<FE, Batched>& L, const FilterView& filter,
+                                          uint64_t (&w)[4]) {
+    const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u;
+    if (AllSlotted || slot < kSlots) {
+        const u32x4* src = reinterpret_cast<const u32x4*>(L.slot_w[AllSlotted ? slot : (slot < kSlots ? slot : 0)]);
+        const u32x4 a = src[0];
+        const u32x4 b = src[1];
+        w[0] = a.x | (static_cast<uint64_t>(a.y) << 32);
+        w[1] = a.z | (static_cast<uint64_t>(a.w) << 32);
+        w[2] = b.x | (static_cast<uint64_t>(b.y) << 32);
+        w[3] = b.z | (static_cast<uint64_t>(b.w) << 32);
+    } else {
+        const uint64_t* src = filter.words + 4ull * lw.shard[o];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = src[q];
+    }
+}
+
+// filter_ref.cuh:70-87 for the 16 windows of the lane: window o is present
+// when every bit of its k-s+1 s-mers' masks is set in its run's shard words.
+// The s-mer masks slide through a ring; with four s-mers per window the ring
+// holds pair masks (s-mers p and p+1), so a window's mask is the OR of two
+// pairs, folded with the and-not of the test into one op per 32-bit half.
+template <int K, int S, int M, int H, class FE, bool Batched, bool AllSlotted>
+__device__ __forceinline__ void findere_test(const LaneWindows<K, S, M, H, FE>& lw, uint32_t heads,
+                                             uint32_t base, const WaveLds<FE, Batched>& L,
+                                             const FilterView& filter, uint32_t (&result)[4]) {
+    constexpr int F = Shape<K, S, M, H, FE>::kFindereSpan;
+    if constexpr (F == 4) {
+        uint64_t last[4];     // mask of the newest s-mer
+        uint64_t pair[3][4];  // pair[p % 3] = mask(p) | mask(p + 1)
+        uint64_t m1[4], m2[4];
+        code_masks<H>(lw.code[0], last);
+        code_masks<H>(lw.code[1], m1);
+        code_masks<H>(lw.code[2], m2);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            pair[0][q] = last[q] | m1[q];
+            pair[1][q] = m1[q] | m2[q];
+            last[q] = m2[q];
+        }
+#pragma unroll
+        for (int o = 0; o < kLaneWindows; ++o) {
+            uint64_t nm[4];
+            code_masks<H>(lw.code[o + 3], nm);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                pair[(o + 2) % 3][q] = last[q] | nm[q];
+                last[q] = nm[q];
+            }
+            if ((lw.valid >> o) & 1u) {
+                uint64_t w[4];
+                run_words<K, S, M, H, FE, Batched, AllSlotted>(lw, o, heads, base, L, filter, w);
+                uint32_t miss = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    miss |= or_and_not(lo32(pair[o % 3][q]), lo32(w[q]), lo32(pair[(o + 2) % 3][q]));
+                    miss |= or_and_not(hi32(pair[o % 3][q]), hi32(w[q]), hi32(pair[(o + 2) % 3][q]));
+                }
+                result[o >> 2] |= (miss == 0 ? 1u : 0u) << (8 * (o & 3));
+            }
+        }
+    } else {
+        uint64_t ring[F][4];
+#pragma unroll
+        for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
+#pragma unroll
+        for (int o = 0; o < kLaneWindows; ++o) {
+            code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
+            if ((lw.valid >> o) & 1u) {
+                uint64_t w[4];
+                run_words<K, S, M, H, FE, Batched, AllSlotted>(lw, o, heads, base, L, filter, w);
+                uint64_t miss = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint64_t x = 0;
+#pragma unroll
+                    for (int d = 0; d < F - 1; ++d) x |= ring[(o + d) % F][q];
+                    miss |= (x | ring[(o + F - 1) % F][q]) & ~w[q];
+                }
+                result[o >> 2] |= (miss == 0 ? 1u : 0u) << (8 * (o & 3));
+            }
+        }
+    }
+}
+
 template <int K, int S, int M, int H, class FE, bool Dense, bool Batched>
 __device__ __forceinline__ void contains_tile(const SequenceInput& in, const FilterView& filter,
                                               uint64_t num_windows, uint8_t* out, uint64_t tile,
@@ -628,33 +745,10 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
 
     // findere test of every window against its run's shard words
     uint32_t result[4] = {0, 0, 0, 0};
-    uint64_t ring[F][4];
-#pragma unroll
-    for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
-#pragma unroll
-    for (int o = 0; o < kLaneWindows; ++o) {
-        code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
-        if ((lw.valid >> o) & 1u) {
-            const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u;
-            uint64_t wm[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int d = 0; d < F; ++d) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) wm[q] |= ring[(o + d) % F][q];
-            }
-            uint64_t w[4];
-            if (slot < kSlots) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) w[q] = L.slot_w[slot][q];
-            } else {
-                const uint64_t* src = filter.words + 4ull * lw.shard[o];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) w[q] = src[q];
-            }
-            const uint64_t miss =
-                (wm[0] & ~w[0]) | (wm[1] & ~w[1]) | (wm[2] & ~w[2]) | (wm[3] & ~w[3]);
-            result[o >> 2] |= (miss == 0 ? 1u : 0u) << (8 * (o & 3));
-        }
+    if (total <= kSlots) {  // wave-uniform: every run of the tile has its LDS slot
+        findere_test<K, S, M, H, FE, Batched, true>(lw, heads, base, L, filter, result);
+    } else {
+        findere_test<K, S, M, H, FE, Batched, false>(lw, heads, base, L, filter, result);
     }
     wave_lds_sync();  // the next tile of this wave reuses the LDS
 
