@@ -69,8 +69,8 @@ def parse_args():
 
 def committed_traffic(workload: str, kernel: str):
     """Calibrated HBM bytes per launch of `kernel` from the committed counter summary."""
-    name = {"insert": "insert_kernel<31, 28, 16, 4, Dna2, false, 4>",
-            "query": "contains_kernel<31, 28, 16, 4, Dna2, false, 4>"}[kernel]
+    name = {"insert": "insert_kernel<31, 28, 16, 4, Dna2, false, 3>",
+            "query": "contains_kernel<31, 28, 16, 4, Dna2, false, 3>"}[kernel]
     path = os.path.join(ROOT, "profiles", f"dna_insert_query_{workload}_pmc_traffic.md")
     if not os.path.exists(path):
         return None

@@ -328,37 +328,30 @@ __global__ __launch_bounds__(kApplyBlock) void apply_kernel(const uint64_t* pair
         u32x4* g = reinterpret_cast<u32x4*>(words + (static_cast<uint64_t>(b) << slice_bits) * 4ull);
         for (uint32_t x = threadIdx.x; x < vecs; x += kApplyBlock) lds4[x] = g[x];
         __syncthreads();
-        for (uint32_t i0 = lo; i0 < hi; i0 += kUnroll * kApplyBlock) {
+        // threads 2r and 2r+1 take the two 16-byte halves of record r's mask in
+        // one instruction: one memory request per mask line
+        const uint32_t half = threadIdx.x & 1u;
+        constexpr uint32_t kPerRound = kApplyBlock / 2;
+        for (uint32_t i0 = lo; i0 < hi; i0 += kUnroll * kPerRound) {
             uint64_t p[kUnroll];
-            u32x4 a[kUnroll], c[kUnroll];
+            u32x4 a[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const uint32_t i = i0 + static_cast<uint32_t>(u * kApplyBlock) + threadIdx.x;
+                const uint32_t i = i0 + static_cast<uint32_t>(u) * kPerRound + (threadIdx.x >> 1);
                 p[u] = i < hi ? pairs[i] : ~0ull;
             }
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                if (p[u] != ~0ull) {
-                    const uint64_t idx = static_cast<uint32_t>(p[u]);
-                    a[u] = masks[2 * idx];
-                    c[u] = masks[2 * idx + 1];
-                }
+                if (p[u] != ~0ull) a[u] = masks[2 * static_cast<uint64_t>(static_cast<uint32_t>(p[u])) + half];
             }
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 if (p[u] != ~0ull) {
-                    uint64_t* s = slice + 4u * (static_cast<uint32_t>(p[u] >> 32) & local_mask);
-                    const uint64_t m[4] = {a[u].x | (static_cast<uint64_t>(a[u].y) << 32),
-                                           a[u].z | (static_cast<uint64_t>(a[u].w) << 32),
-                                           c[u].x | (static_cast<uint64_t>(c[u].y) << 32),
-                                           c[u].z | (static_cast<uint64_t>(c[u].w) << 32)};
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        if (m[q] != 0) {
-                            __hip_atomic_fetch_or(s + q, m[q], __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-                        }
-                    }
+                    uint64_t* s = slice + 4u * (static_cast<uint32_t>(p[u] >> 32) & local_mask) + 2u * half;
+                    const uint64_t m0 = a[u].x | (static_cast<uint64_t>(a[u].y) << 32);
+                    const uint64_t m1 = a[u].z | (static_cast<uint64_t>(a[u].w) << 32);
+                    if (m0 != 0) __hip_atomic_fetch_or(s, m0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (m1 != 0) __hip_atomic_fetch_or(s + 1, m1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
         }

@@ -213,30 +213,36 @@ __device__ __forceinline__ Dna2::Ctx lane_context(const Dna2::Lds& st, int lane)
 // Values of the L-symbol substrings starting at positions 0..15 (MSB-first,
 // filter_common.cuh:128-134); position p is bits [2p, 2p + 2L) of top:w2
 // counted from the top.
+// Value of the L-symbol substring starting at position p (0 <= p, p + L <=
+// 48; MSB-first, filter_common.cuh:128-134): top:w2 as one 96-bit value
+// B = t1:t0:w, the substring is (B >> r) masked to 2L bits, r = 96 - 2p - 2L,
+// taken with funnel shifts.
 template <int L>
-__device__ __forceinline__ void substring_values(const Dna2::Ctx& c, uint64_t (&v)[kLaneWindows]) {
-    // top:w2 as one 96-bit value B = t1:t0:w; the substring at p is
-    // (B >> r) masked to 2L bits, r = 96 - 2p - 2L, taken with funnel shifts
+__device__ __forceinline__ uint64_t substring_at(const Dna2::Ctx& c, int p) {
     const uint32_t t1 = static_cast<uint32_t>(c.top >> 32);
     const uint32_t t0 = static_cast<uint32_t>(c.top);
     const uint32_t w = c.w2;
-#pragma unroll
-    for (int p = 0; p < kLaneWindows; ++p) {
-        const int r = 96 - 2 * p - 2 * L;
-        uint32_t lo, hi;
-        if (r >= 64) {
-            lo = t1 >> (r - 64);
-            hi = 0;
-        } else if (r >= 32) {
-            lo = __builtin_amdgcn_alignbit(t1, t0, static_cast<uint32_t>(r - 32));
-            hi = t1 >> (r - 32);
-        } else {
-            lo = __builtin_amdgcn_alignbit(t0, w, static_cast<uint32_t>(r));
-            hi = __builtin_amdgcn_alignbit(t1, t0, static_cast<uint32_t>(r));
-        }
-        const uint64_t x = (static_cast<uint64_t>(hi) << 32) | lo;
-        v[p] = L == 32 ? x : (x & ((1ull << (2 * L)) - 1ull));
+    const int r = 96 - 2 * p - 2 * L;
+    uint32_t lo, hi;
+    if (r >= 64) {
+        lo = t1 >> (r - 64);
+        hi = 0;
+    } else if (r >= 32) {
+        lo = __builtin_amdgcn_alignbit(t1, t0, static_cast<uint32_t>(r - 32));
+        hi = t1 >> (r - 32);
+    } else {
+        lo = __builtin_amdgcn_alignbit(t0, w, static_cast<uint32_t>(r));
+        hi = __builtin_amdgcn_alignbit(t1, t0, static_cast<uint32_t>(r));
     }
+    const uint64_t x = (static_cast<uint64_t>(hi) << 32) | lo;
+    return L == 32 ? x : (x & ((1ull << (2 * L)) - 1ull));
+}
+
+// The substrings starting at positions 0..15.
+template <int L>
+__device__ __forceinline__ void substring_values(const Dna2::Ctx& c, uint64_t (&v)[kLaneWindows]) {
+#pragma unroll
+    for (int p = 0; p < kLaneWindows; ++p) v[p] = substring_at<L>(c, p);
 }
 
 // --- stage 1: Protein5 --------------------------------------------------------
@@ -301,6 +307,20 @@ __device__ __forceinline__ Protein5::Ctx lane_context(const Protein5::Lds& st, i
     c.d[0] = a.x; c.d[1] = a.y; c.d[2] = a.z; c.d[3] = a.w;
     c.d[4] = b.x; c.d[5] = b.y; c.d[6] = b.z; c.d[7] = b.w;
     return c;
+}
+
+// MSB-first packing of the 5-bit symbols p..p+L-1 (sequence_kmer.cuh:74-89
+// packs `symbol & symbolMask`; p + L <= 32).
+template <int L>
+__device__ __forceinline__ uint64_t substring_at(const Protein5::Ctx& c, int p) {
+    constexpr uint64_t mask = L * 5 >= 64 ? ~0ull : ((1ull << (5 * L)) - 1ull);
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const int j = p + i;
+        acc = (acc << 5) | ((c.d[j >> 2] >> (8 * (j & 3))) & 0x1Fu);
+    }
+    return acc & mask;
 }
 
 // Rolling MSB-first packing of 5-bit symbols (sequence_kmer.cuh:74-89 packs
@@ -391,13 +411,6 @@ __device__ __forceinline__ void code_masks(const PosCode<H>& c, uint64_t (&m)[4]
     for (int i = 0; i < H; ++i) m[i & 3] |= 1ull << code_pos<H>(c, i);
 }
 
-template <int H>
-__device__ __forceinline__ PosCode<H> next_lane_code(const PosCode<H>& c) {
-    PosCode<H> r;
-#pragma unroll
-    for (int j = 0; j < PosCode<H>::kWords; ++j) r.w[j] = __shfl_down(c.w[j], 1, kWave);
-    return r;
-}
 
 // --- per-wave LDS ---------------------------------------------------------------
 //
@@ -540,8 +553,12 @@ __device__ __forceinline__ void lane_windows(const WaveLds<FE, Batched>& L, int 
     substring_values<S>(ctx, sv);
 #pragma unroll
     for (int p = 0; p < kLaneWindows; ++p) lw.code[p] = pos_code<H>(hash64(sv[p]));
+    // s-mers 16 .. 16 + F - 2 (the first ones of the next lane's span) lie in
+    // this lane's context too: computed here, no lane exchange
 #pragma unroll
-    for (int i = 0; i < F - 1; ++i) lw.code[kLaneWindows + i] = next_lane_code<H>(lw.code[i]);
+    for (int i = 0; i < F - 1; ++i) {
+        lw.code[kLaneWindows + i] = pos_code<H>(hash64(substring_at<S>(ctx, kLaneWindows + i)));
+    }
 }
 
 // Bit o of the result: window o starts a run. Bit o of `cont`: window o
@@ -719,27 +736,23 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
     publish_heads<K, S, M, H, FE, Batched>(lw, heads, base, L);
     wave_lds_sync();
 
-    // dense shard loads: lane q fetches slots q, q+64, q+128, q+192
+    // dense shard loads: lanes 2i and 2i+1 fetch the two 16-byte halves of one
+    // slot's shard in the same instruction, so each shard costs the memory
+    // system one request for its line (slots 32 j + i, j = 0..7)
     const u32x4* shards = reinterpret_cast<const u32x4*>(filter.words);
     const uint32_t n_slots = total < kSlots ? total : kSlots;
-    u32x4 wa[kSlots / kWave], wb[kSlots / kWave];
+    constexpr int kHalves = 2 * kSlots / kWave;
+    const uint32_t half = static_cast<uint32_t>(lane) & 1u;
+    u32x4 sw[kHalves];
 #pragma unroll
-    for (int j = 0; j < kSlots / kWave; ++j) {
-        const uint32_t q = static_cast<uint32_t>(lane + j * kWave);
-        if (q < n_slots) {
-            const uint64_t sidx = L.slot_shard[q];
-            wa[j] = shards[2 * sidx];
-            wb[j] = shards[2 * sidx + 1];
-        }
+    for (int j = 0; j < kHalves; ++j) {
+        const uint32_t q = static_cast<uint32_t>(lane + j * kWave) >> 1;
+        if (q < n_slots) sw[j] = shards[2 * static_cast<uint64_t>(L.slot_shard[q]) + half];
     }
 #pragma unroll
-    for (int j = 0; j < kSlots / kWave; ++j) {
-        const uint32_t q = static_cast<uint32_t>(lane + j * kWave);
-        if (q < n_slots) {
-            u32x4* dst = reinterpret_cast<u32x4*>(L.slot_w[q]);
-            dst[0] = wa[j];
-            dst[1] = wb[j];
-        }
+    for (int j = 0; j < kHalves; ++j) {
+        const uint32_t q = static_cast<uint32_t>(lane + j * kWave) >> 1;
+        if (q < n_slots) reinterpret_cast<u32x4*>(L.slot_w[q])[half] = sw[j];
     }
     wave_lds_sync();
 
@@ -943,7 +956,7 @@ __device__ __forceinline__ void for_each_tile(uint32_t* counter, uint32_t num_ti
 }
 
 template <int K, int S, int M, int H, class FE, int WPE, bool Queue>
-__global__ __launch_bounds__(kBlock, WPE) void batched_contains_kernel(SequenceInput in,
+__global__ __launch_bounds__(kBlock, Queue ? 2 : WPE) void batched_contains_kernel(SequenceInput in,
                                                                        FilterView filter,
                                                                        uint64_t num_windows,
                                                                        uint32_t num_tiles,
@@ -960,7 +973,7 @@ __global__ __launch_bounds__(kBlock, WPE) void batched_contains_kernel(SequenceI
 }
 
 template <int K, int S, int M, int H, class FE, int WPE, bool Queue>
-__global__ __launch_bounds__(kBlock, WPE) void batched_insert_kernel(SequenceInput in,
+__global__ __launch_bounds__(kBlock, Queue ? 2 : WPE) void batched_insert_kernel(SequenceInput in,
                                                                      FilterView filter,
                                                                      uint64_t num_windows,
                                                                      uint32_t num_tiles,
@@ -1007,11 +1020,13 @@ uint64_t windows_of(uint64_t num_symbols) {
     return num_symbols < static_cast<uint64_t>(K) ? 0 : num_symbols - K + 1;
 }
 
-// Waves per SIMD the plain kernels are built for: the DNA kernels' register
-// budget (the byte-code front end needs more LDS than four waves per SIMD allow).
-template <class FE>
+// Waves per SIMD the plain kernels are built for: a register budget under
+// which they do not spill (spilled values reloaded while other vector memory
+// operations are in flight have read stale data on this GPU). Eight and more
+// hashes per s-mer need the larger budget of two waves per SIMD.
+template <class FE, int H>
 constexpr int plain_waves() {
-    return FE::kBits == 2 ? SBF_DNA_WAVES_PER_EU : 3;
+    return FE::kBits == 2 ? (H == 4 ? SBF_DNA_WAVES_PER_EU : 2) : 3;
 }
 
 template <int K, int S, int M, int H, class FE>
@@ -1022,7 +1037,7 @@ hipError_t launch_insert(const SequenceInput& in, FilterView f, const RecordSink
     uint64_t tiles = 0;
     const uint32_t grid = n == 0 ? 0u : sink_grid(tile_grid<K, S, M, H, FE>(n, tiles), sink);
     if (grid == 0) return hipSuccess;
-    constexpr int WPE = plain_waves<FE>();
+    constexpr int WPE = plain_waves<FE, H>();
     (void)hipGetLastError();  // report only this launch
     if constexpr (FE::kBits == 2) {
         if (in.dense) {
@@ -1043,7 +1058,7 @@ hipError_t launch_contains(const SequenceInput& in, FilterView f, uint8_t* out,
     if (n == 0) return hipSuccess;
     uint64_t tiles;
     const uint32_t grid = tile_grid<K, S, M, H, FE>(n, tiles);
-    constexpr int WPE = plain_waves<FE>();
+    constexpr int WPE = plain_waves<FE, H>();
     (void)hipGetLastError();  // report only this launch
     if constexpr (FE::kBits == 2) {
         if (in.dense) {
@@ -1107,13 +1122,13 @@ constexpr uint32_t wave_windows() {
 }
 
 // Batched launchers compiled in kernels_strip_batched.hip: (front end, k, s,
-// m, h, waves per SIMD). Four waves per SIMD only for the measured DNA
-// configurations (the byte-code front end's LDS allows three).
+// m, h, waves per SIMD of the strided kernel; the work-queue kernel, which
+// needs more registers for its loop, is built for two so that it does not
+// spill).
 #define SBF_STRIP_BATCHED_INSTANCES(X)                                        \
-    X(Dna2, 31, 28, 16, 4, 3) X(Dna2, 31, 28, 16, 4, 4)                       \
-    X(Dna2, 31, 30, 16, 4, 3) X(Dna2, 31, 30, 16, 4, 4)                       \
-    X(Dna2, 31, 31, 16, 4, 3) X(Dna2, 31, 31, 16, 4, 4)                       \
-    X(Dna2, 21, 19, 11, 4, 3) X(Dna2, 5, 4, 3, 4, 3) X(Dna2, 3, 2, 2, 4, 3)   \
+    X(Dna2, 31, 28, 16, 4, 3) X(Dna2, 31, 30, 16, 4, 3)                       \
+    X(Dna2, 31, 31, 16, 4, 3) X(Dna2, 21, 19, 11, 4, 3)                       \
+    X(Dna2, 5, 4, 3, 4, 3) X(Dna2, 3, 2, 2, 4, 3)                             \
     X(Protein5, 10, 8, 6, 4, 3) X(Protein5, 5, 4, 3, 4, 3)
 
 }  // namespace sbf::strip

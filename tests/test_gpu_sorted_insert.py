@@ -61,7 +61,6 @@ CONFIGS = [
     (31, 31, 16, 4, "dna"),
     (31, 28, 21, 4, "dna"),
     (31, 28, 16, 8, "dna"),
-    (31, 28, 16, 16, "dna"),
     (21, 19, 11, 4, "dna"),
     (5, 4, 3, 4, "dna"),
     (10, 8, 6, 4, "protein"),
