@@ -206,13 +206,13 @@ bool strip_reads(const cusbf_filter* f, const sbf::SequenceInput& in) {
 
 // Records per wave tile the sorted insert reserves: 2.5 / (w + 1) of the
 // tile's windows for a minimizer span of w = k - m + 1 m-mers (a random
-// sequence has 2 / (w + 1) shard runs per window), at most the tile's 256 LDS
-// run slots (runs past them use atomics anyway).
+// sequence has 2 / (w + 1) shard runs per window), at most the tile's windows
+// (a tile has at most one run per window).
 uint32_t reserve_per_tile(const cusbf_filter* f) {
     if (f->reserve_per_tile) return f->reserve_per_tile;
     const double w = static_cast<double>(f->cfg.k - f->cfg.m + 1);
     const double r = std::ceil(f->strip->wave_windows * 2.5 / (w + 1.0));
-    return static_cast<uint32_t>(std::min(256.0, std::max(1.0, r)));
+    return static_cast<uint32_t>(std::min<double>(f->strip->wave_windows, std::max(1.0, r)));
 }
 
 // Cost model of AUTO, from MI355X measurements: the atomic insert does four
@@ -818,8 +818,8 @@ int cusbf_set_insert_path(cusbf_filter* f, uint32_t path, uint32_t slice_bits,
     if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
     if (path > CUSBF_INSERT_SORTED) return fail(CUSBF_ERR_INVALID_ARGUMENT, "unknown insert path");
     if (slice_bits > 12) return fail(CUSBF_ERR_INVALID_ARGUMENT, "slice_bits must be in [1, 12]");
-    if (reserve_per_tile > 256) {
-        return fail(CUSBF_ERR_INVALID_ARGUMENT, "reserve_per_tile must be in [1, 256]");
+    if (reserve_per_tile > 1024) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "reserve_per_tile must be in [1, 1024]");
     }
     f->insert_path = path;
     f->slice_bits = slice_bits ? slice_bits : 12;

@@ -68,6 +68,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // DNA, 2-bit symbols packed in 32-bit words, first symbol in the top bits.
 struct Dna2 {
     static constexpr int kBits = 2;
+    static constexpr int kSlots = 256;  // LDS run slots per wave tile (~100-120 runs used)
     static constexpr int kMaxK = 32;
     static constexpr uint32_t kAlphabet = kDna;
     struct Lds {
@@ -82,6 +83,7 @@ struct Dna2 {
 // Protein (or any 1-byte alphabet of at most 32 symbols): 16 code bytes per lane.
 struct Protein5 {
     static constexpr int kBits = 5;
+    static constexpr int kSlots = 256;  // ~330 runs per tile with a 5-m-mer span
     static constexpr int kMaxK = 12;
     static constexpr uint32_t kAlphabet = kProtein;
     struct Lds {
@@ -415,7 +417,7 @@ __device__ __forceinline__ void code_masks(const PosCode<H>& c, uint64_t (&m)[4]
 // --- per-wave LDS ---------------------------------------------------------------
 //
 // A run is a maximal sequence of consecutive valid windows of the wave tile
-// with the same shard (a super-k-mer cut at the tile edges). The first kSlots
+// with the same shard (a super-k-mer cut at the tile edges). The first FE::kSlots
 // runs of a tile get a slot in the wave's LDS: the query loads each run's
 // 32-byte shard once into its slot with dense, all-lane loads; the insert ORs
 // each run's masks into its slot and then issues one 64-bit atomic OR per
@@ -423,15 +425,13 @@ __device__ __forceinline__ void code_masks(const PosCode<H>& c, uint64_t (&m)[4]
 // changes at nearly every position) handles the excess runs with direct
 // global loads / atomics in the same pass.
 
-constexpr int kSlots = 256;
-
 template <class FE, bool Batched>
 struct WaveLds {
     typename FE::Lds st;
     uint32_t inv[kLdsWords];
     uint32_t bnd[Batched ? kLdsWords : 1];  // bit i of word l: symbol 16l+i starts a record
-    uint32_t slot_shard[kSlots];
-    alignas(16) uint64_t slot_w[kSlots][4];
+    uint32_t slot_shard[FE::kSlots];
+    alignas(16) uint64_t slot_w[FE::kSlots][4];
 };
 
 // Orders this wave's LDS writes before later LDS reads by its other lanes. LDS
@@ -578,7 +578,9 @@ __device__ __forceinline__ uint32_t run_heads(const LaneWindows<K, S, M, H, FE>&
     return lw.valid & ~same;
 }
 
-// Publishes the shard of every run head that has an LDS slot.
+// Publishes the shard of every run head that has an LDS slot; `base` is the
+// slot of the lane's first head (wrapping below zero for heads of an earlier
+// chunk, which are skipped like those past the slots).
 template <int K, int S, int M, int H, class FE, bool Batched>
 __device__ __forceinline__ void publish_heads(const LaneWindows<K, S, M, H, FE>& lw, uint32_t heads,
                                               uint32_t base, WaveLds<FE, Batched>& L) {
@@ -586,8 +588,20 @@ __device__ __forceinline__ void publish_heads(const LaneWindows<K, S, M, H, FE>&
     for (int o = 0; o < kLaneWindows; ++o) {
         if ((heads >> o) & 1u) {
             const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((1u << o) - 1u)));
-            if (slot < kSlots) L.slot_shard[slot] = lw.shard[o];
+            if (slot < static_cast<uint32_t>(FE::kSlots)) L.slot_shard[slot] = lw.shard[o];
         }
+    }
+}
+
+// Zeroes the slots' mask words (each lane its share).
+template <class FE, bool Batched>
+__device__ __forceinline__ void zero_slots(WaveLds<FE, Batched>& L, int lane) {
+#pragma unroll
+    for (int j = 0; j < FE::kSlots / kWave; ++j) {
+        const u32x4 z = {0u, 0u, 0u, 0u};
+        u32x4* dst = reinterpret_cast<u32x4*>(L.slot_w[lane + j * kWave]);
+        dst[0] = z;
+        dst[1] = z;
     }
 }
 
@@ -630,8 +644,8 @@ __device__ __forceinline__ void run_words(const LaneWindows<K, S, M, H, FE>& lw,
                                           const WaveLds<FE, Batched>& L, const FilterView& filter,
                                           uint64_t (&w)[4]) {
     const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u;
-    if (AllSlotted || slot < kSlots) {
-        const u32x4* src = reinterpret_cast<const u32x4*>(L.slot_w[AllSlotted ? slot : (slot < kSlots ? slot : 0)]);
+    if (AllSlotted || slot < FE::kSlots) {
+        const u32x4* src = reinterpret_cast<const u32x4*>(L.slot_w[AllSlotted ? slot : (slot < FE::kSlots ? slot : 0)]);
         const u32x4 a = src[0];
         const u32x4 b = src[1];
         w[0] = a.x | (static_cast<uint64_t>(a.y) << 32);
@@ -740,8 +754,8 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
     // slot's shard in the same instruction, so each shard costs the memory
     // system one request for its line (slots 32 j + i, j = 0..7)
     const u32x4* shards = reinterpret_cast<const u32x4*>(filter.words);
-    const uint32_t n_slots = total < kSlots ? total : kSlots;
-    constexpr int kHalves = 2 * kSlots / kWave;
+    const uint32_t n_slots = total < FE::kSlots ? total : FE::kSlots;
+    constexpr int kHalves = 2 * FE::kSlots / kWave;
     const uint32_t half = static_cast<uint32_t>(lane) & 1u;
     u32x4 sw[kHalves];
 #pragma unroll
@@ -758,7 +772,7 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
 
     // findere test of every window against its run's shard words
     uint32_t result[4] = {0, 0, 0, 0};
-    if (total <= kSlots) {  // wave-uniform: every run of the tile has its LDS slot
+    if (total <= FE::kSlots) {  // wave-uniform: every run of the tile has its LDS slot
         findere_test<K, S, M, H, FE, Batched, true>(lw, heads, base, L, filter, result);
     } else {
         findere_test<K, S, M, H, FE, Batched, false>(lw, heads, base, L, filter, result);
@@ -782,9 +796,15 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
     }
 }
 
-// Sorted insert: `sink.pairs` non-null makes the wave write each slotted run
-// as a record into its region (`region`, `emitted` records so far) instead of
-// issuing the run's atomics; runs past the region's capacity keep the atomics.
+// Insert tile. Runs get LDS slots in chunks of FE::kSlots: for each chunk the
+// lanes OR the masks of their pieces of the chunk's runs into the slots with
+// LDS atomics, then the wave flushes the slots -- one 64-bit device-scope
+// atomic OR per non-zero word of each run, or (sorted insert, `sink.pairs`
+// non-null) one record per run written into the wave's region (`region`,
+// `emitted` records so far) with coalesced stores; runs past the region's
+// capacity keep the atomics. Tiles with more runs than slots (short minimizer
+// spans, e.g. protein with m = 6) take more than one chunk and recompute the
+// lane windows for each (keeping them live across a flush would spill).
 template <int K, int S, int M, int H, class FE, bool Dense, bool Batched>
 __device__ __forceinline__ void insert_tile(const SequenceInput& in, const FilterView& filter,
                                             uint64_t num_windows, uint64_t tile, int lane,
@@ -792,100 +812,90 @@ __device__ __forceinline__ void insert_tile(const SequenceInput& in, const Filte
                                             uint32_t region, uint32_t& emitted) {
     using Sh = Shape<K, S, M, H, FE>;
     constexpr int F = Sh::kFindereSpan;
+    constexpr uint32_t kSlotsU = static_cast<uint32_t>(FE::kSlots);
     const uint64_t tile_first = tile * Sh::kWaveWindows;
-
-    stage_tile<K, S, M, H, FE, Dense, Batched>(in, tile, lane, L);
-    // zero the mask slots (each lane its four)
-#pragma unroll
-    for (int j = 0; j < kSlots / kWave; ++j) {
-        const u32x4 z = {0u, 0u, 0u, 0u};
-        u32x4* dst = reinterpret_cast<u32x4*>(L.slot_w[lane + j * kWave]);
-        dst[0] = z;
-        dst[1] = z;
-    }
-    wave_lds_sync();
-
     const bool owns = lane < Sh::kWindowLanes;
     const uint64_t first_window = tile_first + 16ull * static_cast<uint64_t>(lane);
-    LaneWindows<K, S, M, H, FE> lw;
-    lane_windows<K, S, M, H, FE, Batched>(L, lane, owns, first_window, num_windows,
-                                          static_cast<uint32_t>(filter.num_shards - 1), lw);
-    uint32_t cont;
-    const uint32_t heads = run_heads<K, S, M, H, FE>(lw, lane, cont);
-    uint32_t total;
-    const uint32_t base = wave_exclusive_sum(static_cast<uint32_t>(__popc(heads)), lane, total);
-    publish_heads<K, S, M, H, FE, Batched>(lw, heads, base, L);
-    // bit o: window o is the last window of its run inside this lane
-    const uint32_t ends = lw.valid & ~(cont >> 1);
 
-    // each lane ORs the masks of its piece of every run into the run's slot
-    uint64_t ring[F][4];
+    stage_tile<K, S, M, H, FE, Dense, Batched>(in, tile, lane, L);
+    uint32_t first = 0;  // first run of the chunk (wave-uniform)
+    for (;;) {
+        zero_slots(L, lane);
+        wave_lds_sync();
+
+        LaneWindows<K, S, M, H, FE> lw;
+        lane_windows<K, S, M, H, FE, Batched>(L, lane, owns, first_window, num_windows,
+                                              static_cast<uint32_t>(filter.num_shards - 1), lw);
+        uint32_t cont;
+        const uint32_t heads = run_heads<K, S, M, H, FE>(lw, lane, cont);
+        uint32_t total;
+        const uint32_t base = wave_exclusive_sum(static_cast<uint32_t>(__popc(heads)), lane, total) - first;
+        publish_heads<K, S, M, H, FE, Batched>(lw, heads, base, L);
+        // bit o: window o is the last window of its run inside this lane
+        const uint32_t ends = lw.valid & ~(cont >> 1);
+
+        // each lane ORs the masks of its piece of every run of the chunk into its slot
+        uint64_t ring[F][4];
 #pragma unroll
-    for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
-    uint64_t acc[4] = {0, 0, 0, 0};
+        for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
+        uint64_t acc[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int o = 0; o < kLaneWindows; ++o) {
-        code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
-        if ((cont >> o) & 1u) {
+        for (int o = 0; o < kLaneWindows; ++o) {
+            code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
+            if ((cont >> o) & 1u) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] |= ring[(o + F - 1) % F][q];
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint64_t x = 0;
-#pragma unroll
-                for (int d = 0; d < F; ++d) x |= ring[(o + d) % F][q];
-                acc[q] = x;
-            }
-        }
-        if ((ends >> o) & 1u) {
-            const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u;
-            if (slot < kSlots) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    __hip_atomic_fetch_or(&L.slot_w[slot][q], acc[q], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
+                for (int q = 0; q < 4; ++q) acc[q] |= ring[(o + F - 1) % F][q];
             } else {
-                uint64_t* dst = filter.words + 4ull * lw.shard[o];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    if (acc[q] != 0) {
-                        __hip_atomic_fetch_or(dst + q, acc[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    uint64_t x = 0;
+#pragma unroll
+                    for (int d = 0; d < F; ++d) x |= ring[(o + d) % F][q];
+                    acc[q] = x;
+                }
+            }
+            if ((ends >> o) & 1u) {
+                const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u;
+                if (slot < kSlotsU) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        __hip_atomic_fetch_or(&L.slot_w[slot][q], acc[q], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                 }
             }
         }
-    }
-    wave_lds_sync();
+        wave_lds_sync();
 
-    // dense global atomics: one 64-bit OR per non-zero word of each slotted
-    // run; or, sorted insert, one record per slotted run with coalesced stores
-    const uint32_t n_slots = total < kSlots ? total : kSlots;
-    const uint32_t room = sink.pairs != nullptr ? sink.cap - emitted : 0u;
+        // flush the chunk's slots
+        const uint32_t n_slots = total - first < kSlotsU ? total - first : kSlotsU;
+        const uint32_t room = sink.pairs != nullptr ? sink.cap - emitted : 0u;
 #pragma unroll
-    for (int j = 0; j < kSlots / kWave; ++j) {
-        const uint32_t q = static_cast<uint32_t>(lane + j * kWave);
-        if (q < n_slots) {
-            if (q < room) {
-                const uint64_t at = static_cast<uint64_t>(region) * sink.cap + emitted + q;
-                sink.pairs[at] = (static_cast<uint64_t>(L.slot_shard[q]) << 32) | at;
-                const u32x4* src = reinterpret_cast<const u32x4*>(L.slot_w[q]);
-                u32x4* dst = reinterpret_cast<u32x4*>(sink.masks + 4 * at);
-                dst[0] = src[0];
-                dst[1] = src[1];
-            } else {
-                uint64_t* dst = filter.words + 4ull * L.slot_shard[q];
+        for (int j = 0; j < FE::kSlots / kWave; ++j) {
+            const uint32_t q = static_cast<uint32_t>(lane + j * kWave);
+            if (q < n_slots) {
+                if (q < room) {
+                    const uint64_t at = static_cast<uint64_t>(region) * sink.cap + emitted + q;
+                    sink.pairs[at] = (static_cast<uint64_t>(L.slot_shard[q]) << 32) | at;
+                    const u32x4* src = reinterpret_cast<const u32x4*>(L.slot_w[q]);
+                    u32x4* dst = reinterpret_cast<u32x4*>(sink.masks + 4 * at);
+                    dst[0] = src[0];
+                    dst[1] = src[1];
+                } else {
+                    uint64_t* dst = filter.words + 4ull * L.slot_shard[q];
 #pragma unroll
-                for (int w = 0; w < 4; ++w) {
-                    const uint64_t m = L.slot_w[q][w];
-                    if (m != 0) __hip_atomic_fetch_or(dst + w, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (int w = 0; w < 4; ++w) {
+                        const uint64_t m = L.slot_w[q][w];
+                        if (m != 0) __hip_atomic_fetch_or(dst + w, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
                 }
             }
         }
+        emitted += n_slots < room ? n_slots : room;
+        wave_lds_sync();  // the next chunk or tile of this wave reuses the LDS
+        first += kSlotsU;
+        if (first >= total) break;
     }
-    emitted += n_slots < room ? n_slots : room;
-    wave_lds_sync();  // the next tile of this wave reuses the LDS
 }
 
 // --- plain kernels: waves stride over the tiles ------------------------------------

@@ -190,7 +190,7 @@ class Filter:
     # insert path (no reference counterpart) ------------------------------------
     def set_insert_path(self, path: int, slice_bits: int = 0, reserve_per_tile: int = 0) -> None:
         """INSERT_AUTO / INSERT_ATOMIC / INSERT_SORTED for the lane-strip inserts;
-        slice_bits (1..12, 0 = 12) and records reserved per wave tile (1..256, 0 =
+        slice_bits (1..12, 0 = 12) and records reserved per wave tile (1..1024, 0 =
         from the minimizer span) shape the sorted path."""
         check(self._lib, self._lib.cusbf_set_insert_path(self._h, int(path), int(slice_bits),
                                                          int(reserve_per_tile)))

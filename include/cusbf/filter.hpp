@@ -737,7 +737,7 @@ class filter {
     };
     /// Atomic ORs per shard word, or records partitioned by filter slice and
     /// applied in LDS; both give identical words. slice_bits 1..12 (0 = 12),
-    /// reserve_per_tile 1..256 (0 = from the minimizer span).
+    /// reserve_per_tile 1..1024 (0 = from the minimizer span).
     [[nodiscard]] Result<void> set_insert_path(InsertPath path, uint32_t slice_bits = 0,
                                                uint32_t reserve_per_tile = 0) {
         return detail::check(cusbf_set_insert_path(handle_, static_cast<uint32_t>(path), slice_bits,

@@ -254,7 +254,7 @@ const char* cusbf_kernel_family(const cusbf_filter* f);
 #define CUSBF_INSERT_ATOMIC 1u
 #define CUSBF_INSERT_SORTED 2u
 /* slice_bits: 1..12 (0 = 12); reserve_per_tile: records reserved per wave tile,
- * 1..256 (0 = 2.5 / (k - m + 2) of the tile's windows, at most 256). */
+ * 1..1024 (0 = 2.5 / (k - m + 2) of the tile's windows, at most the tile's windows). */
 int cusbf_set_insert_path(cusbf_filter* f, uint32_t path, uint32_t slice_bits,
                           uint32_t reserve_per_tile);
 /* Current settings and the path the last insert call took (ATOMIC or SORTED). */

@@ -2,7 +2,7 @@
 
 The sorted path must leave filter words bit-identical to the CPU oracle's and
 to the atomic path's on the same input: plain sequences (DNA and protein front
-ends, several k/s/m/h), dense packed input, the batched record form (direct
+ends, several k/s/m/h, tiles with more runs than LDS slots), dense packed input, the batched record form (direct
 calls and graph-captured plans), one to three radix passes (slice sizes from
 2 to 4096 shards), records that overflow a wave's reservation (atomics), words
 already set before the insert, inserts on two streams, empty and tiny inputs,
@@ -62,8 +62,10 @@ CONFIGS = [
     (31, 28, 21, 4, "dna"),
     (31, 28, 16, 8, "dna"),
     (21, 19, 11, 4, "dna"),
-    (5, 4, 3, 4, "dna"),
-    (10, 8, 6, 4, "protein"),
+    (5, 4, 3, 4, "dna"),      # ~500 runs per tile: two slot chunks
+    (3, 2, 2, 4, "dna"),      # ~660 runs per tile: three slot chunks
+    (10, 8, 6, 4, "protein"),  # ~330 runs per tile
+    (5, 4, 3, 4, "protein"),
 ]
 
 
@@ -256,7 +258,7 @@ def test_insert_path_arguments(gpu):
     from cusbf_amd.errors import CusbfError
 
     f = cs.Filter(make_config(31, 28, 16, 4), 1 << 20)
-    for bad in [(3, 0, 0), (cs.INSERT_SORTED, 13, 0), (cs.INSERT_SORTED, 0, 257)]:
+    for bad in [(3, 0, 0), (cs.INSERT_SORTED, 13, 0), (cs.INSERT_SORTED, 0, 1025)]:
         with pytest.raises(CusbfError):
             f.set_insert_path(*bad)
     f.set_insert_path(cs.INSERT_SORTED, 0, 0)
