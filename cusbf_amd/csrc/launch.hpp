@@ -109,7 +109,7 @@ struct SortedLayout {
     uint32_t partials_len = 0;
     uint64_t off_pairs[2] = {0, 0};
     uint64_t off_masks = 0, off_counts = 0, off_hist = 0, off_partials = 0, off_totals = 0,
-             off_bounds = 0;
+             off_bounds = 0, off_groups = 0;
     uint64_t bytes = 0;
 };
 

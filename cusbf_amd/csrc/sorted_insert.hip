@@ -11,9 +11,15 @@
 //      8-byte pair (shard << 32 | record index) and the run's 32-byte OR mask;
 //      runs beyond the region's capacity (and a tile's runs beyond its LDS
 //      slots) still take the atomic path;
-//   2. partition (here): stable LSD radix passes, 8 bits each, over the pairs,
-//      keyed on the slice index shard >> slice_bits, until the pairs are
-//      grouped by slice;
+//   2. partition (here): LSD radix passes, 8 bits each, over the pairs, keyed
+//      on the slice index shard >> slice_bits, until the pairs are grouped by
+//      slice. Only the grouping matters (the apply ORs a slice's records in any
+//      order), so the first pass ranks with LDS atomics (unstable), and the
+//      second runs its blocks over one first-pass digit group each: any order
+//      inside a block keeps the groups apart, so it ranks with atomics too, and
+//      its scanned histogram already holds every slice's start. A third pass
+//      (more than 2^16 slices) needs a stable rank over the second-pass groups
+//      and uses wave ballots;
 //   3. apply (here): one block per non-empty slice loads the slice's words
 //      into LDS, ORs the masks of its records into them with LDS atomics and
 //      stores the slice back.
@@ -287,6 +293,162 @@ __global__ __launch_bounds__(kRankBlock) void scatter_kernel(PassInput in, uint3
     }
 }
 
+// Unstable scatter of one pass: each block ranks its items with LDS atomics
+// on per-digit counters, orders them by digit in LDS and writes each digit's
+// items as one contiguous run at the block's offset for that digit. Items of
+// a block: Group = false, the pass-0 layout (wave w of block b takes slots
+// b * kBlockItems + w * kWaveItems + lane + 64 j of the emit regions, valid
+// below their region's count); Group = true, the block's chunk of its
+// first-pass digit group (block_range).
+__device__ __forceinline__ void block_range(const uint32_t* first_block, const uint32_t* group_start,
+                                            uint32_t b, uint32_t& lo, uint32_t& hi) {
+    // first_block[g]: blocks before group g (256 + 1 entries); find g with
+    // first_block[g] <= b < first_block[g + 1]
+    uint32_t g = 0;
+#pragma unroll
+    for (uint32_t step = kBins / 2; step > 0; step >>= 1) {
+        if (first_block[g + step] <= b) g += step;
+    }
+    if (b >= first_block[kBins]) {  // past the last group's chunks: no items
+        lo = hi = 0;
+        return;
+    }
+    lo = group_start[g] + (b - first_block[g]) * static_cast<uint32_t>(kBlockItems);
+    const uint32_t end = group_start[g + 1];
+    hi = lo + static_cast<uint32_t>(kBlockItems) < end ? lo + static_cast<uint32_t>(kBlockItems) : end;
+}
+
+template <bool Near, bool Group>
+__device__ __forceinline__ void load_block_items(const PassInput& in, const uint32_t* first_block,
+                                                 const uint32_t* group_start, uint64_t (&item)[kRankItems],
+                                                 uint32_t& valid) {
+    valid = 0;
+    if constexpr (Group) {
+        uint32_t lo, hi;
+        block_range(first_block, group_start, blockIdx.x, lo, hi);
+        const uint32_t first = lo + (threadIdx.x >> 6) * kWaveItems + (threadIdx.x & 63);
+#pragma unroll
+        for (int j = 0; j < kRankItems; ++j) {
+            const uint32_t i = first + 64u * static_cast<uint32_t>(j);
+            const bool v = i < hi;
+            item[j] = v ? in.pairs[i] : 0ull;
+            valid |= (v ? 1u : 0u) << j;
+        }
+    } else {
+        SlotCursor<Near> c(in, wave_first_slot());
+#pragma unroll
+        for (int j = 0; j < kRankItems; ++j) {
+            const bool v = c.valid(in);
+            item[j] = v ? in.pairs[c.i] : 0ull;
+            valid |= (v ? 1u : 0u) << j;
+            c.advance(in, 64);
+        }
+    }
+}
+
+// hist[d * nblocks + b] = items of block b with digit d (every block writes its
+// 256 counts, zeros when it has no items).
+template <bool Near, bool Group>
+__global__ __launch_bounds__(kRankBlock) void hist_any_kernel(PassInput in, const uint32_t* first_block,
+                                                              const uint32_t* group_start, uint32_t shift,
+                                                              uint32_t digit_mask, uint32_t nblocks,
+                                                              uint32_t* hist) {
+    __shared__ uint32_t h[kBins];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t p[kRankItems];
+    uint32_t valid;
+    load_block_items<Near, Group>(in, first_block, group_start, p, valid);
+#pragma unroll
+    for (int j = 0; j < kRankItems; ++j) {
+        if ((valid >> j) & 1u) atomicAdd(&h[digit_of(p[j], shift, digit_mask)], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+template <bool Near, bool Group>
+__global__ __launch_bounds__(kRankBlock) void scatter_any_kernel(PassInput in, const uint32_t* first_block,
+                                                                 const uint32_t* group_start, uint32_t shift,
+                                                                 uint32_t digit_mask, uint32_t nblocks,
+                                                                 const uint32_t* offsets, uint64_t* out) {
+    __shared__ uint32_t count[kBins];
+    __shared__ uint32_t start[kBins];
+    __shared__ uint32_t dest[kBins];
+    __shared__ uint32_t tmp[kRankBlock / 64];
+    __shared__ uint64_t sorted[kBlockItems];
+    const int t = static_cast<int>(threadIdx.x);
+    count[t] = 0;
+    dest[t] = offsets[static_cast<uint32_t>(t) * nblocks + blockIdx.x];
+    __syncthreads();
+    uint64_t item[kRankItems];
+    uint32_t valid;
+    load_block_items<Near, Group>(in, first_block, group_start, item, valid);
+    uint32_t rank[kRankItems];
+#pragma unroll
+    for (int j = 0; j < kRankItems; ++j) {
+        rank[j] = ((valid >> j) & 1u) ? atomicAdd(&count[digit_of(item[j], shift, digit_mask)], 1u) : 0u;
+    }
+    __syncthreads();
+    uint32_t total;
+    const uint32_t st = block_exclusive_scan<kRankBlock>(count[t], tmp, total);
+    start[t] = st;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRankItems; ++j) {
+        if ((valid >> j) & 1u) sorted[start[digit_of(item[j], shift, digit_mask)] + rank[j]] = item[j];
+    }
+    __syncthreads();
+    for (uint32_t x = static_cast<uint32_t>(t); x < total; x += kRankBlock) {
+        const uint64_t p = sorted[x];
+        const uint32_t d = digit_of(p, shift, digit_mask);
+        out[dest[d] + (x - start[d])] = p;
+    }
+}
+
+// Second-pass block table from the first pass's scanned histogram: group g =
+// the items of first-pass digit g, at [group_start[g], group_start[g + 1]);
+// first_block[g] = blocks of kBlockItems before group g. One block of 256.
+__global__ __launch_bounds__(kBins) void groups_kernel(const uint32_t* scanned, uint32_t nblocks,
+                                                       const uint32_t* total, uint32_t* group_start,
+                                                       uint32_t* first_block) {
+    __shared__ uint32_t tmp[kBins / 64];
+    const uint32_t g = threadIdx.x;
+    const uint32_t lo = scanned[g * nblocks];
+    const uint32_t hi = g + 1 < kBins ? scanned[(g + 1) * nblocks] : *total;
+    group_start[g] = lo;
+    if (g + 1 == kBins) group_start[kBins] = hi;
+    const uint32_t nb = (hi - lo + static_cast<uint32_t>(kBlockItems) - 1) / static_cast<uint32_t>(kBlockItems);
+    uint32_t sum;
+    const uint32_t before = block_exclusive_scan<kBins>(nb, tmp, sum);
+    first_block[g] = before;
+    if (g + 1 == kBins) first_block[kBins] = sum;
+}
+
+// Slice starts read off a scanned histogram. One pass (slice index < 2^8
+// values): bounds[s] = scanned[s * nblocks] (first block's offset of digit s).
+// Two passes (slice = hi << 8 | lo, second-pass blocks grouped by lo):
+// bounds[s] = scanned[hi * nblocks + first_block[lo]], the offset of the
+// first block of group lo for digit hi -- exclusive scans count every item of
+// smaller hi, and of equal hi in earlier groups. bounds[num_slices] = total.
+__global__ __launch_bounds__(256) void bounds_from_scan_kernel(const uint32_t* scanned, uint32_t nblocks,
+                                                               const uint32_t* first_block,
+                                                               const uint32_t* total, uint32_t num_slices,
+                                                               uint32_t* bounds) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > num_slices) return;
+    if (s == num_slices) {
+        bounds[s] = *total;
+    } else {
+        // the first block of a group past every item is block nblocks of digit hi,
+        // i.e. the start of digit hi + 1 (the total for hi = 255)
+        const uint64_t i = first_block == nullptr
+                               ? static_cast<uint64_t>(s) * nblocks
+                               : static_cast<uint64_t>(s >> 8) * nblocks + first_block[s & (kBins - 1)];
+        bounds[s] = i < static_cast<uint64_t>(kBins) * nblocks ? scanned[i] : *total;
+    }
+}
+
 // bounds[b] = first record of slice b in the grouped pairs, bounds[num_slices]
 // = record total. Record i writes the starts of the slices after its
 // predecessor's slice up to its own; slot `total` closes the table.
@@ -384,7 +546,8 @@ SortedLayout sorted_layout(uint32_t regions, uint32_t cap, uint64_t num_shards, 
     l.num_slices = 1u << l.slice_index_bits;
     l.passes = l.slice_index_bits == 0 ? 1u : (l.slice_index_bits + 7) / 8;
     const uint32_t nblocks = div_up(l.slots, kBlockItems);
-    l.hist_len = static_cast<uint64_t>(kBins) * nblocks;
+    // second-pass blocks: a group adds at most one partial block
+    l.hist_len = static_cast<uint64_t>(kBins) * (nblocks + kBins);
     l.partials_len = div_up(l.hist_len, kScanChunk);
     uint64_t at = 0;
     l.off_pairs[0] = at;
@@ -403,6 +566,8 @@ SortedLayout sorted_layout(uint32_t regions, uint32_t cap, uint64_t num_shards, 
     at += 256;
     l.off_bounds = at;
     at += align256((static_cast<uint64_t>(l.num_slices) + 1) * 4);
+    l.off_groups = at;  // group_start[257], first_block[257]
+    at += align256(2 * (kBins + 1) * 4);
     l.bytes = at;
     return l;
 }
@@ -433,50 +598,87 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStre
     uint32_t* partials = reinterpret_cast<uint32_t*>(p + l.off_partials);
     uint32_t* totals = reinterpret_cast<uint32_t*>(p + l.off_totals);
     uint32_t* bounds = reinterpret_cast<uint32_t*>(p + l.off_bounds);
+    uint32_t* group_start = reinterpret_cast<uint32_t*>(p + l.off_groups);
+    uint32_t* first_block = group_start + (kBins + 1);
 
     (void)hipGetLastError();  // report only these launches
-    PassInput in{pairs[0], static_cast<uint32_t>(l.slots), l.cap,
-                 reinterpret_cast<const uint32_t*>(p + l.off_counts)};
-    const uint32_t nblocks = div_up(l.slots, kBlockItems);
-    const uint32_t hist_len = static_cast<uint32_t>(l.hist_len);
-    int cur = 0;
-    for (uint32_t pass = 0; pass < l.passes; ++pass) {
+    const uint32_t nblocks0 = div_up(l.slots, kBlockItems);
+    const uint32_t nblocks1 = nblocks0 + kBins;
+    auto scan = [&](uint32_t len, uint32_t* total) {
+        const uint32_t parts = div_up(len, kScanChunk);
+        hipLaunchKernelGGL(scan_partials_kernel, dim3(parts), dim3(kScanBlock), 0, stream, hist, len, partials);
+        hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, partials, parts, total);
+        hipLaunchKernelGGL(scan_finish_kernel, dim3(parts), dim3(kScanBlock), 0, stream, hist, len, partials);
+    };
+    auto digit = [&](uint32_t pass, uint32_t& shift, uint32_t& mask) {
         const uint32_t done = 8 * pass;
         const uint32_t bits = l.slice_index_bits > done
                                   ? (l.slice_index_bits - done < 8 ? l.slice_index_bits - done : 8u)
                                   : 0u;
-        const uint32_t shift = 32 + l.slice_bits + done;
-        const uint32_t mask = (1u << bits) - 1u;
-        uint32_t* total = totals + (pass & 1u);
-        const bool near = in.cap >= static_cast<uint32_t>(kWaveItems) || in.cap == in.slots;
-        if (near) {
-            hipLaunchKernelGGL(hist_kernel<true>, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift,
-                               mask, nblocks, hist);
-        } else {
-            hipLaunchKernelGGL(hist_kernel<false>, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift,
-                               mask, nblocks, hist);
-        }
-        hipLaunchKernelGGL(scan_partials_kernel, dim3(l.partials_len), dim3(kScanBlock), 0, stream,
-                           hist, hist_len, partials);
-        hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, partials,
-                           l.partials_len, total);
-        hipLaunchKernelGGL(scan_finish_kernel, dim3(l.partials_len), dim3(kScanBlock), 0, stream,
-                           hist, hist_len, partials);
-        if (near) {
-            hipLaunchKernelGGL(scatter_kernel<true>, dim3(nblocks), dim3(kRankBlock), 0, stream, in, shift,
-                               bits, nblocks, hist, pairs[cur ^ 1]);
-        } else {
-            hipLaunchKernelGGL(scatter_kernel<false>, dim3(nblocks), dim3(kRankBlock), 0, stream, in,
-                               shift, bits, nblocks, hist, pairs[cur ^ 1]);
-        }
-        cur ^= 1;
-        // the next pass reads the compacted pairs: one region holding `total` records
-        in = PassInput{pairs[cur], static_cast<uint32_t>(l.slots), static_cast<uint32_t>(l.slots),
-                       total};
+        shift = 32 + l.slice_bits + done;
+        mask = (1u << bits) - 1u;
+        return bits;
+    };
+
+    // pass 0: the emit regions, unstable
+    uint32_t shift, mask;
+    uint32_t bits = digit(0, shift, mask);
+    PassInput in{pairs[0], static_cast<uint32_t>(l.slots), l.cap,
+                 reinterpret_cast<const uint32_t*>(p + l.off_counts)};
+    const bool near = in.cap >= static_cast<uint32_t>(kWaveItems) || in.cap == in.slots;
+    if (near) {
+        hipLaunchKernelGGL((hist_any_kernel<true, false>), dim3(nblocks0), dim3(kRankBlock), 0, stream, in,
+                           nullptr, nullptr, shift, mask, nblocks0, hist);
+    } else {
+        hipLaunchKernelGGL((hist_any_kernel<false, false>), dim3(nblocks0), dim3(kRankBlock), 0, stream, in,
+                           nullptr, nullptr, shift, mask, nblocks0, hist);
     }
-    const uint32_t* total = in.counts;
-    hipLaunchKernelGGL(bounds_kernel, dim3(div_up(l.slots + 1, 256)), dim3(256), 0, stream, pairs[cur],
-                       total, 32 + l.slice_bits, l.num_slices, bounds);
+    uint32_t* total = totals;
+    scan(kBins * nblocks0, total);
+    if (near) {
+        hipLaunchKernelGGL((scatter_any_kernel<true, false>), dim3(nblocks0), dim3(kRankBlock), 0, stream, in,
+                           nullptr, nullptr, shift, mask, nblocks0, hist, pairs[1]);
+    } else {
+        hipLaunchKernelGGL((scatter_any_kernel<false, false>), dim3(nblocks0), dim3(kRankBlock), 0, stream, in,
+                           nullptr, nullptr, shift, mask, nblocks0, hist, pairs[1]);
+    }
+    int cur = 1;
+    const uint32_t bgrid = div_up(static_cast<uint64_t>(l.num_slices) + 1, 256);
+    if (l.passes == 1) {
+        hipLaunchKernelGGL(bounds_from_scan_kernel, dim3(bgrid), dim3(256), 0, stream, hist, nblocks0,
+                           nullptr, total, l.num_slices, bounds);
+    } else {
+        // pass 1: blocks over the pass-0 digit groups, unstable within a group
+        hipLaunchKernelGGL(groups_kernel, dim3(1), dim3(kBins), 0, stream, hist, nblocks0, total,
+                           group_start, first_block);
+        bits = digit(1, shift, mask);
+        in = PassInput{pairs[1], static_cast<uint32_t>(l.slots), static_cast<uint32_t>(l.slots), total};
+        hipLaunchKernelGGL((hist_any_kernel<true, true>), dim3(nblocks1), dim3(kRankBlock), 0, stream, in,
+                           first_block, group_start, shift, mask, nblocks1, hist);
+        scan(kBins * nblocks1, totals + 1);
+        hipLaunchKernelGGL((scatter_any_kernel<true, true>), dim3(nblocks1), dim3(kRankBlock), 0, stream, in,
+                           first_block, group_start, shift, mask, nblocks1, hist, pairs[0]);
+        cur = 0;
+        total = totals + 1;
+        if (l.passes == 2) {
+            hipLaunchKernelGGL(bounds_from_scan_kernel, dim3(bgrid), dim3(256), 0, stream, hist, nblocks1,
+                               first_block, total, l.num_slices, bounds);
+        } else {
+            // pass 2: stable (wave-ballot ranks) over the compacted pass-1 output
+            bits = digit(2, shift, mask);
+            in = PassInput{pairs[0], static_cast<uint32_t>(l.slots), static_cast<uint32_t>(l.slots), total};
+            hipLaunchKernelGGL(hist_kernel<true>, dim3(nblocks0), dim3(kRankBlock), 0, stream, in, shift,
+                               mask, nblocks0, hist);
+            scan(kBins * nblocks0, totals);
+            hipLaunchKernelGGL(scatter_kernel<true>, dim3(nblocks0), dim3(kRankBlock), 0, stream, in, shift,
+                               bits, nblocks0, hist, pairs[1]);
+            cur = 1;
+            total = totals;
+            hipLaunchKernelGGL(bounds_kernel, dim3(div_up(l.slots + 1, 256)), dim3(256), 0, stream,
+                               pairs[cur], total, 32 + l.slice_bits, l.num_slices, bounds);
+        }
+    }
+    (void)bits;
     const uint32_t grid = l.num_slices < kApplyGrid ? l.num_slices : kApplyGrid;
     hipLaunchKernelGGL(apply_kernel, dim3(grid), dim3(kApplyBlock), (32u << l.slice_bits), stream,
                        pairs[cur], bounds, masks, f.words, l.slice_bits, l.num_slices);
