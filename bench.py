@@ -50,6 +50,8 @@ def parse_args():
                     help="serving scenario: p50/p99 latency under Poisson arrivals")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-qps", type=float, default=100.0)
+    ap.add_argument("--fpr", default="on", choices=["on", "off"],
+                    help="empirical FPR on the foreign reads at s = 28, 30, 31 (m = 16) and m = 21")
     ap.add_argument("--fastx", default="on", choices=["on", "off"],
                     help="FASTA insert / FASTQ query through files (text ingest throughput)")
     ap.add_argument("--fastx-bases", type=int, default=400_000_000)
@@ -79,6 +81,50 @@ def committed_traffic(workload: str, kernel: str):
         if len(cells) >= 5 and cells[1] == name:
             return float(cells[4])
     return None
+
+
+# Published FPR of the reference at 21.4 bits/k-mer (percent; its benchmark
+# instantiates m = 21, k = 31, h = 4), for context beside the measured values.
+PUBLISHED_FPR_M21 = {"s28": 0.848, "s30": 0.951, "s31": 1.593}
+
+
+def fpr_scenario(genome, reads, wl, out, stream_handle) -> dict:
+    """Empirical false-positive rate: positives among the windows of the foreign
+    (uniform random) reads, which are reads [genome_frac * n, n) of the query
+    buffer, over their valid windows. s = 28 comes from the timed query's
+    output; s = 30, 31 (m = 16) and s = 28 with m = 21 insert the genome into a
+    filter of their own and query the foreign reads."""
+    import torch
+
+    import cusbf_amd as cs
+
+    k, rl = wl.k, wl.reads.read_len
+    n = wl.reads.n_reads
+    g = int(n * wl.reads.genome_frac)
+    start = g * (rl + 1)
+    foreign = reads[start:]
+    n_win = foreign.numel() - k + 1
+    negatives = (n - g) * (rl - k + 1)  # foreign reads hold no invalid symbol
+    genome_windows = genome.numel() - k + 1
+    res = {"negative_kmers": negatives, "foreign_reads": n - g,
+           "bits_per_kmer_requested": wl.bits_per_kmer, "m16": {}, "m21": {},
+           "published_m21_percent": PUBLISHED_FPR_M21}
+    pos = int(cs.count_positive(out[start:start + n_win]))
+    res["m16"]["s28"] = pos / negatives
+    for s_, m_ in [(30, 16), (31, 16), (28, 21)]:
+        f = cs.Filter(cs.Config(k, s_, m_, wl.h), int(wl.bits_per_kmer * genome_windows))
+        f.insert_sequence_async(genome, stream=stream_handle)
+        o = torch.empty(n_win, dtype=torch.uint8, device="cuda")
+        f.contains_sequence_async(foreign, o, stream=stream_handle)
+        pos = int(cs.count_positive(o))
+        res[f"m{m_}"][f"s{s_}"] = pos / negatives
+        res["bits_per_kmer_effective"] = f.filter_bits() / genome_windows
+        f.close()
+        del o
+        torch.cuda.empty_cache()
+    log(f"[bench] FPR m=16 s=28/30/31: " + " / ".join(f"{100 * res['m16'][f's{x}']:.3f}%" for x in (28, 30, 31))
+        + f"; m=21 s=28: {100 * res['m21']['s28']:.3f}%")
+    return res
 
 
 def cpu_baseline(genome, reads, wl, threads: int):
@@ -649,6 +695,8 @@ def main() -> int:
     t_dom, b_dom = (tq, bytes_q) if dom == "query" else (ti, bytes_ins)
     achieved = b_dom / t_dom
 
+    fpr = fpr_scenario(genome, reads, wl, out, sh) if args.fpr == "on" else None
+
     # serving runs on the filter as inserted, before the probes overwrite its buffer
     serving = None
     if args.serve == "on":
@@ -716,6 +764,7 @@ def main() -> int:
                      "kernel": dom, "alg_bytes_per_launch": b_dom},
         "serve_p50_ms": None,
         "serve_p99_ms": None,
+        "fpr": fpr,
     }
     if serving is not None:
         result["serve_p50_ms"] = serving["server_p50_ms"]

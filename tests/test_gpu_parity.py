@@ -331,3 +331,31 @@ def test_large_insert_query_no_false_negatives_and_oracle_words(gpu):
     w = o.new_filter(f.num_shards())
     o.mt_insert(host, w)
     assert (f.words() == w).all()
+
+
+@pytest.mark.parametrize("s", [28, 30, 31])
+def test_fpr_on_seeded_negatives_equals_oracle(gpu, s):
+    """The small plain workload's FPR protocol: insert one seeded uniform 10 Mbp
+    sequence (filter requested at 21.4 bits per k-mer), query it (every k-mer
+    present) and an independent seeded 10 Mbp sequence; the GPU's positives
+    on the negatives equal the oracle's exactly, so the empirical FPR matches."""
+    import cusbf_amd as cs
+
+    torch = _torch()
+    n = 10_000_000
+    a = random_dna(n, seed=101)
+    b = random_dna(n, seed=202)
+    cfg = make_config(31, s, 16, 4)
+    f = cs.Filter(cfg, int(21.4 * (n - 30)))
+    f.insert_sequence(a)
+    hits_a = f.contains_sequence(a)
+    assert int(np.count_nonzero(hits_a)) == n - 30
+    hits_b = f.contains_sequence(b)
+    o = make_oracle(31, s, 16, 4)
+    w = o.new_filter(f.num_shards())
+    o.mt_insert(a, w)
+    want = o.mt_contains(b, w)
+    assert (np.asarray(hits_b, dtype=np.uint8) == want.astype(np.uint8)).all()
+    fpr = np.count_nonzero(want) / (n - 30)
+    assert 0 < fpr < 0.02, fpr
+    torch.cuda.synchronize()
