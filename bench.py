@@ -50,6 +50,9 @@ def parse_args():
                     help="serving scenario: p50/p99 latency under Poisson arrivals")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-qps", type=float, default=100.0)
+    ap.add_argument("--dense", default="on", choices=["on", "off"],
+                    help="dense packed input beside byte input (device and host forms)")
+    ap.add_argument("--dense-host-bases", type=int, default=400_000_000)
     ap.add_argument("--fpr", default="on", choices=["on", "off"],
                     help="empirical FPR on the foreign reads at s = 28, 30, 31 (m = 16) and m = 21")
     ap.add_argument("--fastx", default="on", choices=["on", "off"],
@@ -124,6 +127,101 @@ def fpr_scenario(genome, reads, wl, out, stream_handle) -> dict:
         torch.cuda.empty_cache()
     log(f"[bench] FPR m=16 s=28/30/31: " + " / ".join(f"{100 * res['m16'][f's{x}']:.3f}%" for x in (28, 30, 31))
         + f"; m=21 s=28: {100 * res['m21']['s28']:.3f}%")
+    return res
+
+
+def dense_scenario(genome, wl, stream_handle, host_bases: int, reps: int = 3) -> dict:
+    """Dense packed wire format beside byte input (dense_packed_throughput.cu's
+    fixtures) on a seeded uniform ACGT sequence of the genome's length (dense
+    words hold no invalid symbol, so the chromosome separators of the genome
+    would read differently): packed on the device (2 bits per base,
+    LSB-first), then byte vs dense insert and query -- the query is the
+    inserted sequence, every k-mer present -- timed with HIP events (filter
+    cleared outside the timed region, median of `reps`); and the host forms
+    (host bytes / host words in, one synchronous call each, transfers
+    included) on its first `host_bases` bases."""
+    import numpy as np
+    import torch
+
+    import cusbf_amd as cs
+    from cusbf_amd import workloads as W
+
+    cfg = cs.Config(wl.k, wl.s, wl.m, wl.h)
+    genome = W.uniform_dna(genome.numel(), seed=31)
+    n = genome.numel()
+    windows = n - wl.k + 1
+    words = torch.empty((n + 31) // 32, dtype=torch.int64, device="cuda")
+    f = cs.Filter(cfg, int(wl.bits_per_kmer * windows))
+    out = torch.empty(windows, dtype=torch.uint8, device="cuda")
+
+    def timed(op, setup=None):
+        ts = []
+        for _ in range(reps):
+            if setup:
+                setup()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            op()
+            b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b) / 1e3)
+        return statistics.median(ts)
+
+    clear = lambda: f.clear(stream_handle)
+    t_pack = timed(lambda: cs.pack_dense_sequence_device(cfg, genome, n, words, stream=stream_handle))
+    t_bi = timed(lambda: f.insert_sequence_async(genome, stream=stream_handle), clear)
+    t_bq = timed(lambda: f.contains_sequence_async(genome, out, stream=stream_handle))
+    byte_pos = int(cs.count_positive(out))
+    t_di = timed(lambda: f.insert_dense_packed_async(words, n, stream=stream_handle), clear)
+    t_dq = timed(lambda: f.contains_dense_packed_async(words, n, out, stream=stream_handle))
+    dense_pos = int(cs.count_positive(out))
+    res = {"symbols": n, "kmers": windows, "packed_bytes": words.numel() * 8,
+           "pack_device_GBps": n / t_pack / 1e9,
+           "device": {"byte_insert_kmers_per_s": windows / t_bi, "dense_insert_kmers_per_s": windows / t_di,
+                      "byte_query_kmers_per_s": windows / t_bq, "dense_query_kmers_per_s": windows / t_dq,
+                      "byte_insert_ms": t_bi * 1e3, "dense_insert_ms": t_di * 1e3,
+                      "byte_query_ms": t_bq * 1e3, "dense_query_ms": t_dq * 1e3},
+           "all_positive": byte_pos == windows and dense_pos == windows}
+    # host forms on a sample: host bytes in / host words in
+    hn = min(n, host_bases)
+    hw = hn - wl.k + 1
+    host_bytes = genome[:hn].cpu().numpy()
+    host_words = words[: (hn + 31) // 32].cpu().numpy().view(np.uint64).copy()
+    if hn % 32:  # the sample's last word: only its first hn % 32 bases
+        host_words[-1] &= np.uint64((1 << (2 * (hn % 32))) - 1)
+    g = cs.Filter(cfg, int(wl.bits_per_kmer * hw))
+
+    def wall(op):
+        ts = []
+        for _ in range(reps):
+            g.clear()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            op()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    t_hbi = wall(lambda: g.insert_sequence(host_bytes))
+    t_hdi = wall(lambda: g.insert_dense_packed(host_words, hn))
+    g.insert_sequence(host_bytes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hb = g.contains_sequence(host_bytes)
+    t_hbq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hd = g.contains_dense_packed(host_words, hn)
+    t_hdq = time.perf_counter() - t0
+    res["host"] = {"bases": hn, "byte_insert_kmers_per_s": hw / t_hbi, "dense_insert_kmers_per_s": hw / t_hdi,
+                   "byte_query_kmers_per_s": hw / t_hbq, "dense_query_kmers_per_s": hw / t_hdq,
+                   "answers_equal": bool(np.array_equal(hb, hd))}
+    f.close()
+    g.close()
+    del words, out, genome
+    torch.cuda.empty_cache()
+    d = res["device"]
+    log(f"[bench] dense: device insert byte {d['byte_insert_ms']:.2f} ms / dense {d['dense_insert_ms']:.2f} ms, "
+        f"query byte {d['byte_query_ms']:.2f} ms / dense {d['dense_query_ms']:.2f} ms; host insert "
+        f"{res['host']['byte_insert_kmers_per_s'] / 1e9:.2f} / {res['host']['dense_insert_kmers_per_s'] / 1e9:.2f} G k-mers/s")
     return res
 
 
@@ -770,6 +868,8 @@ def main() -> int:
         result["serve_p50_ms"] = serving["server_p50_ms"]
         result["serve_p99_ms"] = serving["server_p99_ms"]
         result["serving"] = serving
+    if args.dense == "on":
+        result["dense"] = dense_scenario(genome, wl, sh, args.dense_host_bases)
     if args.fastx == "on":
         result["fastx"] = fastx_scenario(genome, reads, wl, args.fastx_bases, args.fastx_reads)
     if args.batched == "on":
