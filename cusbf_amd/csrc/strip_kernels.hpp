@@ -140,6 +140,23 @@ __device__ __forceinline__ void encode16(u32x4 v, uint32_t& packed, uint32_t& in
     invalid = i0 | (i1 << 4) | (i2 << 8) | (i3 << 12);
 }
 
+// The 16 input bytes of a lane's chunk when they lie in bounds and 16-byte
+// aligned (one 128-bit load, issued ahead of its use); otherwise `fast` is
+// false and the chunk is read byte by byte where it is used.
+struct RawChunk {
+    u32x4 v;
+    bool fast;
+};
+
+__device__ __forceinline__ RawChunk load_raw(const SequenceInput& in, uint64_t first) {
+    RawChunk r;
+    r.v = u32x4{0u, 0u, 0u, 0u};
+    const char* src = in.seq + first;
+    r.fast = first + 16 <= in.num_symbols && (reinterpret_cast<uintptr_t>(src) & 15u) == 0;
+    if (r.fast) r.v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(src));
+    return r;
+}
+
 // Byte input: chunk of 16 symbols starting at symbol `first`.
 __device__ __forceinline__ void load_chunk_bytes(const char* seq, uint64_t num_symbols,
                                                  uint64_t first, uint32_t& packed,
@@ -185,6 +202,23 @@ __device__ __forceinline__ void load_chunk_dense(const uint64_t* words, uint64_t
         invalid = 0xFFFFu;
     } else {
         invalid = (0xFFFFu << static_cast<uint32_t>(num_symbols - first)) & 0xFFFFu;
+    }
+}
+
+// Byte input from a chunk loaded ahead (load_raw).
+__device__ __forceinline__ void stage_lane_raw(const RawChunk& r, const SequenceInput& in, uint64_t first,
+                                               int lane, Dna2::Lds& st, uint32_t* inv) {
+    uint32_t packed, invalid;
+    if (r.fast) {
+        encode16(r.v, packed, invalid);
+    } else {
+        load_chunk_bytes(in.seq, in.num_symbols, first, packed, invalid);
+    }
+    st.p[lane] = packed;
+    inv[lane] = invalid;
+    if (lane < kLdsWords - kWave) {
+        st.p[kWave + lane] = 0;
+        inv[kWave + lane] = 0xFFFFu;
     }
 }
 
@@ -264,15 +298,12 @@ __device__ __forceinline__ uint32_t encode_protein4(uint32_t x, uint32_t& invali
     return codes;
 }
 
-template <bool Dense>
-__device__ __forceinline__ void stage_lane(const SequenceInput& in, uint64_t first, int lane,
-                                           Protein5::Lds& st, uint32_t* inv) {
-    static_assert(!Dense, "the byte-code front end reads byte sequences only");
-    const char* src = in.seq + first;
+__device__ __forceinline__ void stage_lane_raw(const RawChunk& r, const SequenceInput& in, uint64_t first,
+                                               int lane, Protein5::Lds& st, uint32_t* inv) {
     u32x4 codes;
     uint32_t invalid;
-    if (first + 16 <= in.num_symbols && (reinterpret_cast<uintptr_t>(src) & 15u) == 0) {
-        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(src));
+    if (r.fast) {
+        const u32x4 v = r.v;
         uint32_t i0, i1, i2, i3;
         codes.x = encode_protein4(v.x, i0);
         codes.y = encode_protein4(v.y, i1);
@@ -300,6 +331,13 @@ __device__ __forceinline__ void stage_lane(const SequenceInput& in, uint64_t fir
         st.c[kWave + lane] = u32x4{0u, 0u, 0u, 0u};
         inv[kWave + lane] = 0xFFFFu;
     }
+}
+
+template <bool Dense>
+__device__ __forceinline__ void stage_lane(const SequenceInput& in, uint64_t first, int lane,
+                                           Protein5::Lds& st, uint32_t* inv) {
+    static_assert(!Dense, "the byte-code front end reads byte sequences only");
+    stage_lane_raw(load_raw(in, first), in, first, lane, st, inv);
 }
 
 __device__ __forceinline__ Protein5::Ctx lane_context(const Protein5::Lds& st, int lane) {
@@ -669,7 +707,56 @@ __device__ __forceinline__ void findere_test(const LaneWindows<K, S, M, H, FE>& 
                                              uint32_t base, const WaveLds<FE, Batched>& L,
                                              const FilterView& filter, uint32_t (&result)[4]) {
     constexpr int F = Shape<K, S, M, H, FE>::kFindereSpan;
-    if constexpr (F == 4) {
+    if constexpr (F == 4 && AllSlotted) {
+        // Every run has a slot: no branches. Window o reads slot (run heads up
+        // to o) - 1 at a running LDS offset; slot -1 (windows before the lane's
+        // first head that continue no run: all invalid) lies in this wave's LDS
+        // just below the slots. Invalid windows start as misses.
+        uint64_t last[4];
+        uint64_t pair[3][4];
+        uint64_t m1[4], m2[4];
+        code_masks<H>(lw.code[0], last);
+        code_masks<H>(lw.code[1], m1);
+        code_masks<H>(lw.code[2], m2);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            pair[0][q] = last[q] | m1[q];
+            pair[1][q] = m1[q] | m2[q];
+            last[q] = m2[q];
+        }
+        uint32_t misses[4];  // byte o & 3 of word o >> 2: window o missed
+        const uint32_t invalid = ~lw.valid;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            // bits 4j..4j+3 of `invalid` to bytes (the shifted copies do not overlap)
+            misses[j] = (((invalid >> (4 * j)) & 0xFu) * 0x00204081u) & 0x01010101u;
+        }
+        const char* slot0 = reinterpret_cast<const char*>(&L.slot_w[0][0]) - 32 + 32 * static_cast<int>(base);
+#pragma unroll
+        for (int o = 0; o < kLaneWindows; ++o) {
+            uint64_t nm[4];
+            code_masks<H>(lw.code[o + 3], nm);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                pair[(o + 2) % 3][q] = last[q] | nm[q];
+                last[q] = nm[q];
+            }
+            slot0 += static_cast<int>(__builtin_amdgcn_ubfe(heads, o, 1)) << 5;
+            const u32x4* src = reinterpret_cast<const u32x4*>(slot0);
+            const u32x4 a = src[0];
+            const u32x4 b = src[1];
+            const uint32_t wv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            uint32_t miss = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                miss |= or_and_not(lo32(pair[o % 3][q]), wv[2 * q], lo32(pair[(o + 2) % 3][q]));
+                miss |= or_and_not(hi32(pair[o % 3][q]), wv[2 * q + 1], hi32(pair[(o + 2) % 3][q]));
+            }
+            misses[o >> 2] |= (miss < 1u ? miss : 1u) << (8 * (o & 3));
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) result[j] = ~misses[j] & 0x01010101u;
+    } else if constexpr (F == 4) {
         uint64_t last[4];     // mask of the newest s-mer
         uint64_t pair[3][4];  // pair[p % 3] = mask(p) | mask(p + 1)
         uint64_t m1[4], m2[4];
@@ -727,15 +814,24 @@ __device__ __forceinline__ void findere_test(const LaneWindows<K, S, M, H, FE>& 
     }
 }
 
-template <int K, int S, int M, int H, class FE, bool Dense, bool Batched>
+// Query tile. Prefetch (plain byte input): `raw` holds this tile's input
+// chunk, loaded ahead; once the shard loads are issued the lane loads the
+// chunk of tile `next` into it (none when next >= num_tiles), so that load's
+// latency hides behind the findere test instead of stalling the next tile.
+template <int K, int S, int M, int H, class FE, bool Dense, bool Batched, bool Prefetch = false>
 __device__ __forceinline__ void contains_tile(const SequenceInput& in, const FilterView& filter,
                                               uint64_t num_windows, uint8_t* out, uint64_t tile,
-                                              int lane, WaveLds<FE, Batched>& L) {
+                                              int lane, WaveLds<FE, Batched>& L, RawChunk* raw = nullptr,
+                                              uint64_t next = 0, uint64_t num_tiles = 0) {
     using Sh = Shape<K, S, M, H, FE>;
     constexpr int F = Sh::kFindereSpan;
     const uint64_t tile_first = tile * Sh::kWaveWindows;
 
-    stage_tile<K, S, M, H, FE, Dense, Batched>(in, tile, lane, L);
+    if constexpr (Prefetch) {
+        stage_lane_raw(*raw, in, tile_first + 16ull * static_cast<uint64_t>(lane), lane, L.st, L.inv);
+    } else {
+        stage_tile<K, S, M, H, FE, Dense, Batched>(in, tile, lane, L);
+    }
     wave_lds_sync();
 
     const bool owns = lane < Sh::kWindowLanes;
@@ -762,6 +858,16 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
     for (int j = 0; j < kHalves; ++j) {
         const uint32_t q = static_cast<uint32_t>(lane + j * kWave) >> 1;
         if (q < n_slots) sw[j] = shards[2 * static_cast<uint64_t>(L.slot_shard[q]) + half];
+    }
+    if constexpr (Prefetch) {
+        if (next < num_tiles) {
+            *raw = load_raw(in, next * Sh::kWaveWindows + 16ull * static_cast<uint64_t>(lane));
+        } else {
+            raw->fast = false;
+        }
+        // compiler-only memory barrier: keeps the load here, ahead of the test,
+        // instead of next to its use in the next tile
+        asm volatile("" ::: "memory");
     }
 #pragma unroll
     for (int j = 0; j < kHalves; ++j) {
@@ -908,9 +1014,19 @@ __global__ __launch_bounds__(kBlock, WPE) void contains_kernel(SequenceInput in,
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = static_cast<int>(threadIdx.x) / kWave;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
-    for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wave; tile < num_tiles;
-         tile += stride) {
-        contains_tile<K, S, M, H, FE, Dense, false>(in, filter, num_windows, out, tile, lane, lds[wave]);
+    const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wave;
+    if constexpr (Dense) {
+        for (uint64_t tile = first; tile < num_tiles; tile += stride) {
+            contains_tile<K, S, M, H, FE, Dense, false>(in, filter, num_windows, out, tile, lane, lds[wave]);
+        }
+    } else {
+        RawChunk raw;
+        raw.fast = false;
+        if (first < num_tiles) raw = load_raw(in, first * Shape<K, S, M, H, FE>::kWaveWindows + 16ull * lane);
+        for (uint64_t tile = first; tile < num_tiles; tile += stride) {
+            contains_tile<K, S, M, H, FE, false, false, true>(in, filter, num_windows, out, tile, lane,
+                                                             lds[wave], &raw, tile + stride, num_tiles);
+        }
     }
 }
 
