@@ -141,6 +141,13 @@ struct cusbf_filter {
     void* d_sorted = nullptr;
     uint64_t sorted_bytes = 0;
     hipEvent_t sorted_done = nullptr;  // recorded after the last sorted insert's kernels
+    // The words are all zero (created or cleared, no insert since): a sorted
+    // insert need not read them. Never trusted again once the words pointer
+    // has been handed out or an insert has been captured into a graph.
+    bool words_zero = false;
+    mutable bool words_shared = false;
+    hipEvent_t insert_done = nullptr;  // recorded after the last insert (outside captures)
+    bool insert_recorded = false;
     bool sorted_used = false;
 };
 
@@ -295,15 +302,25 @@ int end_sorted(cusbf_filter* f, hipStream_t stream) {
     return CUSBF_OK;
 }
 
-// Zeroes the region counts, then runs `emit(sink)` (the insert kernel writing
-// records) and the partition and apply kernels, in stream order.
+// Whether this insert goes into all-zero words (see cusbf_filter::words_zero);
+// every insert consumes the state.
+bool take_fresh(cusbf_filter* f, hipStream_t stream) {
+    if (capturing(stream)) f->words_shared = true;
+    const bool fresh = f->words_zero && !f->words_shared;
+    f->words_zero = false;
+    return fresh;
+}
+
+// Zeroes the region counts and the overflow flag, then runs `emit(sink)` (the
+// insert kernel writing records) and the partition and apply kernels, in
+// stream order.
 template <class Emit>
 hipError_t enqueue_sorted(const cusbf_filter* f, const sbf::SortedLayout& l, void* base,
-                          hipStream_t stream, Emit&& emit) {
+                          hipStream_t stream, bool fresh, Emit&& emit) {
     const sbf::RecordSink sink = sbf::sorted::record_sink(l, base);
-    hipError_t e = sbf::fill_u64(reinterpret_cast<uint64_t*>(sink.counts), 0, (l.regions + 1ull) / 2, stream);
+    hipError_t e = sbf::fill_u64(reinterpret_cast<uint64_t*>(sink.counts), 0, (l.regions + 2ull) / 2, stream);
     if (e == hipSuccess) e = emit(sink);
-    if (e == hipSuccess) e = sbf::sorted::sorted_apply(l, base, view(f), stream);
+    if (e == hipSuccess) e = sbf::sorted::sorted_apply(l, base, view(f), fresh, stream);
     return e;
 }
 
@@ -316,8 +333,38 @@ uint32_t plain_sorted_blocks(const cusbf_filter* f, uint64_t tiles) {
     return static_cast<uint32_t>(std::max<uint64_t>(blocks, 1));
 }
 
+// Inserts into one filter run one after another whatever their streams: the
+// sorted apply rewrites whole slices, so an insert overlapping it could lose
+// updates. Inside a capture the graph's order is its caller's.
+int order_insert(cusbf_filter* f, hipStream_t stream) {
+    if (capturing(stream)) return CUSBF_OK;
+    if (!f->insert_done) {
+        CUSBF_HIP_TRY(hipEventCreateWithFlags(&f->insert_done, hipEventDisableTiming));
+    } else if (f->insert_recorded) {
+        CUSBF_HIP_TRY(hipStreamWaitEvent(stream, f->insert_done, 0));
+    }
+    return CUSBF_OK;
+}
+
+int mark_insert(cusbf_filter* f, hipStream_t stream) {
+    if (capturing(stream) || !f->insert_done) return CUSBF_OK;
+    CUSBF_HIP_TRY(hipEventRecord(f->insert_done, stream));
+    f->insert_recorded = true;
+    return CUSBF_OK;
+}
+
+int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream);
+
 int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
+    int rc = order_insert(f, stream);
+    if (rc == CUSBF_OK) rc = launch_insert_ordered(f, in, stream);
+    if (rc == CUSBF_OK) rc = mark_insert(f, stream);
+    return rc;
+}
+
+int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
     f->last_path = CUSBF_INSERT_ATOMIC;
+    const bool fresh = take_fresh(f, stream);
     if (!strip_reads(f, in)) {
         const hipError_t e = sbf::generic_insert(f->kc, in, view(f), stream);
         if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
@@ -336,7 +383,7 @@ int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t str
     }
     rc = begin_sorted(f, l.bytes, stream);
     if (rc != CUSBF_OK) return rc;
-    const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, [&](const sbf::RecordSink& sink) {
+    const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, fresh, [&](const sbf::RecordSink& sink) {
         return f->strip->insert(in, view(f), &sink, stream);
     });
     if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
@@ -535,12 +582,14 @@ const cusbf_batch_tuning& tuning_for(const cusbf_filter* f, uint32_t op, uint64_
 
 // Reset, prepass and kernel of one batched call, in stream order; allocates
 // nothing, so it can be captured into a graph. `sorted` (insert only, or
-// null): the layout of the sorted insert's workspace at `sorted_base`.
+// null): the layout of the sorted insert's workspace at `sorted_base`;
+// `fresh`: the filter words are all zero.
 int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
                     uint64_t num_symbols, const uint64_t* d_offsets, uint64_t num_records,
                     uint32_t* d_state, uint64_t* d_starts, const cusbf_batch_tuning& t,
                     uint8_t* d_output, hipStream_t stream,
-                    const sbf::SortedLayout* sorted = nullptr, void* sorted_base = nullptr) {
+                    const sbf::SortedLayout* sorted = nullptr, void* sorted_base = nullptr,
+                    bool fresh = false) {
     sbf::SequenceInput in = byte_input(f, d_symbols, num_symbols * f->d.symbol_width);
     in.offsets = d_offsets;
     in.num_records = num_records;
@@ -552,7 +601,7 @@ int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
         const int variant = t.waves_per_simd == 4 && f->strip->batched_insert[1] ? 1 : 0;
         const sbf::BatchLaunch shape{t.blocks_per_cu, f->compute_units, t.claim};
         if (op == CUSBF_BATCH_INSERT && sorted) {
-            e = enqueue_sorted(f, *sorted, sorted_base, stream, [&](const sbf::RecordSink& sink) {
+            e = enqueue_sorted(f, *sorted, sorted_base, stream, fresh, [&](const sbf::RecordSink& sink) {
                 return f->strip->batched_insert[variant](in, view(f), shape, &sink, stream);
             });
         } else {
@@ -768,6 +817,7 @@ void cusbf_destroy(cusbf_filter* f) {
         (void)hipFree(f->d_sorted);
     }
     if (f->sorted_done) (void)hipEventDestroy(f->sorted_done);
+    if (f->insert_done) (void)hipEventDestroy(f->insert_done);
     delete f;
 }
 
@@ -777,6 +827,7 @@ int cusbf_clear(cusbf_filter* f, void* stream) {
     if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
     CUSBF_HIP_TRY(sbf::fill_u64(f->d_words, 0, f->num_shards * 4, as_stream(stream)));
     CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    f->words_zero = true;
     return CUSBF_OK;
 }
 
@@ -796,6 +847,7 @@ uint64_t cusbf_num_shards(const cusbf_filter* f) { return f ? f->num_shards : 0;
 
 int cusbf_words_device_ptr(const cusbf_filter* f, uint64_t** out) {
     if (!f || !out) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    f->words_shared = true;  // the caller may write the words
     *out = f->d_words;
     return CUSBF_OK;
 }
@@ -1260,9 +1312,13 @@ int cusbf_insert_batched_async(cusbf_filter* f, const char* d_symbols, uint64_t 
     }
     const cusbf_batch_tuning& t =
         sorted ? ts : tuning_for(f, CUSBF_BATCH_INSERT, num_symbols, num_records);
+    const bool fresh = take_fresh(f, as_stream(stream));
+    rc = order_insert(f, as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
     rc = enqueue_batched(f, CUSBF_BATCH_INSERT, d_symbols, num_symbols, d_offsets, num_records,
                          f->d_batch_state, f->d_starts, t, nullptr, as_stream(stream),
-                         sorted ? &l : nullptr, f->d_sorted);
+                         sorted ? &l : nullptr, f->d_sorted, fresh);
+    if (rc == CUSBF_OK) rc = mark_insert(f, as_stream(stream));
     if (rc == CUSBF_OK && sorted) {
         f->last_path = CUSBF_INSERT_SORTED;
         rc = end_sorted(f, as_stream(stream));
@@ -1334,6 +1390,10 @@ int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
     hipStream_t capture = nullptr;
     sbf::SortedLayout l;
     bool sorted = false;
+    if (op == CUSBF_BATCH_INSERT) {  // the plan's graph can run at any later time
+        f->words_shared = true;
+        f->words_zero = false;
+    }
     if (op == CUSBF_BATCH_INSERT && batched_strip(f)) {
         rc = choose_batched_sorted(f, num_symbols, ts, nullptr, &l, &sorted);
         if (rc != CUSBF_OK) {

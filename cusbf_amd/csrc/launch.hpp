@@ -48,11 +48,13 @@ struct BatchLaunch {
 // region w, slots [w * cap, (w + 1) * cap): slot i holds the pair
 // (shard << 32) | i and the run's OR mask (4 words at masks + 4 i); the wave
 // writes its record count to counts[w]. Runs past the region's capacity use
-// atomics. The launch uses at most regions / 4 blocks.
+// atomics, and a wave whose region is full sets *overflow to 1. The launch
+// uses at most regions / 4 blocks.
 struct RecordSink {
     uint64_t* pairs = nullptr;
     uint64_t* masks = nullptr;
     uint32_t* counts = nullptr;
+    uint32_t* overflow = nullptr;
     uint32_t cap = 0;
     uint32_t regions = 0;
 };
@@ -120,8 +122,10 @@ RecordSink record_sink(const SortedLayout& l, void* base);
 // Allows the apply kernel its LDS slice; call before launching (outside any capture).
 hipError_t prepare();
 // Partition the records the insert kernels wrote and OR them into the filter,
-// in stream order after those kernels.
-hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStream_t stream);
+// in stream order after those kernels. `fresh`: every filter word was zero
+// before those kernels ran (the counts and overflow flag zeroed before them).
+hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, bool fresh,
+                        hipStream_t stream);
 }  // namespace sorted
 
 // Generic kernels: any alphabet / configuration, one window per thread; in the

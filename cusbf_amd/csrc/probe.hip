@@ -3,8 +3,10 @@
 //
 //  * stream copy: 16-byte vector loads and stores, grid-stride (HBM streaming ceiling);
 //  * random 32-byte gather: each access reads one random 32-byte sector of a
-//    buffer (two 16-byte loads), eight independent accesses in flight per lane
-//    (the query's shard loads);
+//    buffer, lanes 2r and 2r+1 taking its two 16-byte halves in one load
+//    instruction (one memory request per sector line), eight independent
+//    accesses in flight per lane pair (the query's shard loads and the sorted
+//    insert's mask loads have this shape);
 //  * random sector update: four 64-bit atomic ORs into one random 32-byte
 //    sector per access (the insert's per-run update).
 // Sector indices come from a counter-based hash, so every run is reproducible.
@@ -35,23 +37,19 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const u32x4* src, u32x4* d
 __global__ __launch_bounds__(kBlock) void gather_kernel(const u32x4* sectors, uint64_t sector_mask,
                                                         uint64_t accesses, uint64_t seed,
                                                         uint32_t* sink) {
-    const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    const uint64_t threads = static_cast<uint64_t>(gridDim.x) * kBlock;
+    const uint64_t pair = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 1;
+    const uint64_t pairs = static_cast<uint64_t>(gridDim.x) * (kBlock / 2);
+    const uint32_t half = threadIdx.x & 1u;
     uint32_t acc = 0;
-    for (uint64_t base = tid * kInFlight; base < accesses; base += threads * kInFlight) {
-        u32x4 a[kInFlight], b[kInFlight];
+    for (uint64_t base = pair * kInFlight; base < accesses; base += pairs * kInFlight) {
+        u32x4 a[kInFlight];
 #pragma unroll
         for (int j = 0; j < kInFlight; ++j) {
             const uint64_t s = mix64(seed + base + j) & sector_mask;
-            if (base + j < accesses) {
-                a[j] = sectors[2 * s];
-                b[j] = sectors[2 * s + 1];
-            } else {
-                a[j] = b[j] = u32x4{0u, 0u, 0u, 0u};
-            }
+            a[j] = base + j < accesses ? sectors[2 * s + half] : u32x4{0u, 0u, 0u, 0u};
         }
 #pragma unroll
-        for (int j = 0; j < kInFlight; ++j) acc ^= a[j].x ^ a[j].w ^ b[j].y ^ b[j].z;
+        for (int j = 0; j < kInFlight; ++j) acc ^= a[j].x ^ a[j].w;
     }
     if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the loads; practically never stores
 }
@@ -96,7 +94,7 @@ int cusbf_probe_copy(const void* src, void* dst, uint64_t n_bytes, void* stream)
 int cusbf_probe_gather(const void* buffer, uint64_t num_sectors, uint64_t accesses, uint64_t seed,
                        uint32_t* d_sink, void* stream) {
     (void)hipGetLastError();
-    hipLaunchKernelGGL(gather_kernel, dim3(grid_of(accesses, 4 * kInFlight)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_of(accesses, 2 * kInFlight)), dim3(kBlock), 0,
                        reinterpret_cast<hipStream_t>(stream), static_cast<const u32x4*>(buffer),
                        num_sectors - 1, accesses, seed, d_sink);
     return static_cast<int>(hipGetLastError());

@@ -9,8 +9,8 @@
 //   1. emit (strip_kernels.hpp, insert_tile): each wave writes its tile's
 //      slotted runs as records into its own region of a record buffer -- an
 //      8-byte pair (shard << 32 | record index) and the run's 32-byte OR mask;
-//      runs beyond the region's capacity (and a tile's runs beyond its LDS
-//      slots) still take the atomic path;
+//      runs beyond the region's capacity still take the atomic path, and a
+//      region that fills up raises the overflow flag;
 //   2. partition (here): LSD radix passes, 8 bits each, over the pairs, keyed
 //      on the slice index shard >> slice_bits, until the pairs are grouped by
 //      slice. Only the grouping matters (the apply ORs a slice's records in any
@@ -22,7 +22,9 @@
 //      and uses wave ballots;
 //   3. apply (here): one block per non-empty slice loads the slice's words
 //      into LDS, ORs the masks of its records into them with LDS atomics and
-//      stores the slice back.
+//      stores the slice back. Into a filter whose words are all zero (the host
+//      knows: created or cleared, no insert since) it starts each slice from
+//      zeros instead of reading it, unless the overflow flag is set.
 // OR is commutative and idempotent, so the words equal the atomic insert's;
 // the atomics of step 1 land before step 3 reads the slice (stream order).
 #include <hip/hip_runtime.h>
@@ -473,22 +475,30 @@ __global__ __launch_bounds__(256) void bounds_kernel(const uint64_t* pairs, cons
 
 // One block per non-empty slice at a time (blocks stride over the slices):
 // slice words -> LDS, OR of the slice's records, LDS -> slice words.
+// `fresh`: the filter words were all zero before the emit kernel; then a slice
+// is read only when the overflow flag says that atomics reached the filter.
 __global__ __launch_bounds__(kApplyBlock) void apply_kernel(const uint64_t* pairs,
                                                             const uint32_t* bounds,
                                                             const u32x4* masks, uint64_t* words,
                                                             uint32_t slice_bits,
-                                                            uint32_t num_slices) {
+                                                            uint32_t num_slices, bool fresh,
+                                                            const uint32_t* overflow) {
     extern __shared__ uint64_t slice[];  // 4 << slice_bits words
     u32x4* lds4 = reinterpret_cast<u32x4*>(slice);
     const uint32_t vecs = 2u << slice_bits;  // 16-byte vectors of one slice
     const uint32_t local_mask = (1u << slice_bits) - 1u;
+    const bool load = !fresh || *overflow != 0u;  // uniform
     constexpr int kUnroll = 4;
     for (uint32_t b = blockIdx.x; b < num_slices; b += gridDim.x) {
         const uint32_t lo = bounds[b];
         const uint32_t hi = bounds[b + 1];
         if (lo >= hi) continue;  // uniform: every thread reads the same bounds
         u32x4* g = reinterpret_cast<u32x4*>(words + (static_cast<uint64_t>(b) << slice_bits) * 4ull);
-        for (uint32_t x = threadIdx.x; x < vecs; x += kApplyBlock) lds4[x] = g[x];
+        if (load) {
+            for (uint32_t x = threadIdx.x; x < vecs; x += kApplyBlock) lds4[x] = g[x];
+        } else {
+            for (uint32_t x = threadIdx.x; x < vecs; x += kApplyBlock) lds4[x] = u32x4{0u, 0u, 0u, 0u};
+        }
         __syncthreads();
         // threads 2r and 2r+1 take the two 16-byte halves of record r's mask in
         // one instruction: one memory request per mask line
@@ -556,8 +566,8 @@ SortedLayout sorted_layout(uint32_t regions, uint32_t cap, uint64_t num_shards, 
     at += align256(l.slots * 8);
     l.off_masks = at;
     at += align256(l.slots * 32);
-    l.off_counts = at;
-    at += align256(static_cast<uint64_t>(regions) * 4);
+    l.off_counts = at;  // counts[regions], then the overflow flag (and one spare word)
+    at += align256((static_cast<uint64_t>(regions) + 2) * 4);
     l.off_hist = at;
     at += align256(l.hist_len * 4);
     l.off_partials = at;
@@ -578,6 +588,7 @@ RecordSink record_sink(const SortedLayout& l, void* base) {
     s.pairs = reinterpret_cast<uint64_t*>(p + l.off_pairs[0]);
     s.masks = reinterpret_cast<uint64_t*>(p + l.off_masks);
     s.counts = reinterpret_cast<uint32_t*>(p + l.off_counts);
+    s.overflow = s.counts + l.regions;
     s.cap = l.cap;
     s.regions = l.regions;
     return s;
@@ -589,7 +600,8 @@ hipError_t prepare() {
                                hipFuncAttributeMaxDynamicSharedMemorySize, 32 << 12);
 }
 
-hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStream_t stream) {
+hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, bool fresh,
+                        hipStream_t stream) {
     char* p = static_cast<char*>(base);
     uint64_t* pairs[2] = {reinterpret_cast<uint64_t*>(p + l.off_pairs[0]),
                           reinterpret_cast<uint64_t*>(p + l.off_pairs[1])};
@@ -681,7 +693,8 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStre
     (void)bits;
     const uint32_t grid = l.num_slices < kApplyGrid ? l.num_slices : kApplyGrid;
     hipLaunchKernelGGL(apply_kernel, dim3(grid), dim3(kApplyBlock), (32u << l.slice_bits), stream,
-                       pairs[cur], bounds, masks, f.words, l.slice_bits, l.num_slices);
+                       pairs[cur], bounds, masks, f.words, l.slice_bits, l.num_slices, fresh,
+                       reinterpret_cast<const uint32_t*>(p + l.off_counts) + l.regions);
     return hipGetLastError();
 }
 

@@ -1044,7 +1044,10 @@ __global__ __launch_bounds__(kBlock, WPE) void insert_kernel(SequenceInput in, F
         insert_tile<K, S, M, H, FE, Dense, false>(in, filter, num_windows, tile, lane, lds[wave], sink,
                                                   region, emitted);
     }
-    if (sink.pairs != nullptr && lane == 0) sink.counts[region] = emitted;
+    if (sink.pairs != nullptr && lane == 0) {
+        sink.counts[region] = emitted;
+        if (emitted == sink.cap) *sink.overflow = 1u;  // later runs may have used atomics
+    }
 }
 
 // --- batched kernels: persistent waves over the tiles ---------------------------------
@@ -1116,7 +1119,10 @@ __global__ __launch_bounds__(kBlock, Queue ? 2 : WPE) void batched_insert_kernel
                                                      sink, region, emitted);
         });
     }
-    if (sink.pairs != nullptr && lane == 0) sink.counts[region] = emitted;
+    if (sink.pairs != nullptr && lane == 0) {
+        sink.counts[region] = emitted;
+        if (emitted == sink.cap) *sink.overflow = 1u;  // later runs may have used atomics
+    }
 }
 
 // --- host launchers ----------------------------------------------------------

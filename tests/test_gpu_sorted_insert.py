@@ -5,8 +5,9 @@ to the atomic path's on the same input: plain sequences (DNA and protein front
 ends, several k/s/m/h, tiles with more runs than LDS slots), dense packed input, the batched record form (direct
 calls and graph-captured plans), one to three radix passes (slice sizes from
 2 to 4096 shards), records that overflow a wave's reservation (atomics), words
-already set before the insert, inserts on two streams, empty and tiny inputs,
-and the AUTO path choice.
+already set before the insert, inserts on two streams, inserts into all-zero
+words (and each way words change outside an insert), empty and tiny inputs, and
+the AUTO path choice.
 """
 from __future__ import annotations
 
@@ -210,6 +211,72 @@ def test_sorted_two_streams(gpu):
     torch.cuda.synchronize()
     w = oracle_words(k, s, m, h, "dna", [a_seq, b_seq], f.num_shards())
     assert (f.words() == w).all()
+
+
+def test_fresh_filter_tracking(gpu):
+    """An insert into all-zero words (a created or cleared filter) starts each slice from
+    zeros instead of reading it. Words changed outside the library's inserts must still
+    count: written through the words pointer, refilled by a captured plan replayed after
+    a clear, or set by an atomic insert still running on another stream."""
+    import cusbf_amd as cs
+    from cusbf_amd import batched as bt
+    from tests.test_gpu_batched import oracle_words as batched_oracle_words
+
+    torch = _torch()
+    k, s, m, h = 31, 28, 16, 4
+    cfg = make_config(k, s, m, h)
+    bits = 1 << 24
+    a_seq, b_seq = random_dna(400_000, 21), random_dna(300_000, 22)
+    shards = cs.Filter(cfg, bits).num_shards()
+    w_a = oracle_words(k, s, m, h, "dna", [a_seq], shards)
+    w_b = oracle_words(k, s, m, h, "dna", [b_seq], shards)
+
+    # cleared after an insert: the next insert equals one into a new filter
+    f = sorted_filter(cfg, bits, slice_bits=6)
+    f.insert_sequence(a_seq)
+    f.clear()
+    f.insert_sequence(b_seq)
+    assert f.insert_path().last_path == cs.INSERT_SORTED
+    assert (f.words() == w_b).all()
+
+    # words written through the device pointer before the first insert are kept
+    g = sorted_filter(cfg, bits, slice_bits=6)
+    t = g.words_tensor()
+    rng = np.random.default_rng(5)
+    idx = torch.from_numpy(rng.choice(t.numel(), 4096, replace=False)).cuda()
+    t[idx] = torch.from_numpy(rng.integers(1, 1 << 62, 4096, dtype=np.int64)).cuda()
+    torch.cuda.synchronize()
+    planted = g.words()
+    assert planted.any()
+    g.insert_sequence(b_seq)
+    assert g.insert_path().last_path == cs.INSERT_SORTED
+    assert (g.words() == (planted | w_b)).all()
+
+    # a captured insert plan replayed after a clear, then an eager sorted insert
+    data, off, d_data, d_off = _batched_inputs("dna", 500, 400, seed=23)
+    w_plan = batched_oracle_words(make_oracle(k, s, m, h, "dna"), data, off, "dna", shards)
+    p = sorted_filter(cfg, bits, slice_bits=6)
+    with bt.BatchPlan(p, bt.INSERT, d_data, d_off) as plan:
+        p.clear()
+        plan.launch()
+        plan.status()
+        p.insert_sequence(b_seq)
+        assert p.insert_path().last_path == cs.INSERT_SORTED
+        assert (p.words() == (w_plan | w_b)).all()
+
+    # an atomic insert on one stream, then a sorted insert on another without a wait
+    q = cs.Filter(cfg, bits)
+    q.set_insert_path(cs.INSERT_ATOMIC)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    d_a = torch.frombuffer(bytearray(a_seq), dtype=torch.uint8).cuda()
+    d_b = torch.frombuffer(bytearray(b_seq), dtype=torch.uint8).cuda()
+    torch.cuda.synchronize()
+    q.insert_sequence_async(d_a, stream=s1)
+    q.set_insert_path(cs.INSERT_SORTED, 6, 0)
+    q.insert_sequence_async(d_b, stream=s2)
+    torch.cuda.synchronize()
+    assert q.insert_path().last_path == cs.INSERT_SORTED
+    assert (q.words() == (w_a | w_b)).all()
 
 
 def test_sorted_tiny_and_empty(gpu):
