@@ -1093,12 +1093,18 @@ int cusbf_contains_dense_packed(const cusbf_filter* f_const, const uint64_t* h_w
     const uint64_t n = cusbf_dense_packed_kmer_count(&f->cfg, num_symbols);
     if (n == 0) return CUSBF_OK;
     if (h_output_len < n) return fail(CUSBF_ERR_INVALID_ARGUMENT, "host output buffer is too small");
+    if (num_words < cusbf_dense_packed_word_count(&f->cfg, num_symbols)) {
+        return fail(CUSBF_ERR_INVALID_ARGUMENT, "dense packed span is too small for num_symbols");
+    }
     int rc = ensure_result(f, n);
+    if (rc == CUSBF_OK) rc = stage_words(f, h_words, num_words, as_stream(stream));
     if (rc != CUSBF_OK) return rc;
-    rc = cusbf_contains_dense_packed_host_to_device(f, h_words, num_words, num_symbols, f->d_result, n,
-                                                    stream);
+    // words in, answers out and the kernel in one stream order, one wait
+    rc = cusbf_contains_dense_packed_async(f, reinterpret_cast<const uint64_t*>(f->d_stage), num_words,
+                                           num_symbols, f->d_result, n, stream);
     if (rc != CUSBF_OK) return rc;
-    CUSBF_HIP_TRY(hipMemcpy(h_output, f->d_result, n, hipMemcpyDeviceToHost));
+    CUSBF_HIP_TRY(hipMemcpyAsync(h_output, f->d_result, n, hipMemcpyDeviceToHost, as_stream(stream)));
+    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
     return CUSBF_OK;
 }
 

@@ -40,8 +40,9 @@ namespace {
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kRankBlock = 256;                       // threads of histogram / scatter blocks
-constexpr int kRankItems = 16;                        // pairs per thread
+constexpr int kRankItems = 32;                        // pairs per thread (at most 32: bit masks)
 constexpr int kBlockItems = kRankBlock * kRankItems;  // pairs per block
+static_assert(kRankItems <= 32, "validity and rank bit masks are 32-bit");
 constexpr int kBins = 256;                            // 8-bit digits
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(kRankBlock) void hist_kernel(PassInput in, uint32_t
     __shared__ uint32_t h[kBins];
     h[threadIdx.x] = 0;
     __syncthreads();
-    // all loads first (16 in flight per lane), then the LDS counts
+    // all loads first (kRankItems in flight per lane), then the LDS counts
     uint64_t p[kRankItems];
     uint32_t valid = 0;
     SlotCursor<Near> c(in, wave_first_slot());
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(kRankBlock) void scatter_kernel(PassInput in, uint3
     uint32_t valid = 0;
     SlotCursor<Near> c(in, wave_first_slot());
 #pragma unroll
-    for (int j = 0; j < kRankItems; ++j) {  // all loads first (16 in flight per lane)
+    for (int j = 0; j < kRankItems; ++j) {  // all loads first (kRankItems in flight per lane)
         const bool v = c.valid(in);
         item[j] = v ? in.pairs[c.i] : 0ull;
         valid |= (v ? 1u : 0u) << j;
@@ -304,17 +305,21 @@ __global__ __launch_bounds__(kRankBlock) void scatter_kernel(PassInput in, uint3
 // first-pass digit group (block_range).
 __device__ __forceinline__ void block_range(const uint32_t* first_block, const uint32_t* group_start,
                                             uint32_t b, uint32_t& lo, uint32_t& hi) {
-    // first_block[g]: blocks before group g (256 + 1 entries); find g with
-    // first_block[g] <= b < first_block[g + 1]
-    uint32_t g = 0;
-#pragma unroll
-    for (uint32_t step = kBins / 2; step > 0; step >>= 1) {
-        if (first_block[g + step] <= b) g += step;
-    }
+    // first_block[g]: blocks before group g (256 + 1 non-decreasing entries,
+    // first_block[0] = 0); g with first_block[g] <= b < first_block[g + 1] is
+    // the number of entries 1..256 at most b: each lane tests four of them
+    // (independent loads) and ballots count the wave's.
     if (b >= first_block[kBins]) {  // past the last group's chunks: no items
         lo = hi = 0;
         return;
     }
+    const uint32_t lane = threadIdx.x & 63u;
+    bool le[kBins / 64];
+#pragma unroll
+    for (int r = 0; r < kBins / 64; ++r) le[r] = first_block[1u + lane + 64u * r] <= b;
+    uint32_t g = 0;
+#pragma unroll
+    for (int r = 0; r < kBins / 64; ++r) g += static_cast<uint32_t>(__popcll(__ballot(le[r])));
     lo = group_start[g] + (b - first_block[g]) * static_cast<uint32_t>(kBlockItems);
     const uint32_t end = group_start[g + 1];
     hi = lo + static_cast<uint32_t>(kBlockItems) < end ? lo + static_cast<uint32_t>(kBlockItems) : end;
@@ -356,11 +361,11 @@ __global__ __launch_bounds__(kRankBlock) void hist_any_kernel(PassInput in, cons
                                                               uint32_t digit_mask, uint32_t nblocks,
                                                               uint32_t* hist) {
     __shared__ uint32_t h[kBins];
-    h[threadIdx.x] = 0;
-    __syncthreads();
     uint64_t p[kRankItems];
     uint32_t valid;
-    load_block_items<Near, Group>(in, first_block, group_start, p, valid);
+    load_block_items<Near, Group>(in, first_block, group_start, p, valid);  // in flight over the set-up
+    h[threadIdx.x] = 0;
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kRankItems; ++j) {
         if ((valid >> j) & 1u) atomicAdd(&h[digit_of(p[j], shift, digit_mask)], 1u);
@@ -380,12 +385,12 @@ __global__ __launch_bounds__(kRankBlock) void scatter_any_kernel(PassInput in, c
     __shared__ uint32_t tmp[kRankBlock / 64];
     __shared__ uint64_t sorted[kBlockItems];
     const int t = static_cast<int>(threadIdx.x);
+    uint64_t item[kRankItems];
+    uint32_t valid;
+    load_block_items<Near, Group>(in, first_block, group_start, item, valid);  // in flight over the set-up
     count[t] = 0;
     dest[t] = offsets[static_cast<uint32_t>(t) * nblocks + blockIdx.x];
     __syncthreads();
-    uint64_t item[kRankItems];
-    uint32_t valid;
-    load_block_items<Near, Group>(in, first_block, group_start, item, valid);
     uint32_t rank[kRankItems];
 #pragma unroll
     for (int j = 0; j < kRankItems; ++j) {
