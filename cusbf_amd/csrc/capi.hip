@@ -141,10 +141,11 @@ struct cusbf_filter {
     void* d_sorted = nullptr;
     uint64_t sorted_bytes = 0;
     hipEvent_t sorted_done = nullptr;  // recorded after the last sorted insert's kernels
-    // The words are all zero (created or cleared, no insert since): a sorted
-    // insert need not read them. Never trusted again once the words pointer
-    // has been handed out or an insert has been captured into a graph.
-    bool words_zero = false;
+    // Device flag, 1 while every word is known to be zero (a clear sets it,
+    // every insert kernel resets it): a sorted insert then need not read the
+    // words. Once the words pointer has been handed out the library cannot see
+    // every write, so clears stop setting it.
+    uint64_t* d_zero = nullptr;
     mutable bool words_shared = false;
     hipEvent_t insert_done = nullptr;  // recorded after the last insert (outside captures)
     bool insert_recorded = false;
@@ -202,7 +203,7 @@ void release_staging(cusbf_filter* f) {
     f->stage_cap = f->result_cap = 0;
 }
 
-sbf::FilterView view(const cusbf_filter* f) { return sbf::FilterView{f->d_words, f->num_shards}; }
+sbf::FilterView view(const cusbf_filter* f) { return sbf::FilterView{f->d_words, f->num_shards, f->d_zero}; }
 
 // Lane-strip kernels read byte input, and dense packed input for DNA.
 bool strip_reads(const cusbf_filter* f, const sbf::SequenceInput& in) {
@@ -302,25 +303,16 @@ int end_sorted(cusbf_filter* f, hipStream_t stream) {
     return CUSBF_OK;
 }
 
-// Whether this insert goes into all-zero words (see cusbf_filter::words_zero);
-// every insert consumes the state.
-bool take_fresh(cusbf_filter* f, hipStream_t stream) {
-    if (capturing(stream)) f->words_shared = true;
-    const bool fresh = f->words_zero && !f->words_shared;
-    f->words_zero = false;
-    return fresh;
-}
-
-// Zeroes the region counts and the overflow flag, then runs `emit(sink)` (the
-// insert kernel writing records) and the partition and apply kernels, in
-// stream order.
+// Zeroes the region counts and the overflow flag (taking the filter's zero
+// flag), then runs `emit(sink)` (the insert kernel writing records) and the
+// partition and apply kernels, in stream order.
 template <class Emit>
 hipError_t enqueue_sorted(const cusbf_filter* f, const sbf::SortedLayout& l, void* base,
-                          hipStream_t stream, bool fresh, Emit&& emit) {
+                          hipStream_t stream, Emit&& emit) {
     const sbf::RecordSink sink = sbf::sorted::record_sink(l, base);
-    hipError_t e = sbf::fill_u64(reinterpret_cast<uint64_t*>(sink.counts), 0, (l.regions + 2ull) / 2, stream);
+    hipError_t e = sbf::sorted::sorted_begin(l, base, view(f), stream);
     if (e == hipSuccess) e = emit(sink);
-    if (e == hipSuccess) e = sbf::sorted::sorted_apply(l, base, view(f), fresh, stream);
+    if (e == hipSuccess) e = sbf::sorted::sorted_apply(l, base, view(f), stream);
     return e;
 }
 
@@ -364,7 +356,6 @@ int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t str
 
 int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
     f->last_path = CUSBF_INSERT_ATOMIC;
-    const bool fresh = take_fresh(f, stream);
     if (!strip_reads(f, in)) {
         const hipError_t e = sbf::generic_insert(f->kc, in, view(f), stream);
         if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
@@ -383,7 +374,7 @@ int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStre
     }
     rc = begin_sorted(f, l.bytes, stream);
     if (rc != CUSBF_OK) return rc;
-    const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, fresh, [&](const sbf::RecordSink& sink) {
+    const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, [&](const sbf::RecordSink& sink) {
         return f->strip->insert(in, view(f), &sink, stream);
     });
     if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
@@ -582,14 +573,12 @@ const cusbf_batch_tuning& tuning_for(const cusbf_filter* f, uint32_t op, uint64_
 
 // Reset, prepass and kernel of one batched call, in stream order; allocates
 // nothing, so it can be captured into a graph. `sorted` (insert only, or
-// null): the layout of the sorted insert's workspace at `sorted_base`;
-// `fresh`: the filter words are all zero.
+// null): the layout of the sorted insert's workspace at `sorted_base`.
 int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
                     uint64_t num_symbols, const uint64_t* d_offsets, uint64_t num_records,
                     uint32_t* d_state, uint64_t* d_starts, const cusbf_batch_tuning& t,
                     uint8_t* d_output, hipStream_t stream,
-                    const sbf::SortedLayout* sorted = nullptr, void* sorted_base = nullptr,
-                    bool fresh = false) {
+                    const sbf::SortedLayout* sorted = nullptr, void* sorted_base = nullptr) {
     sbf::SequenceInput in = byte_input(f, d_symbols, num_symbols * f->d.symbol_width);
     in.offsets = d_offsets;
     in.num_records = num_records;
@@ -601,7 +590,7 @@ int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
         const int variant = t.waves_per_simd == 4 && f->strip->batched_insert[1] ? 1 : 0;
         const sbf::BatchLaunch shape{t.blocks_per_cu, f->compute_units, t.claim};
         if (op == CUSBF_BATCH_INSERT && sorted) {
-            e = enqueue_sorted(f, *sorted, sorted_base, stream, fresh, [&](const sbf::RecordSink& sink) {
+            e = enqueue_sorted(f, *sorted, sorted_base, stream, [&](const sbf::RecordSink& sink) {
                 return f->strip->batched_insert[variant](in, view(f), shape, &sink, stream);
             });
         } else {
@@ -784,6 +773,7 @@ int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_
     if (e == hipSuccess) e = hipMalloc(&f->d_words, f->num_shards * 32);
     if (e == hipSuccess) e = hipMalloc(&f->d_scalar, 4 * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&f->d_batch_state, 64);
+    if (e == hipSuccess) e = hipMalloc(&f->d_zero, 256);
     if (e == hipSuccess) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, f->device) == hipSuccess &&
@@ -811,6 +801,7 @@ void cusbf_destroy(cusbf_filter* f) {
     if (f->d_words) (void)hipFree(f->d_words);
     if (f->d_scalar) (void)hipFree(f->d_scalar);
     if (f->d_batch_state) (void)hipFree(f->d_batch_state);
+    if (f->d_zero) (void)hipFree(f->d_zero);
     if (f->d_starts) (void)hipFree(f->d_starts);
     if (f->d_sorted) {
         (void)hipEventSynchronize(f->sorted_done);
@@ -826,8 +817,8 @@ void cusbf_destroy(cusbf_filter* f) {
 int cusbf_clear(cusbf_filter* f, void* stream) {
     if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
     CUSBF_HIP_TRY(sbf::fill_u64(f->d_words, 0, f->num_shards * 4, as_stream(stream)));
+    CUSBF_HIP_TRY(sbf::fill_u64(f->d_zero, f->words_shared ? 0 : 1, 1, as_stream(stream)));
     CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
-    f->words_zero = true;
     return CUSBF_OK;
 }
 
@@ -847,7 +838,11 @@ uint64_t cusbf_num_shards(const cusbf_filter* f) { return f ? f->num_shards : 0;
 
 int cusbf_words_device_ptr(const cusbf_filter* f, uint64_t** out) {
     if (!f || !out) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
-    f->words_shared = true;  // the caller may write the words
+    if (!f->words_shared) {  // the caller may write the words: never assume them zero again
+        f->words_shared = true;
+        CUSBF_HIP_TRY(sbf::fill_u64(f->d_zero, 0, 1, nullptr));
+        CUSBF_HIP_TRY(hipStreamSynchronize(nullptr));
+    }
     *out = f->d_words;
     return CUSBF_OK;
 }
@@ -1318,12 +1313,11 @@ int cusbf_insert_batched_async(cusbf_filter* f, const char* d_symbols, uint64_t 
     }
     const cusbf_batch_tuning& t =
         sorted ? ts : tuning_for(f, CUSBF_BATCH_INSERT, num_symbols, num_records);
-    const bool fresh = take_fresh(f, as_stream(stream));
     rc = order_insert(f, as_stream(stream));
     if (rc != CUSBF_OK) return rc;
     rc = enqueue_batched(f, CUSBF_BATCH_INSERT, d_symbols, num_symbols, d_offsets, num_records,
                          f->d_batch_state, f->d_starts, t, nullptr, as_stream(stream),
-                         sorted ? &l : nullptr, f->d_sorted, fresh);
+                         sorted ? &l : nullptr, f->d_sorted);
     if (rc == CUSBF_OK) rc = mark_insert(f, as_stream(stream));
     if (rc == CUSBF_OK && sorted) {
         f->last_path = CUSBF_INSERT_SORTED;
@@ -1396,10 +1390,6 @@ int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
     hipStream_t capture = nullptr;
     sbf::SortedLayout l;
     bool sorted = false;
-    if (op == CUSBF_BATCH_INSERT) {  // the plan's graph can run at any later time
-        f->words_shared = true;
-        f->words_zero = false;
-    }
     if (op == CUSBF_BATCH_INSERT && batched_strip(f)) {
         rc = choose_batched_sorted(f, num_symbols, ts, nullptr, &l, &sorted);
         if (rc != CUSBF_OK) {

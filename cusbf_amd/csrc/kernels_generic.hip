@@ -130,6 +130,7 @@ __device__ __forceinline__ bool batch_rejected(const SequenceInput& in) {
 __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, SequenceInput in,
                                                         FilterView f, uint64_t num_windows) {
     __shared__ uint8_t tile[kBlock + kMaxK];
+    mark_words_written(f);
     if (batch_rejected(in)) return;
     for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
          first += static_cast<uint64_t>(gridDim.x) * kBlock) {

@@ -34,7 +34,15 @@ struct SequenceInput {
 struct FilterView {
     uint64_t* words = nullptr;  // num_shards * 4
     uint64_t num_shards = 0;
+    // 1 while every word is known to be zero: set by a clear, reset to 0 by
+    // thread 0 of block 0 of every insert kernel at its start (null: no flag).
+    uint64_t* zero = nullptr;
 };
+
+// Every insert kernel calls this first: the words may change from here on.
+__device__ __forceinline__ void mark_words_written(const FilterView& f) {
+    if (f.zero != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *f.zero = 0;
+}
 
 // Launch shape of the persistent batched kernels.
 struct BatchLaunch {
@@ -121,11 +129,13 @@ SortedLayout sorted_layout(uint32_t regions, uint32_t cap, uint64_t num_shards, 
 RecordSink record_sink(const SortedLayout& l, void* base);
 // Allows the apply kernel its LDS slice; call before launching (outside any capture).
 hipError_t prepare();
+// First step of a sorted insert, in stream order before the insert kernels:
+// zeroes the region counts and the overflow flag, and moves f.zero into the
+// workspace (f.zero becomes 0), where the apply reads it.
+hipError_t sorted_begin(const SortedLayout& l, void* base, FilterView f, hipStream_t stream);
 // Partition the records the insert kernels wrote and OR them into the filter,
-// in stream order after those kernels. `fresh`: every filter word was zero
-// before those kernels ran (the counts and overflow flag zeroed before them).
-hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, bool fresh,
-                        hipStream_t stream);
+// in stream order after those kernels.
+hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStream_t stream);
 }  // namespace sorted
 
 // Generic kernels: any alphabet / configuration, one window per thread; in the

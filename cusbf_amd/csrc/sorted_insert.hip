@@ -22,9 +22,9 @@
 //      and uses wave ballots;
 //   3. apply (here): one block per non-empty slice loads the slice's words
 //      into LDS, ORs the masks of its records into them with LDS atomics and
-//      stores the slice back. Into a filter whose words are all zero (the host
-//      knows: created or cleared, no insert since) it starts each slice from
-//      zeros instead of reading it, unless the overflow flag is set.
+//      stores the slice back. Into a filter whose words are all zero (its
+//      zero flag: set by a clear, reset by every insert kernel) it starts each
+//      slice from zeros instead of reading it, unless the overflow flag is set.
 // OR is commutative and idempotent, so the words equal the atomic insert's;
 // the atomics of step 1 land before step 3 reads the slice (stream order).
 #include <hip/hip_runtime.h>
@@ -478,21 +478,33 @@ __global__ __launch_bounds__(256) void bounds_kernel(const uint64_t* pairs, cons
     for (uint32_t b = from; b <= cur; ++b) bounds[b] = static_cast<uint32_t>(i);
 }
 
+// Zeroes n words of counts (the region counts and the overflow flag) and
+// moves the filter's zero flag into counts[n] (the flag becomes 0).
+__global__ __launch_bounds__(256) void begin_kernel(uint32_t* counts, uint32_t n, uint64_t* zero) {
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) counts[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const bool was_zero = zero != nullptr && *zero != 0;
+        counts[n] = was_zero ? 1u : 0u;
+        if (zero != nullptr) *zero = 0;
+    }
+}
+
 // One block per non-empty slice at a time (blocks stride over the slices):
 // slice words -> LDS, OR of the slice's records, LDS -> slice words.
-// `fresh`: the filter words were all zero before the emit kernel; then a slice
-// is read only when the overflow flag says that atomics reached the filter.
+// flags[0]: the overflow flag; flags[1]: the filter words were all zero
+// before this insert. A slice is read unless they were and no region
+// overflowed into atomics.
 __global__ __launch_bounds__(kApplyBlock) void apply_kernel(const uint64_t* pairs,
                                                             const uint32_t* bounds,
                                                             const u32x4* masks, uint64_t* words,
                                                             uint32_t slice_bits,
-                                                            uint32_t num_slices, bool fresh,
-                                                            const uint32_t* overflow) {
+                                                            uint32_t num_slices,
+                                                            const uint32_t* flags) {
     extern __shared__ uint64_t slice[];  // 4 << slice_bits words
     u32x4* lds4 = reinterpret_cast<u32x4*>(slice);
     const uint32_t vecs = 2u << slice_bits;  // 16-byte vectors of one slice
     const uint32_t local_mask = (1u << slice_bits) - 1u;
-    const bool load = !fresh || *overflow != 0u;  // uniform
+    const bool load = flags[1] == 0u || flags[0] != 0u;  // uniform
     constexpr int kUnroll = 4;
     for (uint32_t b = blockIdx.x; b < num_slices; b += gridDim.x) {
         const uint32_t lo = bounds[b];
@@ -605,8 +617,15 @@ hipError_t prepare() {
                                hipFuncAttributeMaxDynamicSharedMemorySize, 32 << 12);
 }
 
-hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, bool fresh,
-                        hipStream_t stream) {
+hipError_t sorted_begin(const SortedLayout& l, void* base, FilterView f, hipStream_t stream) {
+    uint32_t* counts = reinterpret_cast<uint32_t*>(static_cast<char*>(base) + l.off_counts);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(begin_kernel, dim3(div_up(l.regions + 1ull, 256 * 8)), dim3(256), 0, stream,
+                       counts, l.regions + 1, f.zero);
+    return hipGetLastError();
+}
+
+hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStream_t stream) {
     char* p = static_cast<char*>(base);
     uint64_t* pairs[2] = {reinterpret_cast<uint64_t*>(p + l.off_pairs[0]),
                           reinterpret_cast<uint64_t*>(p + l.off_pairs[1])};
@@ -698,7 +717,7 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, bool fr
     (void)bits;
     const uint32_t grid = l.num_slices < kApplyGrid ? l.num_slices : kApplyGrid;
     hipLaunchKernelGGL(apply_kernel, dim3(grid), dim3(kApplyBlock), (32u << l.slice_bits), stream,
-                       pairs[cur], bounds, masks, f.words, l.slice_bits, l.num_slices, fresh,
+                       pairs[cur], bounds, masks, f.words, l.slice_bits, l.num_slices,
                        reinterpret_cast<const uint32_t*>(p + l.off_counts) + l.regions);
     return hipGetLastError();
 }

@@ -1035,6 +1035,7 @@ __global__ __launch_bounds__(kBlock, WPE) void insert_kernel(SequenceInput in, F
                                                              uint64_t num_windows,
                                                              uint64_t num_tiles, RecordSink sink) {
     __shared__ WaveLds<FE, false> lds[kWavesPerBlock];
+    mark_words_written(filter);
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = static_cast<int>(threadIdx.x) / kWave;
     const uint32_t region = blockIdx.x * kWavesPerBlock + static_cast<uint32_t>(wave);
@@ -1108,6 +1109,7 @@ __global__ __launch_bounds__(kBlock, Queue ? 2 : WPE) void batched_insert_kernel
                                                                      uint32_t num_tiles,
                                                                      uint32_t claim, RecordSink sink) {
     __shared__ WaveLds<FE, true> lds[kWavesPerBlock];
+    mark_words_written(filter);
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = static_cast<int>(threadIdx.x) / kWave;
     const uint32_t region = blockIdx.x * kWavesPerBlock + static_cast<uint32_t>(wave);

@@ -295,30 +295,24 @@ class Filter:
     def insert_dense_packed(self, words: Sequence[int] | np.ndarray, num_symbols: int,
                             stream=None) -> int:
         """filter.cuh:226-240 (host words, synchronises)."""
-        import torch
-
         w = np.ascontiguousarray(words, dtype=np.uint64)
-        if self.dense_packed_kmer_count(num_symbols) == 0:
-            return 0
-        d = torch.from_numpy(w.view(np.int64)).cuda()
-        n = self.insert_dense_packed_async(d, num_symbols, stream=stream)
-        torch.cuda.synchronize()
-        return n
+        inserted = ctypes.c_uint64(0)
+        check(self._lib, self._lib.cusbf_insert_dense_packed(
+            self._h, w.ctypes.data, w.size, num_symbols, stream_handle(stream), ctypes.byref(inserted)))
+        return int(inserted.value)
 
     def contains_dense_packed(self, words: Sequence[int] | np.ndarray, num_symbols: int,
                               stream=None) -> np.ndarray:
         """filter.cuh:289-312 (host words in, host hits out)."""
-        import torch
-
         n = self.dense_packed_kmer_count(num_symbols)
+        out = np.zeros(n, dtype=np.uint8)
         if n == 0:
-            return np.zeros(0, dtype=np.uint8)
+            return out
         w = np.ascontiguousarray(words, dtype=np.uint64)
-        d = torch.from_numpy(w.view(np.int64)).cuda()
-        out = torch.empty(n, dtype=torch.uint8, device="cuda")
-        self.contains_dense_packed_async(d, num_symbols, out, stream=stream)
-        torch.cuda.synchronize()
-        return out.cpu().numpy()
+        check(self._lib, self._lib.cusbf_contains_dense_packed(
+            self._h, w.ctypes.data, w.size, num_symbols, out.ctypes.data, out.size,
+            stream_handle(stream)))
+        return out
 
     # record batches -------------------------------------------------------------
     def insert_record_batch(self, sequence, ranges: Iterable, stream=None) -> InsertReport:

@@ -279,6 +279,36 @@ def test_fresh_filter_tracking(gpu):
     assert (q.words() == (w_a | w_b)).all()
 
 
+def test_captured_insert_replayed_after_clear(gpu):
+    """An insert captured into the caller's graph (inside a capture the atomic path runs)
+    resets the filter's zero flag on every replay, so a sorted insert after a replay reads
+    the slices; checked on every replay, with eager work between replays."""
+    import cusbf_amd as cs
+
+    torch = _torch()
+    k, s, m, h = 31, 28, 16, 4
+    cfg = make_config(k, s, m, h)
+    a_seq, b_seq = random_dna(300_000, 31), random_dna(200_000, 32)
+    f = sorted_filter(cfg, 1 << 24, slice_bits=6)
+    w_a = oracle_words(k, s, m, h, "dna", [a_seq], f.num_shards())
+    w_b = oracle_words(k, s, m, h, "dna", [b_seq], f.num_shards())
+    d_a = torch.frombuffer(bytearray(a_seq), dtype=torch.uint8).cuda()
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        f.insert_sequence_async(d_a, stream=side)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        f.clear()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert (f.words() == w_a).all(), rep
+        f.insert_sequence(b_seq)
+        assert f.insert_path().last_path == cs.INSERT_SORTED
+        assert (f.words() == (w_a | w_b)).all(), rep
+
+
 def test_sorted_tiny_and_empty(gpu):
     import cusbf_amd as cs
 
