@@ -83,7 +83,9 @@ struct Dna2 {
 // Protein (or any 1-byte alphabet of at most 32 symbols): 16 code bytes per lane.
 struct Protein5 {
     static constexpr int kBits = 5;
-    static constexpr int kSlots = 256;  // ~330 runs per tile with a 5-m-mer span
+    // ~330 runs per tile with a 5-m-mer span: 384 slots take most tiles in one
+    // chunk (one pass of the front end); the LDS then holds two blocks per CU
+    static constexpr int kSlots = 384;
     static constexpr int kMaxK = 12;
     static constexpr uint32_t kAlphabet = kProtein;
     struct Lds {

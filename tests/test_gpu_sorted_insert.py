@@ -65,8 +65,8 @@ CONFIGS = [
     (21, 19, 11, 4, "dna"),
     (5, 4, 3, 4, "dna"),      # ~500 runs per tile: two slot chunks
     (3, 2, 2, 4, "dna"),      # ~660 runs per tile: three slot chunks
-    (10, 8, 6, 4, "protein"),  # ~330 runs per tile
-    (5, 4, 3, 4, "protein"),
+    (10, 8, 6, 4, "protein"),  # ~330 runs per tile: mostly one chunk of 384 slots
+    (5, 4, 3, 4, "protein"),   # ~500 runs per tile: two chunks of 384 slots
 ]
 
 
