@@ -129,8 +129,8 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
 // the block takes the contiguous quarter w of it, 64 slots per round.
 constexpr int kWaveItems = kBlockItems / (kRankBlock / 64);
 
-__device__ __forceinline__ uint32_t wave_first_slot() {
-    return blockIdx.x * kBlockItems + (threadIdx.x >> 6) * kWaveItems + (threadIdx.x & 63);
+__device__ __forceinline__ uint32_t wave_first_slot(uint32_t b) {
+    return b * kBlockItems + (threadIdx.x >> 6) * kWaveItems + (threadIdx.x & 63);
 }
 
 // hist[d * nblocks + b] = records of block b with digit d.
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kRankBlock) void hist_kernel(PassInput in, uint32_t
     // all loads first (kRankItems in flight per lane), then the LDS counts
     uint64_t p[kRankItems];
     uint32_t valid = 0;
-    SlotCursor<Near> c(in, wave_first_slot());
+    SlotCursor<Near> c(in, wave_first_slot(blockIdx.x));
 #pragma unroll
     for (int j = 0; j < kRankItems; ++j) {
         const bool v = c.valid(in);
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kRankBlock) void scatter_kernel(PassInput in, uint3
     uint64_t item[kRankItems];
     uint32_t rank[kRankItems];
     uint32_t valid = 0;
-    SlotCursor<Near> c(in, wave_first_slot());
+    SlotCursor<Near> c(in, wave_first_slot(blockIdx.x));
 #pragma unroll
     for (int j = 0; j < kRankItems; ++j) {  // all loads first (kRankItems in flight per lane)
         const bool v = c.valid(in);
@@ -325,14 +325,15 @@ __device__ __forceinline__ void block_range(const uint32_t* first_block, const u
     hi = lo + static_cast<uint32_t>(kBlockItems) < end ? lo + static_cast<uint32_t>(kBlockItems) : end;
 }
 
+// Items of partition block b (every lane of the wave calls it together).
 template <bool Near, bool Group>
 __device__ __forceinline__ void load_block_items(const PassInput& in, const uint32_t* first_block,
-                                                 const uint32_t* group_start, uint64_t (&item)[kRankItems],
-                                                 uint32_t& valid) {
+                                                 const uint32_t* group_start, uint32_t b,
+                                                 uint64_t (&item)[kRankItems], uint32_t& valid) {
     valid = 0;
     if constexpr (Group) {
         uint32_t lo, hi;
-        block_range(first_block, group_start, blockIdx.x, lo, hi);
+        block_range(first_block, group_start, b, lo, hi);
         const uint32_t first = lo + (threadIdx.x >> 6) * kWaveItems + (threadIdx.x & 63);
 #pragma unroll
         for (int j = 0; j < kRankItems; ++j) {
@@ -342,7 +343,7 @@ __device__ __forceinline__ void load_block_items(const PassInput& in, const uint
             valid |= (v ? 1u : 0u) << j;
         }
     } else {
-        SlotCursor<Near> c(in, wave_first_slot());
+        SlotCursor<Near> c(in, wave_first_slot(b));
 #pragma unroll
         for (int j = 0; j < kRankItems; ++j) {
             const bool v = c.valid(in);
@@ -353,25 +354,44 @@ __device__ __forceinline__ void load_block_items(const PassInput& in, const uint
     }
 }
 
-// hist[d * nblocks + b] = items of block b with digit d (every block writes its
-// 256 counts, zeros when it has no items).
+// hist[d * nblocks + b] = items of partition block b with digit d (all 256
+// counts of every block b < nblocks, zeros when it has no items). One
+// histogram block counts kHistBlocks consecutive partition blocks, so one
+// digit's counts leave as kHistBlocks consecutive words (a digit-major layout
+// written one word per block would cost a write request per count).
+constexpr int kHistBlocks = 8;
+
 template <bool Near, bool Group>
 __global__ __launch_bounds__(kRankBlock) void hist_any_kernel(PassInput in, const uint32_t* first_block,
                                                               const uint32_t* group_start, uint32_t shift,
                                                               uint32_t digit_mask, uint32_t nblocks,
                                                               uint32_t* hist) {
-    __shared__ uint32_t h[kBins];
-    uint64_t p[kRankItems];
-    uint32_t valid;
-    load_block_items<Near, Group>(in, first_block, group_start, p, valid);  // in flight over the set-up
-    h[threadIdx.x] = 0;
-    __syncthreads();
+    __shared__ uint32_t h[kHistBlocks][kBins];
 #pragma unroll
-    for (int j = 0; j < kRankItems; ++j) {
-        if ((valid >> j) & 1u) atomicAdd(&h[digit_of(p[j], shift, digit_mask)], 1u);
+    for (int s = 0; s < kHistBlocks; ++s) h[s][threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t b0 = blockIdx.x * kHistBlocks;
+    for (int s = 0; s < kHistBlocks; ++s) {
+        const uint32_t b = b0 + static_cast<uint32_t>(s);
+        if (b >= nblocks) break;  // uniform
+        uint64_t p[kRankItems];
+        uint32_t valid;
+        load_block_items<Near, Group>(in, first_block, group_start, b, p, valid);
+#pragma unroll
+        for (int j = 0; j < kRankItems; ++j) {
+            if ((valid >> j) & 1u) atomicAdd(&h[s][digit_of(p[j], shift, digit_mask)], 1u);
+        }
     }
     __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    // round r, thread t: word w = r * 256 + t holds digit w / kHistBlocks of
+    // block b0 + w % kHistBlocks
+#pragma unroll
+    for (int r = 0; r < kHistBlocks; ++r) {
+        const uint32_t w = static_cast<uint32_t>(r) * kRankBlock + threadIdx.x;
+        const uint32_t d = w / kHistBlocks;
+        const uint32_t s = w % kHistBlocks;
+        if (b0 + s < nblocks) hist[d * nblocks + b0 + s] = h[s][d];
+    }
 }
 
 template <bool Near, bool Group>
@@ -387,7 +407,7 @@ __global__ __launch_bounds__(kRankBlock) void scatter_any_kernel(PassInput in, c
     const int t = static_cast<int>(threadIdx.x);
     uint64_t item[kRankItems];
     uint32_t valid;
-    load_block_items<Near, Group>(in, first_block, group_start, item, valid);  // in flight over the set-up
+    load_block_items<Near, Group>(in, first_block, group_start, blockIdx.x, item, valid);  // in flight over the set-up
     count[t] = 0;
     dest[t] = offsets[static_cast<uint32_t>(t) * nblocks + blockIdx.x];
     __syncthreads();
@@ -663,10 +683,10 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStre
                  reinterpret_cast<const uint32_t*>(p + l.off_counts)};
     const bool near = in.cap >= static_cast<uint32_t>(kWaveItems) || in.cap == in.slots;
     if (near) {
-        hipLaunchKernelGGL((hist_any_kernel<true, false>), dim3(nblocks0), dim3(kRankBlock), 0, stream, in,
+        hipLaunchKernelGGL((hist_any_kernel<true, false>), dim3(div_up(nblocks0, kHistBlocks)), dim3(kRankBlock), 0, stream, in,
                            nullptr, nullptr, shift, mask, nblocks0, hist);
     } else {
-        hipLaunchKernelGGL((hist_any_kernel<false, false>), dim3(nblocks0), dim3(kRankBlock), 0, stream, in,
+        hipLaunchKernelGGL((hist_any_kernel<false, false>), dim3(div_up(nblocks0, kHistBlocks)), dim3(kRankBlock), 0, stream, in,
                            nullptr, nullptr, shift, mask, nblocks0, hist);
     }
     uint32_t* total = totals;
@@ -689,7 +709,7 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStre
                            group_start, first_block);
         bits = digit(1, shift, mask);
         in = PassInput{pairs[1], static_cast<uint32_t>(l.slots), static_cast<uint32_t>(l.slots), total};
-        hipLaunchKernelGGL((hist_any_kernel<true, true>), dim3(nblocks1), dim3(kRankBlock), 0, stream, in,
+        hipLaunchKernelGGL((hist_any_kernel<true, true>), dim3(div_up(nblocks1, kHistBlocks)), dim3(kRankBlock), 0, stream, in,
                            first_block, group_start, shift, mask, nblocks1, hist);
         scan(kBins * nblocks1, totals + 1);
         hipLaunchKernelGGL((scatter_any_kernel<true, true>), dim3(nblocks1), dim3(kRankBlock), 0, stream, in,
