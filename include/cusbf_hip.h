@@ -116,7 +116,10 @@ int cusbf_load_factor(const cusbf_filter* f, float* out);
 uint64_t cusbf_filter_bits(const cusbf_filter* f);
 uint64_t cusbf_num_shards(const cusbf_filter* f);
 /* ref().blocks() (filter.cuh:137-142, filter_ref.cuh:56-64): device address of
- * shard 0; shard i occupies words [4i, 4i+4). */
+ * shard 0; shard i occupies words [4i, 4i+4). The caller may write the words
+ * through it, so from the first call on the filter no longer assumes its words
+ * are zero after a clear (a sorted insert then always reads the slices it
+ * updates); the first call waits for the null stream. */
 int cusbf_words_device_ptr(const cusbf_filter* f, uint64_t** out);
 /* Parity dump of all num_shards*4 filter words into host memory (synchronous). */
 int cusbf_copy_words_to_host(const cusbf_filter* f, uint64_t* host_words, uint64_t num_words);
