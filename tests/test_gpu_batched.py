@@ -415,3 +415,36 @@ def test_per_record_launch_loop_matches_batched(gpu):
     torch.cuda.synchronize()
     assert torch.equal(a.words_tensor(), b.words_tensor())
     assert torch.equal(out_a, out_b)
+
+
+@pytest.mark.parametrize("alphabet", ["dna", "protein"])
+def test_batched_full_size_words_equal_oracle(gpu, alphabet):
+    """The measured batched workload at full size: 1M seeded log-normal records (~3.08 G
+    symbols) inserted by a graph plan (sorted path) into a 2^36-bit filter; every filter
+    word equals the multithreaded oracle's over the records joined by separators."""
+    import cusbf_amd as cs
+    from cusbf_amd import batched as bt
+    from cusbf_amd import workloads as W
+
+    torch = _torch()
+    torch.cuda.empty_cache()
+    if alphabet == "dna":
+        k, s, m, letters, sep, seed = 31, 28, 16, W.DNA_LETTERS, ord("N"), 31
+        cfg = cs.Config(k, s, m, 4)
+    else:
+        k, s, m, letters, sep, seed = 10, 8, 6, W.PROTEIN_LETTERS, ord("*"), 37
+        cfg = cs.Config(k, s, m, 4, alphabet=cs.ProteinAlphabet)
+    data, d_off, lens = W.generate_batched(W.BatchedSpec(records=1_000_000, letters=letters, seed=seed))
+    kmers = int(np.maximum(lens.astype(np.int64) - k + 1, 0).sum())
+    f = cs.Filter(cfg, int(21.4 * kmers))
+    assert f.num_shards() == 1 << 28
+    f.set_insert_path(cs.INSERT_SORTED)
+    with bt.BatchPlan(f, bt.INSERT, data, d_off) as plan:
+        plan.launch()
+        plan.status()
+    host = W.join_records(data, lens, sep).cpu().numpy()
+    del data, d_off
+    o = make_oracle(k, s, m, 4, alphabet)
+    w = o.new_filter(f.num_shards())
+    o.mt_insert(host, w)
+    assert np.array_equal(f.words(), w)
