@@ -392,3 +392,37 @@ def test_fpr_on_seeded_negatives_equals_oracle(gpu, s):
     fpr = np.count_nonzero(want) / (n - 30)
     assert 0 < fpr < 0.02, fpr
     torch.cuda.synchronize()
+
+
+def test_full_size_query_answers_equal_oracle(gpu):
+    """Query answers at full size: the 1M-record DNA workload joined by separators
+    (~3.08 G windows) is inserted, then the same records with every odd record replaced
+    by fresh symbols are queried; all ~3.08 G answer bytes equal the multithreaded
+    oracle's, positives and negatives both present."""
+    import cusbf_amd as cs
+    from cusbf_amd import workloads as W
+
+    torch = _torch()
+    torch.cuda.empty_cache()
+    spec = W.BatchedSpec(records=1_000_000, letters=W.DNA_LETTERS, seed=31)
+    data, _, lens = W.generate_batched(spec)
+    query = W.mutate_odd_records(data, lens, W.DNA_LETTERS, spec.seed + 1)
+    text = W.join_records(data, lens, ord("N"))
+    qtext = W.join_records(query, lens, ord("N"))
+    del data, query
+    cfg = make_config(31, 28, 16, 4)
+    f = cs.Filter(cfg, int(21.4 * text.numel()))
+    f.insert_sequence_async(text)
+    out = torch.empty(qtext.numel() - 30, dtype=torch.uint8, device="cuda")
+    f.contains_sequence_async(qtext, out)
+    got = out.cpu().numpy()
+    host_t, host_q = text.cpu().numpy(), qtext.cpu().numpy()
+    del out, text, qtext
+    torch.cuda.synchronize()
+    o = make_oracle(31, 28, 16, 4)
+    w = o.new_filter(f.num_shards())
+    o.mt_insert(host_t, w)
+    want = o.mt_contains(host_q, w)
+    assert np.array_equal(got, want)
+    positives = int(np.count_nonzero(want))
+    assert 0 < positives < want.size
