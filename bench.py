@@ -296,29 +296,8 @@ def serving_scenario(filt, genome, reads, wl, n_requests: int, qps: float):
     d_long, long_off = W.generate_long_reads(genome, 20_000, seed=77)
     long_pool = d_long.cpu().numpy()
     del d_long
-    long_len = np.diff(long_off) - 1
-    long_len[-1] += 1
     arrivals = np.cumsum(rng.exponential(1.0 / qps, n_requests))
-    units = np.exp(rng.uniform(0, np.log(1_000_000), n_requests)).astype(np.int64)
-    kinds = rng.random(n_requests) < 0.5
-    reqs = []
-    for i in range(n_requests):
-        if kinds[i]:
-            n = int(units[i])
-            st = int(rng.integers(0, pool_reads - n + 1))
-            seq = short[st * stride:(st + n) * stride - 1]
-            rg = np.stack([np.arange(n, dtype=np.uint64) * stride,
-                           np.full(n, wl.reads.read_len, dtype=np.uint64)], axis=1)
-        else:
-            target = int(units[i]) * wl.reads.read_len
-            st = int(rng.integers(0, 20_000 - 64))
-            c = int(np.searchsorted(np.cumsum(long_len[st:]), target)) + 1
-            c = max(1, min(c, 20_000 - st))
-            base = int(long_off[st])
-            seq = long_pool[base:int(long_off[st + c - 1]) + int(long_len[st + c - 1])]
-            rg = np.stack([(long_off[st:st + c] - base).astype(np.uint64),
-                           long_len[st:st + c].astype(np.uint64)], axis=1)
-        reqs.append((seq, rg))
+    reqs = W.serving_requests(short, wl.reads.read_len, long_pool, long_off, n_requests, seed=4243)
     total_bytes = sum(int(r[0].size) for r in reqs)
     # answers land in one pre-faulted arena reused by every run
     from cusbf_amd.serve import normalized_size, ranges_array

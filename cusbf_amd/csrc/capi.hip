@@ -812,13 +812,18 @@ void cusbf_destroy(cusbf_filter* f) {
     delete f;
 }
 
-// filter.cuh:793-803. A fill kernel rather than a memset node, so the clear
-// can also sit inside captured graphs.
+// filter.cuh:793-803: clears the words and synchronizes the stream. The words
+// are cleared by a fill kernel, not a memset node, so a clear may also be
+// captured into a graph; under capture it skips the synchronize (which would
+// end the capture) and never marks the words as known to be zero, because a
+// replay may follow writes through the words pointer that the library
+// cannot see.
 int cusbf_clear(cusbf_filter* f, void* stream) {
     if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    const bool captured = capturing(as_stream(stream));
     CUSBF_HIP_TRY(sbf::fill_u64(f->d_words, 0, f->num_shards * 4, as_stream(stream)));
-    CUSBF_HIP_TRY(sbf::fill_u64(f->d_zero, f->words_shared ? 0 : 1, 1, as_stream(stream)));
-    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    CUSBF_HIP_TRY(sbf::fill_u64(f->d_zero, f->words_shared || captured ? 0 : 1, 1, as_stream(stream)));
+    if (!captured) CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
     return CUSBF_OK;
 }
 
@@ -1425,11 +1430,19 @@ int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
     return CUSBF_OK;
 }
 
+// An insert plan is ordered like every other insert into its filter: after
+// the previous insert on any stream and before the next (its sorted form
+// rewrites whole slices, so an insert overlapping it could lose updates).
 int cusbf_batch_plan_launch(cusbf_batch_plan* p, void* stream) {
     if (!p) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null plan");
     if (p->empty) return CUSBF_OK;
+    const bool insert = p->op == CUSBF_BATCH_INSERT;
+    if (insert) {
+        const int rc = order_insert(p->f, as_stream(stream));
+        if (rc != CUSBF_OK) return rc;
+    }
     CUSBF_HIP_TRY(hipGraphLaunch(p->exec, as_stream(stream)));
-    return CUSBF_OK;
+    return insert ? mark_insert(p->f, as_stream(stream)) : CUSBF_OK;
 }
 
 int cusbf_batch_plan_status(cusbf_batch_plan* p, void* stream) {

@@ -326,3 +326,45 @@ def join_records(data, lens: np.ndarray, separator: int, rec_idx=None):
         pos = torch.arange(a, b, device=data.device, dtype=torch.int64) + rec_idx[a:b]
         out[pos] = data[a:b]
     return out
+
+
+# --- serving requests ------------------------------------------------------------------
+
+def serving_requests(short: np.ndarray, read_len: int, long_pool: np.ndarray,
+                     long_off: np.ndarray, n_requests: int, seed: int,
+                     max_reads: int = 1_000_000):
+    """Requests of the serving configuration, as (host bytes, (n, 2) uint64 ranges).
+
+    Sizes are log-uniform over 1..max_reads reads of read_len bases. Each request
+    is, with probability 1/2, a run of consecutive short reads from `short` (reads
+    of read_len joined by one separator) or enough consecutive long reads of
+    `long_pool` (reads at `long_off`, as generate_long_reads returns them) to hold
+    at least the same number of bases (at least one read, at most the pool).
+    """
+    rng = np.random.default_rng(seed)
+    stride = read_len + 1
+    pool_reads = (short.size + 1) // stride
+    n_long = long_off.size - 1
+    long_len = np.diff(long_off) - 1
+    long_len[-1] += 1  # no separator after the last read
+    units = np.exp(rng.uniform(0, np.log(max_reads), n_requests)).astype(np.int64)
+    kinds = rng.random(n_requests) < 0.5
+    reqs = []
+    for i in range(n_requests):
+        if kinds[i]:
+            n = int(min(units[i], pool_reads))
+            st = int(rng.integers(0, pool_reads - n + 1))
+            seq = short[st * stride:(st + n) * stride - 1]
+            rg = np.stack([np.arange(n, dtype=np.uint64) * stride,
+                           np.full(n, read_len, dtype=np.uint64)], axis=1)
+        else:
+            target = int(units[i]) * read_len
+            st = int(rng.integers(0, max(1, n_long - 64)))
+            c = int(np.searchsorted(np.cumsum(long_len[st:]), target)) + 1
+            c = max(1, min(c, n_long - st))
+            base = int(long_off[st])
+            seq = long_pool[base:int(long_off[st + c - 1]) + int(long_len[st + c - 1])]
+            rg = np.stack([(long_off[st:st + c] - base).astype(np.uint64),
+                           long_len[st:st + c].astype(np.uint64)], axis=1)
+        reqs.append((seq, rg))
+    return reqs

@@ -108,7 +108,9 @@ int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_
 /* ~filter() (filter.cuh:130). NULL is a no-op. */
 void cusbf_destroy(cusbf_filter* f);
 
-/* clear(stream) (filter.cuh:793-803): zero all shards, synchronise. */
+/* clear(stream) (filter.cuh:793-803): zero all shards, synchronise. The words
+ * are cleared by a fill kernel (no memset node); on a stream that is capturing
+ * a graph the clear is recorded without the synchronize. */
 int cusbf_clear(cusbf_filter* f, void* stream);
 /* load_factor() (filter.cuh:810-823). */
 int cusbf_load_factor(const cusbf_filter* f, float* out);
@@ -335,6 +337,9 @@ int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,
                             uint64_t num_symbols, const uint64_t* d_offsets, uint64_t num_records,
                             uint8_t* d_output, uint64_t d_output_len,
                             const cusbf_batch_tuning* tuning, cusbf_batch_plan** out);
+/* Replays the plan on `stream`. An insert plan runs after the filter's previous
+ * insert on any stream and before its next one, like a direct insert (skipped
+ * while `stream` is capturing). */
 int cusbf_batch_plan_launch(cusbf_batch_plan* p, void* stream);
 /* Synchronises the stream; CUSBF_ERR_INVALID_ARGUMENT if the offsets were malformed. */
 int cusbf_batch_plan_status(cusbf_batch_plan* p, void* stream);

@@ -225,13 +225,13 @@ class Oracle:
     def mt_insert(self, seq, words: np.ndarray, threads: int = 0) -> None:
         b = _buf(seq)
         lib().orc_mt_insert(ctypes.byref(self.cfg), _ptr(b), b.size, _ptr(words),
-                            words.size // 4, threads)
+                            words.size // 4, threads or default_threads())
 
     def mt_contains(self, seq, words: np.ndarray, threads: int = 0) -> np.ndarray:
         b = _buf(seq)
         out = np.zeros(self.kmer_count(b.size), dtype=np.uint8)
         lib().orc_mt_contains(ctypes.byref(self.cfg), _ptr(b), b.size, _ptr(words),
-                              words.size // 4, _ptr(out), threads)
+                              words.size // 4, _ptr(out), threads or default_threads())
         return out
 
 
@@ -249,3 +249,37 @@ def salt(i: int) -> int:
 
 def hardware_threads() -> int:
     return lib().orc_hardware_threads()
+
+
+def normalize_records(seq, ranges, separator: int, width: int = 1) -> bytes:
+    """Records of `seq` joined as a normalised record batch
+    (normalized_record_batch.hpp:208-276): before every record after the first
+    non-empty output, the running size is padded with separator bytes to a
+    multiple of the symbol width and one separator symbol (`width` bytes) is
+    written; then the record's bytes."""
+    b = _buf(seq)
+    parts = []
+    size = 0
+    sep = bytes([separator])
+    for off, n in ranges:
+        off, n = int(off), int(n)
+        if size:
+            pad = (width - size % width) % width + width
+            parts.append(sep * pad)
+            size += pad
+        parts.append(b[off:off + n].tobytes())
+        size += n
+    return b"".join(parts)
+
+
+def default_threads() -> int:
+    """Worker threads for the multithreaded form: the CPUs this process may run on
+    (affinity mask and cgroup CPU quota), at most 16 (one GPU's share of a host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, -(-int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return max(1, min(n, 16))

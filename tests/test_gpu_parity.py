@@ -111,6 +111,24 @@ def test_clear_and_idempotence(gpu):
     assert not f.contains_sequence(seq).any()
 
 
+@pytest.mark.parametrize("h", [4, 16])
+def test_load_factor_equals_oracle_popcount(gpu, h):
+    """load_factor (filter.cuh:810-823) == float(popcount of the oracle's words) /
+    float(filter bits): the lane-strip kernels (h = 4) and the generic ones (h = 16)."""
+    from oracle import image_oracle as im
+
+    cfg = make_config(31, 28, 16, h)
+    o = make_oracle(31, 28, 16, h)
+    seq = random_dna(700_000, seed=40 + h, n_rate=0.001)
+    f = new_filter(cfg, 6 * len(seq))
+    f.insert_sequence(seq)
+    w = o.new_filter(f.num_shards())
+    o.mt_insert(seq, w)
+    want = im.load_factor(w, f.filter_bits())
+    assert 0.0 < want < 1.0
+    assert f.load_factor() == want
+
+
 def test_host_and_device_query_agree(gpu):
     torch = _torch()
     c = KNOWN["host_vs_device"]

@@ -309,6 +309,71 @@ def test_captured_insert_replayed_after_clear(gpu):
         assert (f.words() == (w_a | w_b)).all(), rep
 
 
+def test_plan_replay_ordered_with_direct_insert_on_another_stream(gpu):
+    """An insert plan that took the sorted path (its apply rewrites whole slices)
+    replayed on one stream and a direct sorted insert issued on another stream with
+    no host wait between them: the library orders the two, so the words equal the
+    oracle's for both inputs (an overlap would lose one side's bits)."""
+    import cusbf_amd as cs
+    from cusbf_amd import batched as bt
+    from tests.test_gpu_batched import oracle_words as batched_oracle_words
+
+    torch = _torch()
+    k, s, m, h = 31, 28, 16, 4
+    cfg = make_config(k, s, m, h)
+    bits = 1 << 26
+    data, off, d_data, d_off = _batched_inputs("dna", 2000, 1500, seed=41)
+    b_seq = random_dna(3_000_000, 42)
+    f = sorted_filter(cfg, bits, slice_bits=5)
+    shards = f.num_shards()
+    w = batched_oracle_words(make_oracle(k, s, m, h, "dna"), data, off, "dna", shards)
+    w |= oracle_words(k, s, m, h, "dna", [b_seq], shards)
+    d_b = torch.frombuffer(bytearray(b_seq), dtype=torch.uint8).cuda()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    with bt.BatchPlan(f, bt.INSERT, d_data, d_off) as plan:
+        for rep in range(3):
+            f.clear()
+            torch.cuda.synchronize()
+            plan.launch(s1.cuda_stream)
+            f.insert_sequence_async(d_b, stream=s2.cuda_stream)
+            torch.cuda.synchronize()
+            assert f.insert_path().last_path == cs.INSERT_SORTED
+            assert (f.words() == w).all(), rep
+            # and the other order: direct insert first, then the replay
+            f.clear()
+            torch.cuda.synchronize()
+            f.insert_sequence_async(d_b, stream=s2.cuda_stream)
+            plan.launch(s1.cuda_stream)
+            torch.cuda.synchronize()
+            assert (f.words() == w).all(), rep
+
+
+def test_captured_clear_replays(gpu):
+    """A clear captured into a graph (fill kernels, no synchronize inside the capture)
+    zeroes the words on every replay; an insert after each replay equals the oracle's
+    words for that input alone, with eager work between the replays."""
+    torch = _torch()
+    k, s, m, h = 31, 28, 16, 4
+    cfg = make_config(k, s, m, h)
+    a_seq, b_seq = random_dna(300_000, 51), random_dna(200_000, 52)
+    f = sorted_filter(cfg, 1 << 24, slice_bits=6)
+    w_b = oracle_words(k, s, m, h, "dna", [b_seq], f.num_shards())
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        f.clear(side.cuda_stream)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        f.insert_sequence(a_seq)
+        assert f.words().any()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert not f.words().any(), rep
+        f.insert_sequence(b_seq)
+        assert (f.words() == w_b).all(), rep
+
+
 def test_sorted_tiny_and_empty(gpu):
     import cusbf_amd as cs
 

@@ -258,3 +258,66 @@ def test_random_negatives_fpr_is_small():
     o.mt_insert(seq, w)
     fpr = o.mt_contains(random_dna(1_000_000, seed=22), w).mean()
     assert 1e-6 < fpr < 1e-3, fpr
+
+
+# --- filter images and load_factor (oracle/image_oracle.py) -------------------------------------
+
+def test_splitmix64_finaliser_matches_published_outputs():
+    from oracle import image_oracle as im
+
+    # SplitMix64 from seed 0 (Vigna's splitmix64.c): the published first outputs
+    assert im.splitmix64_sequence(0, 5) == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                            0x06C45D188009454F, 0xF88BB8A8724C81EC,
+                                            0x1B39896A51A8749B]
+
+
+def test_image_checksum_definition():
+    from oracle import image_oracle as im
+
+    assert im.words_checksum(np.zeros(0, np.uint64)) == 0
+    # word i = 0: mix(i * golden) is SplitMix64's output i from seed 0
+    seq = im.splitmix64_sequence(0, 4)
+    assert im.words_checksum(np.zeros(5, np.uint64)) == sum(seq) + int(im.splitmix64_mix(np.zeros(1, np.uint64))[0]) & MASK64
+    # the index enters: swapping two different words changes the sum
+    w = np.array([1, 2, 3], dtype=np.uint64)
+    assert im.words_checksum(w) != im.words_checksum(w[::-1].copy())
+    # chunking does not change the sum
+    r = np.random.default_rng(1).integers(0, 1 << 63, 1000, dtype=np.uint64)
+    assert im.words_checksum(r) == im.words_checksum(r, chunk=7)
+
+
+def test_load_factor_restatement():
+    from oracle import image_oracle as im
+
+    assert im.load_factor(np.zeros(64, np.uint64), 4096) == 0.0
+    assert im.load_factor(np.full(64, MASK64, np.uint64), 4096) == 1.0
+    one = np.zeros(1 << 10, np.uint64)
+    one[77] = 1 << 13
+    assert im.load_factor(one, 1 << 16) == 2.0 ** -16
+    r = np.random.default_rng(2).integers(0, 1 << 63, 4096, dtype=np.uint64)
+    assert im.popcount_words(r) == sum(bin(int(x)).count("1") for x in r)
+    # float32 rounding of the quotient, as the reference's float division
+    assert im.load_factor(r, 4096 * 64) == float(np.float32(im.popcount_words(r)) / np.float32(4096 * 64))
+
+
+def test_normalize_records_against_reference_batches():
+    c = KNOWN["device_record_batch"]
+    seq = (c["record1"] + c["separator"] + c["record2"]).encode()
+    norm = orc.normalize_records(seq, c["ranges"], ord(c["separator"]))
+    assert norm == seq
+    o = oracle_for(c["config"])
+    w = o.new_filter(o.num_shards(c["filter_bits"]))
+    o.insert(c["record1"], w)
+    o.insert(c["record2"], w)
+    assert o.contains(norm, w).tolist() == c["batch_hits"]
+    for case in KNOWN["normalized_record_batch"]:
+        if "total_valid_kmers" not in case:
+            continue
+        o = oracle_for(case["config"])
+        norm = orc.normalize_records(case["sequence"], case["ranges"], o.separator, o.symbol_width)
+        assert len(norm) > case["min_normalized_size_exclusive"]
+        assert o.count(norm, 1)[0] == case["total_valid_kmers"]
+    # width 3: padding to the symbol width, then one separator symbol
+    assert orc.normalize_records(b"ACGTA", [(0, 4), (4, 1)], ord("N"), 3) == b"ACGTNNNNNA"
+    # empty leading records write no separator
+    assert orc.normalize_records(b"ACGT", [(0, 0), (0, 4)], ord("N")) == b"ACGT"

@@ -52,6 +52,20 @@ def test_save_load_round_trip(gpu, tmp_path, alphabet, k, s, m, h):
     assert (g.words() == f.words()).all()
     q = random_bytes_for(alphabet, 50_000, 4, 0.001) + seq[:50_000]
     assert np.array_equal(g.contains_sequence(q), f.contains_sequence(q))
+    # the file itself, against the oracle: payload = the oracle's words, header
+    # fields and checksum = the format's restatement over those words
+    from oracle import image_oracle as im
+    from tests.helpers import make_oracle
+
+    o = make_oracle(k, s, m, h, alphabet)
+    w = o.new_filter(f.num_shards())
+    o.insert(seq, w)
+    hdr, payload = im.read_image(path)
+    assert hdr["magic"] == b"CUSBFIMG" and hdr["version"] == 1 and hdr["header_bytes"] == 336
+    assert hdr["num_shards"] == f.num_shards() and hdr["filter_bits"] == f.filter_bits()
+    assert hdr["word_count"] == w.size
+    assert np.array_equal(payload, w)
+    assert int(hdr["checksum"]) == im.words_checksum(w)
 
 
 @pytest.mark.gpu
