@@ -42,6 +42,11 @@ def parse_args():
     ap.add_argument("--reads", type=int, default=0, help="override the number of reads")
     ap.add_argument("--insert-path", default="auto", choices=["auto", "atomic", "sorted"],
                     help="insert path of the lane-strip kernels (auto: sorted when it pays)")
+    ap.add_argument("--nematode", default="on", choices=["on", "off"],
+                    help="nematode-scale leg (100 Mbp, 85.7 bits/k-mer, 20M reads) and the "
+                         "reference's FPR densities on its genome")
+    ap.add_argument("--fpr-kmers", type=int, default=1_000_000_000,
+                    help="uniform random query k-mers of the FPR densities sweep")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--probes", default="on", choices=["on", "off"],
@@ -127,6 +132,111 @@ def fpr_scenario(genome, reads, wl, out, stream_handle) -> dict:
         torch.cuda.empty_cache()
     log(f"[bench] FPR m=16 s=28/30/31: " + " / ".join(f"{100 * res['m16'][f's{x}']:.3f}%" for x in (28, 30, 31))
         + f"; m=21 s=28: {100 * res['m21']['s28']:.3f}%")
+    return res
+
+
+# The reference's published C. elegans throughput (README.md:37-44, figure (a)),
+# G k-mers/s, and its FPR at 21.4 / 85.7 / 342.6 bits per k-mer (README.md:59-63),
+# percent, with m = 21 (benchmarks/fpr_fastx_sweep.cu:295).
+PUBLISHED_NEMATODE = {"rtx_pro_6000": {"insert": 64.0, "query": 140.0},
+                      "gh200": {"insert": 37.0, "query": 53.0}}
+PUBLISHED_FPR_DENSITIES = {"21.4": {"s28": 0.848, "s30": 0.951, "s31": 1.593},
+                           "85.7": {"s28": 0.091, "s30": 0.107, "s31": 0.210},
+                           "342.6": {"s28": 0.0095, "s30": 0.0114, "s31": 0.0264}}
+
+
+def timed_insert_query(filt, genome, reads, out, sh, steps: int, warmup: int):
+    """Median insert (filter cleared outside the timed region) and query seconds."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    ti, tq = [], []
+    for i in range(warmup + steps):
+        filt.clear(sh)
+        ev = _events(3)
+        ev[0].record(stream)
+        filt.insert_sequence_async(genome, stream=sh)
+        ev[1].record(stream)
+        filt.contains_sequence_async(reads, out, stream=sh)
+        ev[2].record(stream)
+        ev[2].synchronize()
+        if i >= warmup:
+            ti.append(ev[0].elapsed_time(ev[1]) / 1e3)
+            tq.append(ev[1].elapsed_time(ev[2]) / 1e3)
+    return statistics.median(ti), statistics.median(tq)
+
+
+def nematode_scenario(steps: int, warmup: int, fpr_kmers: int) -> dict:
+    """The nematode-scale plain path: 100 Mbp genome (35% GC, 6 chromosomes, 5%
+    repeats) at 85.7 bits per k-mer, 20 M 150 bp reads; insert and query timed like
+    the headline workload. Then the reference's FPR protocol
+    (benchmarks/fpr_fastx_sweep.cu): the genome inserted into filters of 2^31,
+    2^33 and 2^35 bits (21.4 / 85.7 / 342.6 bits per k-mer), `fpr_kmers` uniform
+    random k-mers queried, s = 28 / 30 / 31 with m = 21 (the reference's
+    benchmark) and m = 16."""
+    import torch
+
+    import cusbf_amd as cs
+    from cusbf_amd import workloads as W
+
+    wl = W.WORKLOADS["nematode"]
+    genome, _ = W.generate_genome(wl.genome)
+    reads = W.generate_reads(wl.reads, genome)
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    windows = genome.numel() - wl.k + 1
+    filt = cs.Filter(cs.Config(wl.k, wl.s, wl.m, wl.h), int(wl.bits_per_kmer * windows))
+    out = torch.empty(reads.numel() - wl.k + 1, dtype=torch.uint8, device="cuda")
+    v_ins, s_ins = filt.count_windows_runs(genome)
+    v_q, s_q = filt.count_windows_runs(reads)
+    ti, tq = timed_insert_query(filt, genome, reads, out, sh, steps, warmup)
+    positives = int(cs.count_positive(out))
+    bytes_ins = genome.numel() + 64 * s_ins
+    bytes_q = reads.numel() + out.numel() + 32 * s_q
+    res = {"workload": {"genome_bp": wl.genome.total_len, "chromosomes": wl.genome.n_chrom,
+                        "gc": wl.genome.gc, "repeat_frac": wl.genome.repeat_frac,
+                        "reads": wl.reads.n_reads, "read_len": wl.reads.read_len,
+                        "k": wl.k, "s": wl.s, "m": wl.m, "hash_count": wl.h,
+                        "bits_per_kmer_requested": wl.bits_per_kmer,
+                        "filter_bits": filt.filter_bits(),
+                        "bits_per_kmer_effective": filt.filter_bits() / max(1, v_ins)},
+           "insert_path": {1: "atomic", 2: "sorted"}[filt.insert_path().last_path],
+           "insert_ms": ti * 1e3, "query_ms": tq * 1e3,
+           "insert_kmers_per_s": v_ins / ti, "query_kmers_per_s": v_q / tq,
+           "kmers_per_s": (v_ins + v_q) / (ti + tq),
+           "insert_valid_kmers": v_ins, "insert_shard_runs": s_ins,
+           "query_valid_kmers": v_q, "query_shard_runs": s_q,
+           "insert_roofline_frac": bytes_ins / ti / HBM_PEAK,
+           "query_roofline_frac": bytes_q / tq / HBM_PEAK,
+           "query_positive_frac": positives / max(1, v_q),
+           "published_reference_G_kmers_per_s": PUBLISHED_NEMATODE}
+    log(f"[nematode] insert {ti * 1e3:.3f} ms ({v_ins / ti / 1e9:.1f} G k-mers/s), query "
+        f"{tq * 1e3:.3f} ms ({v_q / tq / 1e9:.1f} G k-mers/s)")
+    filt.close()
+    del reads, out
+    torch.cuda.empty_cache()
+
+    q = W.uniform_dna(fpr_kmers + wl.k - 1, seed=0xDEADBEEF)
+    hits = torch.empty(fpr_kmers, dtype=torch.uint8, device="cuda")
+    fpr = {"query_kmers": fpr_kmers, "insert_kmers": windows,
+           "published_m21_percent": PUBLISHED_FPR_DENSITIES, "m21": {}, "m16": {}}
+    for log2_bits, label in ((31, "21.4"), (33, "85.7"), (35, "342.6")):
+        for m_ in (21, 16):
+            row = {}
+            for s_ in (28, 30, 31):
+                f = cs.Filter(cs.Config(wl.k, s_, m_, wl.h), 1 << log2_bits)
+                f.insert_sequence_async(genome, stream=sh)
+                f.contains_sequence_async(q, hits, stream=sh)
+                row[f"s{s_}"] = 100.0 * int(cs.count_positive(hits)) / fpr_kmers
+                f.close()
+            row["bits_per_kmer"] = (1 << log2_bits) / windows
+            fpr[f"m{m_}"][label] = row
+    log("[nematode] FPR m=21 (percent): " + "; ".join(
+        f"{b}: " + " / ".join(f"{fpr['m21'][b][f's{x}']:.4f}" for x in (28, 30, 31))
+        for b in ("21.4", "85.7", "342.6")))
+    res["fpr_percent"] = fpr
+    del q, hits, genome
+    torch.cuda.empty_cache()
     return res
 
 
@@ -229,14 +339,23 @@ def dense_scenario(genome, wl, stream_handle, host_bases: int, reps: int = 3) ->
     return res
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(genome, reads, wl, threads: int):
     """CPU Super Bloom (the oracle's multithreaded sliding form) on a bounded sample."""
     import numpy as np
 
     from oracle import oracle as orc
 
-    t_avail = min(16, os.cpu_count() or 1)
-    threads = threads or t_avail
+    threads = threads or orc.default_threads()
     ins_bytes = min(genome.numel(), 400_000_000)
     read_stride = wl.reads.read_len + 1
     q_reads = min(wl.reads.n_reads, 4_000_000)
@@ -262,6 +381,9 @@ def cpu_baseline(genome, reads, wl, threads: int):
         "unit": "k-mers/s",
         "cores": threads,
         "kind": "port",
+        "cpu_model": cpu_model(),
+        "hardware_concurrency": orc.hardware_threads(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
         "sample": (f"insert first {ins_bytes / 1e6:.0f} Mbp of the genome, query first "
                    f"{q_reads / 1e6:.0f}M reads; {threads} threads; "
                    f"insert {ins_rate / 1e6:.0f} M/s, query {q_rate / 1e6:.0f} M/s; "
@@ -710,6 +832,10 @@ def main() -> int:
     if not torch.cuda.is_available():
         log("no GPU visible")
         return 1
+    nematode = None
+    if args.nematode == "on" and args.workload == "human":
+        log("[bench] nematode-scale leg")
+        nematode = nematode_scenario(args.steps, args.warmup, args.fpr_kmers)
     wl = W.WORKLOADS[args.workload]
     if args.reads:
         wl.reads.n_reads = args.reads
@@ -763,6 +889,19 @@ def main() -> int:
     ti, tq = statistics.median(t_ins), statistics.median(t_q)
     log(f"[bench] timed {args.steps} steps in {wall:.3f}s; insert median {ti * 1e3:.2f} ms, "
         f"query median {tq * 1e3:.2f} ms")
+
+    # insert into a populated filter: the genome again into the filter holding it
+    # (the words are not known to be zero, so the sorted apply reads every slice)
+    pop = []
+    for _ in range(3):
+        ev = _events(2)
+        ev[0].record(stream)
+        filt.insert_sequence_async(genome, stream=sh)
+        ev[1].record(stream)
+        ev[1].synchronize()
+        pop.append(ev[0].elapsed_time(ev[1]) / 1e3)
+    t_pop = statistics.median(pop)
+    log(f"[bench] insert into the populated filter: {t_pop * 1e3:.2f} ms")
 
     positives = int(cs.count_positive(out))
     ins_rate = v_ins / ti
@@ -828,6 +967,8 @@ def main() -> int:
         "query_kmers_per_s": q_rate,
         "insert_ms": ti * 1e3,
         "query_ms": tq * 1e3,
+        "insert_populated_ms": t_pop * 1e3,
+        "insert_populated_kmers_per_s": v_ins / t_pop,
         "insert_shard_runs": s_ins,
         "query_shard_runs": s_q,
         "insert_roofline_frac": bytes_ins / ti / HBM_PEAK,
@@ -847,6 +988,8 @@ def main() -> int:
         "serve_p99_ms": None,
         "fpr": fpr,
     }
+    if nematode is not None:
+        result["nematode"] = nematode
     if serving is not None:
         result["serve_p50_ms"] = serving["server_p50_ms"]
         result["serve_p99_ms"] = serving["server_p99_ms"]

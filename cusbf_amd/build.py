@@ -70,8 +70,9 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     _link(objs_of(HIP_SOURCES), LIB_PATH, ["-lz"])  # zlib: gzip FASTX input
     _link(objs_of(SYNTH_SOURCES), SYNTH_LIB_PATH)
     build_cpp_tests()
+    build_hip_tests()
     if verbose:
-        print(f"built {LIB_PATH}, {SYNTH_LIB_PATH} and {CPP_TEST_PATH}")
+        print(f"built {LIB_PATH}, {SYNTH_LIB_PATH}, {CPP_TEST_PATH} and {HIP_TEST_PATH}")
     return LIB_PATH
 
 
@@ -92,6 +93,22 @@ def build_cpp_tests() -> str:
     if res.returncode != 0:
         raise RuntimeError(f"C++ test build failed:\n{res.stderr[-6000:]}")
     return CPP_TEST_PATH
+
+
+HIP_TEST_SRC = os.path.join(REPO, "tests", "hip", "test_filter_ref.hip")
+HIP_TEST_PATH = os.path.join(REPO, "tests", "hip", "_build", "test_filter_ref")
+
+
+def build_hip_tests() -> str:
+    """A caller's own HIP kernels over filter<Config>::ref() (cusbf/filter_ref_device.hpp)."""
+    os.makedirs(os.path.dirname(HIP_TEST_PATH), exist_ok=True)
+    cmd = [hipcc(), *[f for f in HIPCC_FLAGS if f not in ("-fPIC", "-std=c++17")], "-std=c++20",
+           f"-I{os.path.join(REPO, 'include')}", HIP_TEST_SRC, "-o", HIP_TEST_PATH, f"-L{HERE}",
+           "-lcusbf_hip", "-Wl,-rpath,$ORIGIN/../../../cusbf_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"HIP test build failed:\n{res.stderr[-6000:]}")
+    return HIP_TEST_PATH
 
 
 if __name__ == "__main__":

@@ -25,15 +25,12 @@
 #include <utility>
 #include <vector>
 
-#if __has_include(<hip/hip_runtime_api.h>)
 #include <hip/hip_runtime_api.h>
-#else
-typedef struct ihipStream_t* hipStream_t;
-#endif
 
 #include "../cusbf_hip.h"
 #include "config.hpp"
 #include "error.hpp"
+#include "filter_ref.hpp"
 
 namespace cusbf {
 
@@ -405,17 +402,6 @@ template <typename Config>
         &c, d_sequence, num_symbols, d_words, dense_packed_word_count<Config>(num_symbols), stream));
 }
 
-/// Non-owning view of the filter's shard storage (filter_ref.cuh:20-64, host side).
-template <typename Config>
-struct filter_ref {
-    uint64_t* words = nullptr;  // shard i = words[4i .. 4i+4)
-    uint64_t num_blocks = 0;
-    [[nodiscard]] uint64_t* blocks() const noexcept { return words; }
-    [[nodiscard]] static uint64_t shard_index(uint64_t minimizer_hash, uint64_t n) noexcept {
-        return minimizer_hash & (n - 1);
-    }
-};
-
 template <typename Config>
 class filter {
    public:
@@ -439,10 +425,20 @@ class filter {
     }
     ~filter() { cusbf_destroy(handle_); }
 
+    /// filter.cuh:137-142: view of the shard storage for device code (device members in
+    /// cusbf/filter_ref_device.hpp). The library stops assuming the words are zero once
+    /// the view has been handed out. If the device refuses that step the view is empty
+    /// (blocks() == nullptr, num_blocks() == 0); checked_ref() returns the error instead.
     [[nodiscard]] filter_ref<Config> ref() const noexcept {
         uint64_t* w = nullptr;
-        cusbf_words_device_ptr(handle_, &w);
-        return filter_ref<Config>{w, num_shards()};
+        if (cusbf_words_device_ptr(handle_, &w) != CUSBF_OK) return filter_ref<Config>{};
+        return filter_ref<Config>{reinterpret_cast<filter_block<Config>*>(w), num_shards()};
+    }
+    [[nodiscard]] Result<filter_ref<Config>> checked_ref() const {
+        uint64_t* w = nullptr;
+        const int st = cusbf_words_device_ptr(handle_, &w);
+        if (st != CUSBF_OK) return detail::status_error(st);
+        return filter_ref<Config>{reinterpret_cast<filter_block<Config>*>(w), num_shards()};
     }
 
     // --- byte sequences (filter.cuh:154-190, 464-533) ---

@@ -19,6 +19,17 @@
 
 #include "cusbf/filter.hpp"
 
+// filter_ref / filter_block (filter_ref.cuh:20-113, filter_impl.cuh:18-77) in host
+// constant expressions: the view and the block arithmetic compile without hipcc.
+using RefCfg = cusbf::Config<31, 28, 16, 4>;
+static_assert(cusbf::filter_ref<RefCfg>::shard_index(0xABCDEFull, 1u << 8) == 0xEF);
+static_assert(cusbf::filter_ref<RefCfg>{nullptr, 64}.num_blocks() == 64);
+static_assert(cusbf::filter_ref<RefCfg>{nullptr, 64}.shard_index(131) == 3);
+static_assert(cusbf::filter_block<RefCfg>::sectorizedBitAddress<1>(0x3Full << 16) == 0x3F);
+static_assert(cusbf::filter_block<cusbf::Config<31, 28, 16, 16>>::sectorizedBitAddress<0>(1) ==
+              (0x9E3779B97F4A7C15ull >> 58));
+static_assert(sizeof(cusbf::filter_block<RefCfg>) == 32 && alignof(cusbf::filter_block<RefCfg>) == 32);
+
 namespace {
 
 struct Case {

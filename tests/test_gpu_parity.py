@@ -351,39 +351,6 @@ def test_large_insert_query_no_false_negatives_and_oracle_words(gpu):
     assert (f.words() == w).all()
 
 
-def test_human_scale_sorted_insert_words_equal_oracle(gpu):
-    """The measured workload at its full size: the seeded 3.1 Gbp genome (24 chromosomes,
-    separators, repeats) inserted into a 2^36-bit filter by the sorted insert (the path
-    the library picks for it), then queried against itself. Every valid window is
-    present, and all 2^30 filter words equal the multithreaded oracle's."""
-    import cusbf_amd as cs
-    from cusbf_amd import workloads as W
-
-    torch = _torch()
-    torch.cuda.empty_cache()  # earlier tests' cached blocks would shrink the free memory
-    wl = W.WORKLOADS["human"]
-    genome, _ = W.generate_genome(wl.genome)
-    n = genome.numel()
-    cfg = make_config(31, 28, 16, 4)
-    f = cs.Filter(cfg, int(wl.bits_per_kmer * (n - 30)))
-    assert f.num_shards() == 1 << 28
-    f.set_insert_path(cs.INSERT_SORTED)
-    f.insert_sequence_async(genome)
-    assert f.insert_path().last_path == cs.INSERT_SORTED
-    out = torch.empty(n - 30, dtype=torch.uint8, device="cuda")
-    f.contains_sequence_async(genome, out)
-    valid, _ = f.count_windows_runs(genome)
-    assert cs.count_positive(out) == valid
-    del out
-    host = genome.cpu().numpy()
-    del genome
-    torch.cuda.synchronize()
-    o = make_oracle(31, 28, 16, 4)
-    w = o.new_filter(f.num_shards())
-    o.mt_insert(host, w)
-    assert np.array_equal(f.words(), w)
-
-
 @pytest.mark.parametrize("s", [28, 30, 31])
 def test_fpr_on_seeded_negatives_equals_oracle(gpu, s):
     """The small plain workload's FPR protocol: insert one seeded uniform 10 Mbp
