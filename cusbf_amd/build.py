@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libcusbf_hip.so")
 SYNTH_LIB_PATH = os.path.join(HERE, "libcusbf_synth.so")
 
 HIP_SOURCES = ["kernels_strip.hip", "kernels_strip_batched.hip", "kernels_generic.hip", "sorted_insert.hip", "capi.hip",
-               "serve.hip", "fastx.hip", "persist.hip"]
+               "serve.hip", "fastx.hip", "persist.hip", "noalloc.hip"]
 SYNTH_SOURCES = ["synth.hip", "probe.hip", "baseline.hip"]
 
 # -packed-fp32-ops: the kernels are integer code, and packed FP32 forms stay off.

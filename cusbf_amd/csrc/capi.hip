@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/cusbf_hip.h"
+#include "filter_internal.hpp"
 #include "launch.hpp"
 #include "sbf_common.hpp"
 
@@ -132,6 +133,7 @@ struct cusbf_filter {
     uint64_t* d_starts = nullptr;
     uint64_t starts_cap = 0;  // words
     bool batch_check_pending = false;
+    const uint32_t* batch_check_state = nullptr;  // state words of the last batched call
     uint64_t fastx_chunk_bytes = 0;  // FASTX chunk capacity override (0 = from fill_fraction)
     // insert path (atomic or sorted) and the sorted insert's workspace
     uint32_t insert_path = CUSBF_INSERT_AUTO;
@@ -662,6 +664,34 @@ int set_fastx_error(const char* source, uint32_t line, uint32_t column, const ch
     g_last_column = column;
     return st;
 }
+int contains_batched(const cusbf_filter* f_const, const char* d_symbols, uint64_t num_symbols,
+                     const uint64_t* d_offsets, uint64_t num_records, uint8_t* d_output,
+                     uint64_t d_output_len, uint32_t* d_state, uint64_t* d_starts,
+                     hipStream_t stream) {
+    int rc = validate_batched(f_const, d_symbols, num_symbols, d_offsets, num_records);
+    if (rc != CUSBF_OK) return rc;
+    // the batch scratch is internal state, like the reference's mutable staging
+    cusbf_filter* f = const_cast<cusbf_filter*>(f_const);
+    f->batch_check_pending = false;
+    const uint64_t windows = batched_windows(f, num_symbols);
+    if (windows == 0) return CUSBF_OK;
+    if (!d_output) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device output");
+    if (d_output_len < windows) return fail(CUSBF_ERR_INVALID_ARGUMENT, "batched query output span is too small");
+    if (d_state == nullptr) {  // the filter's own scratch, grown on demand
+        rc = ensure_starts(f, num_symbols);
+        if (rc != CUSBF_OK) return rc;
+        d_state = f->d_batch_state;
+        d_starts = f->d_starts;
+    }
+    rc = enqueue_batched(f, CUSBF_BATCH_CONTAINS, d_symbols, num_symbols, d_offsets, num_records,
+                         d_state, d_starts,
+                         tuning_for(f, CUSBF_BATCH_CONTAINS, num_symbols, num_records), d_output,
+                         stream);
+    f->batch_check_pending = rc == CUSBF_OK;
+    f->batch_check_state = d_state;
+    return rc;
+}
+
 int launch_request_counts(const cusbf_filter* f, const char* d_seq, uint64_t bytes,
                           const uint8_t* d_hits, const uint64_t* d_req, uint64_t n_req,
                           uint64_t* d_counts, hipStream_t stream) {
@@ -774,6 +804,11 @@ int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_
     if (e == hipSuccess) e = hipMalloc(&f->d_scalar, 4 * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&f->d_batch_state, 64);
     if (e == hipSuccess) e = hipMalloc(&f->d_zero, 256);
+    if (e == hipSuccess && cusbf_internal::noalloc_setup(f->device) != CUSBF_OK) {
+        const int st = g_last_status;
+        cusbf_destroy(f);
+        return st;
+    }
     if (e == hipSuccess) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, f->device) == hipSuccess &&
@@ -1329,37 +1364,25 @@ int cusbf_insert_batched_async(cusbf_filter* f, const char* d_symbols, uint64_t 
         rc = end_sorted(f, as_stream(stream));
     }
     f->batch_check_pending = rc == CUSBF_OK;
+    f->batch_check_state = f->d_batch_state;
     return rc;
 }
 
-int cusbf_contains_batched_async(const cusbf_filter* f_const, const char* d_symbols,
+int cusbf_contains_batched_async(const cusbf_filter* f, const char* d_symbols,
                                  uint64_t num_symbols, const uint64_t* d_offsets,
                                  uint64_t num_records, uint8_t* d_output, uint64_t d_output_len,
                                  void* stream) {
-    int rc = validate_batched(f_const, d_symbols, num_symbols, d_offsets, num_records);
-    if (rc != CUSBF_OK) return rc;
-    // the batch scratch is internal state, like the reference's mutable staging
-    cusbf_filter* f = const_cast<cusbf_filter*>(f_const);
-    f->batch_check_pending = false;
-    const uint64_t windows = batched_windows(f, num_symbols);
-    if (windows == 0) return CUSBF_OK;
-    if (!d_output) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device output");
-    if (d_output_len < windows) return fail(CUSBF_ERR_INVALID_ARGUMENT, "batched query output span is too small");
-    rc = ensure_starts(f, num_symbols);
-    if (rc != CUSBF_OK) return rc;
-    rc = enqueue_batched(f, CUSBF_BATCH_CONTAINS, d_symbols, num_symbols, d_offsets, num_records,
-                         f->d_batch_state, f->d_starts,
-                         tuning_for(f, CUSBF_BATCH_CONTAINS, num_symbols, num_records), d_output,
-                         as_stream(stream));
-    f->batch_check_pending = rc == CUSBF_OK;
-    return rc;
+    return cusbf_internal::contains_batched(f, d_symbols, num_symbols, d_offsets, num_records,
+                                            d_output, d_output_len, nullptr, nullptr,
+                                            as_stream(stream));
 }
 
 int cusbf_batched_status(const cusbf_filter* f, void* stream) {
     if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
     CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
     if (!f->batch_check_pending) return CUSBF_OK;
-    return read_batch_flag(f->d_batch_state, as_stream(stream));
+    return read_batch_flag(f->batch_check_state ? f->batch_check_state : f->d_batch_state,
+                           as_stream(stream));
 }
 
 int cusbf_batch_plan_create(cusbf_filter* f, uint32_t op, const char* d_symbols,

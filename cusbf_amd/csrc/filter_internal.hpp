@@ -33,6 +33,17 @@ int launch_request_counts(const cusbf_filter* f, const char* d_seq, uint64_t byt
                           const uint8_t* d_hits, const uint64_t* d_req, uint64_t n_req,
                           uint64_t* d_counts, hipStream_t stream);
 
+// cusbf_contains_batched_async over the given scratch: d_state (16 words) and
+// d_starts (sbf::start_words(num_symbols) words); both null = the filter's own
+// scratch, allocated or grown here.
+int contains_batched(const cusbf_filter* f, const char* d_symbols, uint64_t num_symbols,
+                     const uint64_t* d_offsets, uint64_t num_records, uint8_t* d_output,
+                     uint64_t d_output_len, uint32_t* d_state, uint64_t* d_starts,
+                     hipStream_t stream);
+
+// Looks up the no-allocation scratch of `device` (current) once; cusbf_create calls it.
+int noalloc_setup(int device);
+
 uint32_t kmer_length(const cusbf_filter* f);
 const cusbf_config& config(const cusbf_filter* f);
 // FASTX chunk capacity override in bytes (0 = from fill_fraction).

@@ -153,7 +153,7 @@ hipError_t pack_dense(const KernelConfig& c, const char* seq, uint64_t num_symbo
 
 // Words of the record-start bitmap for num_symbols symbols (one spare word, so
 // a 16-bit lane chunk just past the end still reads inside the bitmap).
-__host__ __device__ inline uint64_t start_words(uint64_t num_symbols) { return num_symbols / 64 + 2; }
+__host__ __device__ constexpr uint64_t start_words(uint64_t num_symbols) { return num_symbols / 64 + 2; }
 
 // Batched form set-up, in stream order before the batched kernels (fill
 // kernels only, so the sequence can be captured in a graph): zero the state

@@ -328,6 +328,23 @@ int cusbf_contains_batched_async(const cusbf_filter* f, const char* d_symbols,
  * offsets were malformed. */
 int cusbf_batched_status(const cusbf_filter* f, void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): cusbf_contains_batched_async with the same arguments and answers,
+ * whose scratch (status words and the record-start bitmap) lives in a fixed 32 MiB
+ * device array of the library instead of buffers the filter allocates. It calls no
+ * allocating routine. Batches of more than CUSBF_NOALLOC_MAX_SYMBOLS symbols fail
+ * with CUSBF_ERR_INVALID_ARGUMENT. Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other threads, other filters) race
+ * and must be ordered by the caller. The filter's device must be current.
+ * cusbf_batched_status reports this call's malformed offsets until the next
+ * batched call. */
+#define CUSBF_NOALLOC_MAX_SYMBOLS ((1ull << 28) - 4096ull)
+int cusbf_contains_batched_noalloc_async(const cusbf_filter* f, const char* d_symbols,
+                                         uint64_t num_symbols, const uint64_t* d_offsets,
+                                         uint64_t num_records, uint8_t* d_output,
+                                         uint64_t d_output_len, void* stream);
+
 /* A batched insert or query captured once into a hipGraph (reset, prepass,
  * persistent kernel) over fixed buffers and replayed with one launch. The plan
  * owns its scratch; `tuning` NULL = the filter's table entry for the batch's
