@@ -645,11 +645,27 @@ __device__ __forceinline__ void zero_slots(WaveLds<FE, Batched>& L, int lane) {
     }
 }
 
+// First window of tile `tile` (wave-uniform), and the first window of the
+// lane's 16 within it. The empty asm statement pins the tile's offset as a
+// scalar register value of its own: otherwise the compiler fuses
+// tile * windows + 16 * lane into one multiply-add whose 64-bit lane operand
+// stays live in a vector register pair for the whole kernel.
+template <int WaveWindows>
+__device__ __forceinline__ uint64_t tile_start(uint64_t tile) {
+    uint64_t t = tile * static_cast<uint64_t>(WaveWindows);
+    asm volatile("" : "+s"(t));
+    return t;
+}
+
+__device__ __forceinline__ uint64_t lane_start(uint64_t tile_first, int lane) {
+    return tile_first + (static_cast<uint32_t>(lane) << 4);
+}
+
 template <int K, int S, int M, int H, class FE, bool Dense, bool Batched>
 __device__ __forceinline__ void stage_tile(const SequenceInput& in, uint64_t tile, int lane,
                                            WaveLds<FE, Batched>& L) {
     using Sh = Shape<K, S, M, H, FE>;
-    const uint64_t first = tile * Sh::kWaveWindows + 16ull * static_cast<uint64_t>(lane);
+    const uint64_t first = lane_start(tile_start<Sh::kWaveWindows>(tile), lane);
     uint32_t starts = 0;
     if constexpr (Batched) starts = record_starts16(in, first);  // issued beside the symbol load
     stage_lane<Dense>(in, first, lane, L.st, L.inv);
@@ -827,17 +843,17 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
                                               uint64_t next = 0, uint64_t num_tiles = 0) {
     using Sh = Shape<K, S, M, H, FE>;
     constexpr int F = Sh::kFindereSpan;
-    const uint64_t tile_first = tile * Sh::kWaveWindows;
+    const uint64_t tile_first = tile_start<Sh::kWaveWindows>(tile);
 
     if constexpr (Prefetch) {
-        stage_lane_raw(*raw, in, tile_first + 16ull * static_cast<uint64_t>(lane), lane, L.st, L.inv);
+        stage_lane_raw(*raw, in, lane_start(tile_first, lane), lane, L.st, L.inv);
     } else {
         stage_tile<K, S, M, H, FE, Dense, Batched>(in, tile, lane, L);
     }
     wave_lds_sync();
 
     const bool owns = lane < Sh::kWindowLanes;
-    const uint64_t first_window = tile_first + 16ull * static_cast<uint64_t>(lane);
+    const uint64_t first_window = lane_start(tile_first, lane);
     LaneWindows<K, S, M, H, FE> lw;
     lane_windows<K, S, M, H, FE, Batched>(L, lane, owns, first_window, num_windows,
                                           static_cast<uint32_t>(filter.num_shards - 1), lw);
@@ -863,7 +879,7 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
     }
     if constexpr (Prefetch) {
         if (next < num_tiles) {
-            *raw = load_raw(in, next * Sh::kWaveWindows + 16ull * static_cast<uint64_t>(lane));
+            *raw = load_raw(in, lane_start(tile_start<Sh::kWaveWindows>(next), lane));
         } else {
             raw->fast = false;
         }
@@ -921,9 +937,9 @@ __device__ __forceinline__ void insert_tile(const SequenceInput& in, const Filte
     using Sh = Shape<K, S, M, H, FE>;
     constexpr int F = Sh::kFindereSpan;
     constexpr uint32_t kSlotsU = static_cast<uint32_t>(FE::kSlots);
-    const uint64_t tile_first = tile * Sh::kWaveWindows;
+    const uint64_t tile_first = tile_start<Sh::kWaveWindows>(tile);
     const bool owns = lane < Sh::kWindowLanes;
-    const uint64_t first_window = tile_first + 16ull * static_cast<uint64_t>(lane);
+    const uint64_t first_window = lane_start(tile_first, lane);
 
     stage_tile<K, S, M, H, FE, Dense, Batched>(in, tile, lane, L);
     uint32_t first = 0;  // first run of the chunk (wave-uniform)
@@ -1008,13 +1024,20 @@ __device__ __forceinline__ void insert_tile(const SequenceInput& in, const Filte
 
 // --- plain kernels: waves stride over the tiles ------------------------------------
 
+// The wave's index in its block, as a wave-uniform (scalar) value: tile indices,
+// region numbers and the loop bounds derived from it then live in scalar
+// registers instead of taking vector registers for the whole kernel.
+__device__ __forceinline__ int wave_in_block() {
+    return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+}
+
 template <int K, int S, int M, int H, class FE, bool Dense, int WPE>
 __global__ __launch_bounds__(kBlock, WPE) void contains_kernel(SequenceInput in, FilterView filter,
                                                                uint64_t num_windows,
                                                                uint64_t num_tiles, uint8_t* out) {
     __shared__ WaveLds<FE, false> lds[kWavesPerBlock];
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const int wave = wave_in_block();
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
     const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wave;
     if constexpr (Dense) {
@@ -1024,7 +1047,9 @@ __global__ __launch_bounds__(kBlock, WPE) void contains_kernel(SequenceInput in,
     } else {
         RawChunk raw;
         raw.fast = false;
-        if (first < num_tiles) raw = load_raw(in, first * Shape<K, S, M, H, FE>::kWaveWindows + 16ull * lane);
+        if (first < num_tiles) {
+            raw = load_raw(in, lane_start(tile_start<Shape<K, S, M, H, FE>::kWaveWindows>(first), lane));
+        }
         for (uint64_t tile = first; tile < num_tiles; tile += stride) {
             contains_tile<K, S, M, H, FE, false, false, true>(in, filter, num_windows, out, tile, lane,
                                                              lds[wave], &raw, tile + stride, num_tiles);
@@ -1039,7 +1064,7 @@ __global__ __launch_bounds__(kBlock, WPE) void insert_kernel(SequenceInput in, F
     __shared__ WaveLds<FE, false> lds[kWavesPerBlock];
     mark_words_written(filter);
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const int wave = wave_in_block();
     const uint32_t region = blockIdx.x * kWavesPerBlock + static_cast<uint32_t>(wave);
     uint32_t emitted = 0;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
@@ -1095,7 +1120,7 @@ __global__ __launch_bounds__(kBlock, Queue ? 2 : WPE) void batched_contains_kern
                                                                        uint32_t claim, uint8_t* out) {
     __shared__ WaveLds<FE, true> lds[kWavesPerBlock];
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const int wave = wave_in_block();
     if (__builtin_amdgcn_readfirstlane(in.state[1]) != 0) return;  // malformed offsets
     init_wave_lds(lds[wave], lane);
     for_each_tile<Queue>(in.state, num_tiles, claim, lane, wave, [&](uint32_t tile) {
@@ -1113,7 +1138,7 @@ __global__ __launch_bounds__(kBlock, Queue ? 2 : WPE) void batched_insert_kernel
     __shared__ WaveLds<FE, true> lds[kWavesPerBlock];
     mark_words_written(filter);
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const int wave = wave_in_block();
     const uint32_t region = blockIdx.x * kWavesPerBlock + static_cast<uint32_t>(wave);
     uint32_t emitted = 0;
     if (__builtin_amdgcn_readfirstlane(in.state[1]) == 0) {  // else malformed offsets: no work

@@ -1,12 +1,13 @@
-"""The lane-strip kernels must not consume register-spill reloads under partial waits.
+"""No kernel of the library spills registers to scratch memory.
 
 Work-queue batched query kernels whose spilled values were reloaded while other
 vector memory operations were still in flight returned different answers from
-run to run on MI355X. The kernels are therefore built with register budgets
-under which they do not spill, or spill only values that are reloaded after a
-full `s_waitcnt vmcnt(0)`. This test compiles the kernel sources for gfx950 to
-assembly (no GPU needed) and checks every kernel: no `s_waitcnt vmcnt(N)` with
-N > 0 may be issued while a scratch reload is outstanding.
+run to run on MI355X. Every kernel is therefore built with a register budget
+under which it does not spill at all. This test compiles the kernel sources for
+gfx950 (no GPU needed) and checks every kernel in two ways: the compiler's
+resource-usage report must give a private segment (ScratchSize) of 0 bytes per
+lane, and the assembly must hold no scratch access (so no `s_waitcnt vmcnt(N)`
+with N > 0 can be issued while a scratch reload is outstanding).
 """
 from __future__ import annotations
 
@@ -20,7 +21,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "cusbf_amd", "csrc")
 SOURCES = ["kernels_strip.hip", "kernels_strip_batched.hip", "kernels_generic.hip",
-           "sorted_insert.hip"]
+           "sorted_insert.hip", "synth.hip", "probe.hip"]
 
 
 def _hipcc():
@@ -54,17 +55,39 @@ def partial_waits_with_reload_pending(body) -> int:
     return bad
 
 
+def scratch_sizes(remarks: str) -> dict:
+    """Kernel name -> ScratchSize [bytes/lane] from -Rpass-analysis=kernel-resource-usage."""
+    sizes, name = {}, None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and name:
+            sizes[name] = int(m.group(1))
+    return sizes
+
+
 @pytest.mark.parametrize("source", SOURCES)
-def test_no_spill_reload_under_partial_wait(source):
+def test_no_kernel_spills(source):
     from cusbf_amd import build
 
     with tempfile.TemporaryDirectory() as tmp:
         out = os.path.join(tmp, "k.s")
         flags = [f for f in build.HIPCC_FLAGS if f not in ("-fPIC", "-Wall")]
-        cmd = [_hipcc(), *flags, "--cuda-device-only", "-S", os.path.join(CSRC, source), "-o", out]
+        cmd = [_hipcc(), *flags, "--cuda-device-only", "-S", os.path.join(CSRC, source), "-o", out,
+               "-Rpass-analysis=kernel-resource-usage"]
         res = subprocess.run(cmd, capture_output=True, text=True)
         assert res.returncode == 0, res.stderr[-3000:]
         asm = open(out).read()
+    sizes = scratch_sizes(res.stderr)
+    kernels = [name for name, _ in kernel_bodies(asm)]
+    assert kernels and set(kernels) <= set(sizes), "resource-usage remarks missing for some kernels"
+    spilling = {name: n for name, n in sizes.items() if n != 0}
+    assert not spilling, spilling
+    scratch = {name for name, body in kernel_bodies(asm)
+               if any(line.startswith(("scratch_", "buffer_store_dword off")) for line in body)}
+    assert not scratch, scratch
     offenders = {name: n for name, body in kernel_bodies(asm)
                  if (n := partial_waits_with_reload_pending(body))}
     assert not offenders, offenders
