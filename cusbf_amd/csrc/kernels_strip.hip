@@ -32,7 +32,9 @@ SBF_STRIP_BATCHED_INSTANCES(SBF_STRIP_EXTERN)
 
 // Compiled configurations: the measured workloads (DNA k=31, s in {28, 30, 31},
 // m=16, h=4; protein k=10, s=8, m=6, h=4), m=21 as in the reference's FPR
-// sweep, h=8, the reference's small test configurations and a mid-size one.
+// sweep, h=8, the protein configuration of the reference's example
+// (examples/main.cu:164: k=12, s=10, m=6, h=4), the reference's small test
+// configurations and a mid-size one.
 // Other configurations (h=12 and h=16 among them: their salted bit addresses
 // do not fit the lane-strip register budget without spilling) run the generic
 // kernels.
@@ -46,6 +48,7 @@ static const StripKernelSet kSets[] = {
     SBF_STRIP_BATCHED(Dna2, 3, 2, 2, 4),
     SBF_STRIP_BATCHED(Dna2, 21, 19, 11, 4),
     SBF_STRIP_BATCHED(Protein5, 10, 8, 6, 4),
+    SBF_STRIP_BATCHED(Protein5, 12, 10, 6, 4),
     SBF_STRIP_BATCHED(Protein5, 5, 4, 3, 4),
 };
 

@@ -404,7 +404,21 @@ __device__ __forceinline__ uint64_t smear(uint64_t x) {
     }
 }
 
-__device__ __forceinline__ uint64_t next_lane(uint64_t v) { return __shfl_down(v, 1, kWave); }
+// Lane exchanges as DPP wave shifts (one VALU move per 32 bits, no LDS access):
+// wave_shl:1 (DPP control 0x130) gives lane i the value of lane i + 1, wave_shr:1
+// (0x138) the value of lane i - 1; the lane at the wave's edge keeps its own value.
+__device__ __forceinline__ uint32_t from_next_lane(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ uint32_t from_prev_lane(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ uint64_t next_lane(uint64_t v) {
+    return static_cast<uint64_t>(from_next_lane(static_cast<uint32_t>(v))) |
+           (static_cast<uint64_t>(from_next_lane(static_cast<uint32_t>(v >> 32))) << 32);
+}
 
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
@@ -607,8 +621,8 @@ __device__ __forceinline__ void lane_windows(const WaveLds<FE, Batched>& L, int 
 template <int K, int S, int M, int H, class FE>
 __device__ __forceinline__ uint32_t run_heads(const LaneWindows<K, S, M, H, FE>& lw, int lane,
                                               uint32_t& cont) {
-    const uint32_t prev_shard = __shfl_up(lw.shard[kLaneWindows - 1], 1, kWave);
-    const uint32_t prev_valid = __shfl_up((lw.valid >> (kLaneWindows - 1)) & 1u, 1, kWave);
+    const uint32_t prev_shard = from_prev_lane(lw.shard[kLaneWindows - 1]);
+    const uint32_t prev_valid = from_prev_lane((lw.valid >> (kLaneWindows - 1)) & 1u);
     uint32_t same = 0;
 #pragma unroll
     for (int o = 1; o < kLaneWindows; ++o) same |= (lw.shard[o] == lw.shard[o - 1] ? 1u : 0u) << o;
@@ -958,26 +972,15 @@ __device__ __forceinline__ void insert_tile(const SequenceInput& in, const Filte
         // bit o: window o is the last window of its run inside this lane
         const uint32_t ends = lw.valid & ~(cont >> 1);
 
-        // each lane ORs the masks of its piece of every run of the chunk into its slot
-        uint64_t ring[F][4];
-#pragma unroll
-        for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
+        // each lane ORs the masks of its piece of every run of the chunk into its
+        // slot. A run's mask is the OR of its windows' masks and window o's mask is
+        // the OR of s-mers o .. o+F-1, so the accumulator restarts at each run head
+        // and takes every window's whole mask (no branch on run heads).
         uint64_t acc[4] = {0, 0, 0, 0};
+        auto take = [&](int o, const uint64_t (&full)[4]) {
+            const uint64_t keep = 0ull - static_cast<uint64_t>((cont >> o) & 1u);
 #pragma unroll
-        for (int o = 0; o < kLaneWindows; ++o) {
-            code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
-            if ((cont >> o) & 1u) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] |= ring[(o + F - 1) % F][q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    uint64_t x = 0;
-#pragma unroll
-                    for (int d = 0; d < F; ++d) x |= ring[(o + d) % F][q];
-                    acc[q] = x;
-                }
-            }
+            for (int q = 0; q < 4; ++q) acc[q] = (acc[q] & keep) | full[q];
             if ((ends >> o) & 1u) {
                 const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u;
                 if (slot < kSlotsU) {
@@ -987,6 +990,49 @@ __device__ __forceinline__ void insert_tile(const SequenceInput& in, const Filte
                                               __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                 }
+            }
+        };
+        if constexpr (F == 4) {
+            // pair masks as in the query: pair[p % 3] = mask(p) | mask(p + 1), so a
+            // window's mask is the OR of two pairs
+            uint64_t last[4], pair[3][4], m1[4], m2[4];
+            code_masks<H>(lw.code[0], last);
+            code_masks<H>(lw.code[1], m1);
+            code_masks<H>(lw.code[2], m2);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                pair[0][q] = last[q] | m1[q];
+                pair[1][q] = m1[q] | m2[q];
+                last[q] = m2[q];
+            }
+#pragma unroll
+            for (int o = 0; o < kLaneWindows; ++o) {
+                uint64_t nm[4], full[4];
+                code_masks<H>(lw.code[o + 3], nm);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    pair[(o + 2) % 3][q] = last[q] | nm[q];
+                    last[q] = nm[q];
+                    full[q] = pair[o % 3][q] | pair[(o + 2) % 3][q];
+                }
+                take(o, full);
+            }
+        } else {
+            uint64_t ring[F][4];
+#pragma unroll
+            for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
+#pragma unroll
+            for (int o = 0; o < kLaneWindows; ++o) {
+                code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
+                uint64_t full[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint64_t x = 0;
+#pragma unroll
+                    for (int d = 0; d < F; ++d) x |= ring[(o + d) % F][q];
+                    full[q] = x;
+                }
+                take(o, full);
             }
         }
         wave_lds_sync();
@@ -1290,6 +1336,7 @@ constexpr uint32_t wave_windows() {
     X(Dna2, 31, 28, 16, 4, 3) X(Dna2, 31, 30, 16, 4, 3)                       \
     X(Dna2, 31, 31, 16, 4, 3) X(Dna2, 21, 19, 11, 4, 3)                       \
     X(Dna2, 5, 4, 3, 4, 3) X(Dna2, 3, 2, 2, 4, 3)                             \
-    X(Protein5, 10, 8, 6, 4, 3) X(Protein5, 5, 4, 3, 4, 3)
+    X(Protein5, 10, 8, 6, 4, 3) X(Protein5, 12, 10, 6, 4, 3)                 \
+    X(Protein5, 5, 4, 3, 4, 3)
 
 }  // namespace sbf::strip

@@ -115,6 +115,7 @@ CASES = [
     (5, 4, 3, 4, "dna", 40, 1.5, 3000, "lane_strip"),
     (3, 2, 2, 4, "dna", 20, 1.5, 3000, "lane_strip"),
     (10, 8, 6, 4, "protein", 300, 1.5, 2000, "lane_strip"),
+    (12, 10, 6, 4, "protein", 300, 1.5, 2000, "lane_strip"),
     (5, 4, 3, 4, "protein", 40, 1.5, 3000, "lane_strip"),
     (31, 28, 16, 8, "dna", 300, 1.5, 1000, "lane_strip"),  # plain lane-strip, generic batched
     (17, 15, 9, 4, "dna", 200, 1.5, 1000, "generic"),

@@ -66,6 +66,7 @@ CONFIGS = [
     (5, 4, 3, 4, "dna"),      # ~500 runs per tile: two slot chunks
     (3, 2, 2, 4, "dna"),      # ~660 runs per tile: three slot chunks
     (10, 8, 6, 4, "protein"),  # ~330 runs per tile: mostly one chunk of 384 slots
+    (12, 10, 6, 4, "protein"),  # the reference example's protein configuration
     (5, 4, 3, 4, "protein"),   # ~500 runs per tile: two chunks of 384 slots
 ]
 
