@@ -235,9 +235,53 @@ def nematode_scenario(steps: int, warmup: int, fpr_kmers: int) -> dict:
         f"{b}: " + " / ".join(f"{fpr['m21'][b][f's{x}']:.4f}" for x in (28, 30, 31))
         for b in ("21.4", "85.7", "342.6")))
     res["fpr_percent"] = fpr
+    res["param_sweep"] = param_sweep_points(genome, q[:genome.numel()], hits, sh)
     del q, hits, genome
     torch.cuda.empty_cache()
     return res
+
+
+# Points of the reference's (s, m, h) grid (benchmarks/param_sweep.cu:263-300, k = 31),
+# corners and middle: the lane-strip kernels cover k - s + 1 <= 8 and k - m + 1 <= 17
+# with h = 4 or 8; the rest runs the generic kernels.
+SWEEP_POINTS = [(31, 28, 16, 8), (31, 16, 8, 4), (31, 20, 12, 8), (31, 24, 16, 12), (31, 28, 16, 16),
+                (31, 28, 24, 4), (31, 30, 20, 8), (31, 31, 31, 4)]
+
+
+def param_sweep_points(genome, query, hits, sh, reps: int = 3) -> list:
+    """Insert the nematode genome and query a uniform sequence of its length at each
+    sweep point, filter at 21.4 bits per k-mer; medians of `reps` HIP-event timings."""
+    import torch
+
+    import cusbf_amd as cs
+
+    stream = torch.cuda.current_stream()
+    out = []
+    n_ins = genome.numel() - 30
+    n_q = query.numel() - 30
+    for k, s_, m_, h_ in SWEEP_POINTS:
+        f = cs.Filter(cs.Config(k, s_, m_, h_), int(21.4 * n_ins))
+        ti, tq = [], []
+        for _ in range(reps):
+            f.clear(sh)
+            ev = _events(3)
+            ev[0].record(stream)
+            f.insert_sequence_async(genome, stream=sh)
+            ev[1].record(stream)
+            f.contains_sequence_async(query, hits[:n_q], stream=sh)
+            ev[2].record(stream)
+            ev[2].synchronize()
+            ti.append(ev[0].elapsed_time(ev[1]) / 1e3)
+            tq.append(ev[1].elapsed_time(ev[2]) / 1e3)
+        row = {"k": k, "s": s_, "m": m_, "h": h_, "kernels": f.kernel_family(),
+               "insert_path": {1: "atomic", 2: "sorted"}[f.insert_path().last_path],
+               "insert_kmers_per_s": n_ins / statistics.median(ti),
+               "query_kmers_per_s": n_q / statistics.median(tq)}
+        out.append(row)
+        f.close()
+        log(f"[sweep] k={k} s={s_} m={m_} h={h_} {row['kernels']}: insert "
+            f"{row['insert_kmers_per_s'] / 1e9:.1f}, query {row['query_kmers_per_s'] / 1e9:.1f} G k-mers/s")
+    return out
 
 
 def dense_scenario(genome, wl, stream_handle, host_bases: int, reps: int = 3) -> dict:
@@ -432,7 +476,8 @@ def serving_scenario(filt, genome, reads, wl, n_requests: int, qps: float):
     def pct(x, q):
         return float(np.percentile(np.asarray(x), q)) * 1e3
 
-    def run_server(burst: bool):
+    def run_server(burst: bool, arr=None):
+        arr = arrivals if arr is None else arr
         lat = [0.0] * n_requests
         kmers = [0] * n_requests
         positives = [0] * n_requests
@@ -444,10 +489,10 @@ def serving_scenario(filt, genome, reads, wl, n_requests: int, qps: float):
             def submitter():
                 for i, (seq, rg) in enumerate(reqs):
                     if not burst:
-                        dt = t0 + arrivals[i] - time.perf_counter()
+                        dt = t0 + arr[i] - time.perf_counter()
                         if dt > 0:
                             time.sleep(dt)
-                        lag[i] = max(0.0, time.perf_counter() - (t0 + arrivals[i]))
+                        lag[i] = max(0.0, time.perf_counter() - (t0 + arr[i]))
                     pend[i] = srv.submit(seq, rg, out=outs[i])
                     ready[i].set()
 
@@ -490,7 +535,22 @@ def serving_scenario(filt, genome, reads, wl, n_requests: int, qps: float):
     b_lat, b_km, b_pos, b_span, b_st = run_server(True)
     pb_lat, pb_km, pb_pos, pb_span = run_plain(True)
     log(f"[serve] burst: server {sum(b_km) / b_span / 1e9:.2f} G k-mers/s, plain "
-        f"{sum(pb_km) / pb_span / 1e9:.2f} G k-mers/s")
+        f"{sum(pb_km) / pb_span / 1e9:.2f} G k-mers/s; span {b_span:.2f} s: fill "
+        f"{b_st['fill_seconds']:.2f} s, launch {b_st['launch_seconds']:.2f} s, device "
+        f"{b_st['device_seconds']:.2f} s, answer copies {b_st['copy_seconds']:.2f} s")
+    # load sweep: Poisson arrivals at fractions of the saturation rate (the burst's
+    # requests per second)
+    sat_qps = n_requests / b_span
+    unit = np.cumsum(np.random.default_rng(4244).exponential(1.0, n_requests))
+    sweep = []
+    for frac in (0.25, 0.5, 0.8):
+        q_lat, q_km, q_pos, q_span, _ = run_server(False, unit / (frac * sat_qps))
+        identical_q = q_pos == b_pos
+        sweep.append({"fraction_of_saturation": frac, "qps": frac * sat_qps,
+                      "p50_ms": pct(q_lat, 50), "p99_ms": pct(q_lat, 99),
+                      "kmers_per_s": sum(q_km) / q_span, "answers_identical": bool(identical_q)})
+        log(f"[serve] {frac:.2f} x saturation ({frac * sat_qps:.0f} QPS): p50 {pct(q_lat, 50):.1f} ms "
+            f"p99 {pct(q_lat, 99):.1f} ms")
     identical = (s_pos == p_pos == b_pos == pb_pos) and (s_km == p_km)
     return {
         "requests": n_requests, "qps": qps, "bytes": total_bytes,
@@ -501,7 +561,12 @@ def serving_scenario(filt, genome, reads, wl, n_requests: int, qps: float):
         "burst_plain_kmers_per_s": sum(pb_km) / pb_span,
         "burst_speedup": (sum(b_km) / b_span) / max(1e-9, sum(pb_km) / pb_span),
         "server_batches": s_st["batches"], "server_max_requests_per_batch": s_st["max_requests_per_batch"],
+        "burst_span_s": b_span,
+        "burst_stage_seconds": {k: b_st[k] for k in ("fill_seconds", "launch_seconds",
+                                                     "device_seconds", "copy_seconds")},
         "answers_identical": bool(identical),
+        "saturation_qps": sat_qps,
+        "load_sweep": sweep,
         "staging_bytes": 16 << 30, "slots": 4,
     }
 

@@ -48,7 +48,9 @@ class CServerOptions(ctypes.Structure):
 
 class CServerStats(ctypes.Structure):
     _fields_ = [("requests", u64), ("batches", u64), ("max_requests_per_batch", u64),
-                ("bytes", u64), ("windows", u64)]
+                ("bytes", u64), ("windows", u64), ("fill_seconds", ctypes.c_double),
+                ("launch_seconds", ctypes.c_double), ("device_seconds", ctypes.c_double),
+                ("copy_seconds", ctypes.c_double)]
 
 
 class CQueryReport(ctypes.Structure):

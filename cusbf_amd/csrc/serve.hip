@@ -13,11 +13,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -54,6 +56,7 @@ constexpr uint64_t kMaxBatchRequests = 4096;
 
 struct Slot {
     hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr;  // recorded before the batch's host-to-device copy
     hipEvent_t done = nullptr;
     char* d_seq = nullptr;
     uint8_t* d_hits = nullptr;
@@ -69,27 +72,81 @@ struct Slot {
     std::vector<Request*> batch;
 };
 
-// Large copies run on several threads (host memory bandwidth needs more than one core).
-void parallel_copy(void* dst, const void* src, uint64_t bytes) {
-    constexpr uint64_t kChunk = 32ull << 20;
-    const uint64_t parts = std::min<uint64_t>(8, (bytes + kChunk - 1) / kChunk);
-    if (parts <= 1) {
-        std::memcpy(dst, src, bytes);
-        return;
+// Host copies (batch fill, answer copies) need more than one core's memory
+// bandwidth. A persistent pool of worker threads runs the parts of each copy job;
+// the thread that submits a job works on it too and returns when every part is
+// done. The scheduler and the completion thread submit jobs concurrently.
+class WorkerPool {
+   public:
+    explicit WorkerPool(unsigned threads) {
+        for (unsigned i = 0; i < threads; ++i) workers_.emplace_back([this] { work(); });
     }
-    const uint64_t per = (bytes + parts - 1) / parts;
-    std::vector<std::thread> th;
-    for (uint64_t t = 1; t < parts; ++t) {
-        const uint64_t off = t * per;
-        if (off >= bytes) break;
-        th.emplace_back([=] {
-            std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off,
-                        std::min(per, bytes - off));
-        });
+    ~WorkerPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread& t : workers_) t.join();
     }
-    std::memcpy(dst, src, std::min(per, bytes));
-    for (std::thread& t : th) t.join();
-}
+    // Runs fn(i) for every i in [0, n).
+    void run(size_t n, const std::function<void(size_t)>& fn) {
+        if (n == 0) return;
+        auto job = std::make_shared<Job>();
+        job->fn = &fn;
+        job->n = n;
+        if (n > 1) {
+            std::lock_guard<std::mutex> lk(mu_);
+            jobs_.push_back(job);
+        }
+        cv_.notify_all();
+        drain(*job);
+        std::unique_lock<std::mutex> lk(job->mu);
+        job->cv.wait(lk, [&] { return job->finished == job->n; });
+    }
+
+   private:
+    struct Job {
+        const std::function<void(size_t)>* fn = nullptr;
+        size_t n = 0;
+        std::atomic<size_t> next{0};
+        size_t finished = 0;  // guarded by mu
+        std::mutex mu;
+        std::condition_variable cv;
+    };
+    static void drain(Job& job) {
+        for (;;) {
+            const size_t i = job.next.fetch_add(1);
+            if (i >= job.n) return;
+            (*job.fn)(i);
+            std::lock_guard<std::mutex> lk(job.mu);
+            if (++job.finished == job.n) job.cv.notify_all();
+        }
+    }
+    void work() {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
+            if (jobs_.empty()) return;  // stopping
+            std::shared_ptr<Job> job = jobs_.front();
+            if (job->next.load() >= job->n) {  // every part taken: retire the job
+                jobs_.pop_front();
+                continue;
+            }
+            lk.unlock();
+            drain(*job);
+            lk.lock();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::shared_ptr<Job>> jobs_;
+    std::vector<std::thread> workers_;
+    bool stop_ = false;
+};
+
+// Copy parts of at most 4 MiB.
+constexpr uint64_t kCopyPart = 4ull << 20;
 
 // Separator padding of normalized_record_batch.hpp:208-232.
 uint64_t boundary_bytes(uint64_t current, uint32_t width) {
@@ -124,67 +181,111 @@ struct cusbf_server {
     bool stop = false;
     bool scheduler_done = false;
     cusbf_server_stats stats{};
+    std::unique_ptr<WorkerPool> pool;  // host copies of the batch fill and the answers
     std::thread scheduler, completer;
 };
 
 namespace {
 
+// True when the request's records are laid out in its buffer exactly as in its
+// normalised batch apart from the separator bytes: each record starts one symbol
+// after the previous one ends. Such a request is copied as one span, and only the
+// symbol between records is overwritten with the separator.
+// (Empty records are excluded: a leading empty record writes no separator.)
+bool span_layout(const Request& r, uint32_t width) {
+    for (size_t i = 0; i < r.ranges.size(); ++i) {
+        if (r.ranges[i].sequence_bytes == 0) return false;
+        if (i > 0 && r.ranges[i].sequence_offset !=
+                         r.ranges[i - 1].sequence_offset + r.ranges[i - 1].sequence_bytes + width) {
+            return false;
+        }
+    }
+    return !r.ranges.empty();
+}
+
 // Writes the slot's batch: requests one after another, each request's records
 // joined by the separator (padded to the symbol width), a separator between
 // requests. Offsets are laid out first; the copies then run on up to eight
-// threads over balanced groups of records.
+// threads over balanced groups of pieces. A request in span layout is one piece
+// (cut into 16 MiB parts so that large requests spread over the threads) whose
+// copy is followed by separator stores between its records.
 void fill_batch(cusbf_server* s, Slot& slot) {
     struct Piece {
         uint64_t pad_at;  // separator bytes written at [pad_at, pad_at + pad)
         uint64_t pad;
         const char* src;
         uint64_t len;
+        const Request* span;  // span piece: patch the separators between its records
+        uint64_t span_base;   // batch offset of the request's first record
     };
+    constexpr uint64_t kPart = kCopyPart;
     std::vector<Piece> pieces;
     uint64_t pos = 0;
     for (Request* r : slot.batch) {
         const uint64_t b = boundary_bytes(pos, s->width);
-        pieces.push_back(Piece{pos, b, nullptr, 0});
+        pieces.push_back(Piece{pos, b, nullptr, 0, nullptr, 0});
         pos += b;
         r->batch_offset = pos;
+        if (span_layout(*r, s->width)) {
+            const char* src = r->seq + r->ranges.front().sequence_offset;
+            for (uint64_t done = 0; done < r->norm_bytes; done += kPart) {
+                const uint64_t len = std::min(kPart, r->norm_bytes - done);
+                pieces.push_back(Piece{pos + done, 0, src + done, len, r, pos});
+            }
+            pos += r->norm_bytes;
+            continue;
+        }
         uint64_t local = 0;
         for (const cusbf_record_range& rr : r->ranges) {
             const uint64_t lb = boundary_bytes(local, s->width);
-            pieces.push_back(Piece{pos, lb, r->seq + rr.sequence_offset, rr.sequence_bytes});
+            pieces.push_back(Piece{pos, lb, r->seq + rr.sequence_offset, rr.sequence_bytes, nullptr, 0});
             pos += lb + rr.sequence_bytes;
             local += lb + rr.sequence_bytes;
         }
     }
     slot.batch_bytes = pos;
     char* out = slot.h_seq;
+    const uint32_t width = s->width;
+    const char sep = s->sep;
+    auto patch = [&](const Piece& p) {
+        // separator symbols of the request that fall inside this part
+        const Request& r = *p.span;
+        const uint64_t first = r.ranges.front().sequence_offset;
+        const uint64_t lo = p.pad_at - p.span_base;
+        const uint64_t hi = lo + p.len;
+        // record i ends at ranges[i].offset + bytes - first (relative); its separator follows
+        auto it = std::lower_bound(r.ranges.begin(), r.ranges.end(), lo,
+                                   [&](const cusbf_record_range& x, uint64_t v) {
+                                       return x.sequence_offset + x.sequence_bytes + width - first <= v;
+                                   });
+        for (; it != r.ranges.end() && it + 1 != r.ranges.end(); ++it) {
+            const uint64_t at = it->sequence_offset + it->sequence_bytes - first;
+            if (at >= hi) break;
+            for (uint32_t j = 0; j < width; ++j) {
+                if (at + j >= lo && at + j < hi) out[p.span_base + at + j] = sep;
+            }
+        }
+    };
     auto run = [&](size_t first, size_t last) {
         for (size_t i = first; i < last; ++i) {
             const Piece& p = pieces[i];
-            if (p.pad) std::memset(out + p.pad_at, s->sep, p.pad);
+            if (p.pad) std::memset(out + p.pad_at, sep, p.pad);
             if (p.len) std::memcpy(out + p.pad_at + p.pad, p.src, p.len);
+            if (p.span) patch(p);
         }
     };
-    constexpr uint64_t kPerThread = 16ull << 20;
-    const uint64_t threads = std::min<uint64_t>(8, pos / kPerThread + 1);
-    if (threads <= 1 || pieces.size() < 2) {
-        run(0, pieces.size());
-        return;
-    }
-    std::vector<std::thread> th;
-    size_t first = 0;
-    uint64_t done_bytes = 0;
-    for (uint64_t t = 1; t <= threads && first < pieces.size(); ++t) {
-        const uint64_t goal = pos * t / threads;
-        size_t last = first;
-        while (last < pieces.size() && (done_bytes < goal || last == first)) {
-            done_bytes += pieces[last].pad + pieces[last].len;
-            ++last;
+    // units of consecutive pieces holding about kCopyPart bytes each, run on the pool
+    std::vector<size_t> unit_start{0};
+    uint64_t unit_bytes = 0;
+    for (size_t i = 0; i < pieces.size(); ++i) {
+        unit_bytes += pieces[i].pad + pieces[i].len;
+        if (unit_bytes >= kCopyPart && i + 1 < pieces.size()) {
+            unit_start.push_back(i + 1);
+            unit_bytes = 0;
         }
-        if (t == threads) last = pieces.size();
-        th.emplace_back(run, first, last);
-        first = last;
     }
-    for (std::thread& t : th) t.join();
+    unit_start.push_back(pieces.size());
+    s->pool->run(unit_start.size() - 1, [&](size_t u) { run(unit_start[u], unit_start[u + 1]); });
 }
 
 void launch_batch(cusbf_server* s, Slot& slot) {
@@ -195,7 +296,10 @@ void launch_batch(cusbf_server* s, Slot& slot) {
         slot.h_req[2 * i] = slot.batch[i]->batch_offset / s->width;
         slot.h_req[2 * i + 1] = slot.batch[i]->windows;
     }
-    hipError_t e = hipMemcpyAsync(slot.d_seq, slot.h_seq, slot.batch_bytes, hipMemcpyHostToDevice, slot.stream);
+    hipError_t e = hipEventRecord(slot.start, slot.stream);
+    if (e == hipSuccess) {
+        e = hipMemcpyAsync(slot.d_seq, slot.h_seq, slot.batch_bytes, hipMemcpyHostToDevice, slot.stream);
+    }
     if (e == hipSuccess) {
         e = hipMemcpyAsync(slot.d_req, slot.h_req, 2 * n_req * sizeof(uint64_t), hipMemcpyHostToDevice,
                            slot.stream);
@@ -258,9 +362,14 @@ void scheduler_loop(cusbf_server* s) {
         s->stats.max_requests_per_batch = std::max<uint64_t>(s->stats.max_requests_per_batch, slot.batch.size());
         s->stats.bytes += bytes;
         lk.unlock();
+        const auto t0 = Clock::now();
         fill_batch(s, slot);
+        const auto t1 = Clock::now();
         launch_batch(s, slot);
+        const auto t2 = Clock::now();
         lk.lock();
+        s->stats.fill_seconds += std::chrono::duration<double>(t1 - t0).count();
+        s->stats.launch_seconds += std::chrono::duration<double>(t2 - t1).count();
         s->completing.push_back(idx);
         s->cv_complete.notify_one();
     }
@@ -278,29 +387,59 @@ void completion_loop(cusbf_server* s) {
         s->completing.pop_front();
         Slot& slot = s->slots[idx];
         lk.unlock();
+        float device_ms = 0.0f;
         if (slot.status == CUSBF_OK) {
-            const hipError_t e = hipEventSynchronize(slot.done);
+            hipError_t e = hipEventSynchronize(slot.done);
+            if (e == hipSuccess) e = hipEventElapsedTime(&device_ms, slot.start, slot.done);
             if (e != hipSuccess) {
                 slot.status = CUSBF_ERR_DEVICE;
                 slot.message = hipGetErrorString(e);
             }
         }
-        const auto now = Clock::now();
+        const auto copy_start = Clock::now();
         uint64_t windows = 0;
+        struct Part {
+            uint8_t* dst;
+            const uint8_t* src;
+            uint64_t len;
+        };
+        std::vector<Part> parts;
         for (size_t i = 0; i < slot.batch.size(); ++i) {
             Request* r = slot.batch[i];
             r->status = slot.status;
             r->message = slot.message;
             if (slot.status == CUSBF_OK && r->windows) {
-                parallel_copy(r->hits, slot.h_hits + r->batch_offset / s->width, r->windows);
+                const uint8_t* src = slot.h_hits + r->batch_offset / s->width;
+                for (uint64_t done = 0; done < r->windows; done += kCopyPart) {
+                    parts.push_back(Part{r->hits + done, src + done, std::min(kCopyPart, r->windows - done)});
+                }
                 r->report.queried_kmers = slot.h_counts[2 * i];
                 r->report.positive_kmers = slot.h_counts[2 * i + 1];
             }
             windows += r->windows;
-            r->latency = std::chrono::duration<double>(now - r->submitted).count();
         }
+        // small parts are grouped so that one task copies about kCopyPart bytes
+        std::vector<size_t> task_start{0};
+        uint64_t task_bytes = 0;
+        for (size_t j = 0; j < parts.size(); ++j) {
+            task_bytes += parts[j].len;
+            if (task_bytes >= kCopyPart && j + 1 < parts.size()) {
+                task_start.push_back(j + 1);
+                task_bytes = 0;
+            }
+        }
+        task_start.push_back(parts.size());
+        s->pool->run(task_start.size() - 1, [&](size_t t) {
+            for (size_t j = task_start[t]; j < task_start[t + 1]; ++j) {
+                std::memcpy(parts[j].dst, parts[j].src, parts[j].len);
+            }
+        });
+        const auto now = Clock::now();
+        for (Request* r : slot.batch) r->latency = std::chrono::duration<double>(now - r->submitted).count();
         lk.lock();
         s->stats.windows += windows;
+        s->stats.device_seconds += device_ms * 1e-3;
+        s->stats.copy_seconds += std::chrono::duration<double>(now - copy_start).count();
         for (Request* r : slot.batch) r->done = true;
         slot.batch.clear();
         s->free_slots.push_back(idx);
@@ -313,6 +452,7 @@ void free_slots(cusbf_server* s) {
     for (Slot& slot : s->slots) {
         if (slot.stream) (void)hipStreamDestroy(slot.stream);
         if (slot.done) (void)hipEventDestroy(slot.done);
+        if (slot.start) (void)hipEventDestroy(slot.start);
         if (slot.d_seq) (void)hipFree(slot.d_seq);
         if (slot.d_hits) (void)hipFree(slot.d_hits);
         if (slot.h_seq) (void)hipHostFree(slot.h_seq);
@@ -345,7 +485,8 @@ int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* optio
     s->slots.resize(options->slots);
     for (Slot& slot : s->slots) {
         hipError_t e = hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&slot.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreate(&slot.start);
+        if (e == hipSuccess) e = hipEventCreate(&slot.done);
         if (e == hipSuccess) e = hipMalloc(&slot.d_seq, s->seq_cap);
         if (e == hipSuccess) e = hipMalloc(&slot.d_hits, s->seq_cap);
         if (e == hipSuccess) e = hipHostMalloc(&slot.h_seq, s->seq_cap, hipHostMallocDefault);
@@ -361,6 +502,8 @@ int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* optio
         }
     }
     for (int i = static_cast<int>(options->slots) - 1; i >= 0; --i) s->free_slots.push_back(i);
+    const unsigned hw = std::thread::hardware_concurrency();
+    s->pool = std::make_unique<WorkerPool>(std::min(12u, std::max(2u, hw > 2 ? hw - 2 : 2u)));
     cusbf_server* raw = s.release();
     raw->scheduler = std::thread(scheduler_loop, raw);
     raw->completer = std::thread(completion_loop, raw);

@@ -972,15 +972,26 @@ __device__ __forceinline__ void insert_tile(const SequenceInput& in, const Filte
         // bit o: window o is the last window of its run inside this lane
         const uint32_t ends = lw.valid & ~(cont >> 1);
 
-        // each lane ORs the masks of its piece of every run of the chunk into its
-        // slot. A run's mask is the OR of its windows' masks and window o's mask is
-        // the OR of s-mers o .. o+F-1, so the accumulator restarts at each run head
-        // and takes every window's whole mask (no branch on run heads).
-        uint64_t acc[4] = {0, 0, 0, 0};
-        auto take = [&](int o, const uint64_t (&full)[4]) {
-            const uint64_t keep = 0ull - static_cast<uint64_t>((cont >> o) & 1u);
+        // each lane ORs the masks of its piece of every run of the chunk into its slot
+        uint64_t ring[F][4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = (acc[q] & keep) | full[q];
+        for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
+        uint64_t acc[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int o = 0; o < kLaneWindows; ++o) {
+            code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
+            if ((cont >> o) & 1u) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] |= ring[(o + F - 1) % F][q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint64_t x = 0;
+#pragma unroll
+                    for (int d = 0; d < F; ++d) x |= ring[(o + d) % F][q];
+                    acc[q] = x;
+                }
+            }
             if ((ends >> o) & 1u) {
                 const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u;
                 if (slot < kSlotsU) {
@@ -990,49 +1001,6 @@ __device__ __forceinline__ void insert_tile(const SequenceInput& in, const Filte
                                               __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                 }
-            }
-        };
-        if constexpr (F == 4) {
-            // pair masks as in the query: pair[p % 3] = mask(p) | mask(p + 1), so a
-            // window's mask is the OR of two pairs
-            uint64_t last[4], pair[3][4], m1[4], m2[4];
-            code_masks<H>(lw.code[0], last);
-            code_masks<H>(lw.code[1], m1);
-            code_masks<H>(lw.code[2], m2);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                pair[0][q] = last[q] | m1[q];
-                pair[1][q] = m1[q] | m2[q];
-                last[q] = m2[q];
-            }
-#pragma unroll
-            for (int o = 0; o < kLaneWindows; ++o) {
-                uint64_t nm[4], full[4];
-                code_masks<H>(lw.code[o + 3], nm);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    pair[(o + 2) % 3][q] = last[q] | nm[q];
-                    last[q] = nm[q];
-                    full[q] = pair[o % 3][q] | pair[(o + 2) % 3][q];
-                }
-                take(o, full);
-            }
-        } else {
-            uint64_t ring[F][4];
-#pragma unroll
-            for (int p = 0; p < F - 1; ++p) code_masks<H>(lw.code[p], ring[p]);
-#pragma unroll
-            for (int o = 0; o < kLaneWindows; ++o) {
-                code_masks<H>(lw.code[o + F - 1], ring[(o + F - 1) % F]);
-                uint64_t full[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    uint64_t x = 0;
-#pragma unroll
-                    for (int d = 0; d < F; ++d) x |= ring[(o + d) % F][q];
-                    full[q] = x;
-                }
-                take(o, full);
             }
         }
         wave_lds_sync();

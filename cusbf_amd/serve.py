@@ -122,7 +122,9 @@ class Server:
         check(self._lib, self._lib.cusbf_server_stats_get(self._h, ctypes.byref(st)))
         return {"requests": st.requests, "batches": st.batches,
                 "max_requests_per_batch": st.max_requests_per_batch, "bytes": st.bytes,
-                "windows": st.windows}
+                "windows": st.windows, "fill_seconds": st.fill_seconds,
+                "launch_seconds": st.launch_seconds, "device_seconds": st.device_seconds,
+                "copy_seconds": st.copy_seconds}
 
 
 def query_record_batch_hits(filt: Filter, sequence, ranges, stream=None,

@@ -477,6 +477,14 @@ typedef struct cusbf_server_stats {
     uint64_t max_requests_per_batch;
     uint64_t bytes;
     uint64_t windows;
+    /* Where a batch's time goes, summed over batches: the scheduler's normalisation
+     * into pinned staging and its enqueue of copies and kernels (host), the batch's
+     * copies and kernels on its stream (device, from events), and the completion
+     * thread's copies of answers into the callers' buffers (host). */
+    double fill_seconds;
+    double launch_seconds;
+    double device_seconds;
+    double copy_seconds;
 } cusbf_server_stats;
 
 int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* options,

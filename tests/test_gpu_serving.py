@@ -84,6 +84,46 @@ def test_each_request_same_alone_and_in_the_mix(loaded):
     assert st["max_requests_per_batch"] > 1
 
 
+def _gapped_requests(genome: bytes, n: int, seed: int):
+    """Requests whose records sit one byte apart in their buffer (reads or long records
+    with a gap byte between them, which the server copies as one span and patches with
+    separators); the gap byte is sometimes a valid symbol, so a missing separator would
+    change the answers of the windows across it."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        count = int(np.exp(rng.uniform(0, np.log(5000))))
+        lens = (np.full(count, 150) if rng.random() < 0.5
+                else rng.integers(31, 30000, max(1, count // 200)))
+        parts, ranges, off = [], [], 0
+        for ln in lens.tolist():
+            st = int(rng.integers(0, len(genome) - ln))
+            rec = genome[st:st + ln] if rng.random() < 0.7 else random_dna(ln, int(rng.integers(1 << 30)))
+            if parts:
+                parts.append(b"ACGTN"[int(rng.integers(0, 5))].to_bytes(1, "little"))
+                off += 1
+            parts.append(rec)
+            ranges.append((off, ln))
+            off += ln
+        reqs.append((b"".join(parts), ranges))
+    return reqs
+
+
+def test_span_layout_requests_alone_and_in_the_mix(loaded):
+    import cusbf_amd as cs
+
+    f, genome = loaded
+    reqs = _gapped_requests(genome, 40, seed=11) + _requests(genome, 10, seed=12)
+    alone = [cs.query_record_batch_hits(f, s, r) for s, r in reqs]
+    with cs.Server(f, staging_bytes=64 << 20, slots=3) as srv:
+        pend = [srv.submit(*r) for r in reqs]
+        for i, p in enumerate(pend):
+            hits, rep, _ = srv.wait(p)
+            want_hits, want_rep = alone[i]
+            assert np.array_equal(hits, want_hits), f"request {i} differs in the mix"
+            assert rep == want_rep
+
+
 def test_hits_match_plain_query_of_normalized_request(loaded):
     import cusbf_amd as cs
 
