@@ -47,6 +47,9 @@ def parse_args():
                          "reference's FPR densities on its genome")
     ap.add_argument("--fpr-kmers", type=int, default=1_000_000_000,
                     help="uniform random query k-mers of the FPR densities sweep")
+    ap.add_argument("--placement-check", type=int, default=0,
+                    help="after the timed steps, time the query on this many further filters "
+                         "of the same size, each allocated while the earlier ones are kept")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--probes", default="on", choices=["on", "off"],
@@ -968,6 +971,25 @@ def main() -> int:
     t_pop = statistics.median(pop)
     log(f"[bench] insert into the populated filter: {t_pop * 1e3:.2f} ms")
 
+    placement = None
+    if args.placement_check:
+        # the same insert and query on further filters, each allocated while the earlier
+        # ones are kept (their pages land elsewhere in device memory)
+        placement = [{"filter": 0, "insert_ms": ti * 1e3, "query_ms": tq * 1e3}]
+        extra = []
+        for i in range(args.placement_check):
+            g = cs.Filter(cfg, int(wl.bits_per_kmer * genome_windows))
+            g.set_insert_path(INSERT_PATHS[args.insert_path])
+            gi, gq = timed_insert_query(g, genome, reads, out, sh, max(3, args.steps), 1)
+            placement.append({"filter": i + 1, "insert_ms": gi * 1e3, "query_ms": gq * 1e3})
+            log(f"[bench] placement: filter {i + 1}: insert {gi * 1e3:.2f} ms, query {gq * 1e3:.2f} ms")
+            extra.append(g)
+        filt.contains_sequence_async(reads, out, stream=sh)  # the headline filter's answers again
+        for g in extra:
+            g.close()
+        del extra
+        torch.cuda.empty_cache()
+
     positives = int(cs.count_positive(out))
     ins_rate = v_ins / ti
     q_rate = v_q / tq
@@ -1055,6 +1077,8 @@ def main() -> int:
     }
     if nematode is not None:
         result["nematode"] = nematode
+    if placement is not None:
+        result["placement_check"] = placement
     if serving is not None:
         result["serve_p50_ms"] = serving["server_p50_ms"]
         result["serve_p99_ms"] = serving["server_p99_ms"]

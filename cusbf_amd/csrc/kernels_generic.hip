@@ -127,26 +127,58 @@ __device__ __forceinline__ bool batch_rejected(const SequenceInput& in) {
 
 // Blocks stride over tiles of kBlock windows: a dispatch must stay below 2^32
 // work-items, so grids are capped and large inputs loop.
+//
+// Insert: consecutive lanes hold consecutive windows, so a run (consecutive
+// valid windows with one shard) is a segment of the wave. A segmented OR scan
+// over the wave (kernels.cuh:143-166 does the same per warp) leaves each run's
+// masks in its last lane, which issues one 64-bit atomic OR per non-zero word:
+// about one update per run instead of one per window.
 __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, SequenceInput in,
                                                         FilterView f, uint64_t num_windows) {
     __shared__ uint8_t tile[kBlock + kMaxK];
     mark_words_written(f);
     if (batch_rejected(in)) return;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
     for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
          first += static_cast<uint64_t>(gridDim.x) * kBlock) {
         __syncthreads();
         load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, tile);
         __syncthreads();
         const uint64_t j = first + threadIdx.x;
-        if (j >= num_windows) continue;
-        const Window w = read_window(c, tile + threadIdx.x);
-        if (!window_ok(in, w, j, c.k)) continue;
-        const uint32_t shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
-        uint64_t m[4];
-        masks(c, w.packed, m);
-        uint64_t* dst = f.words + 4ull * shard;
-        for (int q = 0; q < 4; ++q) {
-            if (m[q] != 0) __hip_atomic_fetch_or(dst + q, m[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool valid = false;
+        uint32_t shard = 0;
+        uint64_t m[4] = {0, 0, 0, 0};
+        if (j < num_windows) {
+            const Window w = read_window(c, tile + threadIdx.x);
+            if (window_ok(in, w, j, c.k)) {
+                valid = true;
+                shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
+                masks(c, w.packed, m);
+            }
+        }
+        // run heads and the lane of this lane's run head
+        const uint32_t prev_shard = __shfl_up(shard, 1, 64);
+        const int prev_valid = __shfl_up(valid ? 1 : 0, 1, 64);
+        const bool head = valid && (lane == 0 || !prev_valid || prev_shard != shard);
+        const uint64_t heads = __ballot(head);
+        const uint64_t valids = __ballot(valid);
+        const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const int start = upto ? 63 - __clzll(static_cast<long long>(upto)) : 0;
+        for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint64_t t = __shfl_up(m[q], d, 64);
+                if (lane - d >= start) m[q] |= t;
+            }
+        }
+        // the run's last lane applies its masks
+        const bool tail = valid && (lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0 ||
+                                    ((valids >> (lane + 1)) & 1ull) == 0);
+        if (tail) {
+            uint64_t* dst = f.words + 4ull * shard;
+            for (int q = 0; q < 4; ++q) {
+                if (m[q] != 0) __hip_atomic_fetch_or(dst + q, m[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
 }

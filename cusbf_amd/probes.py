@@ -21,7 +21,10 @@ def lib():
         L.cusbf_probe_copy.argtypes = [P, P, u64, P]
         L.cusbf_probe_gather.argtypes = [P, u64, u64, u64, P, P]
         L.cusbf_probe_atomic.argtypes = [P, u64, u64, u64, P]
-        for f in (L.cusbf_probe_copy, L.cusbf_probe_gather, L.cusbf_probe_atomic):
+        L.cusbf_probe_gather_form.argtypes = [P, u64, u64, u64, P, ctypes.c_int, P]
+        L.cusbf_probe_alloc.argtypes = [u64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        for f in (L.cusbf_probe_copy, L.cusbf_probe_gather, L.cusbf_probe_atomic,
+                  L.cusbf_probe_gather_form, L.cusbf_probe_alloc):
             f.restype = ctypes.c_int
         _probe = L
     return _probe
@@ -65,3 +68,78 @@ def random_sector_update(buf_ptr: int, num_sectors: int, accesses: int, reps: in
     """Random 32-byte sector updates (four 64-bit atomic ORs) per second."""
     t = _time(lambda i: lib().cusbf_probe_atomic(buf_ptr, num_sectors, accesses, 2000 + i, 0), reps)
     return accesses / t
+
+
+LOAD_FORMS = ("plain", "nontemporal", "atomic_agent", "atomic_system")
+MEMORY_KINDS = ("hipMalloc", "fine-grained", "uncached", "contiguous")
+
+
+def alloc(nbytes: int, kind: int) -> int:
+    """Device buffer of a memory kind (MEMORY_KINDS index); kept until the process ends."""
+    p = ctypes.c_void_p()
+    rc = lib().cusbf_probe_alloc(nbytes, kind, ctypes.byref(p))
+    if rc != 0:
+        raise RuntimeError(f"probe allocation failed (hip error {rc})")
+    return p.value
+
+
+def gather_form(buf_ptr: int, num_sectors: int, accesses: int, mode: int, reps: int = 3) -> list:
+    """Random 32-byte sector reads per second in load form `mode`, one rate per repetition."""
+    import torch
+
+    sink = torch.zeros(1, dtype=torch.int32, device="cuda")
+    rates = []
+    for i in range(reps + 1):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        rc = lib().cusbf_probe_gather_form(buf_ptr, num_sectors, accesses, 300 + 17 * i + mode,
+                                           sink.data_ptr(), mode, 0)
+        b.record()
+        if rc != 0:
+            raise RuntimeError(f"probe launch failed (hip error {rc})")
+        torch.cuda.synchronize()
+        if i:
+            rates.append(accesses / (a.elapsed_time(b) / 1e3))
+    return rates
+
+
+def main(argv=None) -> int:
+    """python -m cusbf_amd.probes forms <memory kind> | placement
+    forms: the gather in every load form on a 4 GiB buffer of one memory kind;
+    placement: plain load form on seven 8 GiB buffers made one after another."""
+    import argparse
+    import json
+
+    import torch  # noqa: F401  (one HIP runtime per process)
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["forms", "placement"])
+    ap.add_argument("kind", nargs="?", type=int, default=0)
+    ap.add_argument("--accesses", type=int, default=1_000_000_000)
+    a = ap.parse_args(argv)
+    if a.what == "forms":
+        sectors = 1 << 27
+        buf = alloc(sectors * 32, a.kind)
+        for mode, name in enumerate(LOAD_FORMS):
+            r = gather_form(buf, sectors, a.accesses, mode)
+            print(json.dumps({"memory": MEMORY_KINDS[a.kind], "load": name,
+                              "G_sectors_per_s": [round(x / 1e9, 2) for x in r]}), flush=True)
+    else:
+        G = 1 << 30
+        plan = [("hipMalloc, first", 0, 0), ("hipMalloc, second", 0, 0),
+                ("9 GiB, 1 GiB-aligned window", 0, 1), ("contiguous", 3, 0),
+                ("hipMalloc, third", 0, 0), ("10 GiB, 2 GiB-aligned window", 0, 2),
+                ("hipMalloc, fourth", 0, 0)]
+        for name, kind, align in plan:
+            base = alloc(8 * G + align * G, kind)
+            ptr = base if align == 0 else (base + align * G - 1) & ~(align * G - 1)
+            r = gather_form(ptr, (8 * G) // 32, a.accesses, 0)
+            print(json.dumps({"allocation": name, "G_sectors_per_s": [round(x / 1e9, 2) for x in r]}),
+                  flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(main())
