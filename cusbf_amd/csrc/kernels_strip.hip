@@ -34,9 +34,9 @@ SBF_STRIP_BATCHED_INSTANCES(SBF_STRIP_EXTERN)
 // m=16, h=4; protein k=10, s=8, m=6, h=4), m=21 as in the reference's FPR
 // sweep, h=8, the protein configuration of the reference's example
 // (examples/main.cu:164: k=12, s=10, m=6, h=4), the reference's small test
-// configurations and a mid-size one.
-// Other configurations (h=12 and h=16 among them: their salted bit addresses
-// do not fit the lane-strip register budget without spilling) run the generic
+// configurations and a mid-size one; h = 8, 12 and 16 (the salted bit
+// addresses of h = 12 and 16 keep each s-mer's hash and build its masks when
+// they are needed) with plain kernels. Other configurations run the generic
 // kernels.
 static const StripKernelSet kSets[] = {
     SBF_STRIP_BATCHED(Dna2, 31, 28, 16, 4),
@@ -44,6 +44,8 @@ static const StripKernelSet kSets[] = {
     SBF_STRIP_BATCHED(Dna2, 31, 31, 16, 4),
     SBF_STRIP_PLAIN(Dna2, 31, 28, 21, 4),
     SBF_STRIP_PLAIN(Dna2, 31, 28, 16, 8),
+    SBF_STRIP_PLAIN(Dna2, 31, 28, 16, 12),
+    SBF_STRIP_PLAIN(Dna2, 31, 28, 16, 16),
     SBF_STRIP_BATCHED(Dna2, 5, 4, 3, 4),
     SBF_STRIP_BATCHED(Dna2, 3, 2, 2, 4),
     SBF_STRIP_BATCHED(Dna2, 21, 19, 11, 4),

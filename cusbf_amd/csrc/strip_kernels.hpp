@@ -44,6 +44,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "launch.hpp"
 #include "sbf_common.hpp"
@@ -467,6 +468,30 @@ __device__ __forceinline__ void code_masks(const PosCode<H>& c, uint64_t (&m)[4]
     for (int i = 0; i < H; ++i) m[i & 3] |= 1ull << code_pos<H>(c, i);
 }
 
+// What a lane keeps per s-mer: the bit positions when they come from slices of the
+// hash (64 / H >= 6: H = 4 or 8); the s-mer hash itself when they come from salted
+// multiplies (H = 12 or 16), whose 6-bit positions would take more registers than
+// the hash and are computed when the masks are built.
+template <int H>
+using SmerCode = typename std::conditional<(64 / H >= 6), PosCode<H>, uint64_t>::type;
+
+template <int H>
+__device__ __forceinline__ SmerCode<H> smer_code(uint64_t h) {
+    if constexpr (64 / H >= 6) {
+        return pos_code<H>(h);
+    } else {
+        return h;
+    }
+}
+
+// filter_impl.cuh:36-47 salted multiply-shift and :54-77 masks, from the s-mer hash.
+template <int H>
+__device__ __forceinline__ void code_masks(const uint64_t& h, uint64_t (&m)[4]) {
+    m[0] = m[1] = m[2] = m[3] = 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) m[i & 3] |= 1ull << ((h * salt(static_cast<uint32_t>(i))) >> 58);
+}
+
 
 // --- per-wave LDS ---------------------------------------------------------------
 //
@@ -529,7 +554,7 @@ template <int K, int S, int M, int H, class FE>
 struct LaneWindows {
     uint32_t shard[kLaneWindows];
     uint32_t valid;  // bit o: window o valid and in range
-    PosCode<H> code[kLaneWindows + Shape<K, S, M, H, FE>::kFindereSpan - 1];
+    SmerCode<H> code[kLaneWindows + Shape<K, S, M, H, FE>::kFindereSpan - 1];
 };
 
 template <int K, int S, int M, int H, class FE, bool Batched>
@@ -606,12 +631,12 @@ __device__ __forceinline__ void lane_windows(const WaveLds<FE, Batched>& L, int 
     uint64_t sv[kLaneWindows];
     substring_values<S>(ctx, sv);
 #pragma unroll
-    for (int p = 0; p < kLaneWindows; ++p) lw.code[p] = pos_code<H>(hash64(sv[p]));
+    for (int p = 0; p < kLaneWindows; ++p) lw.code[p] = smer_code<H>(hash64(sv[p]));
     // s-mers 16 .. 16 + F - 2 (the first ones of the next lane's span) lie in
     // this lane's context too: computed here, no lane exchange
 #pragma unroll
     for (int i = 0; i < F - 1; ++i) {
-        lw.code[kLaneWindows + i] = pos_code<H>(hash64(substring_at<S>(ctx, kLaneWindows + i)));
+        lw.code[kLaneWindows + i] = smer_code<H>(hash64(substring_at<S>(ctx, kLaneWindows + i)));
     }
 }
 

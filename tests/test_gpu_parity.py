@@ -114,7 +114,7 @@ def test_clear_and_idempotence(gpu):
 @pytest.mark.parametrize("h", [4, 16])
 def test_load_factor_equals_oracle_popcount(gpu, h):
     """load_factor (filter.cuh:810-823) == float(popcount of the oracle's words) /
-    float(filter bits): the lane-strip kernels (h = 4) and the generic ones (h = 16)."""
+    float(filter bits): sliced (h = 4) and salted (h = 16) bit addresses."""
     from oracle import image_oracle as im
 
     cfg = make_config(31, 28, 16, h)
@@ -144,11 +144,12 @@ def test_host_and_device_query_agree(gpu):
 # --- seeded random parity, bit-exact words and answers ------------------------------------------
 
 FAST = [("dna", 31, 28, 16, 4), ("dna", 31, 30, 16, 4), ("dna", 31, 31, 16, 4),
-        ("dna", 31, 28, 21, 4), ("dna", 31, 28, 16, 8), ("dna", 5, 4, 3, 4), ("dna", 3, 2, 2, 4),
+        ("dna", 31, 28, 21, 4), ("dna", 31, 28, 16, 8), ("dna", 31, 28, 16, 12),
+        ("dna", 31, 28, 16, 16), ("dna", 5, 4, 3, 4), ("dna", 3, 2, 2, 4),
         ("dna", 21, 19, 11, 4), ("protein", 10, 8, 6, 4), ("protein", 12, 10, 6, 4),
         ("protein", 5, 4, 3, 4)]
-GENERIC = [("dna", 27, 25, 13, 4), ("dna", 32, 20, 32, 8), ("dna", 31, 28, 16, 12),
-           ("dna", 31, 28, 16, 16), ("protein", 12, 8, 5, 16), ("triplet", 7, 5, 4, 4),
+GENERIC = [("dna", 27, 25, 13, 4), ("dna", 32, 20, 32, 8), ("dna", 31, 24, 16, 12),
+           ("dna", 31, 28, 18, 16), ("protein", 12, 8, 5, 16), ("triplet", 7, 5, 4, 4),
            ("triplet", 10, 9, 3, 12), ("custom_xyz", 12, 10, 6, 4), ("custom_xyz", 20, 14, 9, 8)]
 
 

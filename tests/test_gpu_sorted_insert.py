@@ -62,6 +62,7 @@ CONFIGS = [
     (31, 31, 16, 4, "dna"),
     (31, 28, 21, 4, "dna"),
     (31, 28, 16, 8, "dna"),
+    (31, 28, 16, 16, "dna"),  # salted bit addresses
     (21, 19, 11, 4, "dna"),
     (5, 4, 3, 4, "dna"),      # ~500 runs per tile: two slot chunks
     (3, 2, 2, 4, "dna"),      # ~660 runs per tile: three slot chunks
