@@ -10,11 +10,14 @@
 //  * random sector update: four 64-bit atomic ORs into one random 32-byte
 //    sector per access (the insert's per-run update);
 //  * the gather in other load forms (nontemporal, coherent atomic loads) and on
-//    other memory kinds, to see what an HBM read request costs for each.
+//    other memory kinds, to see what an HBM read request costs for each;
+//  * VALU issue cost: how many cycles one wave64 instruction of a given kind
+//    holds its SIMD (the kernels' ALU time is priced with these).
 // Sector indices come from a counter-based hash, so every run is reproducible.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 
 namespace {
 
@@ -108,6 +111,80 @@ __global__ __launch_bounds__(kBlock) void atomic_kernel(uint64_t* words, uint64_
     }
 }
 
+// VALU issue cost: each lane runs kValuIters rounds of eight independent chains
+// of one instruction (inline asm on registers only, so nothing else is issued in
+// the loop but its counter and branch); time x clock x SIMDs / wave instructions
+// is the number of cycles one wave64 instruction holds its SIMD.
+constexpr int kValuIters = 4096;
+constexpr int kValuOps = 23;
+
+#define SBF_CHAIN8(STMT) STMT(0) STMT(1) STMT(2) STMT(3) STMT(4) STMT(5) STMT(6) STMT(7)
+#define SBF_VALU32(OP, ASM, ...)                                         \
+    if constexpr (Op == OP) {                                            \
+        _Pragma("unroll") for (int j = 0; j < 8; ++j) asm volatile(ASM : "+v"(a[j]) : __VA_ARGS__); \
+    }
+#define SBF_VALU64(OP, ASM, ...)                                         \
+    if constexpr (Op == OP) {                                            \
+        _Pragma("unroll") for (int j = 0; j < 8; ++j) asm volatile(ASM : "+v"(b[j]) : __VA_ARGS__); \
+    }
+
+template <int Op>
+__global__ __launch_bounds__(kBlock) void valu_kernel(uint32_t x, uint32_t y, uint64_t* sink) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t c64 = (static_cast<uint64_t>(y) << 32) | x;
+    uint32_t a[8];
+    uint64_t b[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        a[j] = t * (2u * j + 3u);
+        b[j] = (static_cast<uint64_t>(a[j]) << 32) | (t + j);
+    }
+    for (int i = 0; i < kValuIters; ++i) {
+        SBF_VALU32(0, "v_add_u32 %0, %1, %0", "v"(x))
+        SBF_VALU32(1, "v_or_b32 %0, %1, %0", "v"(x))
+        SBF_VALU32(2, "v_and_b32 %0, %1, %0", "v"(x))
+        SBF_VALU32(3, "v_xor_b32 %0, %1, %0", "v"(x))
+        SBF_VALU32(4, "v_lshrrev_b32 %0, %1, %0", "v"(x))
+        SBF_VALU32(5, "v_lshlrev_b32 %0, %1, %0", "v"(x))
+        SBF_VALU32(6, "v_bitop3_b32 %0, %1, %0, %2 bitop3:0x32", "v"(x), "v"(y))
+        SBF_VALU32(7, "v_or3_b32 %0, %1, %0, %2", "v"(x), "v"(y))
+        SBF_VALU32(8, "v_add3_u32 %0, %1, %0, %2", "v"(x), "v"(y))
+        SBF_VALU32(9, "v_min_u32 %0, %1, %0", "v"(x))
+        SBF_VALU32(10, "v_bfe_u32 %0, %0, %1, 6", "v"(x))
+        SBF_VALU32(11, "v_alignbit_b32 %0, %1, %0, 7", "v"(x))
+        SBF_VALU32(12, "v_perm_b32 %0, %1, %0, %2", "v"(x), "v"(y))
+        SBF_VALU32(13, "v_lshl_or_b32 %0, %0, 2, %1", "v"(x))
+        SBF_VALU32(14, "v_mul_lo_u32 %0, %1, %0", "v"(x))
+        SBF_VALU32(15, "v_mul_hi_u32 %0, %1, %0", "v"(x))
+        SBF_VALU64(16, "v_mad_u64_u32 %0, vcc, %1, %2, %0", "v"(x), "v"(y) : "vcc")
+        SBF_VALU64(17, "v_lshlrev_b64 %0, %1, %0", "v"(x))
+        SBF_VALU64(18, "v_lshl_add_u64 %0, %0, 2, %1", "v"(c64))
+        SBF_VALU64(19, "v_cmp_lt_u64_e64 vcc, %0, %1", "v"(c64) : "vcc")
+        SBF_VALU32(20, "v_mov_b32_dpp %0, %0 wave_shl:1 row_mask:0xf bank_mask:0xf", "v"(x))
+        SBF_VALU32(21, "v_bcnt_u32_b32 %0, %1, %0", "v"(x))
+        SBF_VALU32(22, "v_mov_b32 %0, %1", "v"(x))
+    }
+    uint64_t r = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r ^= a[j] ^ b[j];
+    if (r == 0x9E3779B97F4A7C15ull) sink[0] = r;  // keeps the chains; practically never stores
+}
+
+#undef SBF_VALU64
+#undef SBF_VALU32
+#undef SBF_CHAIN8
+
+template <int Op>
+void launch_valu(uint32_t blocks, uint64_t* sink, hipStream_t s) {
+    hipLaunchKernelGGL(valu_kernel<Op>, dim3(blocks), dim3(kBlock), 0, s, 0x9E3779B9u, 0x7F4A7C15u, sink);
+}
+
+template <int... Ops>
+void launch_valu_op(int op, uint32_t blocks, uint64_t* sink, hipStream_t s,
+                    std::integer_sequence<int, Ops...>) {
+    ((op == Ops ? launch_valu<Ops>(blocks, sink, s) : void()), ...);
+}
+
 uint32_t grid_of(uint64_t items, uint64_t per_thread) {
     uint64_t blocks = (items / per_thread + kBlock - 1) / kBlock;
     if (blocks < 1) blocks = 1;
@@ -170,6 +247,47 @@ int cusbf_probe_alloc(uint64_t bytes, int kind, void** out) {
     }
     if (e == hipSuccess) e = hipMemset(*out, 0x5A, bytes);
     if (e == hipSuccess) e = hipDeviceSynchronize();
+    return static_cast<int>(e);
+}
+
+// Cycles one wave64 instruction of VALU kind `op` (0 .. 22; the names are listed in
+// cusbf_amd/probes.py) holds its SIMD, with `waves_per_simd` waves on every SIMD of
+// the device: the second of two identical launches, timed with events (synchronous).
+int cusbf_probe_valu(int op, int waves_per_simd, double* cycles) {
+    if (op < 0 || op >= kValuOps || waves_per_simd < 1 || waves_per_simd > 8 || cycles == nullptr) {
+        return static_cast<int>(hipErrorInvalidValue);
+    }
+    int device = 0, cus = 0, clock_khz = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeClockRate, device);
+    if (e != hipSuccess) return static_cast<int>(e);
+    // 256-thread blocks are four waves: CUs x waves_per_simd blocks put that many on each SIMD
+    const uint32_t blocks = static_cast<uint32_t>(cus) * static_cast<uint32_t>(waves_per_simd);
+    uint64_t* sink = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    e = hipMalloc(&sink, sizeof(uint64_t));
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    if (e == hipSuccess) {
+        const auto ops = std::make_integer_sequence<int, kValuOps>{};
+        launch_valu_op(op, blocks, sink, nullptr, ops);
+        e = hipEventRecord(t0, nullptr);
+        if (e == hipSuccess) launch_valu_op(op, blocks, sink, nullptr, ops);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(t1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e == hipSuccess) {
+            const double wave_instructions = static_cast<double>(blocks) * (kBlock / 64) * kValuIters * 8.0;
+            const double simd_cycles = ms * 1e-3 * clock_khz * 1e3 * (4.0 * cus);
+            *cycles = simd_cycles / wave_instructions;
+        }
+    }
+    if (t0 != nullptr) (void)hipEventDestroy(t0);
+    if (t1 != nullptr) (void)hipEventDestroy(t1);
+    if (sink != nullptr) (void)hipFree(sink);
     return static_cast<int>(e);
 }
 

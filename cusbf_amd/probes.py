@@ -1,7 +1,8 @@
-"""Memory-system ceilings (probe.hip): stream copy, random 32-byte gather, random sector update.
+"""Ceilings the kernels are measured against (probe.hip): stream copy, random 32-byte
+gather, random sector update, and the VALU issue cost of single instructions.
 
-Each function runs its probe `reps` times on the current stream and returns the
-median rate measured with HIP events.
+Each memory probe runs `reps` times on the current stream and returns the median
+rate measured with HIP events.
 """
 from __future__ import annotations
 
@@ -23,8 +24,9 @@ def lib():
         L.cusbf_probe_atomic.argtypes = [P, u64, u64, u64, P]
         L.cusbf_probe_gather_form.argtypes = [P, u64, u64, u64, P, ctypes.c_int, P]
         L.cusbf_probe_alloc.argtypes = [u64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.cusbf_probe_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         for f in (L.cusbf_probe_copy, L.cusbf_probe_gather, L.cusbf_probe_atomic,
-                  L.cusbf_probe_gather_form, L.cusbf_probe_alloc):
+                  L.cusbf_probe_gather_form, L.cusbf_probe_alloc, L.cusbf_probe_valu):
             f.restype = ctypes.c_int
         _probe = L
     return _probe
@@ -103,21 +105,45 @@ def gather_form(buf_ptr: int, num_sectors: int, accesses: int, mode: int, reps: 
     return rates
 
 
+# Instruction kinds of cusbf_probe_valu, in its numbering.
+VALU_OPS = ["v_add_u32", "v_or_b32", "v_and_b32", "v_xor_b32", "v_lshrrev_b32", "v_lshlrev_b32",
+            "v_bitop3_b32", "v_or3_b32", "v_add3_u32", "v_min_u32", "v_bfe_u32", "v_alignbit_b32",
+            "v_perm_b32", "v_lshl_or_b32", "v_mul_lo_u32", "v_mul_hi_u32", "v_mad_u64_u32",
+            "v_lshlrev_b64", "v_lshl_add_u64", "v_cmp_lt_u64", "v_mov_b32_dpp", "v_bcnt_u32_b32",
+            "v_mov_b32"]
+
+
+def valu_cycles(op: str, waves_per_simd: int = 8) -> float:
+    """Cycles one wave64 instruction of kind `op` holds its SIMD (eight independent
+    chains per lane, `waves_per_simd` waves on every SIMD)."""
+    out = ctypes.c_double(0.0)
+    rc = lib().cusbf_probe_valu(VALU_OPS.index(op), waves_per_simd, ctypes.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"valu probe failed (hip error {rc})")
+    return out.value
+
+
 def main(argv=None) -> int:
-    """python -m cusbf_amd.probes forms <memory kind> | placement
+    """python -m cusbf_amd.probes forms <memory kind> | placement | valu
     forms: the gather in every load form on a 4 GiB buffer of one memory kind;
-    placement: plain load form on seven 8 GiB buffers made one after another."""
+    placement: plain load form on seven 8 GiB buffers made one after another;
+    valu: issue cycles of each instruction kind at 8 and 3 waves per SIMD."""
     import argparse
     import json
 
     import torch  # noqa: F401  (one HIP runtime per process)
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["forms", "placement"])
+    ap.add_argument("what", choices=["forms", "placement", "valu"])
     ap.add_argument("kind", nargs="?", type=int, default=0)
     ap.add_argument("--accesses", type=int, default=1_000_000_000)
     a = ap.parse_args(argv)
-    if a.what == "forms":
+    if a.what == "valu":
+        for op in VALU_OPS:
+            c8 = [round(valu_cycles(op, 8), 2) for _ in range(3)]
+            c3 = [round(valu_cycles(op, 3), 2) for _ in range(3)]
+            print(json.dumps({"instruction": op, "cycles_8_waves": c8, "cycles_3_waves": c3}), flush=True)
+    elif a.what == "forms":
         sectors = 1 << 27
         buf = alloc(sectors * 32, a.kind)
         for mode, name in enumerate(LOAD_FORMS):
