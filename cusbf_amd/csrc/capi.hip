@@ -787,6 +787,12 @@ int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_
     f->kc.symbol_bits = d.symbol_bits;
     f->kc.invalid_symbol = d.invalid_symbol;
     std::memcpy(f->kc.lut, cfg->lut, 256);
+    // test hooks: CUSBF_FORCE_GENERIC=1 skips the lane-strip kernels;
+    // CUSBF_GENERIC_FORM=direct|hashed fixes the generic kernels' form
+    const char* form = std::getenv("CUSBF_GENERIC_FORM");
+    f->kc.form = form && std::strcmp(form, "direct") == 0   ? sbf::kFormDirect
+                 : form && std::strcmp(form, "hashed") == 0 ? sbf::kFormHashed
+                                                            : sbf::kFormAuto;
     const char* force_generic = std::getenv("CUSBF_FORCE_GENERIC");
     if (!(force_generic && force_generic[0] == '1')) {
         f->strip = sbf::find_strip_kernels(cfg->k, cfg->s, cfg->m, cfg->hash_count, cfg->alphabet);

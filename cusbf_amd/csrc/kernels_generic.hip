@@ -1,12 +1,15 @@
 // kernels_generic.hip -- Super Bloom kernels for any configuration and alphabet,
 // plus the packing and reduction kernels.
 //
-// One window per thread over an LDS tile of encoded symbols, computed the way
-// the reference computes each window (include/cusbf/detail/kernels.cuh:40-172:
-// pack the k-mer, minimum of the k-m+1 m-mer hashes, k-s+1 s-mer hashes).
-// Configurations without lane-strip kernels (triplets, custom alphabets, other
-// k/s/m/h, dense protein input) run here; the lane-strip kernels are checked
-// against these and against the CPU oracle.
+// One window per thread over an LDS tile of encoded symbols. The insert and
+// query kernels hash every m-mer and s-mer of the tile once and build sparse
+// tables over them in LDS (hashed_tile below), so a window's minimizer and
+// findere masks cost two table reads each instead of the k-m+1 m-mer and k-s+1
+// s-mer hashes the reference computes per window (include/cusbf/detail/
+// kernels.cuh:40-172); the diagnostic run counter recomputes each window the
+// reference's way. Configurations without lane-strip kernels (triplets, custom
+// alphabets, other k/s/m/h, dense protein input) run here; the lane-strip
+// kernels are checked against these and against the CPU oracle.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -125,6 +128,181 @@ __device__ __forceinline__ bool batch_rejected(const SequenceInput& in) {
     return in.state != nullptr && in.state[1] != 0;
 }
 
+// --- hashed tiles -------------------------------------------------------------
+//
+// A block's tile is kBlock windows, thread t owning window first + t. The block
+// hashes each of the tile's m-mers and s-mers once (sequence_kmer.cuh:40-59
+// hashes k-m+1 m-mers and k-s+1 s-mers per window), keeps the m-mer hashes and
+// the s-mers' word masks (filter_impl.cuh:54-77) in LDS, and builds sparse
+// tables over them: level l holds the minimum (for the masks: the OR) over 2^l
+// consecutive positions. A window's minimizer hash is the minimum of two
+// overlapping level-L entries, L = floor(log2(k - m + 1)), and its findere masks
+// are the OR of two level-L' entries, L' = floor(log2(k - s + 1)); min and OR are
+// idempotent, so the overlap does not change either. A bit per position marks
+// invalid symbols (and positions past the input), and a window is valid when
+// its k bits are clear.
+constexpr int kTileSymbols = kBlock + kMaxK;  // the tile's windows and the k - 1 symbols after them
+static_assert(kTileSymbols % 64 == 0, "the invalid bits are ballots of whole waves");
+
+struct HashedTile {
+    uint8_t sym[kTileSymbols];
+    uint64_t invalid[kTileSymbols / 64];  // bit p of the tile: symbol p invalid or past the input
+    uint64_t mh[2][kTileSymbols];         // m-mer hashes, two sparse-table levels in turn
+    uint64_t mk[2][4][kTileSymbols];      // s-mer masks (word q of position p at [.][q][p])
+};
+
+// filter_common.cuh:27-93: the 16 salts, as a table indexed by a loop counter.
+__constant__ const uint64_t kSaltTable[16] = {salt(0),  salt(1),  salt(2),  salt(3), salt(4),  salt(5),
+                                              salt(6),  salt(7),  salt(8),  salt(9), salt(10), salt(11),
+                                              salt(12), salt(13), salt(14), salt(15)};
+
+__device__ __forceinline__ uint32_t floor_log2(uint32_t x) { return 31u - static_cast<uint32_t>(__builtin_clz(x)); }
+
+// sequence_kmer.cuh:74-82: len symbols packed MSB-first.
+__device__ __forceinline__ uint64_t pack_symbols(const KernelConfig& c, const uint8_t* p, uint32_t len) {
+    const uint64_t sym_mask = (1ull << c.symbol_bits) - 1ull;
+    uint64_t v = 0;
+    for (uint32_t i = 0; i < len; ++i) v = (v << c.symbol_bits) | (p[i] & sym_mask);
+    return v;
+}
+
+// Windows [first, first + kBlock): thread t learns whether window first + t is
+// valid and in range, its shard and its masks. Every thread of the block calls.
+__device__ __forceinline__ void hashed_tile(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
+                            uint64_t first, uint64_t num_windows, HashedTile& T, bool& valid,
+                            uint32_t& shard, uint64_t (&m)[4]) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t count = kBlock + c.k - 1;  // symbols the tile's windows cover
+    __syncthreads();                          // the previous tile's readers are done
+    load_tile(c, in, static_cast<int64_t>(first), static_cast<int>(count), T.sym);
+    __syncthreads();
+
+    const uint32_t span_m = c.k - c.m + 1;    // m-mers per window
+    const uint32_t span_s = c.k - c.s + 1;    // s-mers per window
+    const uint32_t pm = kBlock + span_m - 1;  // m-mer positions of the tile
+    const uint32_t ps = kBlock + span_s - 1;  // s-mer positions of the tile
+    const uint32_t both = pm < ps ? pm : ps;
+    const uint32_t longest = c.m > c.s ? c.m : c.s;
+    // level 0: one hash per m-mer, one mask per s-mer (positions p < pm or ps
+    // read symbols p .. p + len - 1 <= count - 1)
+    for (uint32_t p = t; p < static_cast<uint32_t>(kTileSymbols); p += kBlock) {
+        const bool bad = p >= count || T.sym[p] == static_cast<uint8_t>(c.invalid_symbol);
+        const uint64_t ballot = __ballot(bad);  // the loop is uniform per wave
+        if ((p & 63u) == 0) T.invalid[p >> 6] = ballot;
+        uint64_t vm = 0, vs = 0;
+        if (p < both) {
+            const uint64_t v = pack_symbols(c, T.sym + p, longest);
+            vm = v >> (c.symbol_bits * (longest - c.m));
+            vs = v >> (c.symbol_bits * (longest - c.s));
+        } else if (p < pm) {
+            vm = pack_symbols(c, T.sym + p, c.m);
+        } else if (p < ps) {
+            vs = pack_symbols(c, T.sym + p, c.s);
+        }
+        if (p < pm) T.mh[0][p] = minimizer_hash64(vm);
+        if (p < ps) {
+            // kernels.cuh:294-304 with filter_impl.cuh:54-77, one s-mer: hash i
+            // sets a bit of word i % 4 (the hash count is a multiple of 4)
+            const uint64_t h = hash64(vs);
+            const uint32_t slice = 64u / c.h;
+            uint64_t w[4] = {0, 0, 0, 0};
+            for (uint32_t g = 0; g < (c.h >> 2); ++g) {
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    const uint32_t i = 4 * g + q;
+                    const uint32_t pos = slice >= 6u ? static_cast<uint32_t>((h >> (slice * i)) & 63u)
+                                                     : static_cast<uint32_t>((h * kSaltTable[i]) >> 58);
+                    w[q] |= 1ull << pos;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) T.mk[0][q][p] = w[q];
+        }
+    }
+    // sparse-table levels: an entry of level l covers positions p .. p + 2^l - 1
+    const uint32_t lm = floor_log2(span_m), ls = floor_log2(span_s);
+    const uint32_t levels = lm > ls ? lm : ls;
+    uint32_t cm = 0, cs = 0;
+    for (uint32_t l = 1; l <= levels; ++l) {
+        __syncthreads();
+        const uint32_t half = 1u << (l - 1), span = 1u << l;
+        for (uint32_t p = t; p < static_cast<uint32_t>(kTileSymbols); p += kBlock) {
+            if (l <= lm && p + span <= pm) {
+                const uint64_t a = T.mh[cm][p], b = T.mh[cm][p + half];
+                T.mh[cm ^ 1u][p] = a < b ? a : b;
+            }
+            if (l <= ls && p + span <= ps) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) T.mk[cs ^ 1u][q][p] = T.mk[cs][q][p] | T.mk[cs][q][p + half];
+            }
+        }
+        if (l <= lm) cm ^= 1u;
+        if (l <= ls) cs ^= 1u;
+    }
+    __syncthreads();
+
+    const uint64_t j = first + t;
+    valid = false;
+    if (j < num_windows) {
+        // symbols t .. t + k - 1 of the tile (t + k - 1 < kTileSymbols)
+        const uint32_t wi = t >> 6, sh = t & 63u;
+        uint64_t bits = T.invalid[wi] >> sh;
+        if (sh != 0) bits |= T.invalid[wi + 1] << (64u - sh);
+        const uint64_t kmask = c.k >= 64 ? ~0ull : ((1ull << c.k) - 1ull);
+        valid = (bits & kmask) == 0 && (in.offsets == nullptr || !crosses_record(in, j, c.k));
+    }
+    shard = 0;
+    m[0] = m[1] = m[2] = m[3] = 0;
+    if (valid) {
+        // sequence_kmer.cuh:40-50 (minimum over the window's m-mers) and
+        // filter_ref.cuh:52-55
+        const uint64_t a = T.mh[cm][t], b = T.mh[cm][t + span_m - (1u << lm)];
+        shard = static_cast<uint32_t>((a < b ? a : b) & (f.num_shards - 1));
+        const uint32_t u = t + span_s - (1u << ls);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) m[q] = T.mk[cs][q][t] | T.mk[cs][q][u];
+    }
+}
+
+// Per-window form: each thread packs its window's k symbols and hashes its
+// m-mers and s-mers, as the reference does (kernels.cuh:40-172). For short
+// spans this is cheaper than a hashed tile's tables.
+struct DirectTile {
+    uint8_t sym[kTileSymbols];
+};
+
+__device__ __forceinline__ void direct_tile(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
+                                            uint64_t first, uint64_t num_windows, DirectTile& T, bool& valid,
+                                            uint32_t& shard, uint64_t (&m)[4]) {
+    __syncthreads();  // the previous tile's readers are done
+    load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, T.sym);
+    __syncthreads();
+    const uint64_t j = first + threadIdx.x;
+    valid = false;
+    shard = 0;
+    m[0] = m[1] = m[2] = m[3] = 0;
+    if (j < num_windows) {
+        const Window w = read_window(c, T.sym + threadIdx.x);
+        if (window_ok(in, w, j, c.k)) {
+            valid = true;
+            shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
+            masks(c, w.packed, m);
+        }
+    }
+}
+
+__device__ __forceinline__ void tile_windows(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
+                                             uint64_t first, uint64_t num_windows, HashedTile& T, bool& valid,
+                                             uint32_t& shard, uint64_t (&m)[4]) {
+    hashed_tile(c, in, f, first, num_windows, T, valid, shard, m);
+}
+
+__device__ __forceinline__ void tile_windows(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
+                                             uint64_t first, uint64_t num_windows, DirectTile& T, bool& valid,
+                                             uint32_t& shard, uint64_t (&m)[4]) {
+    direct_tile(c, in, f, first, num_windows, T, valid, shard, m);
+}
+
 // Blocks stride over tiles of kBlock windows: a dispatch must stay below 2^32
 // work-items, so grids are capped and large inputs loop.
 //
@@ -133,29 +311,19 @@ __device__ __forceinline__ bool batch_rejected(const SequenceInput& in) {
 // over the wave (kernels.cuh:143-166 does the same per warp) leaves each run's
 // masks in its last lane, which issues one 64-bit atomic OR per non-zero word:
 // about one update per run instead of one per window.
+template <class Tile>
 __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, SequenceInput in,
                                                         FilterView f, uint64_t num_windows) {
-    __shared__ uint8_t tile[kBlock + kMaxK];
+    __shared__ Tile T;
     mark_words_written(f);
     if (batch_rejected(in)) return;
     const int lane = static_cast<int>(threadIdx.x) & 63;
     for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
          first += static_cast<uint64_t>(gridDim.x) * kBlock) {
-        __syncthreads();
-        load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, tile);
-        __syncthreads();
-        const uint64_t j = first + threadIdx.x;
-        bool valid = false;
-        uint32_t shard = 0;
-        uint64_t m[4] = {0, 0, 0, 0};
-        if (j < num_windows) {
-            const Window w = read_window(c, tile + threadIdx.x);
-            if (window_ok(in, w, j, c.k)) {
-                valid = true;
-                shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
-                masks(c, w.packed, m);
-            }
-        }
+        bool valid;
+        uint32_t shard;
+        uint64_t m[4];
+        tile_windows(c, in, f, first, num_windows, T, valid, shard, m);
         // run heads and the lane of this lane's run head
         const uint32_t prev_shard = __shfl_up(shard, 1, 64);
         const int prev_valid = __shfl_up(valid ? 1 : 0, 1, 64);
@@ -164,6 +332,7 @@ __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, Sequence
         const uint64_t valids = __ballot(valid);
         const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
         const int start = upto ? 63 - __clzll(static_cast<long long>(upto)) : 0;
+#pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -176,6 +345,7 @@ __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, Sequence
                                     ((valids >> (lane + 1)) & 1ull) == 0);
         if (tail) {
             uint64_t* dst = f.words + 4ull * shard;
+#pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (m[q] != 0) __hip_atomic_fetch_or(dst + q, m[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
@@ -183,26 +353,25 @@ __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, Sequence
     }
 }
 
+template <class Tile>
 __global__ __launch_bounds__(kBlock) void contains_kernel(KernelConfig c, SequenceInput in,
                                                           FilterView f, uint64_t num_windows,
                                                           uint8_t* out) {
-    __shared__ uint8_t tile[kBlock + kMaxK];
+    __shared__ Tile T;
     if (batch_rejected(in)) return;
     for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
          first += static_cast<uint64_t>(gridDim.x) * kBlock) {
-        __syncthreads();
-        load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, tile);
-        __syncthreads();
+        bool valid;
+        uint32_t shard;
+        uint64_t m[4];
+        tile_windows(c, in, f, first, num_windows, T, valid, shard, m);
         const uint64_t j = first + threadIdx.x;
         if (j >= num_windows) continue;
-        const Window w = read_window(c, tile + threadIdx.x);
-        if (!window_ok(in, w, j, c.k)) {
+        if (!valid) {
             out[j] = 0;
             continue;
         }
-        const uint32_t shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
-        uint64_t m[4];
-        masks(c, w.packed, m);
+        // filter_ref.cuh:70-87
         const uint64_t* src = f.words + 4ull * shard;
         uint64_t miss = 0;
         for (int q = 0; q < 4; ++q) miss |= m[q] & ~src[q];
@@ -548,13 +717,27 @@ static uint64_t window_count(const KernelConfig& c, const SequenceInput& in) {
     return in.num_symbols < c.k ? 0 : in.num_symbols - c.k + 1;
 }
 
+// Hashed tiles pay a fixed LDS cost per window (the hash tables and their
+// sparse-table levels); recomputing pays for each of the window's k - m + 1
+// m-mers and k - s + 1 s-mers (h bit addresses each). Measured on MI355X (k = 31,
+// DNA): at (k-m+1) + (k-s+1) h/4 = 40-44 hashed tiles ran 1.7-3.5x faster, at
+// 2-16 about 20% slower.
+bool hashed_tiles(const KernelConfig& c) {
+    if (c.form != kFormAuto) return c.form == kFormHashed;
+    return (c.k - c.m + 1) + (c.k - c.s + 1) * (c.h / 4) >= 24;
+}
+
 hipError_t generic_insert(const KernelConfig& c, const SequenceInput& in, FilterView f,
                           hipStream_t stream) {
     const uint64_t n = window_count(c, in);
     if (n == 0) return hipSuccess;
     (void)hipGetLastError();  // report only this launch
-    hipLaunchKernelGGL(insert_kernel, dim3(grid_for(n, kBlock, kMaxBlocks)), dim3(kBlock), 0,
-                       stream, c, in, f, n);
+    const dim3 grid(grid_for(n, kBlock, kMaxBlocks)), block(kBlock);
+    if (hashed_tiles(c)) {
+        hipLaunchKernelGGL(insert_kernel<HashedTile>, grid, block, 0, stream, c, in, f, n);
+    } else {
+        hipLaunchKernelGGL(insert_kernel<DirectTile>, grid, block, 0, stream, c, in, f, n);
+    }
     return hipGetLastError();
 }
 
@@ -563,8 +746,12 @@ hipError_t generic_contains(const KernelConfig& c, const SequenceInput& in, Filt
     const uint64_t n = window_count(c, in);
     if (n == 0) return hipSuccess;
     (void)hipGetLastError();  // report only this launch
-    hipLaunchKernelGGL(contains_kernel, dim3(grid_for(n, kBlock, kMaxBlocks)), dim3(kBlock), 0,
-                       stream, c, in, f, n, out);
+    const dim3 grid(grid_for(n, kBlock, kMaxBlocks)), block(kBlock);
+    if (hashed_tiles(c)) {
+        hipLaunchKernelGGL(contains_kernel<HashedTile>, grid, block, 0, stream, c, in, f, n, out);
+    } else {
+        hipLaunchKernelGGL(contains_kernel<DirectTile>, grid, block, 0, stream, c, in, f, n, out);
+    }
     return hipGetLastError();
 }
 

@@ -80,6 +80,9 @@ __host__ __device__ __forceinline__ constexpr uint64_t window_mask(uint32_t bits
 // Alphabets (Alphabet.cuh:113-218). Codes used by the kernels.
 enum AlphabetKind : uint32_t { kDna = 0, kDnaTriplet = 1, kProtein = 2, kCustom = 3 };
 
+// Generic-kernel form: chosen by the window spans, or fixed (tests compare both).
+enum GenericForm : uint32_t { kFormAuto = 0, kFormDirect = 1, kFormHashed = 2 };
+
 // Runtime configuration handed to the generic kernels by value.
 struct KernelConfig {
     uint32_t k, s, m, h;
@@ -87,6 +90,7 @@ struct KernelConfig {
     uint32_t symbol_width;
     uint32_t symbol_bits;
     uint32_t invalid_symbol;
+    uint32_t form;  // GenericForm
     uint8_t lut[256];
 };
 

@@ -149,8 +149,9 @@ FAST = [("dna", 31, 28, 16, 4), ("dna", 31, 30, 16, 4), ("dna", 31, 31, 16, 4),
         ("dna", 21, 19, 11, 4), ("protein", 10, 8, 6, 4), ("protein", 12, 10, 6, 4),
         ("protein", 5, 4, 3, 4)]
 GENERIC = [("dna", 27, 25, 13, 4), ("dna", 32, 20, 32, 8), ("dna", 31, 24, 16, 12),
-           ("dna", 31, 28, 18, 16), ("protein", 12, 8, 5, 16), ("triplet", 7, 5, 4, 4),
-           ("triplet", 10, 9, 3, 12), ("custom_xyz", 12, 10, 6, 4), ("custom_xyz", 20, 14, 9, 8)]
+           ("dna", 31, 28, 18, 16), ("dna", 32, 16, 8, 4), ("protein", 12, 8, 5, 16),
+           ("triplet", 7, 5, 4, 4), ("triplet", 10, 9, 3, 12), ("triplet", 10, 6, 2, 16),
+           ("custom_xyz", 12, 10, 6, 4), ("custom_xyz", 20, 14, 9, 8)]
 
 
 def _parity(alphabet, k, s, m, h, n, seed, invalid_rate, bits_per_kmer=8.0, offset=0):
@@ -190,8 +191,12 @@ def test_lane_strip_parity(gpu, alphabet, k, s, m, h):
     assert f.kernel_family() == "lane_strip"
 
 
+@pytest.mark.parametrize("form", ["direct", "hashed"])
 @pytest.mark.parametrize("alphabet,k,s,m,h", GENERIC)
-def test_generic_parity(gpu, alphabet, k, s, m, h):
+def test_generic_parity(gpu, monkeypatch, form, alphabet, k, s, m, h):
+    # both forms of the generic kernels (per-window recomputation, hashed tiles
+    # with sparse tables) against the oracle, whichever the spans would pick
+    monkeypatch.setenv("CUSBF_GENERIC_FORM", form)
     f = _parity(alphabet, k, s, m, h, 60_000, seed=k * 17 + h, invalid_rate=0.003)
     assert f.kernel_family() == "generic"
 

@@ -34,7 +34,8 @@ def _hipcc():
 
 
 def kernel_bodies(asm: str):
-    for m in re.finditer(r"\n(_Z\w+):[^\n]*\n", asm):
+    # function labels carry the "; @name" comment; data objects (constant tables) do not
+    for m in re.finditer(r"\n(_Z\w+):[ \t]*; @[^\n]*\n", asm):
         end = asm.find(".Lfunc_end", m.end())
         yield m.group(1), [line.strip() for line in asm[m.end():end].split("\n")]
 
