@@ -245,8 +245,9 @@ def nematode_scenario(steps: int, warmup: int, fpr_kmers: int) -> dict:
 
 
 # Points of the reference's (s, m, h) grid (benchmarks/param_sweep.cu:263-300, k = 31),
-# corners and middle: the lane-strip kernels cover k - s + 1 <= 8 and k - m + 1 <= 17
-# with h = 4 or 8; the rest runs the generic kernels.
+# corners and middle: (31, 28, 16, 8) and (31, 28, 16, 16) have lane-strip kernels; the
+# rest run the generic kernels (hashed tiles where the windows span many m-mers and
+# s-mer bit addresses, per-window recomputation elsewhere).
 SWEEP_POINTS = [(31, 28, 16, 8), (31, 16, 8, 4), (31, 20, 12, 8), (31, 24, 16, 12), (31, 28, 16, 16),
                 (31, 28, 24, 4), (31, 30, 20, 8), (31, 31, 31, 4)]
 
