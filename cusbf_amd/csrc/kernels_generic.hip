@@ -23,27 +23,24 @@ constexpr int kBlock = 256;
 constexpr int kMaxK = 64;
 constexpr uint64_t kMaxBlocks = 1ull << 16;  // 2^24 work-items per dispatch at most
 
-// Tile of encoded symbols [first, first + count); symbols outside [0, n) get
-// the invalid sentinel (they only occur in windows that are never reported).
-__device__ void load_tile(const KernelConfig& c, const SequenceInput& in, int64_t first,
-                          int count, uint8_t* tile) {
-    const uint64_t per_word = 64u / c.symbol_bits;
-    const uint64_t sym_mask = (1ull << c.symbol_bits) - 1ull;
-    for (int i = static_cast<int>(threadIdx.x); i < count; i += kBlock) {
-        const int64_t p = first + i;
-        uint8_t v = static_cast<uint8_t>(c.invalid_symbol);
-        if (p >= 0 && static_cast<uint64_t>(p) < in.num_symbols) {
-            const uint64_t up = static_cast<uint64_t>(p);
-            if (in.dense) {
-                // dense_packed.cuh:56-65: LSB-first decode, masked to symbolBits.
-                v = static_cast<uint8_t>((in.words[up / per_word] >> ((up % per_word) * c.symbol_bits)) &
-                                         sym_mask);
-            } else {
-                v = encode_symbol(c, in.seq + up * c.symbol_width);
-            }
-        }
-        tile[i] = v;
+// Encoded symbol p of the input; positions outside [0, n) get the invalid
+// sentinel (they only occur in windows that are never reported).
+__device__ __forceinline__ uint8_t symbol_at(const KernelConfig& c, const SequenceInput& in, int64_t p) {
+    if (p < 0 || static_cast<uint64_t>(p) >= in.num_symbols) return static_cast<uint8_t>(c.invalid_symbol);
+    const uint64_t up = static_cast<uint64_t>(p);
+    if (in.dense) {
+        // dense_packed.cuh:56-65: LSB-first decode, masked to symbolBits.
+        const uint64_t per_word = 64u / c.symbol_bits;
+        const uint64_t sym_mask = (1ull << c.symbol_bits) - 1ull;
+        return static_cast<uint8_t>((in.words[up / per_word] >> ((up % per_word) * c.symbol_bits)) & sym_mask);
     }
+    return encode_symbol(c, in.seq + up * c.symbol_width);
+}
+
+// Tile of encoded symbols [first, first + count).
+__device__ __forceinline__ void load_tile(const KernelConfig& c, const SequenceInput& in, int64_t first,
+                                          int count, uint8_t* tile) {
+    for (int i = static_cast<int>(threadIdx.x); i < count; i += kBlock) tile[i] = symbol_at(c, in, first + i);
 }
 
 struct Window {
@@ -166,16 +163,14 @@ __device__ __forceinline__ uint64_t pack_symbols(const KernelConfig& c, const ui
     return v;
 }
 
-// Windows [first, first + kBlock): thread t learns whether window first + t is
-// valid and in range, its shard and its masks. Every thread of the block calls.
+// Windows [first, first + kBlock), whose symbols the block has loaded into T.sym:
+// thread t learns whether window first + t is valid and in range, its shard and
+// its masks. Every thread of the block calls.
 __device__ __forceinline__ void hashed_tile(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
                             uint64_t first, uint64_t num_windows, HashedTile& T, bool& valid,
                             uint32_t& shard, uint64_t (&m)[4]) {
     const uint32_t t = threadIdx.x;
-    const uint32_t count = kBlock + c.k - 1;  // symbols the tile's windows cover
-    __syncthreads();                          // the previous tile's readers are done
-    load_tile(c, in, static_cast<int64_t>(first), static_cast<int>(count), T.sym);
-    __syncthreads();
+    const uint32_t count = kBlock + c.k - 1;  // symbols the tile's windows cover (loaded in T.sym)
 
     const uint32_t span_m = c.k - c.m + 1;    // m-mers per window
     const uint32_t span_s = c.k - c.s + 1;    // s-mers per window
@@ -274,9 +269,6 @@ struct DirectTile {
 __device__ __forceinline__ void direct_tile(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
                                             uint64_t first, uint64_t num_windows, DirectTile& T, bool& valid,
                                             uint32_t& shard, uint64_t (&m)[4]) {
-    __syncthreads();  // the previous tile's readers are done
-    load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, T.sym);
-    __syncthreads();
     const uint64_t j = first + threadIdx.x;
     valid = false;
     shard = 0;
@@ -320,6 +312,9 @@ __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, Sequence
     const int lane = static_cast<int>(threadIdx.x) & 63;
     for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
          first += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        __syncthreads();  // the previous tile's readers are done
+        load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, T.sym);
+        __syncthreads();
         bool valid;
         uint32_t shard;
         uint64_t m[4];
@@ -361,6 +356,9 @@ __global__ __launch_bounds__(kBlock) void contains_kernel(KernelConfig c, Sequen
     if (batch_rejected(in)) return;
     for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
          first += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        __syncthreads();  // the previous tile's readers are done
+        load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, T.sym);
+        __syncthreads();
         bool valid;
         uint32_t shard;
         uint64_t m[4];

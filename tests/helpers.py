@@ -24,14 +24,31 @@ def xyz_lut() -> list:
     return [xyz_encode(b) for b in range(256)]
 
 
+# A two-symbol custom alphabet: one bit per symbol, so k reaches 64 (the longest
+# k-mer a packed uint64_t holds).
+AB = {"symbols": {"a": 0, "b": 1}, "symbol_count": 2, "invalid_symbol": 2, "separator": "!"}
+
+
+def ab_encode(b: int) -> int:
+    return AB["symbols"].get(chr(b), AB["invalid_symbol"]) if b < 128 else AB["invalid_symbol"]
+
+
+def ab_lut() -> list:
+    return [ab_encode(b) for b in range(256)]
+
+
 ALPHA_CODES = {"dna": orc.ALPHA_DNA, "triplet": orc.ALPHA_DNA_TRIPLET,
-               "protein": orc.ALPHA_PROTEIN, "custom_xyz": orc.ALPHA_CUSTOM}
+               "protein": orc.ALPHA_PROTEIN, "custom_xyz": orc.ALPHA_CUSTOM,
+               "custom_ab": orc.ALPHA_CUSTOM}
 
 
 def make_oracle(k, s, m, h=4, alphabet="dna") -> orc.Oracle:
     if alphabet == "custom_xyz":
         return orc.Oracle(k, s, m, h, orc.ALPHA_CUSTOM, xyz_lut(), XYZ["symbol_count"],
                           XYZ["invalid_symbol"], ord(XYZ["separator"]))
+    if alphabet == "custom_ab":
+        return orc.Oracle(k, s, m, h, orc.ALPHA_CUSTOM, ab_lut(), AB["symbol_count"],
+                          AB["invalid_symbol"], ord(AB["separator"]))
     return orc.Oracle(k, s, m, h, ALPHA_CODES[alphabet])
 
 
@@ -53,6 +70,10 @@ def make_config(k, s, m, h=4, alphabet="dna"):
     if alphabet == "custom_xyz":
         alpha = cs.Alphabet.custom("xyz", XYZ["symbol_count"], XYZ["invalid_symbol"],
                                    XYZ["separator"], b"xyz", xyz_encode)
+        return cs.Config(k, s, m, h, alphabet=alpha)
+    if alphabet == "custom_ab":
+        alpha = cs.Alphabet.custom("ab", AB["symbol_count"], AB["invalid_symbol"],
+                                   AB["separator"], b"ab", ab_encode)
         return cs.Config(k, s, m, h, alphabet=alpha)
     raise ValueError(alphabet)
 
@@ -90,7 +111,8 @@ def random_bytes_for(alphabet: str, n: int, seed: int, invalid_rate: float = 0.0
     if alphabet == "protein":
         return random_protein(n, seed, invalid_rate)
     rng = np.random.default_rng(seed)
-    seq = np.frombuffer(b"xyz", dtype=np.uint8)[rng.integers(0, 3, n)].copy()
+    letters = b"ab" if alphabet == "custom_ab" else b"xyz"
+    seq = np.frombuffer(letters, dtype=np.uint8)[rng.integers(0, len(letters), n)].copy()
     if invalid_rate > 0:
         seq[rng.random(n) < invalid_rate] = ord("!")
     return seq.tobytes()

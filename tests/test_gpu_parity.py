@@ -151,7 +151,8 @@ FAST = [("dna", 31, 28, 16, 4), ("dna", 31, 30, 16, 4), ("dna", 31, 31, 16, 4),
 GENERIC = [("dna", 27, 25, 13, 4), ("dna", 32, 20, 32, 8), ("dna", 31, 24, 16, 12),
            ("dna", 31, 28, 18, 16), ("dna", 32, 16, 8, 4), ("protein", 12, 8, 5, 16),
            ("triplet", 7, 5, 4, 4), ("triplet", 10, 9, 3, 12), ("triplet", 10, 6, 2, 16),
-           ("custom_xyz", 12, 10, 6, 4), ("custom_xyz", 20, 14, 9, 8)]
+           ("custom_xyz", 12, 10, 6, 4), ("custom_xyz", 20, 14, 9, 8),
+           ("custom_ab", 64, 40, 20, 8), ("custom_ab", 64, 64, 64, 4), ("custom_ab", 64, 33, 1, 16)]
 
 
 def _parity(alphabet, k, s, m, h, n, seed, invalid_rate, bits_per_kmer=8.0, offset=0):

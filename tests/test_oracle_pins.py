@@ -156,6 +156,36 @@ def test_single_kmer_bits_lie_in_its_minimizer_shard(k, s, m, h):
         assert valid and packed == _packed(kmer) and omh == mh and omasks.tolist() == masks
 
 
+@pytest.mark.parametrize("k,s,m,h", [(64, 40, 20, 8), (64, 64, 64, 4), (64, 33, 1, 16)])
+def test_one_bit_alphabet_up_to_k64(k, s, m, h):
+    # a two-symbol custom alphabet packs one bit per symbol, so a k-mer fills the
+    # whole uint64_t at k = 64 (window_mask at 64 bits, shifts by 0 and 63)
+    from helpers import ab_encode
+
+    o = make_oracle(k, s, m, h, "custom_ab")
+    rng = np.random.default_rng(k + s + m)
+
+    def packed(t: str) -> int:
+        v = 0
+        for ch in t:
+            v = (v << 1) | ab_encode(ord(ch))
+        return v
+
+    for _ in range(20):
+        kmer = "".join(rng.choice(list("ab"), k))
+        mh = min((packed(kmer[i:i + m]) * 0x9E3779B97F4A7C15) & MASK64 for i in range(k - m + 1))
+        masks = [0, 0, 0, 0]
+        for i in range(k - s + 1):
+            p = (packed(kmer[i:i + s]) * 0x9E3779B97F4A7C15) & MASK64
+            hv = p ^ (p >> 33)
+            for j in range(h):
+                sw = 64 // h
+                pos = (hv >> (sw * j)) & 63 if sw >= 6 else ((hv * orc.salt(j)) & MASK64) >> 58
+                masks[j % 4] |= 1 << pos
+        valid, pk, omh, omasks = o.window(kmer, 0)
+        assert valid and pk == packed(kmer) and omh == mh and omasks.tolist() == masks
+
+
 def test_exhaustive_small_kmers_have_no_false_negatives():
     o = make_oracle(5, 4, 3, 4)
     shards = o.num_shards(1 << 12)
