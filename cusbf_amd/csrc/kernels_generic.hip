@@ -76,13 +76,31 @@ __device__ __forceinline__ uint64_t minimizer(const KernelConfig& c, uint64_t pa
     return best;
 }
 
+// filter_common.cuh:27-93: the 16 salts, as a table indexed by a loop counter.
+__constant__ const uint64_t kSaltTable[16] = {salt(0),  salt(1),  salt(2),  salt(3), salt(4),  salt(5),
+                                              salt(6),  salt(7),  salt(8),  salt(9), salt(10), salt(11),
+                                              salt(12), salt(13), salt(14), salt(15)};
+
+// filter_impl.cuh:36-77 for one s-mer hash: hash i sets bit bit_address(i) of word
+// i % 4 (the hash count is a multiple of 4, so the word index is a constant of
+// the unrolled inner loop and the words stay in registers).
+__device__ __forceinline__ void or_smer_bits(const KernelConfig& c, uint64_t h, uint64_t (&m)[4]) {
+    const uint32_t slice = 64u / c.h;
+    for (uint32_t g = 0; g < (c.h >> 2); ++g) {
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t i = 4 * g + q;
+            const uint32_t pos = slice >= 6u ? static_cast<uint32_t>((h >> (slice * i)) & 63u)
+                                             : static_cast<uint32_t>((h * kSaltTable[i]) >> 58);
+            m[q] |= 1ull << pos;
+        }
+    }
+}
+
 // kernels.cuh:294-304 with filter_impl.cuh:54-77.
 __device__ __forceinline__ void masks(const KernelConfig& c, uint64_t packed, uint64_t (&m)[4]) {
     m[0] = m[1] = m[2] = m[3] = 0;
-    for (uint32_t o = 0; o + c.s <= c.k; ++o) {
-        const uint64_t h = hash64(subwindow(c, packed, c.s, o));
-        for (uint32_t i = 0; i < c.h; ++i) m[i & 3u] |= 1ull << bit_address(c.h, i, h);
-    }
+    for (uint32_t o = 0; o + c.s <= c.k; ++o) or_smer_bits(c, hash64(subwindow(c, packed, c.s, o)), m);
 }
 
 // Batched form: true when a record starts inside window j after its first
@@ -125,33 +143,54 @@ __device__ __forceinline__ bool batch_rejected(const SequenceInput& in) {
     return in.state != nullptr && in.state[1] != 0;
 }
 
-// --- hashed tiles -------------------------------------------------------------
+// --- tiles ----------------------------------------------------------------------
 //
-// A block's tile is kBlock windows, thread t owning window first + t. The block
-// hashes each of the tile's m-mers and s-mers once (sequence_kmer.cuh:40-59
-// hashes k-m+1 m-mers and k-s+1 s-mers per window), keeps the m-mer hashes and
-// the s-mers' word masks (filter_impl.cuh:54-77) in LDS, and builds sparse
-// tables over them: level l holds the minimum (for the masks: the OR) over 2^l
-// consecutive positions. A window's minimizer hash is the minimum of two
-// overlapping level-L entries, L = floor(log2(k - m + 1)), and its findere masks
-// are the OR of two level-L' entries, L' = floor(log2(k - s + 1)); min and OR are
-// idempotent, so the overlap does not change either. A bit per position marks
-// invalid symbols (and positions past the input), and a window is valid when
-// its k bits are clear.
-constexpr int kTileSymbols = kBlock + kMaxK;  // the tile's windows and the k - 1 symbols after them
-static_assert(kTileSymbols % 64 == 0, "the invalid bits are ballots of whole waves");
-
+// A block's tile is R * kBlock consecutive windows; thread t owns windows
+// first + t + r * kBlock (r < R), so the lanes of a wave hold consecutive windows
+// of each of its R slices. Fixed per-tile latencies (the symbol loads, the
+// barriers, the shard loads of the query) are paid once for the R windows of a
+// thread, whose loads are in flight together.
+//
+// Hashed tiles: the block hashes each of the tile's m-mers and s-mers once
+// (sequence_kmer.cuh:40-59 hashes k-m+1 m-mers and k-s+1 s-mers per window),
+// keeps the m-mer hashes and the s-mers' word masks (filter_impl.cuh:54-77) in
+// LDS, and builds sparse tables over them: level l holds the minimum (for the
+// masks: the OR) over 2^l consecutive positions. A window's minimizer hash is the
+// minimum of two overlapping level-L entries, L = floor(log2(k - m + 1)), and its
+// findere masks are the OR of two level-L' entries, L' = floor(log2(k - s + 1));
+// min and OR are idempotent, so the overlap does not change either. A bit per
+// position marks invalid symbols (and positions past the input), and a window
+// is valid when its k bits are clear.
+template <int R>
 struct HashedTile {
-    uint8_t sym[kTileSymbols];
-    uint64_t invalid[kTileSymbols / 64];  // bit p of the tile: symbol p invalid or past the input
-    uint64_t mh[2][kTileSymbols];         // m-mer hashes, two sparse-table levels in turn
-    uint64_t mk[2][4][kTileSymbols];      // s-mer masks (word q of position p at [.][q][p])
+    static constexpr int kR = R;
+    static constexpr int kWindows = R * kBlock;
+    static constexpr int kSymbols = kWindows + kMaxK;  // the windows and the k - 1 symbols after them
+    static_assert(kSymbols % 64 == 0, "the invalid bits are ballots of whole waves");
+    uint8_t sym[kSymbols];
+    uint64_t invalid[kSymbols / 64];  // bit p of the tile: symbol p invalid or past the input
+    uint64_t mh[2][kSymbols];         // m-mer hashes, two sparse-table levels in turn
+    uint64_t mk[2][4][kSymbols];      // s-mer masks (word q of position p at [.][q][p])
 };
 
-// filter_common.cuh:27-93: the 16 salts, as a table indexed by a loop counter.
-__constant__ const uint64_t kSaltTable[16] = {salt(0),  salt(1),  salt(2),  salt(3), salt(4),  salt(5),
-                                              salt(6),  salt(7),  salt(8),  salt(9), salt(10), salt(11),
-                                              salt(12), salt(13), salt(14), salt(15)};
+// Per-window form: each thread packs its windows' k symbols and hashes their
+// m-mers and s-mers, as the reference does (kernels.cuh:40-172). For short
+// spans this is cheaper than a hashed tile's tables.
+template <int R>
+struct DirectTile {
+    static constexpr int kR = R;
+    static constexpr int kWindows = R * kBlock;
+    static constexpr int kSymbols = kWindows + kMaxK;
+    uint8_t sym[kSymbols];
+};
+
+// What a thread learns about its R windows.
+template <int R>
+struct ThreadWindows {
+    bool valid[R];  // valid and in range
+    uint32_t shard[R];
+    uint64_t m[R][4];
+};
 
 __device__ __forceinline__ uint32_t floor_log2(uint32_t x) { return 31u - static_cast<uint32_t>(__builtin_clz(x)); }
 
@@ -163,24 +202,24 @@ __device__ __forceinline__ uint64_t pack_symbols(const KernelConfig& c, const ui
     return v;
 }
 
-// Windows [first, first + kBlock), whose symbols the block has loaded into T.sym:
-// thread t learns whether window first + t is valid and in range, its shard and
-// its masks. Every thread of the block calls.
-__device__ __forceinline__ void hashed_tile(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
-                            uint64_t first, uint64_t num_windows, HashedTile& T, bool& valid,
-                            uint32_t& shard, uint64_t (&m)[4]) {
+// The windows of a tile whose symbols the block has loaded into T.sym. Every
+// thread of the block calls.
+template <int R>
+__device__ __forceinline__ void tile_windows(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
+                                             uint64_t first, uint64_t num_windows, HashedTile<R>& T,
+                                             ThreadWindows<R>& out) {
+    using Tile = HashedTile<R>;
     const uint32_t t = threadIdx.x;
-    const uint32_t count = kBlock + c.k - 1;  // symbols the tile's windows cover (loaded in T.sym)
-
-    const uint32_t span_m = c.k - c.m + 1;    // m-mers per window
-    const uint32_t span_s = c.k - c.s + 1;    // s-mers per window
-    const uint32_t pm = kBlock + span_m - 1;  // m-mer positions of the tile
-    const uint32_t ps = kBlock + span_s - 1;  // s-mer positions of the tile
+    const uint32_t count = Tile::kWindows + c.k - 1;     // symbols the tile's windows cover
+    const uint32_t span_m = c.k - c.m + 1;               // m-mers per window
+    const uint32_t span_s = c.k - c.s + 1;               // s-mers per window
+    const uint32_t pm = Tile::kWindows + span_m - 1;     // m-mer positions of the tile
+    const uint32_t ps = Tile::kWindows + span_s - 1;     // s-mer positions of the tile
     const uint32_t both = pm < ps ? pm : ps;
     const uint32_t longest = c.m > c.s ? c.m : c.s;
     // level 0: one hash per m-mer, one mask per s-mer (positions p < pm or ps
     // read symbols p .. p + len - 1 <= count - 1)
-    for (uint32_t p = t; p < static_cast<uint32_t>(kTileSymbols); p += kBlock) {
+    for (uint32_t p = t; p < static_cast<uint32_t>(Tile::kSymbols); p += kBlock) {
         const bool bad = p >= count || T.sym[p] == static_cast<uint8_t>(c.invalid_symbol);
         const uint64_t ballot = __ballot(bad);  // the loop is uniform per wave
         if ((p & 63u) == 0) T.invalid[p >> 6] = ballot;
@@ -196,20 +235,9 @@ __device__ __forceinline__ void hashed_tile(const KernelConfig& c, const Sequenc
         }
         if (p < pm) T.mh[0][p] = minimizer_hash64(vm);
         if (p < ps) {
-            // kernels.cuh:294-304 with filter_impl.cuh:54-77, one s-mer: hash i
-            // sets a bit of word i % 4 (the hash count is a multiple of 4)
-            const uint64_t h = hash64(vs);
-            const uint32_t slice = 64u / c.h;
+            // kernels.cuh:294-304 with filter_impl.cuh:54-77, one s-mer
             uint64_t w[4] = {0, 0, 0, 0};
-            for (uint32_t g = 0; g < (c.h >> 2); ++g) {
-#pragma unroll
-                for (uint32_t q = 0; q < 4; ++q) {
-                    const uint32_t i = 4 * g + q;
-                    const uint32_t pos = slice >= 6u ? static_cast<uint32_t>((h >> (slice * i)) & 63u)
-                                                     : static_cast<uint32_t>((h * kSaltTable[i]) >> 58);
-                    w[q] |= 1ull << pos;
-                }
-            }
+            or_smer_bits(c, hash64(vs), w);
 #pragma unroll
             for (int q = 0; q < 4; ++q) T.mk[0][q][p] = w[q];
         }
@@ -221,7 +249,7 @@ __device__ __forceinline__ void hashed_tile(const KernelConfig& c, const Sequenc
     for (uint32_t l = 1; l <= levels; ++l) {
         __syncthreads();
         const uint32_t half = 1u << (l - 1), span = 1u << l;
-        for (uint32_t p = t; p < static_cast<uint32_t>(kTileSymbols); p += kBlock) {
+        for (uint32_t p = t; p < static_cast<uint32_t>(Tile::kSymbols); p += kBlock) {
             if (l <= lm && p + span <= pm) {
                 const uint64_t a = T.mh[cm][p], b = T.mh[cm][p + half];
                 T.mh[cm ^ 1u][p] = a < b ? a : b;
@@ -236,67 +264,58 @@ __device__ __forceinline__ void hashed_tile(const KernelConfig& c, const Sequenc
     }
     __syncthreads();
 
-    const uint64_t j = first + t;
-    valid = false;
-    if (j < num_windows) {
-        // symbols t .. t + k - 1 of the tile (t + k - 1 < kTileSymbols)
-        const uint32_t wi = t >> 6, sh = t & 63u;
-        uint64_t bits = T.invalid[wi] >> sh;
-        if (sh != 0) bits |= T.invalid[wi + 1] << (64u - sh);
-        const uint64_t kmask = c.k >= 64 ? ~0ull : ((1ull << c.k) - 1ull);
-        valid = (bits & kmask) == 0 && (in.offsets == nullptr || !crosses_record(in, j, c.k));
-    }
-    shard = 0;
-    m[0] = m[1] = m[2] = m[3] = 0;
-    if (valid) {
-        // sequence_kmer.cuh:40-50 (minimum over the window's m-mers) and
-        // filter_ref.cuh:52-55
-        const uint64_t a = T.mh[cm][t], b = T.mh[cm][t + span_m - (1u << lm)];
-        shard = static_cast<uint32_t>((a < b ? a : b) & (f.num_shards - 1));
-        const uint32_t u = t + span_s - (1u << ls);
+    const uint64_t kmask = c.k >= 64 ? ~0ull : ((1ull << c.k) - 1ull);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) m[q] = T.mk[cs][q][t] | T.mk[cs][q][u];
-    }
-}
-
-// Per-window form: each thread packs its window's k symbols and hashes its
-// m-mers and s-mers, as the reference does (kernels.cuh:40-172). For short
-// spans this is cheaper than a hashed tile's tables.
-struct DirectTile {
-    uint8_t sym[kTileSymbols];
-};
-
-__device__ __forceinline__ void direct_tile(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
-                                            uint64_t first, uint64_t num_windows, DirectTile& T, bool& valid,
-                                            uint32_t& shard, uint64_t (&m)[4]) {
-    const uint64_t j = first + threadIdx.x;
-    valid = false;
-    shard = 0;
-    m[0] = m[1] = m[2] = m[3] = 0;
-    if (j < num_windows) {
-        const Window w = read_window(c, T.sym + threadIdx.x);
-        if (window_ok(in, w, j, c.k)) {
-            valid = true;
-            shard = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
-            masks(c, w.packed, m);
+    for (int r = 0; r < R; ++r) {
+        const uint32_t tr = t + static_cast<uint32_t>(r) * kBlock;
+        const uint64_t j = first + tr;
+        bool valid = false;
+        if (j < num_windows) {
+            // symbols tr .. tr + k - 1 of the tile (tr + k - 1 < kSymbols)
+            const uint32_t wi = tr >> 6, sh = tr & 63u;
+            uint64_t bits = T.invalid[wi] >> sh;
+            if (sh != 0) bits |= T.invalid[wi + 1] << (64u - sh);
+            valid = (bits & kmask) == 0 && (in.offsets == nullptr || !crosses_record(in, j, c.k));
+        }
+        out.valid[r] = valid;
+        out.shard[r] = 0;
+        out.m[r][0] = out.m[r][1] = out.m[r][2] = out.m[r][3] = 0;
+        if (valid) {
+            // sequence_kmer.cuh:40-50 (minimum over the window's m-mers) and
+            // filter_ref.cuh:52-55
+            const uint64_t a = T.mh[cm][tr], b = T.mh[cm][tr + span_m - (1u << lm)];
+            out.shard[r] = static_cast<uint32_t>((a < b ? a : b) & (f.num_shards - 1));
+            const uint32_t u = tr + span_s - (1u << ls);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) out.m[r][q] = T.mk[cs][q][tr] | T.mk[cs][q][u];
         }
     }
 }
 
+template <int R>
 __device__ __forceinline__ void tile_windows(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
-                                             uint64_t first, uint64_t num_windows, HashedTile& T, bool& valid,
-                                             uint32_t& shard, uint64_t (&m)[4]) {
-    hashed_tile(c, in, f, first, num_windows, T, valid, shard, m);
+                                             uint64_t first, uint64_t num_windows, DirectTile<R>& T,
+                                             ThreadWindows<R>& out) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t tr = threadIdx.x + static_cast<uint32_t>(r) * kBlock;
+        const uint64_t j = first + tr;
+        out.valid[r] = false;
+        out.shard[r] = 0;
+        out.m[r][0] = out.m[r][1] = out.m[r][2] = out.m[r][3] = 0;
+        if (j < num_windows) {
+            const Window w = read_window(c, T.sym + tr);
+            if (window_ok(in, w, j, c.k)) {
+                out.valid[r] = true;
+                out.shard[r] = static_cast<uint32_t>(minimizer(c, w.packed) & (f.num_shards - 1));
+                masks(c, w.packed, out.m[r]);
+            }
+        }
+    }
 }
 
-__device__ __forceinline__ void tile_windows(const KernelConfig& c, const SequenceInput& in, const FilterView& f,
-                                             uint64_t first, uint64_t num_windows, DirectTile& T, bool& valid,
-                                             uint32_t& shard, uint64_t (&m)[4]) {
-    direct_tile(c, in, f, first, num_windows, T, valid, shard, m);
-}
-
-// Blocks stride over tiles of kBlock windows: a dispatch must stay below 2^32
-// work-items, so grids are capped and large inputs loop.
+// Blocks stride over tiles: a dispatch must stay below 2^32 work-items, so grids
+// are capped and large inputs loop.
 //
 // Insert: consecutive lanes hold consecutive windows, so a run (consecutive
 // valid windows with one shard) is a segment of the wave. A segmented OR scan
@@ -306,43 +325,48 @@ __device__ __forceinline__ void tile_windows(const KernelConfig& c, const Sequen
 template <class Tile>
 __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, SequenceInput in,
                                                         FilterView f, uint64_t num_windows) {
+    constexpr int R = Tile::kR;
     __shared__ Tile T;
     mark_words_written(f);
     if (batch_rejected(in)) return;
     const int lane = static_cast<int>(threadIdx.x) & 63;
-    for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
-         first += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * Tile::kWindows; first < num_windows;
+         first += static_cast<uint64_t>(gridDim.x) * Tile::kWindows) {
         __syncthreads();  // the previous tile's readers are done
-        load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, T.sym);
+        load_tile(c, in, static_cast<int64_t>(first), Tile::kWindows + static_cast<int>(c.k) - 1, T.sym);
         __syncthreads();
-        bool valid;
-        uint32_t shard;
-        uint64_t m[4];
-        tile_windows(c, in, f, first, num_windows, T, valid, shard, m);
-        // run heads and the lane of this lane's run head
-        const uint32_t prev_shard = __shfl_up(shard, 1, 64);
-        const int prev_valid = __shfl_up(valid ? 1 : 0, 1, 64);
-        const bool head = valid && (lane == 0 || !prev_valid || prev_shard != shard);
-        const uint64_t heads = __ballot(head);
-        const uint64_t valids = __ballot(valid);
-        const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-        const int start = upto ? 63 - __clzll(static_cast<long long>(upto)) : 0;
+        ThreadWindows<R> tw;
+        tile_windows(c, in, f, first, num_windows, T, tw);
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
+        for (int r = 0; r < R; ++r) {
+            const bool valid = tw.valid[r];
+            const uint32_t shard = tw.shard[r];
+            uint64_t* m = tw.m[r];
+            // run heads and the lane of this lane's run head
+            const uint32_t prev_shard = __shfl_up(shard, 1, 64);
+            const int prev_valid = __shfl_up(valid ? 1 : 0, 1, 64);
+            const bool head = valid && (lane == 0 || !prev_valid || prev_shard != shard);
+            const uint64_t heads = __ballot(head);
+            const uint64_t valids = __ballot(valid);
+            const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+            const int start = upto ? 63 - __clzll(static_cast<long long>(upto)) : 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint64_t t = __shfl_up(m[q], d, 64);
-                if (lane - d >= start) m[q] |= t;
+            for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint64_t t = __shfl_up(m[q], d, 64);
+                    if (lane - d >= start) m[q] |= t;
+                }
             }
-        }
-        // the run's last lane applies its masks
-        const bool tail = valid && (lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0 ||
-                                    ((valids >> (lane + 1)) & 1ull) == 0);
-        if (tail) {
-            uint64_t* dst = f.words + 4ull * shard;
+            // the run's last lane applies its masks
+            const bool tail = valid && (lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0 ||
+                                        ((valids >> (lane + 1)) & 1ull) == 0);
+            if (tail) {
+                uint64_t* dst = f.words + 4ull * shard;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (m[q] != 0) __hip_atomic_fetch_or(dst + q, m[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int q = 0; q < 4; ++q) {
+                    if (m[q] != 0) __hip_atomic_fetch_or(dst + q, m[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
         }
     }
@@ -352,28 +376,33 @@ template <class Tile>
 __global__ __launch_bounds__(kBlock) void contains_kernel(KernelConfig c, SequenceInput in,
                                                           FilterView f, uint64_t num_windows,
                                                           uint8_t* out) {
+    constexpr int R = Tile::kR;
     __shared__ Tile T;
     if (batch_rejected(in)) return;
-    for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * kBlock; first < num_windows;
-         first += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * Tile::kWindows; first < num_windows;
+         first += static_cast<uint64_t>(gridDim.x) * Tile::kWindows) {
         __syncthreads();  // the previous tile's readers are done
-        load_tile(c, in, static_cast<int64_t>(first), kBlock + static_cast<int>(c.k) - 1, T.sym);
+        load_tile(c, in, static_cast<int64_t>(first), Tile::kWindows + static_cast<int>(c.k) - 1, T.sym);
         __syncthreads();
-        bool valid;
-        uint32_t shard;
-        uint64_t m[4];
-        tile_windows(c, in, f, first, num_windows, T, valid, shard, m);
-        const uint64_t j = first + threadIdx.x;
-        if (j >= num_windows) continue;
-        if (!valid) {
-            out[j] = 0;
-            continue;
+        ThreadWindows<R> tw;
+        tile_windows(c, in, f, first, num_windows, T, tw);
+        // the R shard loads go out together (filter_ref.cuh:70-87)
+        uint64_t w[R][4];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint64_t* src = f.words + 4ull * tw.shard[r];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[r][q] = tw.valid[r] ? src[q] : 0;
         }
-        // filter_ref.cuh:70-87
-        const uint64_t* src = f.words + 4ull * shard;
-        uint64_t miss = 0;
-        for (int q = 0; q < 4; ++q) miss |= m[q] & ~src[q];
-        out[j] = miss == 0 ? 1 : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint64_t j = first + threadIdx.x + static_cast<uint64_t>(r) * kBlock;
+            if (j >= num_windows) continue;
+            uint64_t miss = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) miss |= tw.m[r][q] & ~w[r][q];
+            out[j] = tw.valid[r] && miss == 0 ? 1 : 0;
+        }
     }
 }
 
@@ -720,6 +749,12 @@ static uint64_t window_count(const KernelConfig& c, const SequenceInput& in) {
 // m-mers and k - s + 1 s-mers (h bit addresses each). Measured on MI355X (k = 31,
 // DNA): at (k-m+1) + (k-s+1) h/4 = 40-44 hashed tiles ran 1.7-3.5x faster, at
 // 2-16 about 20% slower.
+// Windows per thread of each form's tiles. Measured on MI355X: four per thread in
+// the per-window form halved its occupancy (98-103 VGPRs) and ran no faster (the
+// one-shard-per-window query 25% slower), so both forms take one.
+constexpr int kHashedR = 1;
+constexpr int kDirectR = 1;
+
 bool hashed_tiles(const KernelConfig& c) {
     if (c.form != kFormAuto) return c.form == kFormHashed;
     return (c.k - c.m + 1) + (c.k - c.s + 1) * (c.h / 4) >= 24;
@@ -730,11 +765,14 @@ hipError_t generic_insert(const KernelConfig& c, const SequenceInput& in, Filter
     const uint64_t n = window_count(c, in);
     if (n == 0) return hipSuccess;
     (void)hipGetLastError();  // report only this launch
-    const dim3 grid(grid_for(n, kBlock, kMaxBlocks)), block(kBlock);
     if (hashed_tiles(c)) {
-        hipLaunchKernelGGL(insert_kernel<HashedTile>, grid, block, 0, stream, c, in, f, n);
+        using Tile = HashedTile<kHashedR>;
+        hipLaunchKernelGGL(insert_kernel<Tile>, dim3(grid_for(n, Tile::kWindows, kMaxBlocks)), dim3(kBlock), 0,
+                           stream, c, in, f, n);
     } else {
-        hipLaunchKernelGGL(insert_kernel<DirectTile>, grid, block, 0, stream, c, in, f, n);
+        using Tile = DirectTile<kDirectR>;
+        hipLaunchKernelGGL(insert_kernel<Tile>, dim3(grid_for(n, Tile::kWindows, kMaxBlocks)), dim3(kBlock), 0,
+                           stream, c, in, f, n);
     }
     return hipGetLastError();
 }
@@ -744,11 +782,14 @@ hipError_t generic_contains(const KernelConfig& c, const SequenceInput& in, Filt
     const uint64_t n = window_count(c, in);
     if (n == 0) return hipSuccess;
     (void)hipGetLastError();  // report only this launch
-    const dim3 grid(grid_for(n, kBlock, kMaxBlocks)), block(kBlock);
     if (hashed_tiles(c)) {
-        hipLaunchKernelGGL(contains_kernel<HashedTile>, grid, block, 0, stream, c, in, f, n, out);
+        using Tile = HashedTile<kHashedR>;
+        hipLaunchKernelGGL(contains_kernel<Tile>, dim3(grid_for(n, Tile::kWindows, kMaxBlocks)), dim3(kBlock), 0,
+                           stream, c, in, f, n, out);
     } else {
-        hipLaunchKernelGGL(contains_kernel<DirectTile>, grid, block, 0, stream, c, in, f, n, out);
+        using Tile = DirectTile<kDirectR>;
+        hipLaunchKernelGGL(contains_kernel<Tile>, dim3(grid_for(n, Tile::kWindows, kMaxBlocks)), dim3(kBlock), 0,
+                           stream, c, in, f, n, out);
     }
     return hipGetLastError();
 }

@@ -36,8 +36,9 @@ SBF_STRIP_BATCHED_INSTANCES(SBF_STRIP_EXTERN)
 // (examples/main.cu:164: k=12, s=10, m=6, h=4), the reference's small test
 // configurations and a mid-size one; h = 8, 12 and 16 (the salted bit
 // addresses of h = 12 and 16 keep each s-mer's hash and build its masks when
-// they are needed) with plain kernels. Other configurations run the generic
-// kernels.
+// they are needed) and three short-span points of the reference's parameter
+// sweep (param_sweep.cu:263-300: m = 24, 20 and 31 with s = 28, 30 and 31) with
+// plain kernels. Other configurations run the generic kernels.
 static const StripKernelSet kSets[] = {
     SBF_STRIP_BATCHED(Dna2, 31, 28, 16, 4),
     SBF_STRIP_BATCHED(Dna2, 31, 30, 16, 4),
@@ -46,6 +47,9 @@ static const StripKernelSet kSets[] = {
     SBF_STRIP_PLAIN(Dna2, 31, 28, 16, 8),
     SBF_STRIP_PLAIN(Dna2, 31, 28, 16, 12),
     SBF_STRIP_PLAIN(Dna2, 31, 28, 16, 16),
+    SBF_STRIP_PLAIN(Dna2, 31, 28, 24, 4),
+    SBF_STRIP_PLAIN(Dna2, 31, 30, 20, 8),
+    SBF_STRIP_PLAIN(Dna2, 31, 31, 31, 4),
     SBF_STRIP_BATCHED(Dna2, 5, 4, 3, 4),
     SBF_STRIP_BATCHED(Dna2, 3, 2, 2, 4),
     SBF_STRIP_BATCHED(Dna2, 21, 19, 11, 4),

@@ -145,7 +145,8 @@ def test_host_and_device_query_agree(gpu):
 
 FAST = [("dna", 31, 28, 16, 4), ("dna", 31, 30, 16, 4), ("dna", 31, 31, 16, 4),
         ("dna", 31, 28, 21, 4), ("dna", 31, 28, 16, 8), ("dna", 31, 28, 16, 12),
-        ("dna", 31, 28, 16, 16), ("dna", 5, 4, 3, 4), ("dna", 3, 2, 2, 4),
+        ("dna", 31, 28, 16, 16), ("dna", 31, 28, 24, 4), ("dna", 31, 30, 20, 8),
+        ("dna", 31, 31, 31, 4), ("dna", 5, 4, 3, 4), ("dna", 3, 2, 2, 4),
         ("dna", 21, 19, 11, 4), ("protein", 10, 8, 6, 4), ("protein", 12, 10, 6, 4),
         ("protein", 5, 4, 3, 4)]
 GENERIC = [("dna", 27, 25, 13, 4), ("dna", 32, 20, 32, 8), ("dna", 31, 24, 16, 12),

@@ -63,6 +63,8 @@ CONFIGS = [
     (31, 28, 21, 4, "dna"),
     (31, 28, 16, 8, "dna"),
     (31, 28, 16, 16, "dna"),  # salted bit addresses
+    (31, 28, 24, 4, "dna"),   # minimizer span 8
+    (31, 31, 31, 4, "dna"),   # every window its own run: four slot chunks per tile
     (21, 19, 11, 4, "dna"),
     (5, 4, 3, 4, "dna"),      # ~500 runs per tile: two slot chunks
     (3, 2, 2, 4, "dna"),      # ~660 runs per tile: three slot chunks
