@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_PATH = os.path.join(HERE, "libcusbf_hip.so")
 SYNTH_LIB_PATH = os.path.join(HERE, "libcusbf_synth.so")
 
-HIP_SOURCES = ["kernels_strip.hip", "kernels_strip_batched.hip", "kernels_generic.hip", "sorted_insert.hip", "capi.hip",
+HIP_SOURCES = ["kernels_wide_dna.hip", "kernels_wide_protein.hip", "kernels_strip.hip", "kernels_strip_batched.hip",
+               "kernels_generic.hip", "sorted_insert.hip", "capi.hip",
                "serve.hip", "fastx.hip", "persist.hip", "noalloc.hip"]
 SYNTH_SOURCES = ["synth.hip", "probe.hip", "baseline.hip"]
 

@@ -370,14 +370,14 @@ int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStre
     int rc = choose_sorted(f, windows, tiles, plain_sorted_blocks(f, tiles), 1.0, stream, &l, &sorted);
     if (rc != CUSBF_OK) return rc;
     if (!sorted) {
-        const hipError_t e = f->strip->insert(in, view(f), nullptr, stream);
+        const hipError_t e = f->strip->insert(f->kc, in, view(f), nullptr, stream);
         if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
         return CUSBF_OK;
     }
     rc = begin_sorted(f, l.bytes, stream);
     if (rc != CUSBF_OK) return rc;
     const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, [&](const sbf::RecordSink& sink) {
-        return f->strip->insert(in, view(f), &sink, stream);
+        return f->strip->insert(f->kc, in, view(f), &sink, stream);
     });
     if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
     f->last_path = CUSBF_INSERT_SORTED;
@@ -386,7 +386,7 @@ int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStre
 
 int launch_contains(const cusbf_filter* f, const sbf::SequenceInput& in, uint8_t* out,
                     hipStream_t stream) {
-    const hipError_t e = strip_reads(f, in) ? f->strip->contains(in, view(f), out, stream)
+    const hipError_t e = strip_reads(f, in) ? f->strip->contains(f->kc, in, view(f), out, stream)
                                             : sbf::generic_contains(f->kc, in, view(f), out, stream);
     if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
     return CUSBF_OK;
@@ -593,12 +593,12 @@ int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
         const sbf::BatchLaunch shape{t.blocks_per_cu, f->compute_units, t.claim};
         if (op == CUSBF_BATCH_INSERT && sorted) {
             e = enqueue_sorted(f, *sorted, sorted_base, stream, [&](const sbf::RecordSink& sink) {
-                return f->strip->batched_insert[variant](in, view(f), shape, &sink, stream);
+                return f->strip->batched_insert[variant](f->kc, in, view(f), shape, &sink, stream);
             });
         } else {
             e = op == CUSBF_BATCH_INSERT
-                    ? f->strip->batched_insert[variant](in, view(f), shape, nullptr, stream)
-                    : f->strip->batched_contains[variant](in, view(f), shape, d_output, stream);
+                    ? f->strip->batched_insert[variant](f->kc, in, view(f), shape, nullptr, stream)
+                    : f->strip->batched_contains[variant](f->kc, in, view(f), shape, d_output, stream);
         }
     } else {
         e = op == CUSBF_BATCH_INSERT ? sbf::generic_insert(f->kc, in, view(f), stream)
@@ -788,13 +788,19 @@ int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_
     f->kc.invalid_symbol = d.invalid_symbol;
     std::memcpy(f->kc.lut, cfg->lut, 256);
     // test hooks: CUSBF_FORCE_GENERIC=1 skips the lane-strip kernels;
+    // CUSBF_FORCE_WIDE=1 takes the runtime-span lane strips where they apply;
     // CUSBF_GENERIC_FORM=direct|hashed fixes the generic kernels' form
     const char* form = std::getenv("CUSBF_GENERIC_FORM");
     f->kc.form = form && std::strcmp(form, "direct") == 0   ? sbf::kFormDirect
                  : form && std::strcmp(form, "hashed") == 0 ? sbf::kFormHashed
                                                             : sbf::kFormAuto;
     const char* force_generic = std::getenv("CUSBF_FORCE_GENERIC");
-    if (!(force_generic && force_generic[0] == '1')) {
+    const char* force_wide = std::getenv("CUSBF_FORCE_WIDE");
+    if (force_generic && force_generic[0] == '1') {
+        f->strip = nullptr;
+    } else if (force_wide && force_wide[0] == '1') {
+        f->strip = sbf::find_wide_kernels(cfg->k, cfg->s, cfg->m, cfg->hash_count, cfg->alphabet);
+    } else {
         f->strip = sbf::find_strip_kernels(cfg->k, cfg->s, cfg->m, cfg->hash_count, cfg->alphabet);
     }
     default_tuning(f);
@@ -903,7 +909,8 @@ int cusbf_copy_words_to_host(const cusbf_filter* f, uint64_t* host_words, uint64
 
 const char* cusbf_kernel_family(const cusbf_filter* f) {
     if (!f) return "";
-    return f->strip ? "lane_strip" : "generic";
+    if (!f->strip) return "generic";
+    return f->strip->k == 0 ? "lane_strip_wide" : "lane_strip";
 }
 
 int cusbf_set_insert_path(cusbf_filter* f, uint32_t path, uint32_t slice_bits,

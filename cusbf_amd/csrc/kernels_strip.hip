@@ -5,11 +5,13 @@
 
 namespace sbf::strip {
 
-#define SBF_STRIP_EXTERN(FE, K, S, M, H, W)                                                   \
-    extern template hipError_t launch_batched_insert<K, S, M, H, FE, W>(                      \
-        const SequenceInput&, FilterView, const BatchLaunch&, const RecordSink*, hipStream_t); \
-    extern template hipError_t launch_batched_contains<K, S, M, H, FE, W>(                    \
-        const SequenceInput&, FilterView, const BatchLaunch&, uint8_t*, hipStream_t);
+#define SBF_STRIP_EXTERN(FE, K, S, M, H, W)                                                    \
+    extern template hipError_t launch_batched_insert<K, S, M, H, FE, W>(                       \
+        const KernelConfig&, const SequenceInput&, FilterView, const BatchLaunch&,             \
+        const RecordSink*, hipStream_t);                                                       \
+    extern template hipError_t launch_batched_contains<K, S, M, H, FE, W>(                     \
+        const KernelConfig&, const SequenceInput&, FilterView, const BatchLaunch&, uint8_t*,   \
+        hipStream_t);
 SBF_STRIP_BATCHED_INSTANCES(SBF_STRIP_EXTERN)
 #undef SBF_STRIP_EXTERN
 
@@ -62,6 +64,17 @@ static const StripKernelSet kSets[] = {
 
 namespace sbf {
 
+// kernels_wide_dna.hip, kernels_wide_protein.hip
+const StripKernelSet* find_wide_dna(uint32_t k, uint32_t s, uint32_t m, uint32_t h);
+const StripKernelSet* find_wide_protein(uint32_t k, uint32_t s, uint32_t m, uint32_t h);
+
+const StripKernelSet* find_wide_kernels(uint32_t k, uint32_t s, uint32_t m, uint32_t h,
+                                        uint32_t alphabet) {
+    if (alphabet == kDna) return find_wide_dna(k, s, m, h);
+    if (alphabet == kProtein) return find_wide_protein(k, s, m, h);
+    return nullptr;
+}
+
 const StripKernelSet* find_strip_kernels(uint32_t k, uint32_t s, uint32_t m, uint32_t h,
                                          uint32_t alphabet) {
     for (const StripKernelSet& set : strip::kSets) {
@@ -69,7 +82,7 @@ const StripKernelSet* find_strip_kernels(uint32_t k, uint32_t s, uint32_t m, uin
             return &set;
         }
     }
-    return nullptr;
+    return find_wide_kernels(k, s, m, h, alphabet);
 }
 
 }  // namespace sbf

@@ -5,11 +5,13 @@
 
 namespace sbf::strip {
 
-#define SBF_STRIP_INSTANTIATE(FE, K, S, M, H, W)                                              \
-    template hipError_t launch_batched_insert<K, S, M, H, FE, W>(                             \
-        const SequenceInput&, FilterView, const BatchLaunch&, const RecordSink*, hipStream_t); \
-    template hipError_t launch_batched_contains<K, S, M, H, FE, W>(                           \
-        const SequenceInput&, FilterView, const BatchLaunch&, uint8_t*, hipStream_t);
+#define SBF_STRIP_INSTANTIATE(FE, K, S, M, H, W)                                               \
+    template hipError_t launch_batched_insert<K, S, M, H, FE, W>(                              \
+        const KernelConfig&, const SequenceInput&, FilterView, const BatchLaunch&,             \
+        const RecordSink*, hipStream_t);                                                       \
+    template hipError_t launch_batched_contains<K, S, M, H, FE, W>(                            \
+        const KernelConfig&, const SequenceInput&, FilterView, const BatchLaunch&, uint8_t*,   \
+        hipStream_t);
 SBF_STRIP_BATCHED_INSTANCES(SBF_STRIP_INSTANTIATE)
 #undef SBF_STRIP_INSTANTIATE
 

@@ -67,16 +67,22 @@ struct RecordSink {
     uint32_t regions = 0;
 };
 
-using InsertFn = hipError_t (*)(const SequenceInput&, FilterView, const RecordSink*, hipStream_t);
-using ContainsFn = hipError_t (*)(const SequenceInput&, FilterView, uint8_t*, hipStream_t);
-using BatchedInsertFn = hipError_t (*)(const SequenceInput&, FilterView, const BatchLaunch&,
-                                       const RecordSink*, hipStream_t);
-using BatchedContainsFn = hipError_t (*)(const SequenceInput&, FilterView, const BatchLaunch&,
-                                         uint8_t*, hipStream_t);
+// Every launcher takes the filter's configuration first: the lane strips
+// compiled for one (k, s, m, h) ignore it, those with runtime spans read k, s
+// and m from it.
+using InsertFn = hipError_t (*)(const KernelConfig&, const SequenceInput&, FilterView,
+                                const RecordSink*, hipStream_t);
+using ContainsFn = hipError_t (*)(const KernelConfig&, const SequenceInput&, FilterView, uint8_t*,
+                                  hipStream_t);
+using BatchedInsertFn = hipError_t (*)(const KernelConfig&, const SequenceInput&, FilterView,
+                                       const BatchLaunch&, const RecordSink*, hipStream_t);
+using BatchedContainsFn = hipError_t (*)(const KernelConfig&, const SequenceInput&, FilterView,
+                                         const BatchLaunch&, uint8_t*, hipStream_t);
 
-// Lane-strip kernels for one compile-time (k, s, m, h, alphabet).
+// Lane-strip kernels for one compile-time (k, s, m, h, alphabet), or (runtime
+// spans) for every (k, s, m) of one alphabet and h within the kernels' limits.
 struct StripKernelSet {
-    uint32_t k, s, m, h, alphabet;
+    uint32_t k, s, m, h, alphabet;  // k = s = m = 0: runtime spans
     bool dense;             // plain kernels also read dense packed input
     uint32_t wave_windows;  // windows per wave tile
     InsertFn insert;
@@ -85,9 +91,13 @@ struct StripKernelSet {
     BatchedContainsFn batched_contains[2];
 };
 
-// Returns the compiled lane-strip kernel set or nullptr.
+// Returns the lane-strip kernel set compiled for this configuration, else the
+// runtime-span set that covers it, else nullptr (the generic kernels).
 const StripKernelSet* find_strip_kernels(uint32_t k, uint32_t s, uint32_t m, uint32_t h,
                                          uint32_t alphabet);
+// The runtime-span set for (alphabet, h) when (k, s, m) is within its limits.
+const StripKernelSet* find_wide_kernels(uint32_t k, uint32_t s, uint32_t m, uint32_t h,
+                                        uint32_t alphabet);
 
 // Blocks of the plain lane-strip kernels (a dispatch stays below 2^32
 // work-items) and of the persistent batched kernels.

@@ -643,9 +643,8 @@ __device__ __forceinline__ void lane_windows(const WaveLds<FE, Batched>& L, int 
 // Bit o of the result: window o starts a run. Bit o of `cont`: window o
 // continues the run of the window before it (for o = 0, the previous lane's
 // last window).
-template <int K, int S, int M, int H, class FE>
-__device__ __forceinline__ uint32_t run_heads(const LaneWindows<K, S, M, H, FE>& lw, int lane,
-                                              uint32_t& cont) {
+template <class LW>
+__device__ __forceinline__ uint32_t run_heads(const LW& lw, int lane, uint32_t& cont) {
     const uint32_t prev_shard = from_prev_lane(lw.shard[kLaneWindows - 1]);
     const uint32_t prev_valid = from_prev_lane((lw.valid >> (kLaneWindows - 1)) & 1u);
     uint32_t same = 0;
@@ -660,9 +659,9 @@ __device__ __forceinline__ uint32_t run_heads(const LaneWindows<K, S, M, H, FE>&
 // Publishes the shard of every run head that has an LDS slot; `base` is the
 // slot of the lane's first head (wrapping below zero for heads of an earlier
 // chunk, which are skipped like those past the slots).
-template <int K, int S, int M, int H, class FE, bool Batched>
-__device__ __forceinline__ void publish_heads(const LaneWindows<K, S, M, H, FE>& lw, uint32_t heads,
-                                              uint32_t base, WaveLds<FE, Batched>& L) {
+template <class LW, class FE, bool Batched>
+__device__ __forceinline__ void publish_heads(const LW& lw, uint32_t heads, uint32_t base,
+                                              WaveLds<FE, Batched>& L) {
 #pragma unroll
     for (int o = 0; o < kLaneWindows; ++o) {
         if ((heads >> o) & 1u) {
@@ -897,10 +896,10 @@ __device__ __forceinline__ void contains_tile(const SequenceInput& in, const Fil
     lane_windows<K, S, M, H, FE, Batched>(L, lane, owns, first_window, num_windows,
                                           static_cast<uint32_t>(filter.num_shards - 1), lw);
     uint32_t cont;
-    const uint32_t heads = run_heads<K, S, M, H, FE>(lw, lane, cont);
+    const uint32_t heads = run_heads(lw, lane, cont);
     uint32_t total;
     const uint32_t base = wave_exclusive_sum(static_cast<uint32_t>(__popc(heads)), lane, total);
-    publish_heads<K, S, M, H, FE, Batched>(lw, heads, base, L);
+    publish_heads(lw, heads, base, L);
     wave_lds_sync();
 
     // dense shard loads: lanes 2i and 2i+1 fetch the two 16-byte halves of one
@@ -990,10 +989,10 @@ __device__ __forceinline__ void insert_tile(const SequenceInput& in, const Filte
         lane_windows<K, S, M, H, FE, Batched>(L, lane, owns, first_window, num_windows,
                                               static_cast<uint32_t>(filter.num_shards - 1), lw);
         uint32_t cont;
-        const uint32_t heads = run_heads<K, S, M, H, FE>(lw, lane, cont);
+        const uint32_t heads = run_heads(lw, lane, cont);
         uint32_t total;
         const uint32_t base = wave_exclusive_sum(static_cast<uint32_t>(__popc(heads)), lane, total) - first;
-        publish_heads<K, S, M, H, FE, Batched>(lw, heads, base, L);
+        publish_heads(lw, heads, base, L);
         // bit o: window o is the last window of its run inside this lane
         const uint32_t ends = lw.valid & ~(cont >> 1);
 
@@ -1230,8 +1229,8 @@ constexpr int plain_waves() {
 }
 
 template <int K, int S, int M, int H, class FE>
-hipError_t launch_insert(const SequenceInput& in, FilterView f, const RecordSink* sink,
-                         hipStream_t stream) {
+hipError_t launch_insert(const KernelConfig&, const SequenceInput& in, FilterView f,
+                         const RecordSink* sink, hipStream_t stream) {
     const uint64_t n = windows_of<K>(in.num_symbols);
     const RecordSink s = sink ? *sink : RecordSink{};
     uint64_t tiles = 0;
@@ -1252,7 +1251,7 @@ hipError_t launch_insert(const SequenceInput& in, FilterView f, const RecordSink
 }
 
 template <int K, int S, int M, int H, class FE>
-hipError_t launch_contains(const SequenceInput& in, FilterView f, uint8_t* out,
+hipError_t launch_contains(const KernelConfig&, const SequenceInput& in, FilterView f, uint8_t* out,
                            hipStream_t stream) {
     const uint64_t n = windows_of<K>(in.num_symbols);
     if (n == 0) return hipSuccess;
@@ -1276,8 +1275,8 @@ hipError_t launch_contains(const SequenceInput& in, FilterView f, uint8_t* out,
 // compute unit, never more blocks than the tiles need. A sorted insert with no
 // windows still launches, so that every region count is written.
 template <int K, int S, int M, int H, class FE, int WPE>
-hipError_t launch_batched_insert(const SequenceInput& in, FilterView f, const BatchLaunch& b,
-                                 const RecordSink* sink, hipStream_t stream) {
+hipError_t launch_batched_insert(const KernelConfig&, const SequenceInput& in, FilterView f,
+                                 const BatchLaunch& b, const RecordSink* sink, hipStream_t stream) {
     using Sh = Shape<K, S, M, H, FE>;
     const uint64_t n = windows_of<K>(in.num_symbols);
     if (n == 0 && sink == nullptr) return hipSuccess;
@@ -1298,8 +1297,8 @@ hipError_t launch_batched_insert(const SequenceInput& in, FilterView f, const Ba
 }
 
 template <int K, int S, int M, int H, class FE, int WPE>
-hipError_t launch_batched_contains(const SequenceInput& in, FilterView f, const BatchLaunch& b,
-                                   uint8_t* out, hipStream_t stream) {
+hipError_t launch_batched_contains(const KernelConfig&, const SequenceInput& in, FilterView f,
+                                   const BatchLaunch& b, uint8_t* out, hipStream_t stream) {
     using Sh = Shape<K, S, M, H, FE>;
     const uint64_t n = windows_of<K>(in.num_symbols);
     if (n == 0) return hipSuccess;

@@ -234,7 +234,8 @@ int cusbf_count_positive_per_record_async(const uint8_t* d_hits, const uint64_t*
 int cusbf_count_windows_runs_async(const cusbf_filter* f, const char* d_sequence, uint64_t bytes,
                                    uint64_t* d_counts, void* stream);
 
-/* Name of the kernel family the filter dispatches to ("lane_strip" or "generic"). */
+/* Name of the kernel family the filter dispatches to: "lane_strip" (compiled for this
+ * configuration), "lane_strip_wide" (runtime spans) or "generic". */
 const char* cusbf_kernel_family(const cusbf_filter* f);
 
 /* --- insert path (no reference counterpart) ---------------------------------- */

@@ -118,18 +118,21 @@ CASES = [
     (12, 10, 6, 4, "protein", 300, 1.5, 2000, "lane_strip"),
     (5, 4, 3, 4, "protein", 40, 1.5, 3000, "lane_strip"),
     (31, 28, 16, 8, "dna", 300, 1.5, 1000, "lane_strip"),  # plain lane-strip, generic batched
-    (17, 15, 9, 4, "dna", 200, 1.5, 1000, "generic"),
+    (17, 15, 9, 4, "dna", 200, 1.5, 1000, "lane_strip_wide"),
+    (17, 15, 9, 4, "dna", 200, 1.5, 1000, "generic"),  # forced generic
     (7, 6, 4, 4, "triplet", 60, 1.2, 800, "generic"),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[4]}-k{c[0]}s{c[1]}m{c[2]}h{c[3]}")
-def test_batched_matches_oracle(gpu, case):
+def test_batched_matches_oracle(gpu, monkeypatch, case):
     import cusbf_amd as cs
     from cusbf_amd import batched as bt
 
     torch = _torch()
     k, s, m, h, alphabet, median, sigma, n, family = case
+    if family == "generic":
+        monkeypatch.setenv("CUSBF_FORCE_GENERIC", "1")
     lens = lognormal_lengths(n, median, sigma, seed=k * 100 + s)
     lens[:5] = [0, 1, k - 1, k, k + 1]  # empty, shorter than k, exactly k
     off = offsets_of(lens)
@@ -379,7 +382,7 @@ def test_tuner_picks_a_measured_candidate(gpu):
         assert bt.get_tuning(f, 1, op) == bt.BatchTuning(**table["bucket1"][name]["best"])
     assert f.load_factor() == 0.0  # tuning never writes the target filter
     assert f.insert_path().path == cs.INSERT_AUTO
-    g = cs.Filter(make_config(17, 15, 9, 4), 1 << 20)
+    g = cs.Filter(make_config(7, 6, 4, 4, "triplet"), 1 << 20)  # generic kernels: nothing to tune
     assert tune.candidates(g) == [] and tune.tune(g) == {}
 
 

@@ -145,7 +145,7 @@ def expected(cfg_t, records, query_records, num_shards):
     ("dna", (31, 28, 16, 4, "dna"), "fasta"),
     ("dna", (31, 28, 16, 4, "dna"), "fastq"),
     ("protein", (10, 8, 6, 4, "protein"), "fasta"),
-    ("dna", (17, 15, 9, 4, "dna"), "fastq"),  # generic kernels
+    ("dna", (17, 15, 9, 4, "dna"), "fastq"),  # runtime-span lane strips
 ])
 def test_fastx_file_insert_query_match_oracle(gpu, tmp_path, alphabet, cfg_t, fmt):
     records = make_records(alphabet, 1500, seed=len(fmt) + cfg_t[0])

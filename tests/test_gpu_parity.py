@@ -197,8 +197,10 @@ def test_lane_strip_parity(gpu, alphabet, k, s, m, h):
 @pytest.mark.parametrize("alphabet,k,s,m,h", GENERIC)
 def test_generic_parity(gpu, monkeypatch, form, alphabet, k, s, m, h):
     # both forms of the generic kernels (per-window recomputation, hashed tiles
-    # with sparse tables) against the oracle, whichever the spans would pick
+    # with sparse tables) against the oracle, whichever the spans would pick;
+    # configurations the runtime-span lane strips cover are forced onto them
     monkeypatch.setenv("CUSBF_GENERIC_FORM", form)
+    monkeypatch.setenv("CUSBF_FORCE_GENERIC", "1")
     f = _parity(alphabet, k, s, m, h, 60_000, seed=k * 17 + h, invalid_rate=0.003)
     assert f.kernel_family() == "generic"
 

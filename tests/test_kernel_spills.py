@@ -20,7 +20,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "cusbf_amd", "csrc")
-SOURCES = ["kernels_strip.hip", "kernels_strip_batched.hip", "kernels_generic.hip",
+SOURCES = ["kernels_strip.hip", "kernels_strip_batched.hip", "kernels_wide_dna.hip",
+           "kernels_wide_protein.hip", "kernels_generic.hip",
            "sorted_insert.hip", "synth.hip", "probe.hip"]
 
 

@@ -3,8 +3,8 @@
 Its answers must equal cusbf_contains_batched_async's byte for byte on the same
 batch: two calls in a row on one GPU, the second with a smaller batch (the
 shared scratch keeps the first call's bitmap, which the second must clear),
-for a lane-strip DNA configuration, the protein front end and a generic
-configuration. Malformed offsets are reported by cusbf_batched_status, and a
+for a lane-strip DNA configuration, the protein front end, a configuration of
+the runtime-span lane strips and one of the generic kernels (DNA triplets). Malformed offsets are reported by cusbf_batched_status, and a
 batch above the scratch's limit is refused before anything is launched.
 """
 from __future__ import annotations
@@ -34,7 +34,7 @@ def _query(lib, fn, f, d_data, n_sym, d_off, n_rec, out):
 
 
 @pytest.mark.parametrize("alphabet,k,s,m,h", [("dna", 31, 28, 16, 4), ("protein", 10, 8, 6, 4),
-                                              ("dna", 27, 25, 13, 4)])
+                                              ("dna", 27, 25, 13, 4), ("triplet", 7, 5, 4, 4)])
 def test_noalloc_equals_batched_query_twice(gpu, alphabet, k, s, m, h):
     import torch
 
