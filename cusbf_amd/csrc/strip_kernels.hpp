@@ -301,10 +301,10 @@ __device__ __forceinline__ uint32_t encode_protein4(uint32_t x, uint32_t& invali
     return codes;
 }
 
-__device__ __forceinline__ void stage_lane_raw(const RawChunk& r, const SequenceInput& in, uint64_t first,
-                                               int lane, Protein5::Lds& st, uint32_t* inv) {
-    u32x4 codes;
-    uint32_t invalid;
+// The 16 code bytes and the invalid mask of a lane's chunk (loaded ahead in `r`,
+// or read byte by byte when it is not a fast chunk).
+__device__ __forceinline__ void encode_protein_chunk(const RawChunk& r, const SequenceInput& in, uint64_t first,
+                                                     u32x4& codes, uint32_t& invalid) {
     if (r.fast) {
         const u32x4 v = r.v;
         uint32_t i0, i1, i2, i3;
@@ -328,6 +328,13 @@ __device__ __forceinline__ void stage_lane_raw(const RawChunk& r, const Sequence
         }
         codes = u32x4{w[0], w[1], w[2], w[3]};
     }
+}
+
+__device__ __forceinline__ void stage_lane_raw(const RawChunk& r, const SequenceInput& in, uint64_t first,
+                                               int lane, Protein5::Lds& st, uint32_t* inv) {
+    u32x4 codes;
+    uint32_t invalid;
+    encode_protein_chunk(r, in, first, codes, invalid);
     st.c[lane] = codes;
     inv[lane] = invalid;
     if (lane < kLdsWords - kWave) {
@@ -506,6 +513,8 @@ __device__ __forceinline__ void code_masks(const uint64_t& h, uint64_t (&m)[4]) 
 
 template <class FE, bool Batched>
 struct WaveLds {
+    static constexpr int kSlots = FE::kSlots;
+    static constexpr bool kBatched = Batched;
     typename FE::Lds st;
     uint32_t inv[kLdsWords];
     uint32_t bnd[Batched ? kLdsWords : 1];  // bit i of word l: symbol 16l+i starts a record
@@ -659,23 +668,22 @@ __device__ __forceinline__ uint32_t run_heads(const LW& lw, int lane, uint32_t& 
 // Publishes the shard of every run head that has an LDS slot; `base` is the
 // slot of the lane's first head (wrapping below zero for heads of an earlier
 // chunk, which are skipped like those past the slots).
-template <class LW, class FE, bool Batched>
-__device__ __forceinline__ void publish_heads(const LW& lw, uint32_t heads, uint32_t base,
-                                              WaveLds<FE, Batched>& L) {
+template <class LW, class Lds>
+__device__ __forceinline__ void publish_heads(const LW& lw, uint32_t heads, uint32_t base, Lds& L) {
 #pragma unroll
     for (int o = 0; o < kLaneWindows; ++o) {
         if ((heads >> o) & 1u) {
             const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((1u << o) - 1u)));
-            if (slot < static_cast<uint32_t>(FE::kSlots)) L.slot_shard[slot] = lw.shard[o];
+            if (slot < static_cast<uint32_t>(Lds::kSlots)) L.slot_shard[slot] = lw.shard[o];
         }
     }
 }
 
 // Zeroes the slots' mask words (each lane its share).
-template <class FE, bool Batched>
-__device__ __forceinline__ void zero_slots(WaveLds<FE, Batched>& L, int lane) {
+template <class Lds>
+__device__ __forceinline__ void zero_slots(Lds& L, int lane) {
 #pragma unroll
-    for (int j = 0; j < FE::kSlots / kWave; ++j) {
+    for (int j = 0; j < Lds::kSlots / kWave; ++j) {
         const u32x4 z = {0u, 0u, 0u, 0u};
         u32x4* dst = reinterpret_cast<u32x4*>(L.slot_w[lane + j * kWave]);
         dst[0] = z;
@@ -711,9 +719,9 @@ __device__ __forceinline__ void stage_tile(const SequenceInput& in, uint64_t til
 }
 
 // The halo record-start words stay zero (the halo symbols are all invalid).
-template <class FE, bool Batched>
-__device__ __forceinline__ void init_wave_lds(WaveLds<FE, Batched>& L, int lane) {
-    if constexpr (Batched) {
+template <class Lds>
+__device__ __forceinline__ void init_wave_lds(Lds& L, int lane) {
+    if constexpr (Lds::kBatched) {
         if (lane < kLdsWords - kWave) L.bnd[kWave + lane] = 0;
         wave_lds_sync();
     }

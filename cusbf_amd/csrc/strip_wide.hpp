@@ -112,6 +112,23 @@ Spans wide_spans(const KernelConfig& c) {
     return sp;
 }
 
+// Per-wave LDS of the runtime-span kernels: WaveLds with 512 run slots. A
+// tile has at most one run per window (992 or 1008), and short minimizer spans
+// give several hundred (2 / (w + 1) per window on random input): 512 slots take
+// every tile down to w = 3 in one chunk. 19-20 KB per wave, so two 4-wave blocks
+// (two waves per SIMD) share a compute unit.
+template <class FE, bool Batched>
+struct WideLds {
+    static constexpr int kSlots = 512;
+    static constexpr bool kBatched = Batched;
+    typename FE::Lds st;
+    uint32_t inv[kLdsWords];
+    uint32_t bnd[Batched ? kLdsWords : 1];  // bit i of word l: symbol 16l+i starts a record
+    uint32_t slot_shard[kSlots];
+    alignas(16) uint64_t slot_w[kSlots][4];
+};
+static_assert(4 * sizeof(WideLds<Protein5, true>) <= 80 * 1024, "two blocks per compute unit");
+
 // --- per-lane values ---------------------------------------------------------------
 
 // MSB-first values of the 32 symbols starting at each own position p = 0..15
@@ -241,7 +258,7 @@ struct WideLanes {
 };
 
 template <class FE, int H, bool Batched>
-__device__ __forceinline__ void wide_lane_windows(const WaveLds<FE, Batched>& L, int lane, bool owns_windows,
+__device__ __forceinline__ void wide_lane_windows(const WideLds<FE, Batched>& L, int lane, bool owns_windows,
                                                   uint64_t first_window, uint64_t num_windows,
                                                   uint32_t shard_mask, const Spans& sp,
                                                   WideLanes<FE, H>& lw) {
@@ -335,7 +352,7 @@ __device__ __forceinline__ void window_words(const WideLanes<FE, H>& lw, int q, 
 // or the filter for a run past the slots).
 template <class FE, int H, bool Batched, bool AllSlotted>
 __device__ __forceinline__ void wide_findere(const WideLanes<FE, H>& lw, uint32_t heads, uint32_t base,
-                                             const WaveLds<FE, Batched>& L, const FilterView& filter,
+                                             const WideLds<FE, Batched>& L, const FilterView& filter,
                                              const Spans& sp, uint32_t (&result)[4]) {
     // window o's slot: its run's, counted from the lane's first head (slot base);
     // windows before that head continue the previous lane's last run (base - 1)
@@ -365,7 +382,7 @@ __device__ __forceinline__ void wide_findere(const WideLanes<FE, H>& lw, uint32_
             } else {
                 w = 0;
                 if ((lw.valid >> o) & 1u) {
-                    w = slot[o] < static_cast<uint32_t>(FE::kSlots) ? L.slot_w[slot[o]][q]
+                    w = slot[o] < static_cast<uint32_t>(WideLds<FE, Batched>::kSlots) ? L.slot_w[slot[o]][q]
                                                                     : filter.words[4ull * lw.shard[o] + q];
                 }
             }
@@ -380,25 +397,129 @@ __device__ __forceinline__ void wide_findere(const WideLanes<FE, H>& lw, uint32_
     }
 }
 
+// The findere test of the tiles with many runs (wide_contains_tile_chunked): the
+// slots hold the runs first .. first + 511 of the tile, and only windows whose
+// run is among them are answered.
+template <class FE, int H, bool Batched>
+__device__ __forceinline__ void wide_findere_chunk(const WideLanes<FE, H>& lw, uint32_t heads, uint32_t base,
+                                             uint32_t first, const WideLds<FE, Batched>& L,
+                                             const Spans& sp, uint32_t (&result)[4]) {
+    constexpr uint32_t kSlotsU = static_cast<uint32_t>(WideLds<FE, Batched>::kSlots);
+    // window o's slot: its run's, counted from the lane's first head (slot base);
+    // windows before that head continue the previous lane's last run (base - 1)
+    uint32_t in_chunk = lw.valid;
+#pragma unroll
+    for (int o = 0; o < kLaneWindows; ++o) {
+        const uint32_t slot = base + static_cast<uint32_t>(__popc(heads & ((2u << o) - 1u))) - 1u - first;
+        if (slot >= kSlotsU) in_chunk &= ~(1u << o);
+    }
+    // LDS address of slot base - 1 - first; windows outside the chunk do not read
+    const char* slot0 = reinterpret_cast<const char*>(&L.slot_w[0][0]) +
+                        32 * (static_cast<int>(base) - 1 - static_cast<int>(first));
+    uint32_t miss[kLaneWindows];
+#pragma unroll
+    for (int o = 0; o < kLaneWindows; ++o) miss[o] = 0;
+    // One shard word per iteration, kept a loop so that the words' mask arrays
+    // are never live at once. Word q of window o's mask is T(o) | T(o + fd)
+    // (doubling over the s-mers' bits); since (T | U) & ~w = (T & ~w) | (U & ~w),
+    // T and the shifted U are tested one after the other on one array.
+    using Lim = WideLimits<FE>;
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        uint64_t s[WideShape<FE>::kSmerPositions];
+#pragma unroll
+        for (int p = 0; p < WideShape<FE>::kSmerPositions; ++p) s[p] = word_bits<H>(lw.code[p], q);
+        doubling<Lim::kMaxF>(s, sp.fl, OrOp{});
+        for (int part = 0; part < 2; ++part) {  // wave-uniform
+            if (part == 1) {
+                if (sp.fd == 0) break;
+                shift_down<max_span_rest(Lim::kMaxF)>(s, sp.fd);
+            }
+            const char* at = slot0 + 8 * q;
+#pragma unroll
+            for (int o = 0; o < kLaneWindows; ++o) {
+                at += static_cast<int>(__builtin_amdgcn_ubfe(heads, o, 1)) << 5;  // the run of window o
+                uint64_t w = 0;
+                if ((in_chunk >> o) & 1u) w = *reinterpret_cast<const uint64_t*>(at);
+                const uint64_t m = s[o] & ~w;
+                miss[o] |= lo32(m) | hi32(m);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < kLaneWindows; ++o) {
+        const uint32_t hit = ((in_chunk >> o) & 1u) & (miss[o] == 0 ? 1u : 0u);
+        result[o >> 2] |= hit << (8 * (o & 3));
+    }
+}
+
 // --- tile bodies ---------------------------------------------------------------------
 
-template <class FE, bool Dense, bool Batched>
+// The lane's 16 symbols (and record starts) into the wave's LDS. The halo words
+// past the wave's lanes (no symbols: all invalid, no record starts) are written
+// once per kernel by wide_init.
+template <bool Dense, bool Batched>
 __device__ __forceinline__ void wide_stage(const SequenceInput& in, uint64_t first, int lane,
-                                           WaveLds<FE, Batched>& L) {
+                                           WideLds<Dna2, Batched>& L) {
     uint32_t starts = 0;
     if constexpr (Batched) starts = record_starts16(in, first);  // issued beside the symbol load
-    stage_lane<Dense>(in, first, lane, L.st, L.inv);
+    uint32_t packed, invalid;
+    if constexpr (Dense) {
+        load_chunk_dense(in.words, in.num_words, in.num_symbols, first, packed, invalid);
+    } else {
+        load_chunk_bytes(in.seq, in.num_symbols, first, packed, invalid);
+    }
+    L.st.p[lane] = packed;
+    L.inv[lane] = invalid;
     if constexpr (Batched) L.bnd[lane] = starts;
+}
+
+template <bool Dense, bool Batched>
+__device__ __forceinline__ void wide_stage(const SequenceInput& in, uint64_t first, int lane,
+                                           WideLds<Protein5, Batched>& L) {
+    static_assert(!Dense, "the byte-code front end reads byte sequences only");
+    uint32_t starts = 0;
+    if constexpr (Batched) starts = record_starts16(in, first);
+    u32x4 codes;
+    uint32_t invalid;
+    encode_protein_chunk(load_raw(in, first), in, first, codes, invalid);
+    L.st.c[lane] = codes;
+    L.inv[lane] = invalid;
+    if constexpr (Batched) L.bnd[lane] = starts;
+}
+
+__device__ __forceinline__ void wide_halo(Dna2::Lds& st, int i) { st.p[i] = 0; }
+__device__ __forceinline__ void wide_halo(Protein5::Lds& st, int i) { st.c[i] = u32x4{0u, 0u, 0u, 0u}; }
+
+template <class FE, bool Batched>
+__device__ __forceinline__ void wide_init(WideLds<FE, Batched>& L, int lane) {
+    if (lane < kLdsWords - kWave) {
+        wide_halo(L.st, kWave + lane);
+        L.inv[kWave + lane] = 0xFFFFu;
+        if constexpr (Batched) L.bnd[kWave + lane] = 0;
+    }
+    wave_lds_sync();
+}
+
+// First window of the lane's 16 in tile `tile`. The lane's offset 16 * lane is
+// computed per tile by an instruction the compiler cannot hoist, so that no copy
+// of it is kept live across the tile loop (one that spilled in the kernels
+// closest to the register budget).
+template <int WaveWindows>
+__device__ __forceinline__ uint64_t wide_lane_start(uint64_t tile, int lane) {
+    uint32_t off;
+    asm volatile("v_lshlrev_b32 %0, 4, %1" : "=v"(off) : "v"(lane));
+    return tile_start<WaveWindows>(tile) + off;
 }
 
 // Query tile (contains_tile with runtime spans).
 template <class FE, int H, bool Dense, bool Batched>
 __device__ __forceinline__ void wide_contains_tile(const SequenceInput& in, const FilterView& filter,
                                                    const Spans& sp, uint64_t num_windows, uint8_t* out,
-                                                   uint64_t tile, int lane, WaveLds<FE, Batched>& L) {
+                                                   uint64_t tile, int lane, WideLds<FE, Batched>& L) {
     using WS = WideShape<FE>;
-    const uint64_t first_window = lane_start(tile_start<WS::kWaveWindows>(tile), lane);
-    wide_stage<FE, Dense, Batched>(in, first_window, lane, L);
+    const uint64_t first_window = wide_lane_start<WS::kWaveWindows>(tile, lane);
+    wide_stage<Dense>(in, first_window, lane, L);
     wave_lds_sync();
 
     const bool owns = lane < WS::kWindowLanes;
@@ -415,24 +536,30 @@ __device__ __forceinline__ void wide_contains_tile(const SequenceInput& in, cons
     // dense shard loads: lanes 2i and 2i+1 fetch the two 16-byte halves of one
     // slot's shard in the same instruction (slots 32 j + i)
     const u32x4* shards = reinterpret_cast<const u32x4*>(filter.words);
-    const uint32_t n_slots = total < FE::kSlots ? total : FE::kSlots;
-    constexpr int kHalves = 2 * FE::kSlots / kWave;
+    const uint32_t n_slots = total < WideLds<FE, Batched>::kSlots ? total : WideLds<FE, Batched>::kSlots;
+    constexpr int kHalves = 2 * WideLds<FE, Batched>::kSlots / kWave;
     const uint32_t half = static_cast<uint32_t>(lane) & 1u;
-    u32x4 sw[kHalves];
+    // rounds of 256 slots (8 loads per lane), the second only for tiles with
+    // more runs: fewer load registers live at once
 #pragma unroll
-    for (int j = 0; j < kHalves; ++j) {
-        const uint32_t q = static_cast<uint32_t>(lane + j * kWave) >> 1;
-        if (q < n_slots) sw[j] = shards[2 * static_cast<uint64_t>(L.slot_shard[q]) + half];
-    }
+    for (int r = 0; r < kHalves / 8; ++r) {
+        if (static_cast<uint32_t>(r) * 256u >= n_slots) break;  // wave-uniform
+        u32x4 sw[8];
 #pragma unroll
-    for (int j = 0; j < kHalves; ++j) {
-        const uint32_t q = static_cast<uint32_t>(lane + j * kWave) >> 1;
-        if (q < n_slots) reinterpret_cast<u32x4*>(L.slot_w[q])[half] = sw[j];
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t q = static_cast<uint32_t>(lane + (8 * r + j) * kWave) >> 1;
+            if (q < n_slots) sw[j] = shards[2 * static_cast<uint64_t>(L.slot_shard[q]) + half];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t q = static_cast<uint32_t>(lane + (8 * r + j) * kWave) >> 1;
+            if (q < n_slots) reinterpret_cast<u32x4*>(L.slot_w[q])[half] = sw[j];
+        }
     }
     wave_lds_sync();
 
     uint32_t result[4] = {0, 0, 0, 0};
-    if (total <= FE::kSlots) {  // wave-uniform: every run of the tile has its LDS slot
+    if (total <= WideLds<FE, Batched>::kSlots) {  // wave-uniform: every run of the tile has its LDS slot
         wide_findere<FE, H, Batched, true>(lw, heads, base, L, filter, sp, result);
     } else {
         wide_findere<FE, H, Batched, false>(lw, heads, base, L, filter, sp, result);
@@ -453,21 +580,93 @@ __device__ __forceinline__ void wide_contains_tile(const SequenceInput& in, cons
     }
 }
 
+// Query tile for short minimizer spans (w <= 3: up to one run per window, more
+// than the 512 slots hold): the runs in chunks of 512, each chunk loading its
+// shards into the slots and answering its windows.
+template <class FE, int H, bool Dense, bool Batched>
+__device__ __forceinline__ void wide_contains_tile_chunked(const SequenceInput& in, const FilterView& filter,
+                                                   const Spans& sp, uint64_t num_windows, uint8_t* out,
+                                                   uint64_t tile, int lane, WideLds<FE, Batched>& L) {
+    using WS = WideShape<FE>;
+    const uint64_t first_window = wide_lane_start<WS::kWaveWindows>(tile, lane);
+    wide_stage<Dense>(in, first_window, lane, L);
+    wave_lds_sync();
+
+    // the runs in chunks of kSlots (one chunk unless the tile has more runs than
+    // slots; each further chunk recomputes the lane windows, which kept live
+    // across a chunk would spill): dense shard loads into the slots -- lanes 2i
+    // and 2i+1 fetch the two 16-byte halves of one slot's shard in the same
+    // instruction, slots 32 j + i -- then the findere test of the chunk's windows
+    constexpr uint32_t kSlotsU = static_cast<uint32_t>(WideLds<FE, Batched>::kSlots);
+    constexpr int kHalves = 2 * WideLds<FE, Batched>::kSlots / kWave;
+    const bool owns = lane < WS::kWindowLanes;
+    const u32x4* shards = reinterpret_cast<const u32x4*>(filter.words);
+    const uint32_t half = static_cast<uint32_t>(lane) & 1u;
+    uint32_t result[4] = {0, 0, 0, 0};
+    for (uint32_t first = 0;; first += kSlotsU) {  // wave-uniform
+        WideLanes<FE, H> lw;
+        wide_lane_windows<FE, H, Batched>(L, lane, owns, first_window, num_windows,
+                                          static_cast<uint32_t>(filter.num_shards - 1), sp, lw);
+        uint32_t cont;
+        const uint32_t heads = run_heads(lw, lane, cont);
+        uint32_t total;
+        const uint32_t base = wave_exclusive_sum(static_cast<uint32_t>(__popc(heads)), lane, total);
+        publish_heads(lw, heads, base - first, L);
+        wave_lds_sync();
+        const uint32_t n_slots = total - first < kSlotsU ? total - first : kSlotsU;
+        // rounds of 256 slots (8 loads per lane), the second only for tiles
+        // with more runs: fewer load registers live at once
+#pragma unroll
+        for (int r = 0; r < kHalves / 8; ++r) {
+            if (static_cast<uint32_t>(r) * 256u >= n_slots) break;  // wave-uniform
+            u32x4 sw[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t q = static_cast<uint32_t>(lane + (8 * r + j) * kWave) >> 1;
+                if (q < n_slots) sw[j] = shards[2 * static_cast<uint64_t>(L.slot_shard[q]) + half];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t q = static_cast<uint32_t>(lane + (8 * r + j) * kWave) >> 1;
+                if (q < n_slots) reinterpret_cast<u32x4*>(L.slot_w[q])[half] = sw[j];
+            }
+        }
+        wave_lds_sync();
+        // one form for every chunk: a second inlined form spills
+        wide_findere_chunk<FE, H, Batched>(lw, heads, base, first, L, sp, result);
+        wave_lds_sync();  // the next chunk or tile of this wave reuses the LDS
+        if (first + kSlotsU >= total) break;
+    }
+
+    if (!owns || first_window >= num_windows) return;
+    uint8_t* dst = out + first_window;
+    if (first_window + kLaneWindows <= num_windows && (reinterpret_cast<uintptr_t>(dst) & 15u) == 0) {
+        const u32x4 r = {result[0], result[1], result[2], result[3]};
+        __builtin_nontemporal_store(r, reinterpret_cast<u32x4*>(dst));
+    } else {
+        const uint64_t n = num_windows - first_window;
+#pragma unroll
+        for (int o = 0; o < kLaneWindows; ++o) {
+            if (static_cast<uint64_t>(o) < n) dst[o] = static_cast<uint8_t>((result[o >> 2] >> (8 * (o & 3))) & 0xFFu);
+        }
+    }
+}
+
 // Insert tile (insert_tile with runtime spans): runs get LDS slots in chunks
-// of FE::kSlots; each lane ORs the masks of its piece of every run of the
+// of 512; each lane ORs the masks of its piece of every run of the
 // chunk into the run's slot, one shard word at a time, then the wave flushes
 // the slots as records (sorted insert) or 64-bit atomic ORs.
 template <class FE, int H, bool Dense, bool Batched>
 __device__ __forceinline__ void wide_insert_tile(const SequenceInput& in, const FilterView& filter,
                                                  const Spans& sp, uint64_t num_windows, uint64_t tile,
-                                                 int lane, WaveLds<FE, Batched>& L, const RecordSink& sink,
+                                                 int lane, WideLds<FE, Batched>& L, const RecordSink& sink,
                                                  uint32_t region, uint32_t& emitted) {
     using WS = WideShape<FE>;
-    constexpr uint32_t kSlotsU = static_cast<uint32_t>(FE::kSlots);
+    constexpr uint32_t kSlotsU = static_cast<uint32_t>(WideLds<FE, Batched>::kSlots);
     const bool owns = lane < WS::kWindowLanes;
-    const uint64_t first_window = lane_start(tile_start<WS::kWaveWindows>(tile), lane);
+    const uint64_t first_window = wide_lane_start<WS::kWaveWindows>(tile, lane);
 
-    wide_stage<FE, Dense, Batched>(in, first_window, lane, L);
+    wide_stage<Dense>(in, first_window, lane, L);
     uint32_t first = 0;  // first run of the chunk (wave-uniform)
     for (;;) {
         zero_slots(L, lane);
@@ -507,7 +706,7 @@ __device__ __forceinline__ void wide_insert_tile(const SequenceInput& in, const 
         const uint32_t n_slots = total - first < kSlotsU ? total - first : kSlotsU;
         const uint32_t room = sink.pairs != nullptr ? sink.cap - emitted : 0u;
 #pragma unroll
-        for (int j = 0; j < FE::kSlots / kWave; ++j) {
+        for (int j = 0; j < WideLds<FE, Batched>::kSlots / kWave; ++j) {
             const uint32_t q = static_cast<uint32_t>(lane + j * kWave);
             if (q < n_slots) {
                 if (q < room) {
@@ -538,17 +737,24 @@ __device__ __forceinline__ void wide_insert_tile(const SequenceInput& in, const 
 
 constexpr int kWideWaves = 2;
 
-template <class FE, int H, bool Dense>
+// Chunked: the query tiles of short minimizer spans (wide_contains_tile_chunked),
+// a kernel of its own so that each form gets its own register allocation.
+template <class FE, int H, bool Dense, bool Chunked>
 __global__ __launch_bounds__(kBlock, kWideWaves) void wide_contains_kernel(SequenceInput in, FilterView filter,
                                                                            Spans sp, uint64_t num_windows,
                                                                            uint64_t num_tiles, uint8_t* out) {
-    __shared__ WaveLds<FE, false> lds[kWavesPerBlock];
+    __shared__ WideLds<FE, false> lds[kWavesPerBlock];
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = wave_in_block();
+    wide_init(lds[wave], lane);
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
     for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wave; tile < num_tiles;
          tile += stride) {
-        wide_contains_tile<FE, H, Dense, false>(in, filter, sp, num_windows, out, tile, lane, lds[wave]);
+        if constexpr (Chunked) {
+            wide_contains_tile_chunked<FE, H, Dense, false>(in, filter, sp, num_windows, out, tile, lane, lds[wave]);
+        } else {
+            wide_contains_tile<FE, H, Dense, false>(in, filter, sp, num_windows, out, tile, lane, lds[wave]);
+        }
     }
 }
 
@@ -556,10 +762,11 @@ template <class FE, int H, bool Dense>
 __global__ __launch_bounds__(kBlock, kWideWaves) void wide_insert_kernel(SequenceInput in, FilterView filter,
                                                                          Spans sp, uint64_t num_windows,
                                                                          uint64_t num_tiles, RecordSink sink) {
-    __shared__ WaveLds<FE, false> lds[kWavesPerBlock];
+    __shared__ WideLds<FE, false> lds[kWavesPerBlock];
     mark_words_written(filter);
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = wave_in_block();
+    wide_init(lds[wave], lane);
     const uint32_t region = blockIdx.x * kWavesPerBlock + static_cast<uint32_t>(wave);
     uint32_t emitted = 0;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
@@ -577,11 +784,11 @@ template <class FE, int H, bool Queue>
 __global__ __launch_bounds__(kBlock, kWideWaves) void wide_batched_contains_kernel(
     SequenceInput in, FilterView filter, Spans sp, uint64_t num_windows, uint32_t num_tiles, uint32_t claim,
     uint8_t* out) {
-    __shared__ WaveLds<FE, true> lds[kWavesPerBlock];
+    __shared__ WideLds<FE, true> lds[kWavesPerBlock];
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = wave_in_block();
     if (__builtin_amdgcn_readfirstlane(in.state[1]) != 0) return;  // malformed offsets
-    init_wave_lds(lds[wave], lane);
+    wide_init(lds[wave], lane);
     for_each_tile<Queue>(in.state, num_tiles, claim, lane, wave, [&](uint32_t tile) {
         wide_contains_tile<FE, H, false, true>(in, filter, sp, num_windows, out, tile, lane, lds[wave]);
     });
@@ -591,14 +798,15 @@ template <class FE, int H, bool Queue>
 __global__ __launch_bounds__(kBlock, kWideWaves) void wide_batched_insert_kernel(
     SequenceInput in, FilterView filter, Spans sp, uint64_t num_windows, uint32_t num_tiles, uint32_t claim,
     RecordSink sink) {
-    __shared__ WaveLds<FE, true> lds[kWavesPerBlock];
+    __shared__ WideLds<FE, true> lds[kWavesPerBlock];
     mark_words_written(filter);
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int wave = wave_in_block();
+    wide_init(lds[wave], lane);
     const uint32_t region = blockIdx.x * kWavesPerBlock + static_cast<uint32_t>(wave);
     uint32_t emitted = 0;
     if (__builtin_amdgcn_readfirstlane(in.state[1]) == 0) {  // else malformed offsets: no work
-        init_wave_lds(lds[wave], lane);
+        wide_init(lds[wave], lane);
         for_each_tile<Queue>(in.state, num_tiles, claim, lane, wave, [&](uint32_t tile) {
             wide_insert_tile<FE, H, false, true>(in, filter, sp, num_windows, tile, lane, lds[wave], sink,
                                                  region, emitted);
@@ -654,16 +862,29 @@ hipError_t launch_wide_contains(const KernelConfig& c, const SequenceInput& in, 
     uint64_t tiles;
     const uint32_t grid = wide_grid<FE>(n, tiles);
     const Spans sp = wide_spans<FE>(c);
+    // minimizer spans up to 3 give 2 / (w + 1) >= 1/2 runs per window: most tiles
+    // hold more runs than the 512 slots
+    const bool chunked = sp.w <= 3;
     (void)hipGetLastError();  // report only this launch
     if constexpr (FE::kBits == 2) {
         if (in.dense) {
-            hipLaunchKernelGGL((wide_contains_kernel<FE, H, true>), dim3(grid), dim3(kBlock), 0, stream, in, f,
-                               sp, n, tiles, out);
+            if (chunked) {
+                hipLaunchKernelGGL((wide_contains_kernel<FE, H, true, true>), dim3(grid), dim3(kBlock), 0, stream,
+                                   in, f, sp, n, tiles, out);
+            } else {
+                hipLaunchKernelGGL((wide_contains_kernel<FE, H, true, false>), dim3(grid), dim3(kBlock), 0, stream,
+                                   in, f, sp, n, tiles, out);
+            }
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((wide_contains_kernel<FE, H, false>), dim3(grid), dim3(kBlock), 0, stream, in, f, sp, n,
-                       tiles, out);
+    if (chunked) {
+        hipLaunchKernelGGL((wide_contains_kernel<FE, H, false, true>), dim3(grid), dim3(kBlock), 0, stream, in, f,
+                           sp, n, tiles, out);
+    } else {
+        hipLaunchKernelGGL((wide_contains_kernel<FE, H, false, false>), dim3(grid), dim3(kBlock), 0, stream, in, f,
+                           sp, n, tiles, out);
+    }
     return hipGetLastError();
 }
 

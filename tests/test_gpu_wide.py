@@ -6,8 +6,10 @@ benchmarks/param_sweep.cu:263-530) and every protein configuration with k <= 12
 (its protein grid, :534-591). Filter words and answers must equal the CPU
 oracle's: sweep corners (findere span 1..16, minimizer span 1..24, every hash
 count), small k, protein, the atomic and the sorted insert, dense packed input,
-the batched record form (strided and work-queue kernels), tile seams, and the
-compile-time lane strips' configurations run through the runtime-span kernels.
+the batched record form (strided and work-queue kernels), tile seams, the
+chunked query of minimizer spans up to 3 (more runs per tile than LDS slots),
+and the compile-time lane strips' configurations run through the runtime-span
+kernels.
 """
 from __future__ import annotations
 
@@ -42,6 +44,7 @@ WIDE = [
     ("dna", 31, 25, 18, 8),   # f 7, w 14
     ("dna", 31, 16, 16, 4),   # f 16, w 16
     ("dna", 31, 19, 26, 16),  # f 13, w 6
+    ("dna", 31, 22, 29, 8),   # f 10, w 3 (chunked query: more runs than slots)
     # other k
     ("dna", 32, 20, 32, 8),
     ("dna", 32, 17, 9, 4),
@@ -143,7 +146,8 @@ def test_wide_tile_seams(gpu, n):
     _check(f, "dna", 31, 20, 10, 8, seq, seq + query)
 
 
-@pytest.mark.parametrize("case", [("dna", 31, 16, 8, 4), ("dna", 31, 22, 20, 12)], ids=_ids)
+@pytest.mark.parametrize("case", [("dna", 31, 16, 8, 4), ("dna", 31, 22, 20, 12), ("dna", 31, 24, 31, 8)],
+                         ids=_ids)
 def test_wide_dense_parity(gpu, case):
     import cusbf_amd as cs
 
