@@ -244,47 +244,84 @@ def nematode_scenario(steps: int, warmup: int, fpr_kmers: int) -> dict:
     return res
 
 
-# Points of the reference's (s, m, h) grid (benchmarks/param_sweep.cu:263-300, k = 31),
-# corners and middle: (31, 28, 16, 8) and (31, 28, 16, 16) have lane-strip kernels; the
-# rest run the generic kernels (hashed tiles where the windows span many m-mers and
-# s-mer bit addresses, per-window recomputation elsewhere).
-SWEEP_POINTS = [(31, 28, 16, 8), (31, 16, 8, 4), (31, 20, 12, 8), (31, 24, 16, 12), (31, 28, 16, 16),
-                (31, 28, 24, 4), (31, 30, 20, 8), (31, 31, 31, 4)]
+# Points of the reference's parameter grids (benchmarks/param_sweep.cu:263-591): DNA k = 31,
+# s = 16..31, m = 8..31, h = 4..16, corners and middle, and protein k = 12, s = 6..12,
+# m = 4..12. (31, 28, 16, 8), (31, 28, 16, 16), (31, 28, 24, 4), (31, 30, 20, 8) and
+# (31, 31, 31, 4) have lane strips compiled for them; the others run the runtime-span lane
+# strips. Each point is also timed on the generic kernels (one window per thread,
+# CUSBF_FORCE_GENERIC), which served them before the runtime-span kernels existed.
+SWEEP_POINTS = [(31, 28, 16, 8), (31, 16, 8, 4), (31, 20, 12, 8), (31, 24, 16, 12), (31, 18, 10, 16),
+                (31, 22, 18, 4), (31, 26, 12, 8), (31, 18, 26, 8), (31, 24, 29, 4), (31, 20, 31, 8),
+                (31, 28, 16, 16), (31, 28, 24, 4), (31, 30, 20, 8), (31, 31, 31, 4)]
+PROTEIN_SWEEP_POINTS = [(12, 6, 4, 16), (12, 8, 6, 8), (12, 9, 8, 4), (12, 10, 6, 4), (12, 12, 12, 4)]
+
+
+def _sweep_filter(cfg, bits, generic: bool):
+    import cusbf_amd as cs
+
+    if generic:
+        os.environ["CUSBF_FORCE_GENERIC"] = "1"
+    try:
+        return cs.Filter(cfg, bits)
+    finally:
+        os.environ.pop("CUSBF_FORCE_GENERIC", None)
+
+
+def _sweep_times(f, seq, query, hits, sh, reps):
+    import torch
+
+    stream = torch.cuda.current_stream()
+    n_q = query.numel() - f.config.k + 1
+    ti, tq = [], []
+    for _ in range(reps):
+        f.clear(sh)
+        ev = _events(3)
+        ev[0].record(stream)
+        f.insert_sequence_async(seq, stream=sh)
+        ev[1].record(stream)
+        f.contains_sequence_async(query, hits[:n_q], stream=sh)
+        ev[2].record(stream)
+        ev[2].synchronize()
+        ti.append(ev[0].elapsed_time(ev[1]) / 1e3)
+        tq.append(ev[1].elapsed_time(ev[2]) / 1e3)
+    return statistics.median(ti), statistics.median(tq)
 
 
 def param_sweep_points(genome, query, hits, sh, reps: int = 3) -> list:
-    """Insert the nematode genome and query a uniform sequence of its length at each
-    sweep point, filter at 21.4 bits per k-mer; medians of `reps` HIP-event timings."""
-    import torch
-
+    """Insert the nematode genome and query a uniform sequence of its length at each DNA
+    sweep point (a uniform 20-letter protein sequence of the same length, inserted and
+    queried with a second one, at the protein points), filter at 21.4 bits per k-mer;
+    medians of `reps` HIP-event timings, on the dispatched kernels and on the generic ones."""
     import cusbf_amd as cs
+    from cusbf_amd import workloads as W
 
-    stream = torch.cuda.current_stream()
     out = []
-    n_ins = genome.numel() - 30
-    n_q = query.numel() - 30
-    for k, s_, m_, h_ in SWEEP_POINTS:
-        f = cs.Filter(cs.Config(k, s_, m_, h_), int(21.4 * n_ins))
-        ti, tq = [], []
-        for _ in range(reps):
-            f.clear(sh)
-            ev = _events(3)
-            ev[0].record(stream)
-            f.insert_sequence_async(genome, stream=sh)
-            ev[1].record(stream)
-            f.contains_sequence_async(query, hits[:n_q], stream=sh)
-            ev[2].record(stream)
-            ev[2].synchronize()
-            ti.append(ev[0].elapsed_time(ev[1]) / 1e3)
-            tq.append(ev[1].elapsed_time(ev[2]) / 1e3)
-        row = {"k": k, "s": s_, "m": m_, "h": h_, "kernels": f.kernel_family(),
+    protein = W.uniform_letters(genome.numel(), b"ACDEFGHIKLMNPQRSTVWY", seed=41)
+    protein_q = W.uniform_letters(genome.numel(), b"ACDEFGHIKLMNPQRSTVWY", seed=42)
+    points = [("dna", p) for p in SWEEP_POINTS] + [("protein", p) for p in PROTEIN_SWEEP_POINTS]
+    for alphabet, (k, s_, m_, h_) in points:
+        if alphabet == "dna":
+            cfg, seq, qry = cs.Config(k, s_, m_, h_), genome, query
+        else:
+            cfg, seq, qry = cs.Config(k, s_, m_, h_, alphabet=cs.ProteinAlphabet), protein, protein_q
+        n_ins = seq.numel() - k + 1
+        n_q = qry.numel() - k + 1
+        f = _sweep_filter(cfg, int(21.4 * n_ins), generic=False)
+        ti, tq = _sweep_times(f, seq, qry, hits, sh, reps)
+        row = {"alphabet": alphabet, "k": k, "s": s_, "m": m_, "h": h_, "kernels": f.kernel_family(),
                "insert_path": {1: "atomic", 2: "sorted"}[f.insert_path().last_path],
-               "insert_kmers_per_s": n_ins / statistics.median(ti),
-               "query_kmers_per_s": n_q / statistics.median(tq)}
-        out.append(row)
+               "insert_kmers_per_s": n_ins / ti, "query_kmers_per_s": n_q / tq}
         f.close()
-        log(f"[sweep] k={k} s={s_} m={m_} h={h_} {row['kernels']}: insert "
-            f"{row['insert_kmers_per_s'] / 1e9:.1f}, query {row['query_kmers_per_s'] / 1e9:.1f} G k-mers/s")
+        g = _sweep_filter(cfg, int(21.4 * n_ins), generic=True)
+        gi, gq = _sweep_times(g, seq, qry, hits, sh, reps)
+        g.close()
+        row["generic_insert_kmers_per_s"] = n_ins / gi
+        row["generic_query_kmers_per_s"] = n_q / gq
+        out.append(row)
+        log(f"[sweep] {alphabet} k={k} s={s_} m={m_} h={h_} {row['kernels']}: insert "
+            f"{row['insert_kmers_per_s'] / 1e9:.1f}, query {row['query_kmers_per_s'] / 1e9:.1f} G k-mers/s "
+            f"(generic {row['generic_insert_kmers_per_s'] / 1e9:.1f} / {row['generic_query_kmers_per_s'] / 1e9:.1f})")
+    del protein, protein_q
     return out
 
 
