@@ -785,14 +785,21 @@ def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str =
         f"({v_ins / bi / 1e9:.1f} / {v_q / bq / 1e9:.1f} G k-mers/s)")
 
     # (c) plain single launch over the records joined by separators on the device
+    # (cusbf_join_records_async; checked against a torch join of the same records)
     rec_idx = W.record_index(lens)
-    ev = _events(2)
-    ev[0].record(stream)
-    norm_a = W.join_records(data, lens, sep, rec_idx)
-    norm_q = W.join_records(query, lens, sep, rec_idx)
-    ev[1].record(stream)
-    ev[1].synchronize()
-    join_s = ev[0].elapsed_time(ev[1]) / 1e3 / 2
+    norm_a = bt.join_records_async(filt, data, d_off)
+    norm_q = bt.join_records_async(filt, query, d_off)
+    js = []
+    for _ in range(3):
+        ev = _events(2)
+        ev[0].record(stream)
+        bt.join_records_async(filt, data, d_off, d_out=norm_a, stream=sh)
+        bt.join_records_async(filt, query, d_off, d_out=norm_q, stream=sh)
+        ev[1].record(stream)
+        ev[1].synchronize()
+        js.append(ev[0].elapsed_time(ev[1]) / 1e3 / 2)
+    join_s = statistics.median(js)
+    join_checked = torch.equal(norm_q, W.join_records(query, lens, sep, rec_idx))
     f2 = cs.Filter(cfg, filt.filter_bits())
     out_n = torch.empty(norm_q.numel() - k + 1, dtype=torch.uint8, device="cuda")
     ni, nq = [], []
@@ -850,7 +857,8 @@ def batched_scenario(alphabet: str, n_records: int, reps: int, tuning_out: str =
         "speedup_vs_joined_plain_incl_join": batched_rate / (kmers / (pi_ + pq_ + 2 * join_s)),
         "query_positive_kmers": positives,
         "identical": {"words_joined_plain": bool(words_equal_norm),
-                      "answers_joined_plain": bool(same_norm)},
+                      "answers_joined_plain": bool(same_norm),
+                      "device_join_equals_torch_join": bool(join_checked)},
     }
     if not baselines:
         return result

@@ -123,6 +123,7 @@ SIGNATURES = [
     ("cusbf_insert_batched_async", i32, [P, P, u64, P, u64, P]),
     ("cusbf_contains_batched_async", i32, [P, P, u64, P, u64, P, u64, P]),
     ("cusbf_batched_status", i32, [P, P]),
+    ("cusbf_join_records_async", i32, [P, P, u64, P, u64, P, u64, P]),
     ("cusbf_contains_batched_noalloc_async", i32, [P, P, u64, P, u64, P, u64, P]),
     ("cusbf_batch_plan_create", i32, [P, u32, P, u64, P, u64, P, u64,
                                       ctypes.POINTER(CBatchTuning), ctypes.POINTER(P)]),

@@ -97,6 +97,23 @@ def contains_batched_async(filt: Filter, d_symbols, d_offsets, d_output,
         _records(d_offsets, num_records), dev_ptr(d_output), on, stream_handle(stream)))
 
 
+def join_records_async(filt: Filter, d_symbols, d_offsets, d_out=None, num_symbols: int | None = None,
+                       num_records: int | None = None, stream=None):
+    """The batched records joined by one separator symbol each, on the device (the
+    record-batch layout): returns d_out, allocated when not given."""
+    import torch
+
+    lib = _lib.load()
+    n_sym = _symbols(filt, d_symbols, num_symbols)
+    n_rec = _records(d_offsets, num_records)
+    out_len = (n_sym + max(n_rec, 1) - 1) * filt.config.symbol_width
+    if d_out is None:
+        d_out = torch.empty(max(1, out_len), dtype=torch.uint8, device="cuda")[:out_len]
+    check(lib, lib.cusbf_join_records_async(filt.handle, dev_ptr(d_symbols), n_sym, dev_ptr(d_offsets), n_rec,
+                                            dev_ptr(d_out), nbytes(d_out), stream_handle(stream)))
+    return d_out
+
+
 def batched_status(filt: Filter, stream=None) -> None:
     """Synchronises; raises CusbfError(invalid_argument) for malformed offsets."""
     lib = _lib.load()

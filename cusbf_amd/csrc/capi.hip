@@ -1390,6 +1390,22 @@ int cusbf_contains_batched_async(const cusbf_filter* f, const char* d_symbols,
                                             as_stream(stream));
 }
 
+int cusbf_join_records_async(const cusbf_filter* f, const char* d_symbols, uint64_t num_symbols,
+                             const uint64_t* d_offsets, uint64_t num_records, char* d_out,
+                             uint64_t d_out_len, void* stream) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (num_records == 0) return CUSBF_OK;
+    if (!d_offsets) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device offsets");
+    if (num_symbols && !d_symbols) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device sequence");
+    const uint64_t w = f->d.symbol_width;
+    const uint64_t need = (num_symbols + num_records - 1) * w;
+    if (need && !d_out) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device output");
+    if (d_out_len < need) return fail(CUSBF_ERR_INVALID_ARGUMENT, "joined record batch output is too small");
+    CUSBF_HIP_TRY(sbf::join_records(d_symbols, num_symbols, d_offsets, num_records, f->d.symbol_width,
+                                    f->d.separator, d_out, as_stream(stream)));
+    return CUSBF_OK;
+}
+
 int cusbf_batched_status(const cusbf_filter* f, void* stream) {
     if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
     CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));

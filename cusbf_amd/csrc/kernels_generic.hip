@@ -727,6 +727,57 @@ __global__ __launch_bounds__(kBlock) void batch_prepass_kernel(const uint64_t* o
     }
 }
 
+// Records back to back (`in`, offsets in symbols) -> the same records joined by one
+// separator symbol each (normalized_record_batch.hpp:208-232 for records whose byte
+// lengths are whole symbols): record r moves to symbol off[r] + r of `out`, and the w
+// bytes after it (before record r + 1) are separators. Each thread writes one 16-byte
+// chunk of `out`, found by a binary search over off[r] + r; inside one record the
+// chunk's bytes are 16 contiguous input bytes, read as five aligned dwords and
+// funnel-shifted. Reads stay inside `in` whatever the offsets hold.
+__global__ __launch_bounds__(kBlock) void join_records_kernel(const uint8_t* in, uint64_t in_bytes,
+                                                              const uint64_t* off, uint64_t num_records,
+                                                              uint32_t w, uint8_t sep, uint8_t* out,
+                                                              uint64_t out_bytes) {
+    const uint64_t chunks = (out_bytes + 15) / 16;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; c < chunks; c += stride) {
+        const uint64_t p0 = 16 * c;
+        // last record r with (off[r] + r) * w <= p0 (record 0 starts at 0)
+        uint64_t lo = 0, hi = num_records;
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if ((off[mid] + mid) * w <= p0) {
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        }
+        uint64_t r = lo;
+        uint64_t end = (off[r + 1] + r) * w;  // record r occupies [(off[r] + r) w, end)
+        const uint64_t src0 = p0 - r * w;
+        if (p0 + 16 <= end && (src0 & ~3ull) + 20 <= in_bytes &&
+            (reinterpret_cast<uintptr_t>(out + p0) & 15u) == 0) {
+            const uint32_t* d = reinterpret_cast<const uint32_t*>(in + (src0 & ~3ull));
+            const uint32_t sh = static_cast<uint32_t>(src0 & 3u);
+            const uint32_t a0 = d[0], a1 = d[1], a2 = d[2], a3 = d[3], a4 = d[4];
+            const u32x4 v = {__builtin_amdgcn_alignbyte(a1, a0, sh), __builtin_amdgcn_alignbyte(a2, a1, sh),
+                             __builtin_amdgcn_alignbyte(a3, a2, sh), __builtin_amdgcn_alignbyte(a4, a3, sh)};
+            *reinterpret_cast<u32x4*>(out + p0) = v;
+            continue;
+        }
+        for (uint64_t p = p0; p < p0 + 16 && p < out_bytes; ++p) {
+            while (r + 1 < num_records && p >= (off[r + 1] + r + 1) * w) {
+                ++r;
+                end = (off[r + 1] + r) * w;
+            }
+            uint8_t b = sep;
+            const uint64_t start = (off[r] + r) * w;
+            if (p >= start && p < end && p - r * w < in_bytes) b = in[p - r * w];
+            out[p] = b;
+        }
+    }
+}
+
 uint32_t grid_for(uint64_t items, uint64_t per_block, uint64_t cap) {
     uint64_t g = (items + per_block - 1) / per_block;
     if (g < 1) g = 1;
@@ -872,6 +923,20 @@ hipError_t request_counts(const KernelConfig& c, const char* seq, uint64_t num_s
     hipLaunchKernelGGL(request_counts_kernel, dim3(grid_for((n + 15) / 16, kBlock, kMaxBlocks)),
                        dim3(kBlock), 0, stream, c, seq, num_symbols, hits, req, n_req,
                        reinterpret_cast<unsigned long long*>(counts));
+    return hipGetLastError();
+}
+
+hipError_t join_records(const char* in, uint64_t num_symbols, const uint64_t* offsets,
+                        uint64_t num_records, uint32_t symbol_width, uint8_t separator, char* out,
+                        hipStream_t stream) {
+    if (num_records == 0) return hipSuccess;
+    const uint64_t out_bytes = (num_symbols + num_records - 1) * symbol_width;
+    if (out_bytes == 0) return hipSuccess;
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(join_records_kernel, dim3(grid_for((out_bytes + 15) / 16, kBlock, kMaxBlocks)),
+                       dim3(kBlock), 0, stream, reinterpret_cast<const uint8_t*>(in),
+                       num_symbols * symbol_width, offsets, num_records, symbol_width, separator,
+                       reinterpret_cast<uint8_t*>(out), out_bytes);
     return hipGetLastError();
 }
 

@@ -171,6 +171,12 @@ __host__ __device__ constexpr uint64_t start_words(uint64_t num_symbols) { retur
 hipError_t batch_prepass(const SequenceInput& in, uint64_t* starts, uint32_t* state,
                          hipStream_t stream);
 
+// Records back to back (num_records + 1 symbol offsets) -> the same records joined
+// by one separator symbol each (num_symbols + num_records - 1 symbols).
+hipError_t join_records(const char* in, uint64_t num_symbols, const uint64_t* offsets,
+                        uint64_t num_records, uint32_t symbol_width, uint8_t separator, char* out,
+                        hipStream_t stream);
+
 // Server report counts: request r owns windows [req[2r], req[2r] + req[2r+1])
 // of a batch (sorted, disjoint); counts[2r] += valid windows, counts[2r+1] +=
 // hits equal to 1 (caller zeroes counts).

@@ -329,6 +329,18 @@ int cusbf_contains_batched_async(const cusbf_filter* f, const char* d_symbols,
  * offsets were malformed. */
 int cusbf_batched_status(const cusbf_filter* f, void* stream);
 
+/* No counterpart in the reference, whose device record batches take the records
+ * joined already (insert_record_batch_async / contains_record_batch_async,
+ * filter.cuh:385-400, 603-618): the records of a batched call (back to back,
+ * num_records + 1 non-decreasing symbol offsets from 0 to num_symbols) joined by
+ * one separator symbol of the filter's alphabet each, on the device -- the layout
+ * the record-batch calls read (normalized_record_batch.hpp:208-232). d_out_len
+ * must hold (num_symbols + num_records - 1) symbols. Malformed offsets give an
+ * unspecified layout (no access outside d_symbols and d_out). */
+int cusbf_join_records_async(const cusbf_filter* f, const char* d_symbols, uint64_t num_symbols,
+                             const uint64_t* d_offsets, uint64_t num_records, char* d_out,
+                             uint64_t d_out_len, void* stream);
+
 /* No counterpart in the reference. For callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc): cusbf_contains_batched_async with the same arguments and answers,

@@ -162,6 +162,47 @@ def test_batched_matches_oracle(gpu, monkeypatch, case):
     assert exp.sum() > 0 and exp.sum() < n_win
 
 
+@pytest.mark.parametrize("alphabet,k", [("dna", 31), ("protein", 10), ("triplet", 7)])
+def test_join_records_equals_host_join(gpu, alphabet, k):
+    """cusbf_join_records_async: the batched records joined by the separator on the
+    device equal the host join (empty and one-symbol records, long records, records
+    ending inside and across 16-byte chunks), and the joined buffer through the plain
+    record-batch query answers like the batched query."""
+    from cusbf_amd import batched as bt
+
+    torch = _torch()
+    lens = lognormal_lengths(3000, 120, 1.8, seed=k, hi=5000)
+    lens[:8] = [0, 0, 1, 2, 15, 16, 17, 0]
+    off = offsets_of(lens)
+    data = symbols_for(alphabet, int(off[-1]), seed=k + 11)
+    cfg = make_config(k, k - 2, k // 2, 4, alphabet)
+    import cusbf_amd as cs
+
+    f = cs.Filter(cfg, 1 << 22)
+    want, _ = joined(data, off, alphabet)
+    got = bt.join_records_async(f, device(data), device_offsets(off))
+    torch.cuda.synchronize()
+    assert got.numel() == len(want)
+    assert bytes(got.cpu().numpy().tobytes()) == want
+    with pytest.raises(cs.CusbfError):
+        bt.join_records_async(f, device(data), device_offsets(off), d_out=got[: len(want) - 1])
+    # the joined layout is what the batched query reads without separators
+    bt.insert_batched_async(f, device(data), device_offsets(off))
+    bt.batched_status(f)
+    n_win = int(off[-1]) - k + 1
+    out = torch.zeros(n_win, dtype=torch.uint8, device="cuda")
+    bt.contains_batched_async(f, device(data), device_offsets(off), out)
+    bt.batched_status(f)
+    hits = f.contains_sequence(want)
+    _, starts = joined(data, off, alphabet)
+    got_b = out.cpu().numpy()
+    for i in range(len(lens)):
+        n = int(lens[i]) - k + 1
+        if n > 0:
+            a = int(off[i])
+            assert np.array_equal(got_b[a:a + n], hits[starts[i]:starts[i] + n])
+
+
 def test_batched_equals_plain_record_batch_words(gpu):
     """Batched insert == the plain record-batch path (host records joined by separators)."""
     import cusbf_amd as cs
