@@ -735,15 +735,35 @@ __global__ __launch_bounds__(kBlock) void batch_prepass_kernel(const uint64_t* o
 // chunk's bytes are 16 contiguous input bytes, read as five aligned dwords and
 // funnel-shifted. Reads stay inside `in` whatever the offsets hold.
 __global__ __launch_bounds__(kBlock) void join_records_kernel(const uint8_t* in, uint64_t in_bytes,
-                                                              const uint64_t* off, uint64_t num_records,
+                                                              const uint64_t* __restrict__ off,
+                                                              uint64_t num_records,
                                                               uint32_t w, uint8_t sep, uint8_t* out,
                                                               uint64_t out_bytes) {
     const uint64_t chunks = (out_bytes + 15) / 16;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
     for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; c < chunks; c += stride) {
         const uint64_t p0 = 16 * c;
-        // last record r with (off[r] + r) * w <= p0 (record 0 starts at 0)
+        // last record r with (off[r] + r) * w <= p0 (record 0 starts at 0): a binary
+        // search for the wave's first chunk (the same in every lane: one memory request
+        // per step), then a galloping search from there for the lane's own chunk
+        const uint64_t pw = p0 - 16 * static_cast<uint64_t>(threadIdx.x & 63u);
         uint64_t lo = 0, hi = num_records;
+        while (hi - lo > 1) {  // wave-uniform: scalar loads
+            const uint64_t mid = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>((lo + hi) >> 1)) |
+                                 (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(
+                                      static_cast<uint32_t>(((lo + hi) >> 1) >> 32))) << 32);
+            if ((off[mid] + mid) * w <= pw) {
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        }
+        uint64_t step = 1;
+        while (lo + step < num_records && (off[lo + step] + lo + step) * w <= p0) {
+            lo += step;
+            step <<= 1;
+        }
+        hi = lo + step < num_records ? lo + step : num_records;
         while (hi - lo > 1) {
             const uint64_t mid = lo + (hi - lo) / 2;
             if ((off[mid] + mid) * w <= p0) {
