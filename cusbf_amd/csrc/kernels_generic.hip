@@ -775,7 +775,7 @@ __global__ __launch_bounds__(kBlock) void join_records_kernel(const uint8_t* in,
         uint64_t r = lo;
         uint64_t end = (off[r + 1] + r) * w;  // record r occupies [(off[r] + r) w, end)
         const uint64_t src0 = p0 - r * w;
-        if (p0 + 16 <= end && (src0 & ~3ull) + 20 <= in_bytes &&
+        if (p0 + 16 <= end && p0 + 16 <= out_bytes && (src0 & ~3ull) + 20 <= in_bytes &&
             (reinterpret_cast<uintptr_t>(out + p0) & 15u) == 0) {
             const uint32_t* d = reinterpret_cast<const uint32_t*>(in + (src0 & ~3ull));
             const uint32_t sh = static_cast<uint32_t>(src0 & 3u);
