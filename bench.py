@@ -1066,15 +1066,29 @@ def main() -> int:
         copy_bw = probes.stream_copy(reads.data_ptr(), out.data_ptr(), copy_bytes)
         gather = probes.random_gather(words, sectors, min(s_q, 2_000_000_000))
         update = probes.random_sector_update(words, sectors, s_ins)
+        # the query's memory traffic without its arithmetic: the reads streamed in and
+        # the answers out in 1 KiB tiles, with the query's shard runs as random sectors
+        tiles = min(reads.numel(), out.numel()) // 1024
+        per_tile = max(1, round(s_q / tiles))
+        traffic_s = probes.query_traffic(reads.data_ptr(), out.data_ptr(), tiles, per_tile, words, sectors)
+        parts_s = {name: probes.query_traffic(reads.data_ptr(), out.data_ptr(), tiles, per_tile, words,
+                                              sectors, parts) * 1e3
+                   for name, parts in (("sectors_only_ms", 0), ("sectors_and_input_ms", 1),
+                                       ("sectors_and_output_ms", 2))}
         ceilings = {
             "stream_copy_GBps": copy_bw / 1e9,
             "random_32B_gather_per_s": gather,
             "random_32B_gather_GBps": gather * 32 / 1e9,
             "random_32B_update_per_s": update,
             "buffer_bytes": sectors * 32,
+            "query_traffic_probe": {"tiles": tiles, "stream_bytes_each_way": tiles * 1024,
+                                    "sectors_per_tile": per_tile, "sectors": tiles * per_tile,
+                                    "ms": traffic_s * 1e3, "query_ms_over_probe_ms": tq / traffic_s,
+                                    **parts_s},
         }
         log(f"[bench] copy {copy_bw / 1e9:.0f} GB/s, random gather {gather / 1e9:.2f} G/s, "
-            f"random update {update / 1e9:.2f} G/s")
+            f"random update {update / 1e9:.2f} G/s; query traffic alone {traffic_s * 1e3:.2f} ms "
+            f"(query {tq * 1e3:.2f} ms)")
     result = {
         "metric": METRIC,
         "value": value,

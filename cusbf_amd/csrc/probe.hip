@@ -12,7 +12,11 @@
 //  * the gather in other load forms (nontemporal, coherent atomic loads) and on
 //    other memory kinds, to see what an HBM read request costs for each;
 //  * VALU issue cost: how many cycles one wave64 instruction of a given kind
-//    holds its SIMD (the kernels' ALU time is priced with these).
+//    holds its SIMD (the kernels' ALU time is priced with these);
+//  * the query's memory traffic without its arithmetic: per 1 KiB tile of a
+//    streamed input, a given number of random 32-byte sector reads and 1 KiB
+//    written back (input, answers and shard lines of the query in the same
+//    mix, with the shard loads' sectors taken from a hash).
 // Sector indices come from a counter-based hash, so every run is reproducible.
 #include <hip/hip_runtime.h>
 
@@ -36,6 +40,38 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const u32x4* src, u32x4* d
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
         dst[i] = __builtin_nontemporal_load(src + i);
+    }
+}
+
+// Waves stride over 1 KiB tiles like the lane-strip query: each lane loads its 16
+// input bytes (nontemporal), the wave reads `per_tile` random sectors (lanes 2q
+// and 2q+1 the two halves of sector q, q = lane / 2 + 32 i), and each lane stores
+// 16 bytes (nontemporal) that depend on the loads, so none is dropped.
+// `parts`: bit 0 streams the input in, bit 1 writes the output (3: the query's mix).
+__global__ __launch_bounds__(kBlock) void traffic_kernel(const u32x4* in, u32x4* out, uint64_t tiles,
+                                                         uint32_t per_tile, const u32x4* sectors,
+                                                         uint64_t sector_mask, uint64_t seed,
+                                                         uint32_t parts, uint32_t* sink) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t half = lane & 1u;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * (kBlock / 64);
+    const uint64_t rounds = (per_tile + 31) / 32;
+    for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6); t < tiles;
+         t += stride) {
+        u32x4 acc = {0u, 0u, 0u, 0u};
+        if (parts & 1u) acc = __builtin_nontemporal_load(in + 64 * t + lane);
+        for (uint64_t i = 0; i < rounds; ++i) {
+            const uint64_t q = (lane >> 1) + 32 * i;
+            if (q < per_tile) {
+                const uint64_t sct = mix64(seed + t * per_tile + q) & sector_mask;
+                acc ^= sectors[2 * sct + half];
+            }
+        }
+        if (parts & 2u) {
+            __builtin_nontemporal_store(acc, out + 64 * t + lane);
+        } else if ((acc.x ^ acc.w) == 0x9E3779B9u) {
+            sink[0] = acc.x;  // keeps the loads; practically never stores
+        }
     }
 }
 
@@ -213,6 +249,25 @@ int cusbf_probe_gather(const void* buffer, uint64_t num_sectors, uint64_t access
     hipLaunchKernelGGL(gather_kernel, dim3(grid_of(accesses, 2 * kInFlight)), dim3(kBlock), 0,
                        reinterpret_cast<hipStream_t>(stream), static_cast<const u32x4*>(buffer),
                        num_sectors - 1, accesses, seed, d_sink);
+    return static_cast<int>(hipGetLastError());
+}
+
+// The query's traffic: `tiles` tiles of 1 KiB streamed from `in` to `out` (both
+// tiles * 1 KiB), with `per_tile` random 32-byte sector reads of `buffer`
+// (num_sectors, a power of two) per tile. At most 2^16 blocks of four waves.
+// `parts`: bit 0 streams the input in, bit 1 writes the output; with a bit clear that
+// stream is left out (d_sink receives nothing in practice).
+int cusbf_probe_traffic(const void* in, void* out, uint64_t tiles, uint32_t per_tile, const void* buffer,
+                        uint64_t num_sectors, uint64_t seed, uint32_t parts, uint32_t* d_sink,
+                        void* stream) {
+    uint64_t blocks = (tiles + 3) / 4;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 65536) blocks = 65536;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(traffic_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0,
+                       reinterpret_cast<hipStream_t>(stream), static_cast<const u32x4*>(in),
+                       static_cast<u32x4*>(out), tiles, per_tile, static_cast<const u32x4*>(buffer),
+                       num_sectors - 1, seed, parts, d_sink);
     return static_cast<int>(hipGetLastError());
 }
 

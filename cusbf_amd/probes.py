@@ -25,8 +25,10 @@ def lib():
         L.cusbf_probe_gather_form.argtypes = [P, u64, u64, u64, P, ctypes.c_int, P]
         L.cusbf_probe_alloc.argtypes = [u64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.cusbf_probe_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.cusbf_probe_traffic.argtypes = [P, P, u64, ctypes.c_uint32, P, u64, u64, ctypes.c_uint32, P, P]
         for f in (L.cusbf_probe_copy, L.cusbf_probe_gather, L.cusbf_probe_atomic,
-                  L.cusbf_probe_gather_form, L.cusbf_probe_alloc, L.cusbf_probe_valu):
+                  L.cusbf_probe_gather_form, L.cusbf_probe_alloc, L.cusbf_probe_valu,
+                  L.cusbf_probe_traffic):
             f.restype = ctypes.c_int
         _probe = L
     return _probe
@@ -64,6 +66,19 @@ def random_gather(buf_ptr: int, num_sectors: int, accesses: int, reps: int = 3) 
     t = _time(lambda i: lib().cusbf_probe_gather(buf_ptr, num_sectors, accesses, 1000 + i,
                                                  sink.data_ptr(), 0), reps)
     return accesses / t
+
+
+def query_traffic(in_ptr: int, out_ptr: int, tiles: int, per_tile: int, buf_ptr: int, num_sectors: int,
+                  parts: int = 3, reps: int = 3) -> float:
+    """Seconds (median of `reps`) for the query's memory traffic without its arithmetic:
+    `tiles` 1 KiB tiles streamed in (parts bit 0) and out (bit 1), `per_tile` random
+    32-byte sectors per tile."""
+    import torch
+
+    sink = torch.zeros(1, dtype=torch.int32, device="cuda")
+    return _time(lambda i: lib().cusbf_probe_traffic(in_ptr, out_ptr, tiles, per_tile, buf_ptr,
+                                                     num_sectors, 3000 + i, parts, sink.data_ptr(), 0),
+                 reps)
 
 
 def random_sector_update(buf_ptr: int, num_sectors: int, accesses: int, reps: int = 3) -> float:
