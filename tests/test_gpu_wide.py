@@ -206,3 +206,42 @@ def test_wide_batched(gpu, case, claim, sorted_path):
     assert np.array_equal(got[:n_win], exp)
     assert (got[n_win:] == 0xEE).all()
     assert 0 < exp.sum() < n_win
+
+
+# The reference's parameter grids (benchmarks/param_sweep.cu:263-591): DNA k = 31 with
+# s = 16..31 and m = 8, 10, .., 30, 31, protein k = 12 with s = 6..12 and m = 4, 6, .., 12;
+# each (s, m) point once, the hash count cycling through 4, 8, 12 and 16.
+DNA_GRID = [(31, s, m, (4, 8, 12, 16)[(s + m) % 4]) for s in range(16, 32)
+            for m in list(range(8, 31, 2)) + [31]]
+PROTEIN_GRID = [(12, s, m, (4, 8, 12, 16)[(s + m) % 4]) for s in range(6, 13) for m in range(4, 13, 2)]
+
+
+@pytest.mark.parametrize("alphabet,grid", [("dna", DNA_GRID), ("protein", PROTEIN_GRID)])
+def test_reference_sweep_grids_equal_oracle(gpu, alphabet, grid):
+    """Every (s, m) point of the reference's sweep grids on the dispatched kernels (runtime-span
+    or compiled lane strips): words of a seeded insert and answers of a query half present,
+    half fresh, equal to the oracle's."""
+    import cusbf_amd as cs
+
+    torch = _torch()
+    bad = []
+    for i, (k, s, m, h) in enumerate(grid):
+        seq, query = _inputs(alphabet, 12_001, seed=1000 + i)
+        f = cs.Filter(make_config(k, s, m, h, alphabet), 8 * len(seq))
+        if f.kernel_family() == "generic":
+            bad.append((k, s, m, h, "generic kernels"))
+            continue
+        f.insert_sequence_async(dev(seq))
+        torch.cuda.synchronize()
+        o = make_oracle(k, s, m, h, alphabet)
+        w = o.new_filter(f.num_shards())
+        o.insert(seq, w)
+        out = torch.empty(o.kmer_count(len(query)), dtype=torch.uint8, device="cuda")
+        f.contains_sequence_async(dev(query), out)
+        torch.cuda.synchronize()
+        if not (f.words() == w).all():
+            bad.append((k, s, m, h, "words"))
+        elif not (out.cpu().numpy() == o.contains(query, w)).all():
+            bad.append((k, s, m, h, "answers"))
+        f.close()
+    assert not bad, bad
