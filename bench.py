@@ -1017,6 +1017,35 @@ def main() -> int:
     t_pop = statistics.median(pop)
     log(f"[bench] insert into the populated filter: {t_pop * 1e3:.2f} ms")
 
+    # the insert in window ranges: a sorted workspace limit of a quarter of the single
+    # insert's, so the genome runs as several sorted inserts one after another (each
+    # range after the first loads and stores the slices it touches); the words must
+    # equal the single insert's
+    ranged = None
+    if filt.insert_path().last_path == cs.INSERT_SORTED:
+        whole_ws = filt.sorted_workspace().allocated
+        ref_words = filt.words_tensor().clone()
+        filt.set_sorted_workspace_limit(whole_ws // 4)
+        rt = []
+        for _ in range(3):
+            filt.clear(sh)
+            ev = _events(2)
+            ev[0].record(stream)
+            filt.insert_sequence_async(genome, stream=sh)
+            ev[1].record(stream)
+            ev[1].synchronize()
+            rt.append(ev[0].elapsed_time(ev[1]) / 1e3)
+        info = filt.sorted_workspace()
+        same = bool(torch.equal(filt.words_tensor(), ref_words))
+        del ref_words
+        filt.set_sorted_workspace_limit(0)
+        t_rng = statistics.median(rt)
+        ranged = {"workspace_limit_bytes": whole_ws // 4, "single_insert_workspace_bytes": whole_ws,
+                  "ranges": info.last_chunks, "insert_ms": t_rng * 1e3,
+                  "insert_kmers_per_s": v_ins / t_rng, "words_equal_single_insert": same}
+        log(f"[bench] insert in {info.last_chunks} window ranges (workspace limit "
+            f"{whole_ws / 4 / 2**30:.1f} GiB): {t_rng * 1e3:.2f} ms, words equal: {same}")
+
     placement = None
     if args.placement_check:
         # the same insert and query on further filters, each allocated while the earlier
@@ -1116,6 +1145,7 @@ def main() -> int:
         "query_ms": tq * 1e3,
         "insert_populated_ms": t_pop * 1e3,
         "insert_populated_kmers_per_s": v_ins / t_pop,
+        "insert_in_window_ranges": ranged,
         "insert_shard_runs": s_ins,
         "query_shard_runs": s_q,
         "insert_roofline_frac": bytes_ins / ti / HBM_PEAK,

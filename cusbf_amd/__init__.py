@@ -17,6 +17,7 @@ from .filter import (  # noqa: F401
     INSERT_SORTED,
     Filter,
     InsertPathInfo,
+    SortedWorkspaceInfo,
     InsertReport,
     QueryReport,
     RecordRange,
@@ -34,4 +35,5 @@ __all__ = [
     "count_positive", "encode_symbols", "pack_dense_sequence", "pack_dense_sequence_device",
     "NormalizedRecord", "normalize_record_batch", "query_layout", "Server",
     "query_record_batch_hits", "INSERT_AUTO", "INSERT_ATOMIC", "INSERT_SORTED", "InsertPathInfo",
+    "SortedWorkspaceInfo",
 ]

@@ -116,6 +116,9 @@ SIGNATURES = [
     ("cusbf_set_insert_path", i32, [P, u32, u32, u32]),
     ("cusbf_get_insert_path", i32, [P, ctypes.POINTER(u32), ctypes.POINTER(u32),
                                     ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+    ("cusbf_set_sorted_workspace_limit", i32, [P, u64]),
+    ("cusbf_get_sorted_workspace", i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                         ctypes.POINTER(u32)]),
     ("cusbf_batch_variants", u32, [P]),
     ("cusbf_batch_bucket", u32, [u64, u64]),
     ("cusbf_batch_tuning_set", i32, [P, u32, u32, ctypes.POINTER(CBatchTuning)]),

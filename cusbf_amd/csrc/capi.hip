@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -142,6 +143,8 @@ struct cusbf_filter {
     uint32_t last_path = CUSBF_INSERT_ATOMIC;
     void* d_sorted = nullptr;
     uint64_t sorted_bytes = 0;
+    uint64_t sorted_limit = 0;  // workspace limit in bytes (0: half of the free device memory)
+    uint32_t last_chunks = 0;   // sorted inserts the last plain insert took (0: atomic)
     hipEvent_t sorted_done = nullptr;  // recorded after the last sorted insert's kernels
     // Device flag, 1 while every word is known to be zero (a clear sets it,
     // every insert kernel resets it): a sorted insert then need not read the
@@ -227,16 +230,17 @@ uint32_t reserve_per_tile(const cusbf_filter* f) {
 
 // Cost model of AUTO, from MI355X measurements: the atomic insert does four
 // memory-side atomic ORs per run at ~25.6 G/s; the sorted insert moves ~45 ps
-// of record traffic per run and loads and stores each slice it touches, plus
-// about a dozen launches.
-bool sorted_pays(const cusbf_filter* f, uint64_t windows, const sbf::SortedLayout& l) {
+// of record traffic per run and, in each of its `chunks` sorted inserts, loads
+// and stores every slice it touches, plus about a dozen launches.
+bool sorted_pays(const cusbf_filter* f, uint64_t windows, const sbf::SortedLayout& l, uint64_t chunks) {
     const double w = static_cast<double>(f->cfg.k - f->cfg.m + 1);
     const double runs = static_cast<double>(windows) * 2.0 / (w + 1.0);
     const double atomic_s = runs * 4.0 / 25.6e9;
     const double slices = static_cast<double>(l.num_slices);
-    const double touched = slices * (1.0 - std::exp(-runs / slices));
+    const double n = static_cast<double>(chunks);
+    const double touched = slices * (1.0 - std::exp(-runs / n / slices));
     const double slice_bytes = 32.0 * static_cast<double>(1u << l.slice_bits);
-    const double sorted_s = runs * 45e-12 + touched * slice_bytes * 2.0 / 4.0e12 + 80e-6;
+    const double sorted_s = runs * 45e-12 + n * (touched * slice_bytes * 2.0 / 4.0e12 + 80e-6);
     return sorted_s < 0.8 * atomic_s;
 }
 
@@ -245,38 +249,144 @@ bool capturing(hipStream_t stream) {
     return hipStreamIsCapturing(stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
 }
 
+// Whether a sorted workspace of `bytes` is allowed: within the filter's limit,
+// or, with no limit set, already held or within half of the free device memory.
+bool workspace_fits(const cusbf_filter* f, uint64_t bytes) {
+    if (f->sorted_limit) return bytes <= f->sorted_limit;
+    if (bytes <= f->sorted_bytes) return true;
+    size_t free_b = 0, total_b = 0;
+    return hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+           bytes <= (static_cast<uint64_t>(free_b) + f->sorted_bytes) / 2;
+}
+
+// Layout of a lane-strip insert over `tiles` wave tiles run by `blocks` blocks
+// (4 regions each; `slack` scales the tiles a region reserves for). *ok is
+// false when the records would need 2^31 slots or more.
+sbf::SortedLayout region_layout(const cusbf_filter* f, uint64_t tiles, uint32_t blocks, double slack,
+                                bool* ok) {
+    const uint64_t regions = 4ull * blocks;
+    const uint64_t per_region = static_cast<uint64_t>(
+        std::ceil(static_cast<double>((tiles + regions - 1) / regions) * slack));
+    const uint64_t cap = std::max<uint64_t>(1, per_region * reserve_per_tile(f));
+    *ok = regions * cap < (1ull << 31);
+    if (!*ok) return sbf::SortedLayout{};
+    return sbf::sorted::sorted_layout(static_cast<uint32_t>(regions), static_cast<uint32_t>(cap),
+                                      f->num_shards, f->slice_bits);
+}
+
 // Decides the path of a lane-strip insert over `tiles` wave tiles run by
-// `blocks` blocks (4 regions each; `slack` scales the tiles a region
-// reserves for); *layout is filled when the sorted path is taken.
+// `blocks` blocks, as one sorted insert; *layout is filled when the sorted
+// path is taken.
 int choose_sorted(const cusbf_filter* f, uint64_t windows, uint64_t tiles, uint32_t blocks,
                   double slack, hipStream_t stream, sbf::SortedLayout* layout, bool* use) {
     *use = false;
     if (f->insert_path == CUSBF_INSERT_ATOMIC || windows == 0 || capturing(stream)) return CUSBF_OK;
     const bool forced = f->insert_path == CUSBF_INSERT_SORTED;
-    const uint64_t regions = 4ull * blocks;
-    const uint64_t per_region = static_cast<uint64_t>(
-        std::ceil(static_cast<double>((tiles + regions - 1) / regions) * slack));
-    const uint64_t cap = std::max<uint64_t>(1, per_region * reserve_per_tile(f));
-    if (regions * cap >= (1ull << 31)) {
+    bool ok = false;
+    const sbf::SortedLayout l = region_layout(f, tiles, blocks, slack, &ok);
+    if (!ok) {
         if (forced) return fail(CUSBF_ERR_RESOURCE, "insert is too large for one sorted insert");
         return CUSBF_OK;
     }
-    const sbf::SortedLayout l = sbf::sorted::sorted_layout(static_cast<uint32_t>(regions),
-                                                           static_cast<uint32_t>(cap), f->num_shards,
-                                                           f->slice_bits);
-    if (!forced && !sorted_pays(f, windows, l)) return CUSBF_OK;
-    if (l.bytes > f->sorted_bytes) {
-        size_t free_b = 0, total_b = 0;
-        const bool fits = hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
-                          l.bytes <= (static_cast<uint64_t>(free_b) + f->sorted_bytes) / 2;
-        if (!fits) {
-            if (forced) return fail(CUSBF_ERR_RESOURCE, "sorted insert workspace does not fit in device memory");
-            return CUSBF_OK;
-        }
+    if (!forced && !sorted_pays(f, windows, l, 1)) return CUSBF_OK;
+    if (!workspace_fits(f, l.bytes)) {
+        if (forced) return fail(CUSBF_ERR_RESOURCE, "sorted insert workspace does not fit in device memory");
+        return CUSBF_OK;
     }
     *layout = l;
     *use = true;
     return CUSBF_OK;
+}
+
+// Blocks of a sorted plain insert: 16 per compute unit at most, so that every
+// region reserves for a few hundred tiles.
+uint32_t plain_sorted_blocks(const cusbf_filter* f, uint64_t tiles) {
+    uint64_t blocks = (tiles + sbf::kStripWavesPerBlock - 1) / sbf::kStripWavesPerBlock;
+    blocks = std::min<uint64_t>(blocks, 16ull * f->compute_units);
+    blocks = std::min<uint64_t>(blocks, sbf::kStripMaxBlocks);
+    return static_cast<uint32_t>(std::max<uint64_t>(blocks, 1));
+}
+
+sbf::SortedLayout plain_layout(const cusbf_filter* f, uint64_t windows, bool* ok) {
+    const uint64_t tiles = (windows + f->strip->wave_windows - 1) / f->strip->wave_windows;
+    return region_layout(f, tiles, plain_sorted_blocks(f, tiles), 1.0, ok);
+}
+
+// A plain insert on the sorted path, as `chunks` sorted inserts over
+// consecutive ranges of `chunk_windows` windows (the last may be shorter), so
+// that one range's records fit the workspace limit and 2^31 slots.
+struct SortedChunks {
+    bool use = false;
+    uint64_t chunk_windows = 0;
+    uint64_t chunks = 0;
+    uint64_t bytes = 0;  // workspace of the first (largest) range
+};
+
+// Range starts are multiples of whole wave tiles and of the symbols of a dense
+// packed word, so a range of dense input starts on a word.
+uint64_t chunk_align(const cusbf_filter* f) {
+    const uint64_t a = f->strip->wave_windows;
+    const uint64_t b = f->d.symbol_bits ? 64u / f->d.symbol_bits : 1u;
+    return a / std::gcd(a, b) * b;
+}
+
+int plan_plain_sorted(const cusbf_filter* f, uint64_t windows, hipStream_t stream, SortedChunks* c) {
+    *c = SortedChunks{};
+    if (f->insert_path == CUSBF_INSERT_ATOMIC || windows == 0 || capturing(stream)) return CUSBF_OK;
+    const bool forced = f->insert_path == CUSBF_INSERT_SORTED;
+    const uint64_t align = chunk_align(f);
+    size_t free_b = 0, total_b = 0;
+    const double limit = f->sorted_limit ? static_cast<double>(f->sorted_limit)
+                         : hipMemGetInfo(&free_b, &total_b) == hipSuccess
+                             ? (static_cast<double>(free_b) + static_cast<double>(f->sorted_bytes)) / 2.0
+                             : 0.0;
+    uint64_t chunks = 1;
+    for (;;) {
+        uint64_t per = (windows + chunks - 1) / chunks;
+        if (chunks > 1) per = (per + align - 1) / align * align;
+        bool ok = false;
+        const sbf::SortedLayout l = plain_layout(f, per, &ok);
+        if (ok && workspace_fits(f, l.bytes)) {
+            const uint64_t n = (windows + per - 1) / per;
+            if (!forced && !sorted_pays(f, windows, l, n)) return CUSBF_OK;
+            c->use = true;
+            c->chunk_windows = per;
+            c->chunks = n;
+            c->bytes = l.bytes;
+            return CUSBF_OK;
+        }
+        if (per <= align) break;
+        // next try: scaled by how far this one is over its bounds
+        double over = 2.0;
+        if (ok && limit > 0.0) over = std::max(1.0, static_cast<double>(l.bytes) / limit);
+        if (!ok) {
+            bool fine = false;
+            const sbf::SortedLayout one = plain_layout(f, align, &fine);
+            const double slots_per_window = fine ? static_cast<double>(one.slots) / align : 1.0;
+            over = std::max(over, slots_per_window * static_cast<double>(per) / 2147483648.0);
+        }
+        chunks = std::max<uint64_t>(chunks + 1,
+                                    static_cast<uint64_t>(std::ceil(static_cast<double>(chunks) * over * 1.02)));
+    }
+    if (forced) return fail(CUSBF_ERR_RESOURCE, "sorted insert workspace does not fit its limit");
+    return CUSBF_OK;
+}
+
+// Windows [first, first + n) of `in` as an input of their own: n + k - 1
+// symbols from symbol `first` (a multiple of the symbols of a dense word).
+sbf::SequenceInput window_range(const cusbf_filter* f, const sbf::SequenceInput& in, uint64_t first,
+                                uint64_t n) {
+    sbf::SequenceInput part = in;
+    part.num_symbols = n + f->cfg.k - 1;
+    if (in.dense) {
+        const uint64_t per_word = 64u / f->d.symbol_bits;
+        const uint64_t w0 = first / per_word;
+        part.words = in.words + w0;
+        part.num_words = std::min(in.num_words - w0, (part.num_symbols + per_word - 1) / per_word);
+    } else {
+        part.seq = in.seq + first * f->d.symbol_width;
+    }
+    return part;
 }
 
 // Makes the filter's workspace hold `bytes` and orders this stream after the
@@ -318,15 +428,6 @@ hipError_t enqueue_sorted(const cusbf_filter* f, const sbf::SortedLayout& l, voi
     return e;
 }
 
-// Blocks of a sorted plain insert: 16 per compute unit at most, so that every
-// region reserves for a few hundred tiles.
-uint32_t plain_sorted_blocks(const cusbf_filter* f, uint64_t tiles) {
-    uint64_t blocks = (tiles + sbf::kStripWavesPerBlock - 1) / sbf::kStripWavesPerBlock;
-    blocks = std::min<uint64_t>(blocks, 16ull * f->compute_units);
-    blocks = std::min<uint64_t>(blocks, sbf::kStripMaxBlocks);
-    return static_cast<uint32_t>(std::max<uint64_t>(blocks, 1));
-}
-
 // Inserts into one filter run one after another whatever their streams: the
 // sorted apply rewrites whole slices, so an insert overlapping it could lose
 // updates. Inside a capture the graph's order is its caller's.
@@ -358,29 +459,38 @@ int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t str
 
 int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
     f->last_path = CUSBF_INSERT_ATOMIC;
+    f->last_chunks = 0;
     if (!strip_reads(f, in)) {
         const hipError_t e = sbf::generic_insert(f->kc, in, view(f), stream);
         if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
         return CUSBF_OK;
     }
     const uint64_t windows = in.num_symbols < f->cfg.k ? 0 : in.num_symbols - f->cfg.k + 1;
-    const uint64_t tiles = (windows + f->strip->wave_windows - 1) / f->strip->wave_windows;
-    sbf::SortedLayout l;
-    bool sorted = false;
-    int rc = choose_sorted(f, windows, tiles, plain_sorted_blocks(f, tiles), 1.0, stream, &l, &sorted);
+    SortedChunks c;
+    int rc = plan_plain_sorted(f, windows, stream, &c);
     if (rc != CUSBF_OK) return rc;
-    if (!sorted) {
+    if (!c.use) {
         const hipError_t e = f->strip->insert(f->kc, in, view(f), nullptr, stream);
         if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
         return CUSBF_OK;
     }
-    rc = begin_sorted(f, l.bytes, stream);
+    rc = begin_sorted(f, c.bytes, stream);
     if (rc != CUSBF_OK) return rc;
-    const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, [&](const sbf::RecordSink& sink) {
-        return f->strip->insert(f->kc, in, view(f), &sink, stream);
-    });
-    if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+    // Ranges one after another in stream order: each one's first kernel finds
+    // the zero-words flag reset by the previous one's insert kernel, so only
+    // the first range may skip loading its slices.
+    for (uint64_t first = 0; first < windows; first += c.chunk_windows) {
+        const uint64_t n = std::min(c.chunk_windows, windows - first);
+        const sbf::SequenceInput part = c.chunks == 1 ? in : window_range(f, in, first, n);
+        bool ok = false;
+        const sbf::SortedLayout l = plain_layout(f, n, &ok);  // no larger than the first range's
+        const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, [&](const sbf::RecordSink& sink) {
+            return f->strip->insert(f->kc, part, view(f), &sink, stream);
+        });
+        if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+    }
     f->last_path = CUSBF_INSERT_SORTED;
+    f->last_chunks = static_cast<uint32_t>(std::min<uint64_t>(c.chunks, UINT32_MAX));
     return end_sorted(f, stream);
 }
 
@@ -934,6 +1044,28 @@ int cusbf_get_insert_path(const cusbf_filter* f, uint32_t* path, uint32_t* slice
     if (slice_bits) *slice_bits = f->slice_bits;
     if (reserve_per_tile) *reserve_per_tile = f->reserve_per_tile;
     if (last_path) *last_path = f->last_path;
+    return CUSBF_OK;
+}
+
+int cusbf_set_sorted_workspace_limit(cusbf_filter* f, uint64_t bytes) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    f->sorted_limit = bytes;
+    if (bytes != 0 && f->d_sorted != nullptr && f->sorted_bytes > bytes) {
+        CUSBF_HIP_TRY(hipEventSynchronize(f->sorted_done));
+        CUSBF_HIP_TRY(hipFree(f->d_sorted));
+        f->d_sorted = nullptr;
+        f->sorted_bytes = 0;
+        f->sorted_used = false;
+    }
+    return CUSBF_OK;
+}
+
+int cusbf_get_sorted_workspace(const cusbf_filter* f, uint64_t* limit, uint64_t* allocated,
+                               uint32_t* last_chunks) {
+    if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
+    if (limit) *limit = f->sorted_limit;
+    if (allocated) *allocated = f->sorted_bytes;
+    if (last_chunks) *last_chunks = f->last_chunks;
     return CUSBF_OK;
 }
 

@@ -113,6 +113,13 @@ class InsertPathInfo:
     last_path: int  # INSERT_ATOMIC or INSERT_SORTED: the path the last insert call took
 
 
+@dataclass
+class SortedWorkspaceInfo:
+    limit: int  # bytes (0: half of the free device memory)
+    allocated: int  # bytes the workspace holds now
+    last_chunks: int  # sorted inserts the last plain insert ran as (0: atomic path)
+
+
 class Filter:
     """GPU-resident Super Bloom filter (filter.cuh:68-1848)."""
 
@@ -199,6 +206,18 @@ class Filter:
         v = [ctypes.c_uint32() for _ in range(4)]
         check(self._lib, self._lib.cusbf_get_insert_path(self._h, *[ctypes.byref(x) for x in v]))
         return InsertPathInfo(*(int(x.value) for x in v))
+
+    def set_sorted_workspace_limit(self, nbytes: int) -> None:
+        """Bound the sorted insert's workspace (0: half of the free device memory);
+        a plain insert whose records do not fit runs as several sorted inserts over
+        consecutive window ranges."""
+        check(self._lib, self._lib.cusbf_set_sorted_workspace_limit(self._h, int(nbytes)))
+
+    def sorted_workspace(self) -> SortedWorkspaceInfo:
+        lim, alloc, chunks = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()
+        check(self._lib, self._lib.cusbf_get_sorted_workspace(self._h, ctypes.byref(lim), ctypes.byref(alloc),
+                                                              ctypes.byref(chunks)))
+        return SortedWorkspaceInfo(int(lim.value), int(alloc.value), int(chunks.value))
 
     def words_device_ptr(self) -> int:
         p = ctypes.c_void_p()

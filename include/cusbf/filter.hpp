@@ -745,6 +745,18 @@ class filter {
         (void)cusbf_get_insert_path(handle_, nullptr, nullptr, nullptr, &last);
         return static_cast<InsertPath>(last);
     }
+    /// Bounds the sorted insert's workspace in bytes (0 = half of the free device
+    /// memory); a plain insert whose records do not fit runs as several sorted
+    /// inserts over consecutive window ranges.
+    [[nodiscard]] Result<void> set_sorted_workspace_limit(uint64_t bytes) {
+        return detail::check(cusbf_set_sorted_workspace_limit(handle_, bytes));
+    }
+    /// Sorted inserts the last plain insert ran as (0 when it took the atomic path).
+    [[nodiscard]] uint32_t last_sorted_chunks() const {
+        uint32_t chunks = 0;
+        (void)cusbf_get_sorted_workspace(handle_, nullptr, nullptr, &chunks);
+        return chunks;
+    }
 
     // --- filter images (no reference counterpart) ---
     /// Writes header + words to `path` (cusbf_save).

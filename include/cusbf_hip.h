@@ -266,6 +266,21 @@ int cusbf_set_insert_path(cusbf_filter* f, uint32_t path, uint32_t slice_bits,
 /* Current settings and the path the last insert call took (ATOMIC or SORTED). */
 int cusbf_get_insert_path(const cusbf_filter* f, uint32_t* path, uint32_t* slice_bits,
                           uint32_t* reserve_per_tile, uint32_t* last_path);
+/* Workspace limit of the sorted insert (no reference counterpart), in bytes;
+ * 0 (the default) allows half of the free device memory. A plain insert (byte
+ * or dense packed input) whose records would not fit the limit, or would need
+ * 2^31 record slots or more, runs as several sorted inserts over consecutive
+ * window ranges of whole wave tiles, one after another in stream order; the
+ * words are the same as from one insert (OR is idempotent). Each range after
+ * the first loads and stores the slices it touches, which the cost model of
+ * AUTO counts against the atomic path. A limit below the workspace the filter
+ * holds frees it (after the last sorted insert has finished). Batched inserts
+ * run as one sorted insert within the limit, else on the atomic path. */
+int cusbf_set_sorted_workspace_limit(cusbf_filter* f, uint64_t bytes);
+/* The limit (0: default), the bytes the workspace holds now, and the number of
+ * sorted inserts the last plain insert ran as (0 when it took the atomic path). */
+int cusbf_get_sorted_workspace(const cusbf_filter* f, uint64_t* limit, uint64_t* allocated,
+                               uint32_t* last_chunks);
 
 /* --- batched record form (no single-call counterpart in the reference) -------- */
 /*

@@ -134,6 +134,95 @@ def test_sorted_dense_packed(gpu):
     assert (f.words() == w).all()
 
 
+RANGED = [
+    (31, 28, 16, 4, "dna"),
+    (10, 8, 6, 4, "protein"),
+    (31, 23, 14, 4, "dna"),  # runtime-span kernels
+]
+
+
+@pytest.mark.parametrize("cfg_t", RANGED, ids=lambda c: f"{c[4]}-k{c[0]}s{c[1]}m{c[2]}h{c[3]}")
+def test_sorted_in_window_ranges(gpu, cfg_t):
+    """A workspace limit below one insert's records: the insert runs as several
+    sorted inserts over consecutive window ranges (each range after the first ORs
+    into the words the earlier ones left) and leaves the oracle's words."""
+    import cusbf_amd as cs
+
+    k, s, m, h, alphabet = cfg_t
+    seq = seq_for(alphabet, 1_000_003, seed=k + 3 * m)
+    cfg = make_config(k, s, m, h, alphabet)
+    bits = 1 << 22
+    whole = sorted_filter(cfg, bits)
+    whole.insert_sequence(seq)
+    one = whole.sorted_workspace()
+    assert one.last_chunks == 1 and one.limit == 0 and one.allocated > 0
+    f = sorted_filter(cfg, bits)
+    f.set_sorted_workspace_limit(one.allocated // 4)
+    assert f.insert_sequence(seq) == len(seq) - k + 1
+    info = f.sorted_workspace()
+    assert info.last_chunks >= 4, info
+    assert 0 < info.allocated <= one.allocated // 4
+    assert f.insert_path().last_path == cs.INSERT_SORTED
+    w = oracle_words(k, s, m, h, alphabet, [seq], f.num_shards())
+    assert (whole.words() == w).all()
+    assert (f.words() == w).all()
+    # a second insert of other symbols ORs into the words in ranges again
+    more = seq_for(alphabet, 700_001, seed=k + 3 * m + 1)
+    f.insert_sequence(more)
+    assert f.sorted_workspace().last_chunks >= 3
+    w2 = oracle_words(k, s, m, h, alphabet, [seq, more], f.num_shards())
+    assert (f.words() == w2).all()
+
+
+def test_sorted_in_window_ranges_dense(gpu):
+    """Dense packed input in window ranges: each range starts on a packed word."""
+    import cusbf_amd as cs
+
+    k, s, m, h = 31, 28, 16, 4
+    seq = random_dna(1_000_003, 8)
+    cfg = make_config(k, s, m, h)
+    words_in = cs.pack_dense_sequence(cfg, seq)
+    f = sorted_filter(cfg, 1 << 22)
+    f.set_sorted_workspace_limit(400_000)
+    assert f.insert_dense_packed(words_in, len(seq)) == len(seq) - k + 1
+    assert f.sorted_workspace().last_chunks >= 4
+    o = make_oracle(k, s, m, h, "dna")
+    w = o.new_filter(f.num_shards())
+    o.insert_dense(o.pack_dense(seq), len(seq), w)
+    assert (f.words() == w).all()
+
+
+def test_sorted_workspace_limit_too_small(gpu):
+    """A limit below one wave tile's workspace: SORTED reports a resource error and
+    leaves the words alone; AUTO takes the atomic path. A limit below the workspace
+    the filter holds frees it."""
+    import cusbf_amd as cs
+
+    k, s, m, h = 31, 28, 16, 4
+    seq = random_dna(200_001, 9, n_rate=0.001)
+    cfg = make_config(k, s, m, h)
+    f = sorted_filter(cfg, 1 << 22)
+    f.insert_sequence(seq)
+    assert f.sorted_workspace().allocated > 0
+    f.clear()
+    f.set_sorted_workspace_limit(4096)
+    assert f.sorted_workspace().allocated == 0
+    with pytest.raises(cs.CusbfError):
+        f.insert_sequence(seq)
+    assert not f.words().any()
+    f.set_insert_path(cs.INSERT_AUTO)
+    f.insert_sequence(seq)
+    assert f.insert_path().last_path == cs.INSERT_ATOMIC
+    assert f.sorted_workspace().last_chunks == 0
+    w = oracle_words(k, s, m, h, "dna", [seq], f.num_shards())
+    assert (f.words() == w).all()
+    f.set_sorted_workspace_limit(0)
+    f.set_insert_path(cs.INSERT_SORTED)
+    f.insert_sequence(seq)
+    assert f.sorted_workspace().last_chunks == 1
+    assert (f.words() == w).all()
+
+
 def test_sorted_record_batch(gpu):
     """Host record batch (records joined by separators, one insert launch)."""
     k, s, m, h = 31, 28, 16, 4
