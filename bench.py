@@ -410,17 +410,33 @@ def dense_scenario(genome, wl, stream_handle, host_bases: int, reps: int = 3) ->
     g.insert_sequence(host_bytes)
     t_hbq, hb = wall_query(lambda: g.contains_sequence(host_bytes))
     t_hdq, hd = wall_query(lambda: g.contains_dense_packed(host_words, hn))
+    # cusbf_host_overhead.cu: host bytes in, answers into device memory, against
+    # the device forms on the same bases (HIP events)
+    d_out = torch.empty(hw, dtype=torch.uint8, device="cuda")
+    t_hbqd, _ = wall_query(lambda: g.contains_sequence(host_bytes, d_output=d_out))
+    hd_dev_equal = bool(np.array_equal(d_out.cpu().numpy(), hb))
+    sample = genome[:hn]
+    t_dbi = timed(lambda: g.insert_sequence_async(sample, stream=stream_handle),
+                  lambda: g.clear(stream_handle))
+    t_dbq = timed(lambda: g.contains_sequence_async(sample, d_out, stream=stream_handle))
     res["host"] = {"bases": hn, "byte_insert_kmers_per_s": hw / t_hbi, "dense_insert_kmers_per_s": hw / t_hdi,
                    "byte_query_kmers_per_s": hw / t_hbq, "dense_query_kmers_per_s": hw / t_hdq,
-                   "answers_equal": bool(np.array_equal(hb, hd))}
+                   "byte_query_to_device_kmers_per_s": hw / t_hbqd,
+                   "byte_insert_ms": t_hbi * 1e3, "byte_query_ms": t_hbq * 1e3,
+                   "byte_query_to_device_ms": t_hbqd * 1e3,
+                   "device_insert_ms": t_dbi * 1e3, "device_query_ms": t_dbq * 1e3,
+                   "answers_equal": bool(np.array_equal(hb, hd)) and hd_dev_equal}
     f.close()
     g.close()
-    del words, out, genome
+    del words, out, genome, d_out, sample
     torch.cuda.empty_cache()
     d = res["device"]
     log(f"[bench] dense: device insert byte {d['byte_insert_ms']:.2f} ms / dense {d['dense_insert_ms']:.2f} ms, "
         f"query byte {d['byte_query_ms']:.2f} ms / dense {d['dense_query_ms']:.2f} ms; host insert "
-        f"{res['host']['byte_insert_kmers_per_s'] / 1e9:.2f} / {res['host']['dense_insert_kmers_per_s'] / 1e9:.2f} G k-mers/s")
+        f"{res['host']['byte_insert_kmers_per_s'] / 1e9:.2f} / {res['host']['dense_insert_kmers_per_s'] / 1e9:.2f} G k-mers/s; "
+        f"host forms on {hn / 1e6:.0f} Mbp: insert {res['host']['byte_insert_ms']:.1f} ms (device "
+        f"{res['host']['device_insert_ms']:.1f}), query to host {res['host']['byte_query_ms']:.1f} ms, to device "
+        f"{res['host']['byte_query_to_device_ms']:.1f} ms (device {res['host']['device_query_ms']:.1f})")
     return res
 
 

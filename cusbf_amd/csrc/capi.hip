@@ -16,10 +16,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cusbf_hip.h"
 #include "filter_internal.hpp"
+#include "host_pool.hpp"
 #include "launch.hpp"
 #include "sbf_common.hpp"
 
@@ -112,6 +114,12 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 }  // namespace
 
+// Pinned ring of the host-memory forms: buffers of kRingBytes; copies below
+// kRingDirect bytes go straight from the caller's memory.
+constexpr uint32_t kRingSlots = 3;
+constexpr uint64_t kRingBytes = 32ull << 20;
+constexpr uint64_t kRingDirect = 4ull << 20;
+
 struct cusbf_filter {
     cusbf_config cfg{};
     Derived d{};
@@ -153,6 +161,12 @@ struct cusbf_filter {
     uint64_t* d_zero = nullptr;
     mutable bool words_shared = false;
     hipEvent_t insert_done = nullptr;  // recorded after the last insert (outside captures)
+    // Host forms: host threads copy the caller's bytes into one pinned buffer
+    // while the DMA engine reads (or fills) another.
+    char* h_ring[kRingSlots] = {};
+    hipEvent_t ring_done[kRingSlots] = {};  // recorded after the last copy that used the buffer
+    bool ring_pending[kRingSlots] = {};
+    uint32_t ring_next = 0;
     bool insert_recorded = false;
     bool sorted_used = false;
 };
@@ -197,6 +211,109 @@ int ensure_result(cusbf_filter* f, uint64_t bytes) {
     if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
     f->result_cap = bytes;
     return CUSBF_OK;
+}
+
+// Host copy of `bytes` bytes, in parts of 4 MiB run on the host pool.
+void parallel_copy(void* dst, const void* src, uint64_t bytes) {
+    constexpr uint64_t kPart = 4ull << 20;
+    const uint64_t parts = (bytes + kPart - 1) / kPart;
+    if (parts <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    cusbf_internal::host_pool().run(parts, [&](size_t i) {
+        const uint64_t at = i * kPart;
+        std::memcpy(static_cast<char*>(dst) + at, static_cast<const char*>(src) + at,
+                    std::min(kPart, bytes - at));
+    });
+}
+
+// The next ring buffer, once the copy that last used it has finished.
+int ring_slot(cusbf_filter* f, uint32_t* slot) {
+    const uint32_t s = f->ring_next;
+    if (f->h_ring[s] == nullptr) {
+        if (f->ring_done[s] == nullptr) {
+            CUSBF_HIP_TRY(hipEventCreateWithFlags(&f->ring_done[s], hipEventDisableTiming));
+        }
+        CUSBF_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&f->h_ring[s]), kRingBytes, hipHostMallocDefault));
+    } else if (f->ring_pending[s]) {
+        CUSBF_HIP_TRY(hipEventSynchronize(f->ring_done[s]));
+    }
+    f->ring_pending[s] = false;
+    f->ring_next = (s + 1) % kRingSlots;
+    *slot = s;
+    return CUSBF_OK;
+}
+
+// Host bytes into device memory, in stream order (does not wait): host
+// threads fill one pinned buffer while the DMA engine reads the previous one.
+int copy_in(cusbf_filter* f, void* d_dst, const void* h_src, uint64_t bytes, hipStream_t stream) {
+    if (bytes < kRingDirect) {
+        CUSBF_HIP_TRY(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, stream));
+        return CUSBF_OK;
+    }
+    for (uint64_t at = 0; at < bytes; at += kRingBytes) {
+        const uint64_t n = std::min(kRingBytes, bytes - at);
+        uint32_t s = 0;
+        const int rc = ring_slot(f, &s);
+        if (rc != CUSBF_OK) return rc;
+        parallel_copy(f->h_ring[s], static_cast<const char*>(h_src) + at, n);
+        CUSBF_HIP_TRY(hipMemcpyAsync(static_cast<char*>(d_dst) + at, f->h_ring[s], n,
+                                     hipMemcpyHostToDevice, stream));
+        CUSBF_HIP_TRY(hipEventRecord(f->ring_done[s], stream));
+        f->ring_pending[s] = true;
+    }
+    return CUSBF_OK;
+}
+
+// Device bytes (ready in stream order) into host memory, and waits: the DMA
+// engine fills pinned buffers ahead while host threads empty the oldest one.
+int copy_out(cusbf_filter* f, void* h_dst, const void* d_src, uint64_t bytes, hipStream_t stream) {
+    if (bytes < kRingDirect) {
+        CUSBF_HIP_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, stream));
+        CUSBF_HIP_TRY(hipStreamSynchronize(stream));
+        return CUSBF_OK;
+    }
+    const uint64_t chunks = (bytes + kRingBytes - 1) / kRingBytes;
+    uint32_t slot_of[kRingSlots] = {};
+    auto size_of = [&](uint64_t c) { return std::min(kRingBytes, bytes - c * kRingBytes); };
+    auto issue = [&](uint64_t c) -> int {
+        uint32_t s = 0;
+        const int rc = ring_slot(f, &s);
+        if (rc != CUSBF_OK) return rc;
+        CUSBF_HIP_TRY(hipMemcpyAsync(f->h_ring[s], static_cast<const char*>(d_src) + c * kRingBytes,
+                                     size_of(c), hipMemcpyDeviceToHost, stream));
+        CUSBF_HIP_TRY(hipEventRecord(f->ring_done[s], stream));
+        f->ring_pending[s] = true;
+        slot_of[c % kRingSlots] = s;
+        return CUSBF_OK;
+    };
+    for (uint64_t c = 0; c < std::min<uint64_t>(kRingSlots, chunks); ++c) {
+        const int rc = issue(c);
+        if (rc != CUSBF_OK) return rc;
+    }
+    for (uint64_t c = 0; c < chunks; ++c) {
+        const uint32_t s = slot_of[c % kRingSlots];
+        CUSBF_HIP_TRY(hipEventSynchronize(f->ring_done[s]));
+        f->ring_pending[s] = false;
+        parallel_copy(static_cast<char*>(h_dst) + c * kRingBytes, f->h_ring[s], size_of(c));
+        if (c + kRingSlots < chunks) {  // the buffer just emptied is the next in the ring
+            const int rc = issue(c + kRingSlots);
+            if (rc != CUSBF_OK) return rc;
+        }
+    }
+    return CUSBF_OK;
+}
+
+void release_ring(cusbf_filter* f) {
+    for (uint32_t s = 0; s < kRingSlots; ++s) {
+        if (f->ring_pending[s]) (void)hipEventSynchronize(f->ring_done[s]);
+        if (f->h_ring[s]) (void)hipHostFree(f->h_ring[s]);
+        if (f->ring_done[s]) (void)hipEventDestroy(f->ring_done[s]);
+        f->h_ring[s] = nullptr;
+        f->ring_done[s] = nullptr;
+        f->ring_pending[s] = false;
+    }
 }
 
 // fastx_pipeline_state.cuh:230-234 (release_all).
@@ -745,6 +862,12 @@ int read_batch_flag(const uint32_t* d_state, hipStream_t stream) {
 
 namespace cusbf_internal {
 
+WorkerPool& host_pool() {
+    // never destroyed: its threads wait for work until the process ends
+    static WorkerPool* pool = new WorkerPool(std::clamp(std::thread::hardware_concurrency() / 2, 2u, 8u));
+    return *pool;
+}
+
 int set_error(int status, const char* message) { return fail(status, message); }
 int set_hip_error(hipError_t err, const char* file, int line) { return fail_hip(err, file, line); }
 int device(const cusbf_filter* f) { return f->device; }
@@ -955,6 +1078,7 @@ int cusbf_create(const cusbf_config* cfg, uint64_t requested_filter_bits, cusbf_
 void cusbf_destroy(cusbf_filter* f) {
     if (!f) return;
     release_staging(f);
+    release_ring(f);
     if (f->d_words) (void)hipFree(f->d_words);
     if (f->d_scalar) (void)hipFree(f->d_scalar);
     if (f->d_batch_state) (void)hipFree(f->d_batch_state);
@@ -1013,8 +1137,8 @@ int cusbf_copy_words_to_host(const cusbf_filter* f, uint64_t* host_words, uint64
     if (!f || (!host_words && num_words)) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
     if (num_words < f->num_shards * 4) return fail(CUSBF_ERR_INVALID_ARGUMENT, "host word buffer is too small");
     CUSBF_HIP_TRY(hipDeviceSynchronize());
-    CUSBF_HIP_TRY(hipMemcpy(host_words, f->d_words, f->num_shards * 32, hipMemcpyDeviceToHost));
-    return CUSBF_OK;
+    // the pinned ring is mutable state of the filter, like its staging
+    return copy_out(const_cast<cusbf_filter*>(f), host_words, f->d_words, f->num_shards * 32, nullptr);
 }
 
 const char* cusbf_kernel_family(const cusbf_filter* f) {
@@ -1088,7 +1212,8 @@ int cusbf_insert_sequence(cusbf_filter* f, const char* h_sequence, uint64_t byte
     if (symbols_of(f, bytes) < f->cfg.k) return CUSBF_OK;
     int rc = ensure_stage(f, bytes);
     if (rc != CUSBF_OK) return rc;
-    CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, h_sequence, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    rc = copy_in(f, f->d_stage, h_sequence, bytes, as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
     rc = launch_insert(f, byte_input(f, f->d_stage, bytes), as_stream(stream));
     if (rc != CUSBF_OK) return rc;
     CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
@@ -1117,12 +1242,11 @@ int cusbf_contains_sequence(const cusbf_filter* f_const, const char* h_sequence,
     int rc = ensure_stage(f, bytes);
     if (rc == CUSBF_OK) rc = ensure_result(f, n);
     if (rc != CUSBF_OK) return rc;
-    CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, h_sequence, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    rc = copy_in(f, f->d_stage, h_sequence, bytes, as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
     rc = launch_contains(f, byte_input(f, f->d_stage, bytes), f->d_result, as_stream(stream));
     if (rc != CUSBF_OK) return rc;
-    CUSBF_HIP_TRY(hipMemcpyAsync(h_output, f->d_result, n, hipMemcpyDeviceToHost, as_stream(stream)));
-    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
-    return CUSBF_OK;
+    return copy_out(f, h_output, f->d_result, n, as_stream(stream));
 }
 
 // filter.cuh:486-501
@@ -1136,7 +1260,8 @@ int cusbf_contains_sequence_host_to_device(const cusbf_filter* f_const, const ch
     if (!d_output) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null device buffer");
     int rc = ensure_stage(f, bytes);
     if (rc != CUSBF_OK) return rc;
-    CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, h_sequence, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    rc = copy_in(f, f->d_stage, h_sequence, bytes, as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
     rc = launch_contains(f, byte_input(f, f->d_stage, bytes), d_output, as_stream(stream));
     if (rc != CUSBF_OK) return rc;
     CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
@@ -1221,9 +1346,7 @@ static int stage_words(cusbf_filter* f, const uint64_t* h_words, uint64_t num_wo
                        hipStream_t stream) {
     const int rc = ensure_stage(f, num_words * sizeof(uint64_t));
     if (rc != CUSBF_OK) return rc;
-    CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, h_words, num_words * sizeof(uint64_t),
-                                 hipMemcpyHostToDevice, stream));
-    return CUSBF_OK;
+    return copy_in(f, f->d_stage, h_words, num_words * sizeof(uint64_t), stream);
 }
 
 // filter.cuh:226-240
@@ -1283,9 +1406,7 @@ int cusbf_contains_dense_packed(const cusbf_filter* f_const, const uint64_t* h_w
     rc = cusbf_contains_dense_packed_async(f, reinterpret_cast<const uint64_t*>(f->d_stage), num_words,
                                            num_symbols, f->d_result, n, stream);
     if (rc != CUSBF_OK) return rc;
-    CUSBF_HIP_TRY(hipMemcpyAsync(h_output, f->d_result, n, hipMemcpyDeviceToHost, as_stream(stream)));
-    CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
-    return CUSBF_OK;
+    return copy_out(f, h_output, f->d_result, n, as_stream(stream));
 }
 
 // filter.cuh:385-400
@@ -1337,7 +1458,8 @@ int cusbf_insert_record_batch(cusbf_filter* f, const char* h_sequence, uint64_t 
     if (!norm.empty()) {
         rc = ensure_stage(f, norm.size());
         if (rc != CUSBF_OK) return rc;
-        CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, norm.data(), norm.size(), hipMemcpyHostToDevice, as_stream(stream)));
+        rc = copy_in(f, f->d_stage, norm.data(), norm.size(), as_stream(stream));
+        if (rc != CUSBF_OK) return rc;
         rc = launch_insert(f, byte_input(f, f->d_stage, norm.size()), as_stream(stream));
         if (rc != CUSBF_OK) return rc;
         CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
@@ -1369,7 +1491,8 @@ int cusbf_query_record_batch(const cusbf_filter* f_const, const char* h_sequence
         rc = ensure_stage(f, norm.size());
         if (rc == CUSBF_OK && hits) rc = ensure_result(f, hits);
         if (rc != CUSBF_OK) return rc;
-        CUSBF_HIP_TRY(hipMemcpyAsync(f->d_stage, norm.data(), norm.size(), hipMemcpyHostToDevice, as_stream(stream)));
+        rc = copy_in(f, f->d_stage, norm.data(), norm.size(), as_stream(stream));
+        if (rc != CUSBF_OK) return rc;
         if (hits) {
             rc = launch_contains(f, byte_input(f, f->d_stage, norm.size()), f->d_result, as_stream(stream));
             if (rc != CUSBF_OK) return rc;

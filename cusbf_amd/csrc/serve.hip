@@ -29,11 +29,13 @@
 
 #include "../../include/cusbf_hip.h"
 #include "filter_internal.hpp"
+#include "host_pool.hpp"
 
 namespace {
 
 using Clock = std::chrono::steady_clock;
 namespace ci = cusbf_internal;
+using cusbf_internal::WorkerPool;
 
 struct Request {
     uint64_t ticket = 0;
@@ -70,79 +72,6 @@ struct Slot {
     int status = CUSBF_OK;
     std::string message;
     std::vector<Request*> batch;
-};
-
-// Host copies (batch fill, answer copies) need more than one core's memory
-// bandwidth. A persistent pool of worker threads runs the parts of each copy job;
-// the thread that submits a job works on it too and returns when every part is
-// done. The scheduler and the completion thread submit jobs concurrently.
-class WorkerPool {
-   public:
-    explicit WorkerPool(unsigned threads) {
-        for (unsigned i = 0; i < threads; ++i) workers_.emplace_back([this] { work(); });
-    }
-    ~WorkerPool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (std::thread& t : workers_) t.join();
-    }
-    // Runs fn(i) for every i in [0, n).
-    void run(size_t n, const std::function<void(size_t)>& fn) {
-        if (n == 0) return;
-        auto job = std::make_shared<Job>();
-        job->fn = &fn;
-        job->n = n;
-        if (n > 1) {
-            std::lock_guard<std::mutex> lk(mu_);
-            jobs_.push_back(job);
-        }
-        cv_.notify_all();
-        drain(*job);
-        std::unique_lock<std::mutex> lk(job->mu);
-        job->cv.wait(lk, [&] { return job->finished == job->n; });
-    }
-
-   private:
-    struct Job {
-        const std::function<void(size_t)>* fn = nullptr;
-        size_t n = 0;
-        std::atomic<size_t> next{0};
-        size_t finished = 0;  // guarded by mu
-        std::mutex mu;
-        std::condition_variable cv;
-    };
-    static void drain(Job& job) {
-        for (;;) {
-            const size_t i = job.next.fetch_add(1);
-            if (i >= job.n) return;
-            (*job.fn)(i);
-            std::lock_guard<std::mutex> lk(job.mu);
-            if (++job.finished == job.n) job.cv.notify_all();
-        }
-    }
-    void work() {
-        std::unique_lock<std::mutex> lk(mu_);
-        for (;;) {
-            cv_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
-            if (jobs_.empty()) return;  // stopping
-            std::shared_ptr<Job> job = jobs_.front();
-            if (job->next.load() >= job->n) {  // every part taken: retire the job
-                jobs_.pop_front();
-                continue;
-            }
-            lk.unlock();
-            drain(*job);
-            lk.lock();
-        }
-    }
-    std::mutex mu_;
-    std::condition_variable cv_;
-    std::deque<std::shared_ptr<Job>> jobs_;
-    std::vector<std::thread> workers_;
-    bool stop_ = false;
 };
 
 // Copy parts of at most 4 MiB.

@@ -271,9 +271,16 @@ class Filter:
             self._h, dev_ptr(d_sequence), n, stream_handle(stream), ctypes.byref(out)))
         return int(out.value)
 
-    def contains_sequence(self, sequence, stream=None) -> np.ndarray:
-        """filter.cuh:514-533: host bytes in, one byte per window out."""
+    def contains_sequence(self, sequence, d_output=None, stream=None) -> np.ndarray | None:
+        """filter.cuh:514-533: host bytes in, one byte per window out. With a device
+        d_output (filter.cuh:486-501) the answers are written there instead and
+        nothing is returned."""
         b = _host_bytes(sequence)
+        if d_output is not None:
+            check(self._lib, self._lib.cusbf_contains_sequence_host_to_device(
+                self._h, b.ctypes.data, b.size, dev_ptr(d_output), nbytes(d_output),
+                stream_handle(stream)))
+            return None
         out = np.zeros(self.record_kmer_count(b.size), dtype=np.uint8)
         if out.size == 0:
             return out
@@ -321,8 +328,15 @@ class Filter:
         return int(inserted.value)
 
     def contains_dense_packed(self, words: Sequence[int] | np.ndarray, num_symbols: int,
-                              stream=None) -> np.ndarray:
-        """filter.cuh:289-312 (host words in, host hits out)."""
+                              d_output=None, stream=None) -> np.ndarray | None:
+        """filter.cuh:289-312 (host words in, host hits out); with a device d_output,
+        filter.cuh:267-282 (answers written there, nothing returned)."""
+        if d_output is not None:
+            w = np.ascontiguousarray(words, dtype=np.uint64)
+            check(self._lib, self._lib.cusbf_contains_dense_packed_host_to_device(
+                self._h, w.ctypes.data, w.size, int(num_symbols), dev_ptr(d_output),
+                nbytes(d_output), stream_handle(stream)))
+            return None
         n = self.dense_packed_kmer_count(num_symbols)
         out = np.zeros(n, dtype=np.uint8)
         if n == 0:

@@ -1,0 +1,96 @@
+"""GPU parity of the host-memory forms at sizes that stage through the pinned ring.
+
+The host forms (filter.cuh:154-167 insert_sequence, :486-533 contains_sequence
+into device or host memory, :226-240 / :267-312 the dense packed forms) copy
+the caller's bytes through a ring of 32 MiB pinned buffers: host threads fill
+one buffer while the DMA engine reads another, and answers come back the same
+way. These inputs take several buffers and a ragged last one; the words and
+answers must equal the device forms' on the same bytes and the oracle's.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.helpers import make_config, make_oracle, random_dna
+
+pytestmark = pytest.mark.gpu
+
+K, S, M, H = 31, 28, 16, 4
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _dev(b: bytes):
+    torch = _torch()
+    return torch.from_numpy(np.frombuffer(b, dtype=np.uint8).copy()).cuda()
+
+
+def test_host_byte_forms_through_the_ring(gpu):
+    import cusbf_amd as cs
+
+    torch = _torch()
+    cfg = make_config(K, S, M, H)
+    seq = random_dna(150_000_003, 21, n_rate=0.001, lower_rate=0.01)  # 5 ring buffers
+    bits = 1 << 30  # the words copy back through 4 buffers
+    f = cs.Filter(cfg, bits)
+    assert f.insert_sequence(seq) == len(seq) - K + 1
+    g = cs.Filter(cfg, bits)
+    g.insert_sequence_async(_dev(seq))
+    torch.cuda.synchronize()
+    o = make_oracle(K, S, M, H, "dna")
+    w = o.new_filter(f.num_shards())
+    o.mt_insert(seq, w)
+    assert (g.words() == w).all()
+    assert (f.words() == w).all()
+
+    query = random_dna(70_000_001, 22, n_rate=0.001) + seq[:40_000_000]
+    want = o.mt_contains(query, w)
+    host = f.contains_sequence(query)
+    bad = np.nonzero(host != want)[0]
+    assert bad.size == 0, f"{bad.size} host answers differ (first window {bad[0]})"
+    out = torch.full((want.size + 5,), 7, dtype=torch.uint8, device="cuda")
+    assert f.contains_sequence(query, d_output=out) is None
+    got = out.cpu().numpy()
+    assert (got[: want.size] == want).all()
+    assert (got[want.size:] == 7).all()
+    # a second call reuses the ring buffers (the first call's copies have finished)
+    assert (f.contains_sequence(query[:9_000_000]) == want[: 9_000_000 - K + 1]).all()
+
+
+def test_host_dense_forms_through_the_ring(gpu):
+    import cusbf_amd as cs
+
+    torch = _torch()
+    cfg = make_config(K, S, M, H)
+    n = 300_000_017  # 9.4 M packed words: 3 ring buffers in; 300 M answers: 9 out
+    seq = random_dna(n, 23)
+    d_seq = _dev(seq)
+    d_words = torch.empty((n + 31) // 32, dtype=torch.int64, device="cuda")
+    cs.pack_dense_sequence_device(cfg, d_seq, n, d_words)
+    torch.cuda.synchronize()
+    words = d_words.cpu().numpy().view(np.uint64).copy()
+    bits = 1 << 31
+    f = cs.Filter(cfg, bits)
+    assert f.insert_dense_packed(words, n) == n - K + 1
+    g = cs.Filter(cfg, bits)
+    g.insert_dense_packed_async(d_words, n)
+    torch.cuda.synchronize()
+    o = make_oracle(K, S, M, H, "dna")
+    w = o.new_filter(f.num_shards())
+    o.insert_dense(words, n, w)
+    assert (g.words() == w).all()
+    assert (f.words() == w).all()
+    host = f.contains_dense_packed(words, n)
+    dev_out = torch.empty(n - K + 1, dtype=torch.uint8, device="cuda")
+    g.contains_dense_packed_async(d_words, n, dev_out)
+    ref = dev_out.cpu().numpy()
+    assert ref.all()  # every window of the inserted sequence is present
+    assert (host == ref).all()
+    out = torch.zeros(n - K + 1, dtype=torch.uint8, device="cuda")
+    assert f.contains_dense_packed(words, n, d_output=out) is None
+    assert torch.equal(out, dev_out)
