@@ -649,13 +649,15 @@ def fastx_scenario(genome, reads, wl, n_bases: int, n_reads: int) -> dict:
     whole call (mmap, parse, stage, kernels, reports); beside it the parse-only
     time of the same text on the host, and a check that the filter words equal
     the record-batch path's."""
+    import ctypes
     import shutil
     import tempfile
 
     import numpy as np
 
     import cusbf_amd as cs
-    from cusbf_amd import fastx
+    from cusbf_amd import _lib as cs_lib
+    from cusbf_amd.errors import check
 
     n_bases = min(n_bases, genome.numel())
     g = genome[:n_bases].cpu().numpy()
@@ -698,9 +700,18 @@ def fastx_scenario(genome, reads, wl, n_bases: int, n_reads: int) -> dict:
         t0 = time.perf_counter()
         q = filt.query_fastx_file(fq_path)
         t_q = time.perf_counter() - t0
+        # parse alone, no device work and no consumer: FASTA split by host threads,
+        # FASTQ parsed line by line
+        lib = cs_lib.load()
+        n_rec = ctypes.c_uint64()
         t0 = time.perf_counter()
-        n_fa = len(fastx.parse_fastx(fasta[: 64 << 20]))
-        t_parse_fa = (time.perf_counter() - t0) * len(fasta) / min(len(fasta), 64 << 20)
+        check(lib, lib.cusbf_parse_fastx_buffer_chunked(fasta, len(fasta), b"genome.fa", 0, 0, None, None,
+                                                        ctypes.byref(n_rec)))
+        t_parse_fa = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        check(lib, lib.cusbf_parse_fastx_buffer_chunked(fastq, len(fastq), b"reads.fq", 0, 0, None, None,
+                                                        ctypes.byref(n_rec)))
+        t_parse_fq = time.perf_counter() - t0
         # the record-batch path on the same records gives the same words
         ref = cs.Filter(cfg, filt.filter_bits())
         seqs = [p.split(b"\n", 1)[1].replace(b"\n", b"") for p in parts]
@@ -721,6 +732,7 @@ def fastx_scenario(genome, reads, wl, n_bases: int, n_reads: int) -> dict:
         "insert_report": [ins.records_indexed, ins.indexed_bases, ins.inserted_kmers],
         "query_report": [q.records_queried, q.queried_bases, q.queried_kmers, q.positive_kmers],
         "host_parse_only_fasta_GBps": len(fasta) / t_parse_fa / 1e9,
+        "host_parse_only_fastq_GBps": len(fastq) / t_parse_fq / 1e9,
         "words_equal_record_batch_path": same,
     }
 

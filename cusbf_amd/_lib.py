@@ -148,6 +148,8 @@ SIGNATURES = [
                                      ctypes.POINTER(CQueryReport)]),
     ("cusbf_parse_fastx_buffer", i32, [P, u64, ctypes.c_char_p, FASTX_CONSUMER, P,
                                        ctypes.POINTER(u64)]),
+    ("cusbf_parse_fastx_buffer_chunked", i32, [P, u64, ctypes.c_char_p, u64, u64, FASTX_CONSUMER, P,
+                                               ctypes.POINTER(u64)]),
     ("cusbf_fastx_set_chunk_bytes", i32, [P, u64]),
     ("cusbf_last_error_location", i32, [ctypes.c_char_p, sz, ctypes.POINTER(u32),
                                         ctypes.POINTER(u32)]),

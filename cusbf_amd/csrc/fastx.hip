@@ -48,6 +48,7 @@
 
 #include "../../include/cusbf_hip.h"
 #include "filter_internal.hpp"
+#include "host_pool.hpp"
 
 namespace {
 
@@ -379,6 +380,225 @@ class Parser {
     std::string_view last_;
 };
 
+// --- FASTA text in memory, split by host threads ---------------------------------------
+//
+// After its first header a FASTA text cannot fail to parse (Fastx.hpp:240-460:
+// a line starting with '>' starts a record, every other line is sequence), so a
+// text held in memory is split into records by the host pool: pass 1 finds the
+// header lines and counts each record's sequence bytes in parts of the text,
+// pass 2 copies the sequence lines to their offsets. Lines, '\r' trimming and
+// headers are those of the line reader and parser above.
+
+struct FastaRecord {
+    uint64_t header_at, header_len;  // in the text: after '>', trailing '\r' dropped
+    uint64_t bytes;                  // sequence bytes (before padding)
+    uint64_t out_at;                 // offset in the output
+};
+
+// One chunk's worth of records: text [pos, cut), output bytes after padding.
+struct FastaBatch {
+    std::vector<FastaRecord> records;
+    uint64_t cut = 0;
+    uint64_t out_bytes = 0;
+};
+
+// Start of the first line that is not empty once one trailing '\r' is dropped
+// (the parser's first header line), or text.size() when there is none.
+uint64_t first_content_line(std::string_view t) {
+    uint64_t at = 0;
+    while (at < t.size()) {
+        const void* nl = std::memchr(t.data() + at, '\n', t.size() - at);
+        const uint64_t stop = nl ? static_cast<uint64_t>(static_cast<const char*>(nl) - t.data()) : t.size();
+        uint64_t len = stop - at;
+        if (len != 0 && t[at + len - 1] == '\r') --len;
+        if (len != 0) return at;
+        at = nl ? stop + 1 : t.size();
+    }
+    return t.size();
+}
+
+class FastaSplitter {
+   public:
+    FastaSplitter(std::string_view text, uint32_t width, char sep) : t_(text.data()), end_(text.size()), width_(width), sep_(sep) {}
+
+    // Records from the header line at `pos` (a line start whose line begins with
+    // '>') holding at most about `limit` bytes of text and at most `max_records`
+    // records; a record larger than `limit` is taken alone, whole.
+    void scan(uint64_t pos, uint64_t limit, uint64_t max_records, FastaBatch* b) {
+        uint64_t span_end = end_ - pos > limit ? pos + limit : end_;
+        if (span_end < end_) span_end = line_start_at_or_after(std::max(span_end, pos + 1));
+        parts_.clear();
+        split(pos, span_end);
+        pass1();
+        std::vector<FastaRecord>& recs = b->records;
+        recs.clear();
+        collect(recs);
+        uint64_t cut = span_end;
+        if (span_end < end_) {
+            if (recs.size() >= 2) {
+                // the last record may continue past the span: it starts the next batch
+                cut = recs.back().header_at - 1;
+                recs.pop_back();
+                clip_parts(cut);
+            } else {
+                // one record larger than the span: extend it to the next header
+                const uint64_t next = next_header(span_end);
+                const size_t first_new = parts_.size();
+                split(span_end, next);
+                pass1(first_new);
+                cut = next;
+            }
+        }
+        if (recs.size() > max_records) {
+            cut = recs[max_records].header_at - 1;
+            recs.resize(max_records);
+            clip_parts(cut);
+        }
+        // totals: each part's leading bytes belong to the record before it
+        recs.clear();
+        collect(recs);
+        uint64_t at = 0;
+        for (FastaRecord& r : recs) {
+            r.out_at = at;
+            at += padded(r.bytes);
+        }
+        b->cut = cut;
+        b->out_bytes = at;
+    }
+
+    // Pass 2: the batch's sequence bytes (and separator padding) into `out`.
+    void copy(const FastaBatch& b, char* out) {
+        const FastaRecord* recs = b.records.data();
+        uint64_t next = 0;  // global index of the next record to start
+        uint64_t tail = 0;  // output position after the bytes so far of record next - 1
+        for (Part& p : parts_) {
+            p.lead_out = tail;  // zero lead bytes in the first part: the text starts at a header
+            tail += p.lead;
+            p.first_own = next;
+            for (const FastaRecord& r : p.recs) {
+                tail = recs[next].out_at + r.bytes;
+                ++next;
+            }
+        }
+        cusbf_internal::host_pool().run(parts_.size(), [&](size_t i) { copy_part(parts_[i], recs, out); });
+        for (const FastaRecord& r : b.records) {
+            const uint64_t pad = padded(r.bytes) - r.bytes;
+            if (pad) std::memset(out + r.out_at + r.bytes, sep_, pad);
+        }
+    }
+
+   private:
+    struct Part {
+        uint64_t a = 0, b = 0;  // line starts
+        uint64_t lead = 0;      // sequence bytes before the part's first header
+        std::vector<FastaRecord> recs;  // records whose header is in the part, bytes within it
+        uint64_t first_own = 0;  // global index of recs[0]
+        uint64_t lead_out = 0;   // output offset of the lead bytes
+    };
+
+    uint64_t padded(uint64_t bytes) const {
+        const uint64_t rem = bytes % width_;
+        return rem ? bytes + (width_ - rem) : bytes;
+    }
+    uint64_t line_start_at_or_after(uint64_t at) const {
+        if (at == 0 || t_[at - 1] == '\n') return at;
+        const void* nl = std::memchr(t_ + at, '\n', end_ - at);
+        return nl ? static_cast<uint64_t>(static_cast<const char*>(nl) - t_) + 1 : end_;
+    }
+    // The first line start after `at` (itself a line start) whose line begins with '>'.
+    uint64_t next_header(uint64_t at) const {
+        if (at < end_ && t_[at] == '>') return at;
+        const void* hit = memmem(t_ + at - 1, end_ - at + 1, "\n>", 2);
+        return hit ? static_cast<uint64_t>(static_cast<const char*>(hit) - t_) + 1 : end_;
+    }
+    // Parts of about 1 MiB and more, one per pool task, cut at line starts.
+    void split(uint64_t a, uint64_t b) {
+        if (a >= b) return;
+        constexpr uint64_t kMinPart = 1ull << 20;
+        const uint64_t n = std::clamp<uint64_t>((b - a) / kMinPart, 1, 64);
+        uint64_t start = a;
+        for (uint64_t i = 1; i <= n && start < b; ++i) {
+            uint64_t stop = i == n ? b : std::min(b, line_start_at_or_after(a + (b - a) / n * i));
+            if (stop <= start) continue;
+            Part p;
+            p.a = start;
+            p.b = stop;
+            parts_.push_back(std::move(p));
+            start = stop;
+        }
+    }
+    // A line [at, next) of the text: its length after dropping one trailing '\r'.
+    uint64_t line_at(uint64_t at, uint64_t limit, uint64_t* len) const {
+        const void* nl = std::memchr(t_ + at, '\n', limit - at);
+        const uint64_t stop = nl ? static_cast<uint64_t>(static_cast<const char*>(nl) - t_) : limit;
+        uint64_t l = stop - at;
+        if (l != 0 && t_[at + l - 1] == '\r') --l;
+        *len = l;
+        return nl ? stop + 1 : limit;
+    }
+    void scan_part(Part& p) const {
+        p.lead = 0;
+        p.recs.clear();
+        for (uint64_t at = p.a; at < p.b;) {
+            uint64_t len = 0;
+            const uint64_t next = line_at(at, p.b, &len);
+            if (len != 0 && t_[at] == '>') {
+                p.recs.push_back(FastaRecord{at + 1, len - 1, 0, 0});
+            } else if (p.recs.empty()) {
+                p.lead += len;
+            } else {
+                p.recs.back().bytes += len;
+            }
+            at = next;
+        }
+    }
+    void pass1(size_t from = 0) {
+        const size_t n = parts_.size() - from;
+        cusbf_internal::host_pool().run(n, [&](size_t i) { scan_part(parts_[from + i]); });
+    }
+    // Records in text order with their whole sequence bytes.
+    void collect(std::vector<FastaRecord>& recs) const {
+        for (const Part& p : parts_) {
+            if (!recs.empty()) recs.back().bytes += p.lead;
+            for (const FastaRecord& r : p.recs) recs.push_back(r);
+        }
+    }
+    // Drops the parts and records from `cut` (a header line start) on.
+    void clip_parts(uint64_t cut) {
+        while (!parts_.empty() && parts_.back().a >= cut) parts_.pop_back();
+        if (parts_.empty()) return;
+        Part& p = parts_.back();
+        if (p.b > cut) {
+            p.b = cut;
+            while (!p.recs.empty() && p.recs.back().header_at - 1 >= cut) p.recs.pop_back();
+        }
+    }
+    void copy_part(const Part& p, const FastaRecord* recs, char* out) const {
+        uint64_t rec = p.first_own;
+        uint64_t to = p.lead_out;
+        bool started = false;
+        for (uint64_t at = p.a; at < p.b;) {
+            uint64_t len = 0;
+            const uint64_t next = line_at(at, p.b, &len);
+            if (len != 0 && t_[at] == '>') {
+                if (started) ++rec;
+                started = true;
+                to = recs[rec].out_at;
+            } else if (len != 0) {
+                std::memcpy(out + to, t_ + at, len);
+                to += len;
+            }
+            at = next;
+        }
+    }
+
+    const char* t_;
+    uint64_t end_;
+    uint32_t width_;
+    char sep_;
+    std::vector<Part> parts_;
+};
+
 // --- chunk pipeline -------------------------------------------------------------------
 
 struct RecordMeta {
@@ -444,6 +664,11 @@ class Pipeline final : public SequenceSink {
     }
 
     int run(Source& src) {
+        const std::string_view text = src.whole();
+        if (text.data() != nullptr) {
+            const uint64_t first = first_content_line(text);
+            if (first < text.size() && text[first] == '>') return run_fasta_text(text, first);
+        }
         Parser parser(src);
         std::string header;
         ParseStatus st;
@@ -470,6 +695,46 @@ class Pipeline final : public SequenceSink {
         }
         int rc = stopped_ ? CUSBF_OK : flush();
         if (rc == CUSBF_OK) rc = drain();
+        if (rc == CUSBF_OK && stopped_) {
+            return ci::set_error(CUSBF_ERR_IO, "FASTX record consumer stopped the query");
+        }
+        return rc;
+    }
+
+    // FASTA text in memory: each chunk's records split out by the host pool
+    // straight into the chunk's pinned staging.
+    int run_fasta_text(std::string_view text, uint64_t pos) {
+        FastaSplitter sp(text, width_, sep_);
+        FastaBatch b;
+        uint64_t index = 0;
+        while (pos < text.size() && !stopped_) {
+            sp.scan(pos, target_, max_records_, &b);
+            int rc = CUSBF_OK;
+            if (b.out_bytes > alloc_) {  // one record larger than the staging
+                rc = finish(chunks_[cur_ ^ 1]);
+                if (rc == CUSBF_OK) rc = allocate(b.out_bytes);
+                if (rc != CUSBF_OK) return rc;
+            }
+            Chunk& c = chunks_[cur_];
+            sp.copy(b, c.h_seq);
+            c.used = b.out_bytes;
+            for (const FastaRecord& r : b.records) {
+                RecordMeta m{index++, 0, 0, r.out_at, r.bytes};
+                if (query_ && consumer_) {
+                    m.header_at = c.headers.size();
+                    m.header_len = r.header_len;
+                    c.headers.append(text.data() + r.header_at, r.header_len);
+                }
+                c.records.push_back(m);
+                ++records_;
+                bases_ += r.bytes;
+            }
+            rc = flush();
+            if (rc == CUSBF_OK) rc = take_next();
+            if (rc != CUSBF_OK) return rc;
+            pos = b.cut;
+        }
+        const int rc = drain();
         if (rc == CUSBF_OK && stopped_) {
             return ci::set_error(CUSBF_ERR_IO, "FASTX record consumer stopped the query");
         }
@@ -804,7 +1069,45 @@ int cusbf_query_fastx_file(const cusbf_filter* f, const char* path, double fill_
 
 int cusbf_parse_fastx_buffer(const char* text, uint64_t bytes, const char* source_name,
                              cusbf_fastx_consumer consumer, void* user, uint64_t* records) {
+    return cusbf_parse_fastx_buffer_chunked(text, bytes, source_name, 0, 0, consumer, user, records);
+}
+
+int cusbf_parse_fastx_buffer_chunked(const char* text, uint64_t bytes, const char* source_name,
+                                     uint64_t chunk_bytes, uint64_t max_records,
+                                     cusbf_fastx_consumer consumer, void* user, uint64_t* records) {
     if (!text && bytes) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "null argument");
+    const std::string_view whole(text ? text : "", bytes);
+    const uint64_t first = first_content_line(whole);
+    if (first < whole.size() && whole[first] == '>') {
+        // FASTA: the pipeline's split into chunks, without device work
+        FastaSplitter sp(whole, 1, 'N');
+        FastaBatch b;
+        std::vector<char> out;
+        uint64_t n = 0;
+        for (uint64_t pos = first; pos < whole.size(); pos = b.cut) {
+            sp.scan(pos, chunk_bytes ? chunk_bytes : kDefaultMaxChunk, max_records ? max_records : UINT64_MAX, &b);
+            out.resize(std::max<uint64_t>(b.out_bytes, 1));
+            sp.copy(b, out.data());
+            for (const FastaRecord& r : b.records) {
+                if (consumer) {
+                    cusbf_fastx_record_view v{};
+                    v.record_index = n;
+                    v.header = whole.data() + r.header_at;
+                    v.header_len = r.header_len;
+                    v.sequence = out.data() + r.out_at;
+                    v.sequence_len = r.bytes;
+                    v.queried_bases = r.bytes;
+                    if (consumer(&v, user) != 0) {
+                        if (records) *records = n + 1;
+                        return ci::set_error(CUSBF_ERR_IO, "FASTX record consumer stopped the query");
+                    }
+                }
+                ++n;
+            }
+        }
+        if (records) *records = n;
+        return CUSBF_OK;
+    }
     MemorySource src(text, bytes);
     Parser parser(src);
     const std::string name = source_name ? source_name : "<stream>";

@@ -223,8 +223,11 @@ def query_fastx_file_detailed(filt: Filter, path, fill_fraction: float = 0.7,
                                                            stream))
 
 
-def parse_fastx(source, source_name: str = "<stream>") -> list:
-    """The library's parser alone (no device work): [(header, sequence bytes), ...]."""
+def parse_fastx(source, source_name: str = "<stream>", chunk_bytes: int | None = None,
+                max_records: int = 0) -> list:
+    """The library's parser alone (no device work): [(header, sequence bytes), ...].
+    With chunk_bytes, the records are grouped as the FASTX pipeline groups a text in
+    memory (chunks of about chunk_bytes of text, at most max_records records each)."""
     lib = _lib.load()
     out = []
 
@@ -234,8 +237,13 @@ def parse_fastx(source, source_name: str = "<stream>") -> list:
     cfn, cbox = _consumer(keep)
     text = _text(source)
     n = ctypes.c_uint64()
-    st = lib.cusbf_parse_fastx_buffer(text, len(text), source_name.encode(), cfn, None,
-                                      ctypes.byref(n))
+    if chunk_bytes is None:
+        st = lib.cusbf_parse_fastx_buffer(text, len(text), source_name.encode(), cfn, None,
+                                          ctypes.byref(n))
+    else:
+        st = lib.cusbf_parse_fastx_buffer_chunked(text, len(text), source_name.encode(),
+                                                  int(chunk_bytes), int(max_records), cfn, None,
+                                                  ctypes.byref(n))
     _raise_kept(cbox)
     check(lib, st)
     return out

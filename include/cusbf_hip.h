@@ -458,6 +458,14 @@ int cusbf_query_fastx_file(const cusbf_filter* f, const char* path, double fill_
  * hits); no device work. */
 int cusbf_parse_fastx_buffer(const char* text, uint64_t bytes, const char* source_name,
                              cusbf_fastx_consumer consumer, void* user, uint64_t* records);
+/* The same records grouped as the FASTX pipeline groups a FASTA text held in
+ * memory (no reference counterpart): chunks of about chunk_bytes of text (0 =
+ * 512 MiB) and at most max_records records (0 = no limit), a record larger than
+ * a chunk taken alone; FASTA text is split by host threads, FASTQ text parsed as
+ * by cusbf_parse_fastx_buffer. No device work. */
+int cusbf_parse_fastx_buffer_chunked(const char* text, uint64_t bytes, const char* source_name,
+                                     uint64_t chunk_bytes, uint64_t max_records,
+                                     cusbf_fastx_consumer consumer, void* user, uint64_t* records);
 /* No reference counterpart: fixed FASTX chunk capacity in bytes (0 = from fill_fraction). */
 int cusbf_fastx_set_chunk_bytes(cusbf_filter* f, uint64_t bytes);
 /* SourceLocation of the last error on this thread when it is a FASTX parse error

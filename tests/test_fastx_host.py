@@ -145,3 +145,56 @@ def test_parser_matches_oracle_on_random_texts():
         assert l == o, (i, text)
         errors += o[1] is not None
     assert 100 < errors < 2000  # both kinds of outcome are exercised
+
+
+def random_fasta(rng: random.Random, records: int, max_len: int) -> bytes:
+    """FASTA with wrapping, CRLF, blank lines, empty and header-only records,
+    '@' / '+' / '\\r' inside sequence lines, and sometimes no final newline."""
+    eol = [b"\n", b"\r\n"]
+    out = bytearray()
+    for _ in range(rng.choice([0, 0, 1, 3])):
+        out += rng.choice(eol)  # leading blank lines
+    for _ in range(records):
+        out += b">" + bytes(rng.choice(b"abc r1:>\t") for _ in range(rng.randint(0, 12))) + rng.choice(eol)
+        n = rng.choice([0, rng.randint(0, max_len), rng.randint(0, 50)])
+        seq = bytes(rng.choice(b"ACGTNacgt@+\r") for _ in range(n))
+        wrap = rng.choice([1, 7, 60, 80, 1 << 30])
+        for i in range(0, len(seq), wrap):
+            out += seq[i:i + wrap] + rng.choice(eol)
+            if rng.random() < 0.02:
+                out += rng.choice(eol)  # blank line inside the sequence
+    if out and rng.random() < 0.3:
+        out = out.rstrip(b"\r\n")
+    return bytes(out)
+
+
+@pytest.mark.parametrize("chunk_bytes", [1, 5, 64, 777, 1 << 12, 3 << 20, 0])
+@pytest.mark.parametrize("max_records", [0, 1, 3])
+def test_fasta_split_in_chunks_matches_oracle(chunk_bytes, max_records):
+    """FASTA text in memory is split into records by host threads, chunk by chunk
+    (the FASTX pipeline's grouping): the records equal the oracle's for every chunk
+    size and record limit, records larger than a chunk included."""
+    from cusbf_amd import fastx
+
+    rng = random.Random(7 * chunk_bytes + max_records)
+    for i in range(40):
+        text = random_fasta(rng, rng.randint(0, 12), 3000)
+        want, err = oracle_parse(text)
+        assert err is None
+        got = fastx.parse_fastx(text, chunk_bytes=chunk_bytes, max_records=max_records)
+        assert [(h.encode("latin-1"), s) for h, s in got] == want, (i, text[:200])
+
+
+def test_fasta_split_large_text_matches_oracle():
+    """Texts of several MiB: the split uses parts of about 1 MiB on several host
+    threads, with records and lines crossing part boundaries."""
+    from cusbf_amd import fastx
+
+    rng = random.Random(99)
+    for records, max_len in [(3, 3_000_000), (4000, 3000), (1, 7_000_000)]:
+        text = random_fasta(rng, records, max_len)
+        want, err = oracle_parse(text)
+        assert err is None
+        for chunk in (None, 1 << 20, 2_500_001, 10_000):
+            got = fastx.parse_fastx(text, chunk_bytes=chunk)
+            assert [(h.encode("latin-1"), s) for h, s in got] == want, (records, chunk)

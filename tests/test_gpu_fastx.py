@@ -173,15 +173,23 @@ def test_fastx_file_insert_query_match_oracle(gpu, tmp_path, alphabet, cfg_t, fm
     assert det.summary.positive_kmers == sum(int(h.sum()) for _, h in per)
 
 
-def test_fastx_sources_and_chunking_agree(gpu, tmp_path):
-    """File, gzip file, host buffer, reader, tiny chunks, fill_fraction 0: same words and answers."""
+@pytest.mark.parametrize("fmt,cfg_t", [
+    ("fastq", (31, 28, 16, 4, "dna")),
+    ("fasta", (31, 28, 16, 4, "dna")),
+    ("fasta", (3, 2, 2, 4, "triplet")),  # records padded to whole codons
+])
+def test_fastx_sources_and_chunking_agree(gpu, tmp_path, fmt, cfg_t):
+    """File, gzip file, host buffer, reader, tiny chunks, fill_fraction 0: same words and
+    answers. FASTA files and buffers are split by host threads, gzip files and readers
+    parsed line by line."""
     from cusbf_amd import fastx
 
-    cfg_t = (31, 28, 16, 4, "dna")
     records = make_records("dna", 800, seed=5, median=2000)
-    text = fastq_text(records)
-    plain = write(tmp_path, "r.fq", text)
-    gz = write(tmp_path, "r.fq.gz", text, gz=True)
+    if cfg_t[4] == "triplet":
+        records = [r[: len(r) - len(r) % 3 + (1 if i % 5 == 0 else 0)] for i, r in enumerate(records)]
+    text = fastq_text(records) if fmt == "fastq" else fasta_text(records, width=70)
+    plain = write(tmp_path, "r." + fmt, text)
+    gz = write(tmp_path, "r." + fmt + ".gz", text, gz=True)
     results = []
     for mode in ("file", "gzip", "buffer", "reader", "tiny_chunks", "fill0", "record_larger_than_chunk"):
         f = new_filter(cfg_t, bits=1 << 24)
