@@ -705,11 +705,12 @@ def fastx_scenario(genome, reads, wl, n_bases: int, n_reads: int) -> dict:
         lib = cs_lib.load()
         n_rec = ctypes.c_uint64()
         t0 = time.perf_counter()
-        check(lib, lib.cusbf_parse_fastx_buffer_chunked(fasta, len(fasta), b"genome.fa", 0, 0, None, None,
+        none = cs_lib.FASTX_CONSUMER()  # no consumer
+        check(lib, lib.cusbf_parse_fastx_buffer_chunked(fasta, len(fasta), b"genome.fa", 0, 0, none, None,
                                                         ctypes.byref(n_rec)))
         t_parse_fa = time.perf_counter() - t0
         t0 = time.perf_counter()
-        check(lib, lib.cusbf_parse_fastx_buffer_chunked(fastq, len(fastq), b"reads.fq", 0, 0, None, None,
+        check(lib, lib.cusbf_parse_fastx_buffer_chunked(fastq, len(fastq), b"reads.fq", 0, 0, none, None,
                                                         ctypes.byref(n_rec)))
         t_parse_fq = time.perf_counter() - t0
         # the record-batch path on the same records gives the same words

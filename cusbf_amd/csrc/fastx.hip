@@ -24,9 +24,11 @@
 //    source order once a chunk's event completes.
 // Chunk size follows the reference's budget rule (filter.cuh:402-425,
 // fastx_chunk.cuh:94-152): fill_fraction of free device memory minus a 64 MiB
-// slack, divided over the chunk's device buffers, is the flush threshold; a
-// zero budget puts every record in a chunk of its own, and a record larger
-// than the staging grows it (the reference processes such a record alone).
+// slack, divided over the chunk's device buffers, is the flush threshold (at
+// most 64 MiB); a zero budget puts every record in a chunk of its own, and a
+// record larger than the staging grows it (the reference processes such a
+// record alone). Text held in memory is split into records by host threads
+// (FASTA always, FASTQ where its record starts can be vouched for).
 // A consumer returning non-zero stops the query (the filter stays usable).
 #include <hip/hip_runtime.h>
 
@@ -254,6 +256,10 @@ class SequenceSink {
 class Parser {
    public:
     explicit Parser(Source& src) : lines_(src) {}
+    // A parser resuming inside a text: `lines_before` lines precede the source,
+    // whose records are of `format` (1 FASTA, 2 FASTQ).
+    Parser(Source& src, uint64_t lines_before, int format)
+        : lines_(src), format_(format), line_no_(lines_before) {}
 
     // 1: a record (header into `header`, sequence into `sink`), 0: end, -1: error.
     int next(std::string& header, SequenceSink& sink, uint64_t& seq_len, ParseStatus& st) {
@@ -389,15 +395,15 @@ class Parser {
 // pass 2 copies the sequence lines to their offsets. Lines, '\r' trimming and
 // headers are those of the line reader and parser above.
 
-struct FastaRecord {
+struct SplitRecord {
     uint64_t header_at, header_len;  // in the text: after '>', trailing '\r' dropped
     uint64_t bytes;                  // sequence bytes (before padding)
     uint64_t out_at;                 // offset in the output
 };
 
 // One chunk's worth of records: text [pos, cut), output bytes after padding.
-struct FastaBatch {
-    std::vector<FastaRecord> records;
+struct SplitBatch {
+    std::vector<SplitRecord> records;
     uint64_t cut = 0;
     uint64_t out_bytes = 0;
 };
@@ -419,18 +425,18 @@ uint64_t first_content_line(std::string_view t) {
 
 class FastaSplitter {
    public:
-    FastaSplitter(std::string_view text, uint32_t width, char sep) : t_(text.data()), end_(text.size()), width_(width), sep_(sep) {}
+    FastaSplitter(std::string_view text, uint32_t width) : t_(text.data()), end_(text.size()), width_(width) {}
 
     // Records from the header line at `pos` (a line start whose line begins with
     // '>') holding at most about `limit` bytes of text and at most `max_records`
     // records; a record larger than `limit` is taken alone, whole.
-    void scan(uint64_t pos, uint64_t limit, uint64_t max_records, FastaBatch* b) {
+    void scan(uint64_t pos, uint64_t limit, uint64_t max_records, SplitBatch* b) {
         uint64_t span_end = end_ - pos > limit ? pos + limit : end_;
         if (span_end < end_) span_end = line_start_at_or_after(std::max(span_end, pos + 1));
         parts_.clear();
         split(pos, span_end);
         pass1();
-        std::vector<FastaRecord>& recs = b->records;
+        std::vector<SplitRecord>& recs = b->records;
         recs.clear();
         collect(recs);
         uint64_t cut = span_end;
@@ -458,7 +464,7 @@ class FastaSplitter {
         recs.clear();
         collect(recs);
         uint64_t at = 0;
-        for (FastaRecord& r : recs) {
+        for (SplitRecord& r : recs) {
             r.out_at = at;
             at += padded(r.bytes);
         }
@@ -466,32 +472,28 @@ class FastaSplitter {
         b->out_bytes = at;
     }
 
-    // Pass 2: the batch's sequence bytes (and separator padding) into `out`.
-    void copy(const FastaBatch& b, char* out) {
-        const FastaRecord* recs = b.records.data();
+    // Pass 2: the batch's sequence bytes into `out` (padding is the caller's).
+    void copy(const SplitBatch& b, char* out) {
+        const SplitRecord* recs = b.records.data();
         uint64_t next = 0;  // global index of the next record to start
         uint64_t tail = 0;  // output position after the bytes so far of record next - 1
         for (Part& p : parts_) {
             p.lead_out = tail;  // zero lead bytes in the first part: the text starts at a header
             tail += p.lead;
             p.first_own = next;
-            for (const FastaRecord& r : p.recs) {
+            for (const SplitRecord& r : p.recs) {
                 tail = recs[next].out_at + r.bytes;
                 ++next;
             }
         }
         cusbf_internal::host_pool().run(parts_.size(), [&](size_t i) { copy_part(parts_[i], recs, out); });
-        for (const FastaRecord& r : b.records) {
-            const uint64_t pad = padded(r.bytes) - r.bytes;
-            if (pad) std::memset(out + r.out_at + r.bytes, sep_, pad);
-        }
     }
 
    private:
     struct Part {
         uint64_t a = 0, b = 0;  // line starts
         uint64_t lead = 0;      // sequence bytes before the part's first header
-        std::vector<FastaRecord> recs;  // records whose header is in the part, bytes within it
+        std::vector<SplitRecord> recs;  // records whose header is in the part, bytes within it
         uint64_t first_own = 0;  // global index of recs[0]
         uint64_t lead_out = 0;   // output offset of the lead bytes
     };
@@ -505,11 +507,17 @@ class FastaSplitter {
         const void* nl = std::memchr(t_ + at, '\n', end_ - at);
         return nl ? static_cast<uint64_t>(static_cast<const char*>(nl) - t_) + 1 : end_;
     }
-    // The first line start after `at` (itself a line start) whose line begins with '>'.
+    // The first line start at or after `at` (itself a line start) whose line
+    // begins with '>' (sequence lines rarely hold a '>', so memchr leaps to it).
     uint64_t next_header(uint64_t at) const {
-        if (at < end_ && t_[at] == '>') return at;
-        const void* hit = memmem(t_ + at - 1, end_ - at + 1, "\n>", 2);
-        return hit ? static_cast<uint64_t>(static_cast<const char*>(hit) - t_) + 1 : end_;
+        for (uint64_t from = at; from < end_;) {
+            const void* hit = std::memchr(t_ + from, '>', end_ - from);
+            if (!hit) break;
+            const uint64_t h = static_cast<uint64_t>(static_cast<const char*>(hit) - t_);
+            if (h == at || t_[h - 1] == '\n') return h;
+            from = h + 1;
+        }
+        return end_;
     }
     // Parts of about 1 MiB and more, one per pool task, cut at line starts.
     void split(uint64_t a, uint64_t b) {
@@ -543,7 +551,7 @@ class FastaSplitter {
             uint64_t len = 0;
             const uint64_t next = line_at(at, p.b, &len);
             if (len != 0 && t_[at] == '>') {
-                p.recs.push_back(FastaRecord{at + 1, len - 1, 0, 0});
+                p.recs.push_back(SplitRecord{at + 1, len - 1, 0, 0});
             } else if (p.recs.empty()) {
                 p.lead += len;
             } else {
@@ -557,10 +565,10 @@ class FastaSplitter {
         cusbf_internal::host_pool().run(n, [&](size_t i) { scan_part(parts_[from + i]); });
     }
     // Records in text order with their whole sequence bytes.
-    void collect(std::vector<FastaRecord>& recs) const {
+    void collect(std::vector<SplitRecord>& recs) const {
         for (const Part& p : parts_) {
             if (!recs.empty()) recs.back().bytes += p.lead;
-            for (const FastaRecord& r : p.recs) recs.push_back(r);
+            for (const SplitRecord& r : p.recs) recs.push_back(r);
         }
     }
     // Drops the parts and records from `cut` (a header line start) on.
@@ -573,7 +581,7 @@ class FastaSplitter {
             while (!p.recs.empty() && p.recs.back().header_at - 1 >= cut) p.recs.pop_back();
         }
     }
-    void copy_part(const Part& p, const FastaRecord* recs, char* out) const {
+    void copy_part(const Part& p, const SplitRecord* recs, char* out) const {
         uint64_t rec = p.first_own;
         uint64_t to = p.lead_out;
         bool started = false;
@@ -595,9 +603,224 @@ class FastaSplitter {
     const char* t_;
     uint64_t end_;
     uint32_t width_;
-    char sep_;
     std::vector<Part> parts_;
 };
+
+// --- FASTQ text in memory, split by host threads ---------------------------------------
+//
+// A FASTQ text can fail anywhere, so its parts are parsed by the serial rules
+// above (blank lines before a header skipped; sequence lines up to a '+' line;
+// quality lines until their bytes reach the sequence's) and checked: the parts
+// start at record starts found by a pattern ('@' line, one sequence line, '+'
+// line, a quality line as long as the sequence, then an '@' line or the end),
+// and every part must end exactly where the next one starts. The first part
+// starts where the serial parser is between records, and a part that ends on
+// the next part's start leaves it there too, so every part then parses as the
+// serial parser would. Anything else -- an error, a record running past its
+// part, no record start near a chunk's end -- hands the text from the chunk's
+// start to the serial parser, which reports errors at their line and column.
+
+class FastqSplitter {
+   public:
+    FastqSplitter(std::string_view text, uint32_t width) : t_(text.data()), end_(text.size()), width_(width) {}
+
+    // Records from the record start `pos`, about `limit` bytes of text and at most
+    // `max_records` of them; false when the text from `pos` is left to the serial parser.
+    bool scan(uint64_t pos, uint64_t limit, uint64_t max_records, SplitBatch* b) {
+        uint64_t span_end = end_;
+        if (end_ - pos > limit) {
+            span_end = record_start_near(std::max(pos + limit, pos + 1));
+            if (span_end == kNone) return false;
+        }
+        parts_.clear();
+        constexpr uint64_t kMinPart = 1ull << 20;
+        const uint64_t n = std::clamp<uint64_t>((span_end - pos) / kMinPart, 1, 64);
+        uint64_t start = pos;
+        for (uint64_t i = 1; i <= n; ++i) {
+            uint64_t stop = span_end;
+            if (i < n) {
+                stop = record_start_near(pos + (span_end - pos) / n * i);
+                if (stop == kNone || stop <= start || stop >= span_end) continue;
+            }
+            Part p;
+            p.a = start;
+            p.b = stop;
+            parts_.push_back(std::move(p));
+            start = stop;
+        }
+        cusbf_internal::host_pool().run(parts_.size(), [&](size_t i) { parse_part(parts_[i]); });
+        std::vector<SplitRecord>& recs = b->records;
+        recs.clear();
+        for (const Part& p : parts_) {
+            if (!p.ok) return false;
+            recs.insert(recs.end(), p.recs.begin(), p.recs.end());
+        }
+        uint64_t cut = span_end;
+        if (recs.size() > max_records) {
+            cut = recs[max_records].header_at - 1;
+            recs.resize(max_records);
+            while (!parts_.empty() && parts_.back().a >= cut) parts_.pop_back();
+            Part& last = parts_.back();
+            last.b = std::min(last.b, cut);
+            while (!last.recs.empty() && last.recs.back().header_at - 1 >= cut) last.recs.pop_back();
+        }
+        uint64_t at = 0, next = 0;
+        for (Part& p : parts_) {
+            p.first_own = next;
+            next += p.recs.size();
+        }
+        for (SplitRecord& r : recs) {
+            r.out_at = at;
+            const uint64_t rem = r.bytes % width_;
+            at += rem ? r.bytes + (width_ - rem) : r.bytes;
+        }
+        b->cut = cut;
+        b->out_bytes = at;
+        return true;
+    }
+
+    // The batch's sequence bytes into `out` (padding is the caller's).
+    void copy(const SplitBatch& b, char* out) {
+        cusbf_internal::host_pool().run(parts_.size(), [&](size_t i) { copy_part(parts_[i], b.records.data(), out); });
+    }
+
+   private:
+    static constexpr uint64_t kNone = ~0ull;
+    static constexpr uint64_t kSearch = 4ull << 20;  // how far a record start is looked for
+
+    struct Part {
+        uint64_t a = 0, b = 0;
+        std::vector<SplitRecord> recs;
+        uint64_t first_own = 0;
+        bool ok = false;
+    };
+
+    uint64_t line_at(uint64_t at, uint64_t* len) const {
+        const void* nl = std::memchr(t_ + at, '\n', end_ - at);
+        const uint64_t stop = nl ? static_cast<uint64_t>(static_cast<const char*>(nl) - t_) : end_;
+        uint64_t l = stop - at;
+        if (l != 0 && t_[at + l - 1] == '\r') --l;
+        *len = l;
+        return nl ? stop + 1 : end_;
+    }
+
+    // A line start at or after `x` where a record by the pattern begins; end_ when
+    // the text ends first, kNone when none is found within kSearch bytes.
+    uint64_t record_start_near(uint64_t x) const {
+        uint64_t at = x;
+        if (at < end_ && at > 0 && t_[at - 1] != '\n') {
+            const void* nl = std::memchr(t_ + at, '\n', end_ - at);
+            at = nl ? static_cast<uint64_t>(static_cast<const char*>(nl) - t_) + 1 : end_;
+        }
+        while (at < end_) {
+            if (at - x > kSearch) return kNone;
+            uint64_t l0 = 0;
+            const uint64_t a1 = line_at(at, &l0);
+            if (l0 != 0 && t_[at] == '@' && a1 < end_) {
+                uint64_t l1 = 0, l2 = 0, l3 = 0;
+                const uint64_t a2 = line_at(a1, &l1);
+                if (a2 < end_ && !(l1 != 0 && t_[a1] == '+')) {
+                    const uint64_t a3 = line_at(a2, &l2);
+                    if (l2 != 0 && t_[a2] == '+' && a3 < end_) {
+                        uint64_t a4 = line_at(a3, &l3);
+                        if (l3 == l1) {
+                            uint64_t l4 = 0;
+                            while (a4 < end_) {  // blank lines, then a header or the end
+                                const uint64_t a5 = line_at(a4, &l4);
+                                if (l4 != 0) break;
+                                a4 = a5;
+                            }
+                            if (a4 >= end_ || t_[a4] == '@') return at;
+                        }
+                    }
+                }
+            }
+            at = a1;
+        }
+        return end_;
+    }
+
+    // Pass 1: the part's records by the serial rules; ok when it ends on p.b.
+    void parse_part(Part& p) const {
+        p.recs.clear();
+        p.ok = false;
+        uint64_t at = p.a;
+        while (at < p.b) {
+            uint64_t len = 0;
+            uint64_t next = line_at(at, &len);
+            if (len == 0) {
+                at = next;
+                continue;
+            }
+            if (t_[at] != '@') return;  // an error of the serial parser
+            SplitRecord r{at + 1, len - 1, 0, 0};
+            at = next;
+            for (;;) {  // sequence lines up to the '+' line
+                if (at >= p.b) return;
+                next = line_at(at, &len);
+                const bool plus = len != 0 && t_[at] == '+';
+                at = next;
+                if (plus) break;
+                r.bytes += len;
+            }
+            uint64_t got = 0;
+            while (got < r.bytes) {  // quality lines
+                if (at >= p.b) return;
+                at = line_at(at, &len);
+                got += len;
+            }
+            if (got != r.bytes) return;
+            p.recs.push_back(r);
+        }
+        p.ok = at == p.b;
+    }
+
+    // Pass 2: each record's sequence lines (from the line after its header up to
+    // its '+' line) to the record's offset.
+    void copy_part(const Part& p, const SplitRecord* recs, char* out) const {
+        for (size_t i = 0; i < p.recs.size(); ++i) {
+            const SplitRecord& own = p.recs[i];
+            char* to = out + recs[p.first_own + i].out_at;
+            uint64_t len = 0;
+            uint64_t at = line_at(own.header_at + own.header_len, &len);  // past the header line
+            for (;;) {
+                const uint64_t next = line_at(at, &len);
+                if (len != 0 && t_[at] == '+') break;
+                std::memcpy(to, t_ + at, len);
+                to += len;
+                at = next;
+            }
+        }
+    }
+
+    const char* t_;
+    uint64_t end_;
+    uint32_t width_;
+    std::vector<Part> parts_;
+};
+
+// Lines before `bytes` of text (each ends with '\n'), counted on the host pool.
+uint64_t count_newlines(const char* t, uint64_t bytes) {
+    constexpr uint64_t kPart = 16ull << 20;
+    const uint64_t parts = (bytes + kPart - 1) / kPart;
+    std::vector<uint64_t> counts(parts, 0);
+    ci::host_pool().run(parts, [&](size_t i) {
+        const char* a = t + i * kPart;
+        const char* e = t + std::min(bytes, (i + 1) * kPart);
+        uint64_t n = 0;
+        while (a < e) {
+            const void* nl = std::memchr(a, '\n', static_cast<size_t>(e - a));
+            if (!nl) break;
+            ++n;
+            a = static_cast<const char*>(nl) + 1;
+        }
+        counts[i] = n;
+    });
+    uint64_t total = 0;
+    for (uint64_t c : counts) total += c;
+    return total;
+}
+
 
 // --- chunk pipeline -------------------------------------------------------------------
 
@@ -621,7 +844,7 @@ struct Chunk {
 };
 
 constexpr uint64_t kSlackBytes = 64ull << 20;
-constexpr uint64_t kDefaultMaxChunk = 512ull << 20;
+constexpr uint64_t kDefaultMaxChunk = 64ull << 20;
 constexpr uint64_t kMinStaging = 1ull << 20;
 constexpr uint64_t kBytesPerRecordSlot = 32;  // staging bytes per record-table entry
 
@@ -668,11 +891,82 @@ class Pipeline final : public SequenceSink {
         if (text.data() != nullptr) {
             const uint64_t first = first_content_line(text);
             if (first < text.size() && text[first] == '>') return run_fasta_text(text, first);
+            if (first < text.size() && text[first] == '@') {
+                uint64_t resume = 0, index = 0;
+                const int rc = run_fastq_text(text, first, &resume, &index);
+                if (rc != CUSBF_OK || stopped_ || resume >= text.size()) return finish_run(rc);
+                // the rest by the serial parser, with the lines before it counted
+                MemorySource rest(text.data() + resume, text.size() - resume);
+                const uint64_t lines = count_newlines(text.data(), resume);
+                Parser parser(rest, lines, 2);
+                return run_serial(parser, index);
+            }
         }
         Parser parser(src);
+        return run_serial(parser, 0);
+    }
+
+    // FASTQ text in memory: chunks split by the host pool while the splitter can
+    // vouch for them; *resume = where the serial parser takes over (text.size()
+    // when nothing is left), *index = the next record's index.
+    int run_fastq_text(std::string_view text, uint64_t pos, uint64_t* resume, uint64_t* index) {
+        FastqSplitter sp(text, width_);
+        SplitBatch b;
+        while (pos < text.size() && !stopped_) {
+            if (!sp.scan(pos, target_, max_records_, &b)) break;
+            int rc = CUSBF_OK;
+            if (b.out_bytes > alloc_) {
+                rc = finish(chunks_[cur_ ^ 1]);
+                if (rc == CUSBF_OK) rc = allocate(b.out_bytes);
+                if (rc != CUSBF_OK) return rc;
+            }
+            Chunk& c = chunks_[cur_];
+            sp.copy(b, c.h_seq);
+            rc = add_split_records(c, text, b, index);
+            if (rc == CUSBF_OK) rc = flush();
+            if (rc == CUSBF_OK) rc = take_next();
+            if (rc != CUSBF_OK) return rc;
+            pos = b.cut;
+        }
+        *resume = pos;
+        return CUSBF_OK;
+    }
+
+    // A split batch's records into chunk c (its staging already holds their bytes):
+    // separator padding, record table entries, headers for a consumer, totals.
+    int add_split_records(Chunk& c, std::string_view text, const SplitBatch& b, uint64_t* index) {
+        c.used = b.out_bytes;
+        for (const SplitRecord& r : b.records) {
+            const uint64_t pad = padded(r.bytes) - r.bytes;
+            if (pad) std::memset(c.h_seq + r.out_at + r.bytes, sep_, pad);
+            RecordMeta m{(*index)++, 0, 0, r.out_at, r.bytes};
+            if (query_ && consumer_) {
+                m.header_at = c.headers.size();
+                m.header_len = r.header_len;
+                c.headers.append(text.data() + r.header_at, r.header_len);
+            }
+            c.records.push_back(m);
+            ++records_;
+            bases_ += r.bytes;
+        }
+        return CUSBF_OK;
+    }
+
+    // After the last chunk: the device work drained, the consumer's stop reported.
+    int finish_run(int rc) {
+        if (rc != CUSBF_OK) return rc;
+        rc = stopped_ ? CUSBF_OK : flush();
+        if (rc == CUSBF_OK) rc = drain();
+        if (rc == CUSBF_OK && stopped_) {
+            return ci::set_error(CUSBF_ERR_IO, "FASTX record consumer stopped the query");
+        }
+        return rc;
+    }
+
+    int run_serial(Parser& parser, uint64_t first_index) {
         std::string header;
         ParseStatus st;
-        for (uint64_t index = 0;; ++index) {
+        for (uint64_t index = first_index;; ++index) {
             Chunk& c = chunks_[cur_];
             rec_at_ = c.used;
             spilled_ = false;
@@ -693,19 +987,14 @@ class Pipeline final : public SequenceSink {
             if (rc != CUSBF_OK) return rc;
             if (stopped_) break;
         }
-        int rc = stopped_ ? CUSBF_OK : flush();
-        if (rc == CUSBF_OK) rc = drain();
-        if (rc == CUSBF_OK && stopped_) {
-            return ci::set_error(CUSBF_ERR_IO, "FASTX record consumer stopped the query");
-        }
-        return rc;
+        return finish_run(CUSBF_OK);
     }
 
     // FASTA text in memory: each chunk's records split out by the host pool
     // straight into the chunk's pinned staging.
     int run_fasta_text(std::string_view text, uint64_t pos) {
-        FastaSplitter sp(text, width_, sep_);
-        FastaBatch b;
+        FastaSplitter sp(text, width_);
+        SplitBatch b;
         uint64_t index = 0;
         while (pos < text.size() && !stopped_) {
             sp.scan(pos, target_, max_records_, &b);
@@ -717,28 +1006,13 @@ class Pipeline final : public SequenceSink {
             }
             Chunk& c = chunks_[cur_];
             sp.copy(b, c.h_seq);
-            c.used = b.out_bytes;
-            for (const FastaRecord& r : b.records) {
-                RecordMeta m{index++, 0, 0, r.out_at, r.bytes};
-                if (query_ && consumer_) {
-                    m.header_at = c.headers.size();
-                    m.header_len = r.header_len;
-                    c.headers.append(text.data() + r.header_at, r.header_len);
-                }
-                c.records.push_back(m);
-                ++records_;
-                bases_ += r.bytes;
-            }
-            rc = flush();
+            rc = add_split_records(c, text, b, &index);
+            if (rc == CUSBF_OK) rc = flush();
             if (rc == CUSBF_OK) rc = take_next();
             if (rc != CUSBF_OK) return rc;
             pos = b.cut;
         }
-        const int rc = drain();
-        if (rc == CUSBF_OK && stopped_) {
-            return ci::set_error(CUSBF_ERR_IO, "FASTX record consumer stopped the query");
-        }
-        return rc;
+        return finish_run(CUSBF_OK);
     }
 
     // Sequence bytes of the record being parsed go to the current chunk while
@@ -833,7 +1107,24 @@ class Pipeline final : public SequenceSink {
         if (e == hipSuccess) e = hipMalloc(&d_counts_, 2 * max_records_ * 8);
         if (e == hipSuccess && query_) e = hipMalloc(&d_hits_, alloc_);
         if (e != hipSuccess) return ci::set_hip_error(e, __FILE__, __LINE__);
+        // The host's first touch of a pinned page costs a fault (~3 GB/s on one
+        // thread): touch the staging the parser writes, and the hits a consumer
+        // reads, here on the host pool rather than page by page later. The record
+        // tables are touched as records use them.
+        for (Chunk& c : chunks_) {
+            prefault(c.h_seq, alloc_);
+            if (query_ && consumer_) prefault(c.h_hits, alloc_);
+        }
         return CUSBF_OK;
+    }
+
+    static void prefault(void* p, uint64_t bytes) {
+        constexpr uint64_t kPart = 4ull << 20;
+        const uint64_t parts = (bytes + kPart - 1) / kPart;
+        ci::host_pool().run(parts, [&](size_t i) {
+            const uint64_t at = i * kPart;
+            std::memset(static_cast<char*>(p) + at, 0, std::min(kPart, bytes - at));
+        });
     }
 
     void free_buffers() {
@@ -1080,15 +1371,15 @@ int cusbf_parse_fastx_buffer_chunked(const char* text, uint64_t bytes, const cha
     const uint64_t first = first_content_line(whole);
     if (first < whole.size() && whole[first] == '>') {
         // FASTA: the pipeline's split into chunks, without device work
-        FastaSplitter sp(whole, 1, 'N');
-        FastaBatch b;
+        FastaSplitter sp(whole, 1);
+        SplitBatch b;
         std::vector<char> out;
         uint64_t n = 0;
         for (uint64_t pos = first; pos < whole.size(); pos = b.cut) {
             sp.scan(pos, chunk_bytes ? chunk_bytes : kDefaultMaxChunk, max_records ? max_records : UINT64_MAX, &b);
             out.resize(std::max<uint64_t>(b.out_bytes, 1));
             sp.copy(b, out.data());
-            for (const FastaRecord& r : b.records) {
+            for (const SplitRecord& r : b.records) {
                 if (consumer) {
                     cusbf_fastx_record_view v{};
                     v.record_index = n;
@@ -1108,13 +1399,47 @@ int cusbf_parse_fastx_buffer_chunked(const char* text, uint64_t bytes, const cha
         if (records) *records = n;
         return CUSBF_OK;
     }
-    MemorySource src(text, bytes);
-    Parser parser(src);
+    uint64_t n = 0, resume = 0;
+    if (first < whole.size() && whole[first] == '@') {
+        // FASTQ: split while the splitter can vouch for the chunks, the rest serially
+        FastqSplitter sp(whole, 1);
+        SplitBatch b;
+        std::vector<char> out;
+        uint64_t pos = first;
+        while (pos < whole.size() &&
+               sp.scan(pos, chunk_bytes ? chunk_bytes : kDefaultMaxChunk, max_records ? max_records : UINT64_MAX, &b)) {
+            out.resize(std::max<uint64_t>(b.out_bytes, 1));
+            sp.copy(b, out.data());
+            for (const SplitRecord& r : b.records) {
+                if (consumer) {
+                    cusbf_fastx_record_view v{};
+                    v.record_index = n;
+                    v.header = whole.data() + r.header_at;
+                    v.header_len = r.header_len;
+                    v.sequence = out.data() + r.out_at;
+                    v.sequence_len = r.bytes;
+                    v.queried_bases = r.bytes;
+                    if (consumer(&v, user) != 0) {
+                        if (records) *records = n + 1;
+                        return ci::set_error(CUSBF_ERR_IO, "FASTX record consumer stopped the query");
+                    }
+                }
+                ++n;
+            }
+            pos = b.cut;
+        }
+        if (pos >= whole.size()) {
+            if (records) *records = n;
+            return CUSBF_OK;
+        }
+        resume = pos;
+    }
+    MemorySource src(text + resume, bytes - resume);
+    Parser parser = resume ? Parser(src, count_newlines(text, resume), 2) : Parser(src);
     const std::string name = source_name ? source_name : "<stream>";
     std::string header;
     ParseOnly sink;
     ParseStatus st;
-    uint64_t n = 0;
     for (;;) {
         sink.seq.clear();
         uint64_t len = 0;

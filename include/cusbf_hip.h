@@ -460,7 +460,7 @@ int cusbf_parse_fastx_buffer(const char* text, uint64_t bytes, const char* sourc
                              cusbf_fastx_consumer consumer, void* user, uint64_t* records);
 /* The same records grouped as the FASTX pipeline groups a FASTA text held in
  * memory (no reference counterpart): chunks of about chunk_bytes of text (0 =
- * 512 MiB) and at most max_records records (0 = no limit), a record larger than
+ * 64 MiB) and at most max_records records (0 = no limit), a record larger than
  * a chunk taken alone; FASTA text is split by host threads, FASTQ text parsed as
  * by cusbf_parse_fastx_buffer. No device work. */
 int cusbf_parse_fastx_buffer_chunked(const char* text, uint64_t bytes, const char* source_name,

@@ -198,3 +198,77 @@ def test_fasta_split_large_text_matches_oracle():
         for chunk in (None, 1 << 20, 2_500_001, 10_000):
             got = fastx.parse_fastx(text, chunk_bytes=chunk)
             assert [(h.encode("latin-1"), s) for h, s in got] == want, (records, chunk)
+
+
+def lib_parse_chunked(text: bytes, chunk_bytes: int, max_records: int = 0):
+    from cusbf_amd import fastx
+    from cusbf_amd.errors import CusbfError, ErrorCategory
+
+    try:
+        recs = fastx.parse_fastx(text, chunk_bytes=chunk_bytes, max_records=max_records)
+        return [(h.encode("latin-1"), s) for h, s in recs], None
+    except CusbfError as e:
+        assert e.category == ErrorCategory.fastx_parse, e
+        detail = e.message.split(": ", 1)[1]
+        return None, (e.location.line, e.location.column, detail, e.message, e.location.file)
+
+
+@pytest.mark.parametrize("chunk_bytes", [1, 64, 4096, 0])
+@pytest.mark.parametrize("max_records", [0, 2])
+def test_chunked_parse_matches_oracle_on_random_texts(chunk_bytes, max_records):
+    """FASTQ text in memory is split by host threads where a pattern check and the
+    part ends vouch for it, the rest parsed serially: records and errors (line,
+    column, detail) equal the oracle's for every chunk size."""
+    rng = random.Random(31 * chunk_bytes + max_records + 5)
+    for i in range(500):
+        text = random_text(rng)
+        assert lib_parse_chunked(text, chunk_bytes, max_records) == oracle_parse(text), (i, text)
+
+
+def random_fastq(rng: random.Random, records: int, max_len: int, wrap: bool = False) -> bytes:
+    """Four-line FASTQ (CRLF or LF, blank lines between records, qualities that
+    start with '@' or '+'); wrapped sequence and quality lines when `wrap`."""
+    eol = rng.choice([b"\n", b"\r\n"])
+    out = bytearray()
+    for i in range(records):
+        if rng.random() < 0.01:
+            out += eol  # blank line between records
+        n = rng.choice([0, rng.randint(1, max_len), rng.randint(1, 200)])
+        seq = bytes(rng.choice(b"ACGTN") for _ in range(n))
+        qual = bytes(rng.choice(b"@+I#5") for _ in range(n))
+        out += b"@r%d x" % i + eol
+        w = rng.choice([7, 60]) if wrap else max(1, n)
+        for j in range(0, n, w):
+            out += seq[j:j + w] + eol
+        if n == 0:
+            out += eol
+        out += b"+" + eol
+        for j in range(0, n, w):
+            out += qual[j:j + w] + eol
+        if n == 0:
+            out += eol
+    return bytes(out)
+
+
+def test_fastq_split_large_texts_match_oracle():
+    """Texts of several MiB split into parts of about 1 MiB, with and without
+    wrapped lines (which the pattern does not vouch for: parsed serially), and with
+    a defect deep inside (reported at its line and column by the serial parser)."""
+    rng = random.Random(17)
+    for records, max_len, wrap in [(40000, 300, False), (300, 60000, False), (20000, 300, True)]:
+        text = random_fastq(rng, records, max_len, wrap)
+        want = oracle_parse(text)
+        assert want[1] is None
+        for chunk in (0, 1 << 20, 300_001):
+            assert lib_parse_chunked(text, chunk) == want, (records, wrap, chunk)
+    text = random_fastq(rng, 40000, 300)
+    lines = text.split(b"\n")
+    k = len(lines) * 3 // 4
+    while not lines[k].startswith(b"+"):
+        k += 1
+    lines[k + 1] = lines[k + 1][:-2] + b"\r" if lines[k + 1].endswith(b"\r") else lines[k + 1][:-1]
+    bad = b"\n".join(lines)
+    want = oracle_parse(bad)
+    assert want[1] is not None
+    for chunk in (0, 1 << 20, 300_001):
+        assert lib_parse_chunked(bad, chunk) == want, chunk
