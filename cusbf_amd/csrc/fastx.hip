@@ -1107,24 +1107,14 @@ class Pipeline final : public SequenceSink {
         if (e == hipSuccess) e = hipMalloc(&d_counts_, 2 * max_records_ * 8);
         if (e == hipSuccess && query_) e = hipMalloc(&d_hits_, alloc_);
         if (e != hipSuccess) return ci::set_hip_error(e, __FILE__, __LINE__);
-        // The host's first touch of a pinned page costs a fault (~3 GB/s on one
-        // thread): touch the staging the parser writes, and the hits a consumer
-        // reads, here on the host pool rather than page by page later. The record
-        // tables are touched as records use them.
+        // Touch the staging the parser writes, and the hits a consumer reads, on
+        // the host pool now rather than page by page later (first-touch faults);
+        // the record tables are touched as records use them.
         for (Chunk& c : chunks_) {
-            prefault(c.h_seq, alloc_);
-            if (query_ && consumer_) prefault(c.h_hits, alloc_);
+            ci::touch_pages(ci::host_pool(), c.h_seq, alloc_);
+            if (query_ && consumer_) ci::touch_pages(ci::host_pool(), c.h_hits, alloc_);
         }
         return CUSBF_OK;
-    }
-
-    static void prefault(void* p, uint64_t bytes) {
-        constexpr uint64_t kPart = 4ull << 20;
-        const uint64_t parts = (bytes + kPart - 1) / kPart;
-        ci::host_pool().run(parts, [&](size_t i) {
-            const uint64_t at = i * kPart;
-            std::memset(static_cast<char*>(p) + at, 0, std::min(kPart, bytes - at));
-        });
     }
 
     void free_buffers() {

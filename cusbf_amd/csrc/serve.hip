@@ -433,6 +433,12 @@ int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* optio
     for (int i = static_cast<int>(options->slots) - 1; i >= 0; --i) s->free_slots.push_back(i);
     const unsigned hw = std::thread::hardware_concurrency();
     s->pool = std::make_unique<WorkerPool>(std::min(12u, std::max(2u, hw > 2 ? hw - 2 : 2u)));
+    // the slots' pinned buffers touched once now, on the pool, rather than page by
+    // page (first-touch faults) inside the batch fills and answer copies
+    for (Slot& slot : s->slots) {
+        ci::touch_pages(*s->pool, slot.h_seq, s->seq_cap);
+        ci::touch_pages(*s->pool, slot.h_hits, s->seq_cap);
+    }
     cusbf_server* raw = s.release();
     raw->scheduler = std::thread(scheduler_loop, raw);
     raw->completer = std::thread(completion_loop, raw);
