@@ -601,6 +601,7 @@ int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStre
         const sbf::SequenceInput part = c.chunks == 1 ? in : window_range(f, in, first, n);
         bool ok = false;
         const sbf::SortedLayout l = plain_layout(f, n, &ok);  // no larger than the first range's
+        if (!ok || l.bytes > c.bytes) return fail(CUSBF_ERR_RESOURCE, "sorted insert range exceeds its workspace");
         const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, [&](const sbf::RecordSink& sink) {
             return f->strip->insert(f->kc, part, view(f), &sink, stream);
         });

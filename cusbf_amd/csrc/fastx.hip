@@ -660,6 +660,7 @@ class FastqSplitter {
             cut = recs[max_records].header_at - 1;
             recs.resize(max_records);
             while (!parts_.empty() && parts_.back().a >= cut) parts_.pop_back();
+            if (parts_.empty()) return false;  // cannot happen: cut lies past the first record
             Part& last = parts_.back();
             last.b = std::min(last.b, cut);
             while (!last.recs.empty() && last.recs.back().header_at - 1 >= cut) last.recs.pop_back();
