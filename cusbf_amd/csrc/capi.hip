@@ -135,6 +135,8 @@ struct cusbf_filter {
     uint8_t* d_result = nullptr;
     uint64_t result_cap = 0;
     uint64_t* d_scalar = nullptr;  // reductions
+    uint64_t* d_rec = nullptr;     // host record batches: per-record requests and counts
+    uint64_t rec_cap = 0;          // words
     // batched record form: tuning table, work-queue state, record-start bitmap
     uint32_t compute_units = 256;
     cusbf_batch_tuning tuning[CUSBF_BATCH_BUCKETS][3] = {};
@@ -649,11 +651,6 @@ int validate_ranges(const cusbf_filter* f, uint64_t bytes, const cusbf_record_ra
     return CUSBF_OK;
 }
 
-struct NormRecord {
-    uint64_t size;
-    uint64_t valid_kmers;
-};
-
 uint64_t record_kmers(const cusbf_filter* f, uint64_t bytes) { return windows_of(f, bytes); }
 
 // normalized_record_batch.hpp:87-110 (valid_record_kmer_count): windows of
@@ -698,30 +695,113 @@ uint64_t valid_record_kmers(const cusbf_filter* f, const char* seq, uint64_t byt
 
 // normalized_record_batch.hpp:208-276: records joined by the alphabet
 // separator, padded to the symbol width.
-void normalize(const cusbf_filter* f, const char* seq, const cusbf_record_range* ranges, uint64_t n,
-               std::vector<char>& out, std::vector<NormRecord>& records) {
+// A host record batch as the buffer the reference normalises it into
+// (normalized_record_batch.hpp:208-276): records joined by the separator, each
+// boundary padded to the symbol width. The bytes are produced chunk by chunk
+// straight into the pinned ring by the host pool, never as a buffer of their own.
+struct NormPlan {
+    std::vector<uint64_t> at;  // per record: first byte in the normalised buffer
+    uint64_t total = 0;        // bytes of the normalised buffer
+    uint64_t bases = 0;        // record bytes
+};
+
+void norm_plan(const cusbf_filter* f, const cusbf_record_range* r, uint64_t n, NormPlan* p) {
     const uint32_t w = f->d.symbol_width;
+    p->at.resize(n);
+    p->total = 0;
+    p->bases = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (p->total) p->total += (p->total % w ? w - p->total % w : 0) + w;
+        p->at[i] = p->total;
+        p->total += r[i].sequence_bytes;
+        p->bases += r[i].sequence_bytes;
+    }
+}
+
+// Bytes [a, a + len) of the normalised buffer into dst.
+void norm_fill(const cusbf_filter* f, const char* seq, const cusbf_record_range* r, const NormPlan& p,
+               uint64_t a, uint64_t len, char* dst) {
     const char sep = static_cast<char>(f->d.separator);
-    uint64_t total = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-        if (total) total += (total % w ? w - total % w : 0) + w;
-        total += ranges[i].sequence_bytes;
+    const uint64_t n = p.at.size();
+    // the first record that ends past a
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (p.at[mid] + r[mid].sequence_bytes <= a) lo = mid + 1; else hi = mid;
     }
-    out.resize(total);
-    records.clear();
-    records.reserve(n);
-    uint64_t pos = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-        if (pos) {
-            const uint64_t pad = (pos % w ? w - pos % w : 0) + w;
-            std::memset(out.data() + pos, sep, pad);
-            pos += pad;
+    uint64_t i = lo, pos = a;
+    const uint64_t end = a + len;
+    while (pos < end) {
+        if (i < n && pos >= p.at[i]) {
+            const uint64_t rec_end = p.at[i] + r[i].sequence_bytes;
+            const uint64_t m = std::min(end, rec_end) - pos;
+            std::memcpy(dst + (pos - a), seq + r[i].sequence_offset + (pos - p.at[i]), m);
+            pos += m;
+            if (pos == rec_end) ++i;
+        } else {
+            const uint64_t stop = i < n ? std::min(end, p.at[i]) : end;
+            std::memset(dst + (pos - a), sep, stop - pos);
+            pos = stop;
         }
-        const char* rec = seq + ranges[i].sequence_offset;
-        std::memcpy(out.data() + pos, rec, ranges[i].sequence_bytes);
-        pos += ranges[i].sequence_bytes;
-        records.push_back(NormRecord{ranges[i].sequence_bytes, valid_record_kmers(f, rec, ranges[i].sequence_bytes)});
     }
+}
+
+// The normalised batch into device memory, in stream order (does not wait).
+int copy_in_normalized(cusbf_filter* f, void* d_dst, const char* seq, const cusbf_record_range* r,
+                       const NormPlan& p, hipStream_t stream) {
+    constexpr uint64_t kPart = 4ull << 20;
+    for (uint64_t at = 0; at < p.total; at += kRingBytes) {
+        const uint64_t n = std::min(kRingBytes, p.total - at);
+        uint32_t s = 0;
+        const int rc = ring_slot(f, &s);
+        if (rc != CUSBF_OK) return rc;
+        char* buf = f->h_ring[s];
+        cusbf_internal::host_pool().run((n + kPart - 1) / kPart, [&](size_t i) {
+            const uint64_t off = i * kPart;
+            norm_fill(f, seq, r, p, at + off, std::min(kPart, n - off), buf + off);
+        });
+        CUSBF_HIP_TRY(hipMemcpyAsync(static_cast<char*>(d_dst) + at, buf, n, hipMemcpyHostToDevice, stream));
+        CUSBF_HIP_TRY(hipEventRecord(f->ring_done[s], stream));
+        f->ring_pending[s] = true;
+    }
+    return CUSBF_OK;
+}
+
+// Per record of a staged normalised batch, on the device: windows with no
+// invalid symbol (normalized_record_batch.hpp:87-110) and, with d_hits, answers
+// equal to 1; summed into *valid and *positive. Waits for the stream.
+int record_counts(cusbf_filter* f, const cusbf_record_range* r, const NormPlan& p, const uint8_t* d_hits,
+                  hipStream_t stream, uint64_t* valid, uint64_t* positive) {
+    const uint64_t n = p.at.size();
+    *valid = 0;
+    if (positive) *positive = 0;
+    if (n == 0 || p.total == 0) return CUSBF_OK;
+    const uint32_t w = f->d.symbol_width;
+    std::vector<uint64_t> req(2 * n), counts(2 * n);
+    for (uint64_t i = 0; i < n; ++i) {
+        req[2 * i] = p.at[i] / w;
+        req[2 * i + 1] = record_kmers(f, r[i].sequence_bytes);
+    }
+    if (f->rec_cap < 4 * n) {
+        if (f->d_rec) CUSBF_HIP_TRY(hipFree(f->d_rec));
+        f->d_rec = nullptr;
+        f->rec_cap = 0;
+        CUSBF_HIP_TRY(hipMalloc(&f->d_rec, 4 * n * sizeof(uint64_t)));
+        f->rec_cap = 4 * n;
+    }
+    uint64_t* d_req = f->d_rec;
+    uint64_t* d_counts = f->d_rec + 2 * n;
+    int rc = copy_in(f, d_req, req.data(), 2 * n * sizeof(uint64_t), stream);
+    if (rc == CUSBF_OK) {
+        rc = cusbf_internal::launch_request_counts(f, f->d_stage, p.total, d_hits, d_req, n, d_counts, stream);
+    }
+    if (rc == CUSBF_OK) rc = copy_out(f, counts.data(), d_counts, 2 * n * sizeof(uint64_t), stream);
+    if (rc != CUSBF_OK) return rc;
+    for (uint64_t i = 0; i < n; ++i) {
+        *valid += counts[2 * i];
+        if (positive) *positive += counts[2 * i + 1];
+    }
+    return CUSBF_OK;
 }
 
 // --- batched record form ------------------------------------------------------
@@ -1082,6 +1162,7 @@ void cusbf_destroy(cusbf_filter* f) {
     release_ring(f);
     if (f->d_words) (void)hipFree(f->d_words);
     if (f->d_scalar) (void)hipFree(f->d_scalar);
+    if (f->d_rec) (void)hipFree(f->d_rec);
     if (f->d_batch_state) (void)hipFree(f->d_batch_state);
     if (f->d_zero) (void)hipFree(f->d_zero);
     if (f->d_starts) (void)hipFree(f->d_starts);
@@ -1448,22 +1529,15 @@ int cusbf_insert_record_batch(cusbf_filter* f, const char* h_sequence, uint64_t 
     if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
     int rc = validate_ranges(f, bytes, ranges, num_ranges, "record batch range exceeds the source sequence");
     if (rc != CUSBF_OK) return rc;
-    std::vector<char> norm;
-    std::vector<NormRecord> records;
-    normalize(f, h_sequence, ranges, num_ranges, norm, records);
-    cusbf_insert_report rep{records.size(), 0, 0};
-    for (const NormRecord& r : records) {
-        rep.indexed_bases += r.size;
-        rep.inserted_kmers += r.valid_kmers;
-    }
-    if (!norm.empty()) {
-        rc = ensure_stage(f, norm.size());
+    NormPlan p;
+    norm_plan(f, ranges, num_ranges, &p);
+    cusbf_insert_report rep{num_ranges, p.bases, 0};
+    if (p.total) {
+        rc = ensure_stage(f, p.total);
+        if (rc == CUSBF_OK) rc = copy_in_normalized(f, f->d_stage, h_sequence, ranges, p, as_stream(stream));
+        if (rc == CUSBF_OK) rc = launch_insert(f, byte_input(f, f->d_stage, p.total), as_stream(stream));
+        if (rc == CUSBF_OK) rc = record_counts(f, ranges, p, nullptr, as_stream(stream), &rep.inserted_kmers, nullptr);
         if (rc != CUSBF_OK) return rc;
-        rc = copy_in(f, f->d_stage, norm.data(), norm.size(), as_stream(stream));
-        if (rc != CUSBF_OK) return rc;
-        rc = launch_insert(f, byte_input(f, f->d_stage, norm.size()), as_stream(stream));
-        if (rc != CUSBF_OK) return rc;
-        CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
     }
     release_staging(f);
     if (report) *report = rep;
@@ -1478,30 +1552,20 @@ int cusbf_query_record_batch(const cusbf_filter* f_const, const char* h_sequence
     if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
     int rc = validate_ranges(f, bytes, ranges, num_ranges, "record batch range exceeds the source sequence");
     if (rc != CUSBF_OK) return rc;
-    std::vector<char> norm;
-    std::vector<NormRecord> records;
-    normalize(f, h_sequence, ranges, num_ranges, norm, records);
-    cusbf_query_report rep{records.size(), 0, 0, 0};
-    for (const NormRecord& r : records) {
-        rep.queried_bases += r.size;
-        rep.queried_kmers += r.valid_kmers;
-    }
-    if (!norm.empty()) {
-        // query_layout.cuh:66-67: total hits = windows of the normalised buffer.
-        const uint64_t hits = windows_of(f, norm.size());
-        rc = ensure_stage(f, norm.size());
-        if (rc == CUSBF_OK && hits) rc = ensure_result(f, hits);
-        if (rc != CUSBF_OK) return rc;
-        rc = copy_in(f, f->d_stage, norm.data(), norm.size(), as_stream(stream));
-        if (rc != CUSBF_OK) return rc;
-        if (hits) {
-            rc = launch_contains(f, byte_input(f, f->d_stage, norm.size()), f->d_result, as_stream(stream));
-            if (rc != CUSBF_OK) return rc;
-            CUSBF_HIP_TRY(sbf::fill_u64(f->d_scalar, 0, 1, as_stream(stream)));
-            CUSBF_HIP_TRY(sbf::count_ones(f->d_result, hits, f->d_scalar, as_stream(stream)));
-            CUSBF_HIP_TRY(hipMemcpyAsync(&rep.positive_kmers, f->d_scalar, sizeof(uint64_t), hipMemcpyDeviceToHost, as_stream(stream)));
+    NormPlan p;
+    norm_plan(f, ranges, num_ranges, &p);
+    cusbf_query_report rep{num_ranges, p.bases, 0, 0};
+    // query_layout.cuh:66-67: total hits = windows of the normalised buffer.
+    const uint64_t hits = windows_of(f, p.total);
+    if (hits) {
+        rc = ensure_stage(f, p.total);
+        if (rc == CUSBF_OK) rc = ensure_result(f, hits);
+        if (rc == CUSBF_OK) rc = copy_in_normalized(f, f->d_stage, h_sequence, ranges, p, as_stream(stream));
+        if (rc == CUSBF_OK) rc = launch_contains(f, byte_input(f, f->d_stage, p.total), f->d_result, as_stream(stream));
+        if (rc == CUSBF_OK) {
+            rc = record_counts(f, ranges, p, f->d_result, as_stream(stream), &rep.queried_kmers, &rep.positive_kmers);
         }
-        CUSBF_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+        if (rc != CUSBF_OK) return rc;
     }
     release_staging(f);
     if (report) *report = rep;
@@ -1517,20 +1581,21 @@ int cusbf_query_record_batch_hits(const cusbf_filter* f_const, const char* h_seq
     if (!f) return fail(CUSBF_ERR_INVALID_ARGUMENT, "null filter");
     int rc = validate_ranges(f, bytes, ranges, num_ranges, "record batch range exceeds the source sequence");
     if (rc != CUSBF_OK) return rc;
-    std::vector<char> norm;
-    std::vector<NormRecord> records;
-    normalize(f, h_sequence, ranges, num_ranges, norm, records);
-    const uint64_t hits = windows_of(f, norm.size());
+    NormPlan p;
+    norm_plan(f, ranges, num_ranges, &p);
+    const uint64_t hits = windows_of(f, p.total);
     if (hits_len < hits) return fail(CUSBF_ERR_INVALID_ARGUMENT, "record batch hit buffer is too small");
-    cusbf_query_report rep{records.size(), 0, 0, 0};
-    for (const NormRecord& r : records) {
-        rep.queried_bases += r.size;
-        rep.queried_kmers += r.valid_kmers;
-    }
+    cusbf_query_report rep{num_ranges, p.bases, 0, 0};
     if (hits) {
-        rc = cusbf_contains_sequence(f, norm.data(), norm.size(), h_hits, hits_len, stream);
+        rc = ensure_stage(f, p.total);
+        if (rc == CUSBF_OK) rc = ensure_result(f, hits);
+        if (rc == CUSBF_OK) rc = copy_in_normalized(f, f->d_stage, h_sequence, ranges, p, as_stream(stream));
+        if (rc == CUSBF_OK) rc = launch_contains(f, byte_input(f, f->d_stage, p.total), f->d_result, as_stream(stream));
+        if (rc == CUSBF_OK) {
+            rc = record_counts(f, ranges, p, f->d_result, as_stream(stream), &rep.queried_kmers, &rep.positive_kmers);
+        }
+        if (rc == CUSBF_OK) rc = copy_out(f, h_hits, f->d_result, hits, as_stream(stream));
         if (rc != CUSBF_OK) return rc;
-        for (uint64_t i = 0; i < hits; ++i) rep.positive_kmers += h_hits[i] == 1;
     }
     if (report) *report = rep;
     return CUSBF_OK;

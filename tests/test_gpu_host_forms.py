@@ -94,3 +94,56 @@ def test_host_dense_forms_through_the_ring(gpu):
     out = torch.zeros(n - K + 1, dtype=torch.uint8, device="cuda")
     assert f.contains_dense_packed(words, n, d_output=out) is None
     assert torch.equal(out, dev_out)
+
+
+@pytest.mark.parametrize("alphabet,cfg_t", [("dna", (31, 28, 16, 4)), ("triplet", (3, 2, 2, 4))])
+def test_host_record_batch_through_the_ring(gpu, alphabet, cfg_t):
+    """Host record batches (filter.cuh:338-364, 549-582, 930-990) of ~70 MB: the
+    normalised buffer (records joined by separators padded to the symbol width) is
+    produced chunk by chunk into the pinned ring by host threads, and the reports'
+    valid windows are counted per record on the device. Words, reports and the
+    per-window hits of the normalised batch equal the oracle's."""
+    import cusbf_amd as cs
+    from cusbf_amd.serve import query_record_batch_hits
+
+    k, s, m, h = cfg_t
+    width = 3 if alphabet == "triplet" else 1
+    rng = np.random.default_rng(5)
+    lens = np.clip(np.round(rng.lognormal(np.log(1500), 1.3, 20_000)), 0, 2_000_000).astype(np.int64)
+    lens[:4] = [0, 2, 31, 1]
+    gaps = rng.integers(0, 5, lens.size)
+    if alphabet == "triplet":  # ranges align to whole codons (filter.cuh:429-457)
+        lens, gaps = lens - lens % 3, 3 * gaps
+    total = int(lens.sum() + gaps.sum())
+    source = random_dna(total, 31, n_rate=0.002)
+    ranges, at = [], 0
+    for ln, gp in zip(lens.tolist(), gaps.tolist()):
+        at += gp
+        ranges.append((at, ln))
+        at += ln
+    records = [source[a:a + n] for a, n in ranges]
+    cfg = make_config(k, s, m, h, alphabet)
+    bits = 1 << 30
+    f = cs.Filter(cfg, bits)
+    rep = f.insert_record_batch(source, ranges)
+    o = make_oracle(k, s, m, h, alphabet)
+    w = o.new_filter(f.num_shards())
+    # normalized_record_batch.hpp:208-232: a boundary (pad to the symbol width, then one
+    # separator symbol) before every record that has bytes before it
+    buf = bytearray()
+    for r in records:
+        if buf:
+            buf += b"N" * ((-len(buf)) % width + width)
+        buf += r
+    joined = bytes(buf)
+    o.mt_insert(joined, w)
+    assert (f.words() == w).all()
+    valid = sum(o.count(r, f.num_shards())[0] for r in records if len(r) >= k * width)
+    assert (rep.records_indexed, rep.indexed_bases, rep.inserted_kmers) == (len(records), int(lens.sum()), valid)
+    want = o.mt_contains(joined, w)
+    q = f.query_record_batch(source, ranges)
+    assert (q.records_queried, q.queried_bases, q.queried_kmers) == (len(records), int(lens.sum()), valid)
+    assert q.positive_kmers == int(want.sum())
+    hits, qh = query_record_batch_hits(f, source, ranges)
+    assert (hits[: want.size] == want).all()
+    assert (qh.queried_kmers, qh.positive_kmers) == (valid, int(want.sum()))
