@@ -216,6 +216,19 @@ def nematode_scenario(steps: int, warmup: int, fpr_kmers: int) -> dict:
     log(f"[nematode] insert {ti * 1e3:.3f} ms ({v_ins / ti / 1e9:.1f} G k-mers/s), query "
         f"{tq * 1e3:.3f} ms ({v_q / tq / 1e9:.1f} G k-mers/s)")
     filt.close()
+    # the sizing of the reference's published figures, bit_ceil(16 x k-mers) bits
+    # (benchmarks/benchmark_common.cuh:178-186): 2^31 bits for this genome
+    ref_bits = 1 << (16 * windows - 1).bit_length()
+    g = cs.Filter(cs.Config(wl.k, wl.s, wl.m, wl.h), ref_bits)
+    ti2, tq2 = timed_insert_query(g, genome, reads, out, sh, steps, warmup)
+    res["reference_sizing"] = {"filter_bits": g.filter_bits(),
+                               "bits_per_kmer_effective": g.filter_bits() / max(1, v_ins),
+                               "insert_ms": ti2 * 1e3, "query_ms": tq2 * 1e3,
+                               "insert_kmers_per_s": v_ins / ti2, "query_kmers_per_s": v_q / tq2,
+                               "insert_path": {1: "atomic", 2: "sorted"}[g.insert_path().last_path]}
+    log(f"[nematode] at 2^{ref_bits.bit_length() - 1} bits (the reference's sizing): insert "
+        f"{v_ins / ti2 / 1e9:.1f}, query {v_q / tq2 / 1e9:.1f} G k-mers/s")
+    g.close()
     del reads, out
     torch.cuda.empty_cache()
 
