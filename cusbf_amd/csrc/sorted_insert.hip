@@ -358,10 +358,10 @@ __device__ __forceinline__ void load_block_items(const PassInput& in, const uint
 // counts of every block b < nblocks, zeros when it has no items). One
 // histogram block counts kHistBlocks consecutive partition blocks, so one
 // digit's counts leave as kHistBlocks consecutive words (a digit-major layout
-// written one word per block would cost a write request per count).
-constexpr int kHistBlocks = 8;
-
-template <bool Near, bool Group>
+// written one word per block would cost a write request per count). Small
+// inserts, whose partition blocks would give too few histogram blocks to
+// fill the GPU, count one partition block per histogram block.
+template <bool Near, bool Group, int kHistBlocks>
 __global__ __launch_bounds__(kRankBlock) void hist_any_kernel(PassInput in, const uint32_t* first_block,
                                                               const uint32_t* group_start, uint32_t shift,
                                                               uint32_t digit_mask, uint32_t nblocks,
@@ -676,6 +676,16 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStre
         return bits;
     };
 
+    // histogram blocks of eight partition blocks when that still gives 1024 or more
+    auto histogram = [&](auto kernel8, auto kernel1, const PassInput& pin, const uint32_t* fb,
+                         const uint32_t* gs, uint32_t sh, uint32_t mk, uint32_t nb) {
+        if (nb >= 8u * 1024u) {
+            hipLaunchKernelGGL(kernel8, dim3(div_up(nb, 8)), dim3(kRankBlock), 0, stream, pin, fb, gs, sh, mk, nb, hist);
+        } else {
+            hipLaunchKernelGGL(kernel1, dim3(nb), dim3(kRankBlock), 0, stream, pin, fb, gs, sh, mk, nb, hist);
+        }
+    };
+
     // pass 0: the emit regions, unstable
     uint32_t shift, mask;
     uint32_t bits = digit(0, shift, mask);
@@ -683,11 +693,11 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStre
                  reinterpret_cast<const uint32_t*>(p + l.off_counts)};
     const bool near = in.cap >= static_cast<uint32_t>(kWaveItems) || in.cap == in.slots;
     if (near) {
-        hipLaunchKernelGGL((hist_any_kernel<true, false>), dim3(div_up(nblocks0, kHistBlocks)), dim3(kRankBlock), 0, stream, in,
-                           nullptr, nullptr, shift, mask, nblocks0, hist);
+        histogram(hist_any_kernel<true, false, 8>, hist_any_kernel<true, false, 1>, in, nullptr, nullptr, shift,
+                  mask, nblocks0);
     } else {
-        hipLaunchKernelGGL((hist_any_kernel<false, false>), dim3(div_up(nblocks0, kHistBlocks)), dim3(kRankBlock), 0, stream, in,
-                           nullptr, nullptr, shift, mask, nblocks0, hist);
+        histogram(hist_any_kernel<false, false, 8>, hist_any_kernel<false, false, 1>, in, nullptr, nullptr,
+                  shift, mask, nblocks0);
     }
     uint32_t* total = totals;
     scan(kBins * nblocks0, total);
@@ -709,8 +719,8 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStre
                            group_start, first_block);
         bits = digit(1, shift, mask);
         in = PassInput{pairs[1], static_cast<uint32_t>(l.slots), static_cast<uint32_t>(l.slots), total};
-        hipLaunchKernelGGL((hist_any_kernel<true, true>), dim3(div_up(nblocks1, kHistBlocks)), dim3(kRankBlock), 0, stream, in,
-                           first_block, group_start, shift, mask, nblocks1, hist);
+        histogram(hist_any_kernel<true, true, 8>, hist_any_kernel<true, true, 1>, in, first_block, group_start,
+                  shift, mask, nblocks1);
         scan(kBins * nblocks1, totals + 1);
         hipLaunchKernelGGL((scatter_any_kernel<true, true>), dim3(nblocks1), dim3(kRankBlock), 0, stream, in,
                            first_block, group_start, shift, mask, nblocks1, hist, pairs[0]);
