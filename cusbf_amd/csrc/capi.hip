@@ -418,10 +418,13 @@ int choose_sorted(const cusbf_filter* f, uint64_t windows, uint64_t tiles, uint3
 }
 
 // Blocks of a sorted plain insert: 16 per compute unit at most, so that every
-// region reserves for a few hundred tiles.
+// region reserves for a few hundred tiles; a smaller insert takes no more blocks
+// than give each wave 16 tiles (four per compute unit at least), so that its
+// regions still hold a whole partition wave's items.
 uint32_t plain_sorted_blocks(const cusbf_filter* f, uint64_t tiles) {
     uint64_t blocks = (tiles + sbf::kStripWavesPerBlock - 1) / sbf::kStripWavesPerBlock;
     blocks = std::min<uint64_t>(blocks, 16ull * f->compute_units);
+    blocks = std::min<uint64_t>(blocks, std::max<uint64_t>(4ull * f->compute_units, (tiles + 63) / 64));
     blocks = std::min<uint64_t>(blocks, sbf::kStripMaxBlocks);
     return static_cast<uint32_t>(std::max<uint64_t>(blocks, 1));
 }
