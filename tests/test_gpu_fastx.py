@@ -145,6 +145,7 @@ def expected(cfg_t, records, query_records, num_shards):
     ("dna", (31, 28, 16, 4, "dna"), "fasta"),
     ("dna", (31, 28, 16, 4, "dna"), "fastq"),
     ("protein", (10, 8, 6, 4, "protein"), "fasta"),
+    ("protein", (10, 8, 6, 4, "protein"), "fastq"),
     ("dna", (17, 15, 9, 4, "dna"), "fastq"),  # runtime-span lane strips
 ])
 def test_fastx_file_insert_query_match_oracle(gpu, tmp_path, alphabet, cfg_t, fmt):
@@ -177,6 +178,7 @@ def test_fastx_file_insert_query_match_oracle(gpu, tmp_path, alphabet, cfg_t, fm
     ("fastq", (31, 28, 16, 4, "dna")),
     ("fasta", (31, 28, 16, 4, "dna")),
     ("fasta", (3, 2, 2, 4, "triplet")),  # records padded to whole codons
+    ("fastq", (3, 2, 2, 4, "triplet")),
 ])
 def test_fastx_sources_and_chunking_agree(gpu, tmp_path, fmt, cfg_t):
     """File, gzip file, host buffer, reader, tiny chunks, fill_fraction 0: same words and
