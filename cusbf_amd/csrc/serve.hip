@@ -17,6 +17,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -74,6 +75,8 @@ struct Slot {
     std::vector<Request*> batch;
 };
 
+constexpr uint64_t kBatchTarget = 256ull << 20;
+
 // Copy parts of at most 4 MiB.
 constexpr uint64_t kCopyPart = 4ull << 20;
 
@@ -97,7 +100,8 @@ struct cusbf_server {
     cusbf_server_options options{};
     uint32_t width = 1;
     char sep = 'N';
-    uint64_t seq_cap = 0;  // per slot
+    uint64_t seq_cap = 0;       // per slot
+    uint64_t batch_target = 0;  // a batch closes once it holds this many bytes
     std::vector<Slot> slots;
 
     std::mutex mu;
@@ -269,7 +273,7 @@ void scheduler_loop(cusbf_server* s) {
             s->cv_queue.wait_until(lk, until, [&] {
                 uint64_t bytes = 0;
                 for (Request* r : s->queue) bytes += r->norm_bytes + s->width * 2;
-                return s->stop || bytes >= s->seq_cap;
+                return s->stop || bytes >= s->batch_target;
             });
         }
         s->cv_slots.wait(lk, [&] { return !s->free_slots.empty(); });
@@ -281,7 +285,7 @@ void scheduler_loop(cusbf_server* s) {
         while (!s->queue.empty() && slot.batch.size() < kMaxBatchRequests) {
             Request* r = s->queue.front();
             const uint64_t need = boundary_bytes(bytes, s->width) + r->norm_bytes;
-            if (bytes + need > s->seq_cap && !slot.batch.empty()) break;
+            if (!slot.batch.empty() && (bytes + need > s->seq_cap || bytes >= s->batch_target)) break;
             s->queue.pop_front();
             slot.batch.push_back(r);
             bytes += need;
@@ -411,6 +415,15 @@ int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* optio
     s->sep = static_cast<char>(ci::separator(f));
     s->seq_cap = options->staging_bytes / options->slots / 2;
     if (s->seq_cap < 4096) return ci::set_error(CUSBF_ERR_INVALID_ARGUMENT, "staging budget is too small");
+    // Batches of at most kBatchTarget bytes (a larger request still goes alone, up to
+    // the slot's staging): under a backlog the slots then take turns, one batch's
+    // answers coming back while the next one's bytes go out, instead of every slot
+    // filling its whole staging at once.
+    s->batch_target = std::min(s->seq_cap, kBatchTarget);
+    if (const char* env = std::getenv("CUSBF_SERVER_BATCH_BYTES")) {
+        const uint64_t v = std::strtoull(env, nullptr, 10);
+        if (v) s->batch_target = std::min(s->seq_cap, v);
+    }
     s->slots.resize(options->slots);
     for (Slot& slot : s->slots) {
         hipError_t e = hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking);
