@@ -147,3 +147,28 @@ def test_host_record_batch_through_the_ring(gpu, alphabet, cfg_t):
     hits, qh = query_record_batch_hits(f, source, ranges)
     assert (hits[: want.size] == want).all()
     assert (qh.queried_kmers, qh.positive_kmers) == (valid, int(want.sum()))
+
+
+@pytest.mark.parametrize("n", [(4 << 20) - 1, 4 << 20, 32 << 20, (96 << 20) + 1])
+def test_host_forms_at_ring_sizes(gpu, n):
+    """Sizes at the ring's edges: below the 4 MiB direct-copy limit, exactly 4 MiB, one
+    whole 32 MiB buffer, and one byte past three buffers (the ring wraps)."""
+    import cusbf_amd as cs
+
+    torch = _torch()
+    cfg = make_config(K, S, M, H)
+    seq = random_dna(n, n % 1000, n_rate=0.001)
+    f = cs.Filter(cfg, 1 << 28)
+    g = cs.Filter(cfg, 1 << 28)
+    assert f.insert_sequence(seq) == n - K + 1
+    g.insert_sequence_async(_dev(seq))
+    torch.cuda.synchronize()
+    assert (f.words() == g.words()).all()
+    query = random_dna(n, n % 1000 + 1, n_rate=0.001)
+    out = torch.empty(n - K + 1, dtype=torch.uint8, device="cuda")
+    g.contains_sequence_async(_dev(query), out)
+    want = out.cpu().numpy()
+    assert (f.contains_sequence(query) == want).all()
+    out2 = torch.full((n - K + 1,), 9, dtype=torch.uint8, device="cuda")
+    f.contains_sequence(query, d_output=out2)
+    assert torch.equal(out2, out)
