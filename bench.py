@@ -33,6 +33,16 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def guarded(name: str, fn, *args, **kwargs):
+    """Runs an auxiliary leg: a failure is logged and recorded in the JSON line as
+    {"error": ...} instead of ending the run (the headline figures do not depend on it)."""
+    try:
+        return fn(*args, **kwargs)
+    except Exception as e:  # noqa: BLE001 - any failure of a side leg is reported, not raised
+        log(f"[bench] {name} failed: {type(e).__name__}: {e}")
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -992,7 +1002,7 @@ def main() -> int:
     nematode = None
     if args.nematode == "on" and args.workload == "human":
         log("[bench] nematode-scale leg")
-        nematode = nematode_scenario(args.steps, args.warmup, args.fpr_kmers)
+        nematode = guarded("nematode", nematode_scenario, args.steps, args.warmup, args.fpr_kmers)
     wl = W.WORKLOADS[args.workload]
     if args.reads:
         wl.reads.n_reads = args.reads
@@ -1066,30 +1076,32 @@ def main() -> int:
     # equal the single insert's
     ranged = None
     if filt.insert_path().last_path == cs.INSERT_SORTED:
-        whole_ws = filt.sorted_workspace().allocated
-        # host copies of the words: a device view of them would hand out the words
-        # pointer, after which clears no longer mark the words as zero
-        ref_words = filt.words()
-        filt.set_sorted_workspace_limit(whole_ws // 4)
-        rt = []
-        for _ in range(3):
-            filt.clear(sh)
-            ev = _events(2)
-            ev[0].record(stream)
-            filt.insert_sequence_async(genome, stream=sh)
-            ev[1].record(stream)
-            ev[1].synchronize()
-            rt.append(ev[0].elapsed_time(ev[1]) / 1e3)
-        info = filt.sorted_workspace()
-        same = bool(np.array_equal(filt.words(), ref_words))
-        del ref_words
+        def measure_ranges() -> dict:
+            whole_ws = filt.sorted_workspace().allocated
+            # host copies of the words: a device view of them would hand out the words
+            # pointer, after which clears no longer mark the words as zero
+            ref_words = filt.words()
+            filt.set_sorted_workspace_limit(whole_ws // 4)
+            rt = []
+            for _ in range(3):
+                filt.clear(sh)
+                ev = _events(2)
+                ev[0].record(stream)
+                filt.insert_sequence_async(genome, stream=sh)
+                ev[1].record(stream)
+                ev[1].synchronize()
+                rt.append(ev[0].elapsed_time(ev[1]) / 1e3)
+            info = filt.sorted_workspace()
+            same = bool(np.array_equal(filt.words(), ref_words))
+            t_rng = statistics.median(rt)
+            log(f"[bench] insert in {info.last_chunks} window ranges (workspace limit "
+                f"{whole_ws / 4 / 2**30:.1f} GiB): {t_rng * 1e3:.2f} ms, words equal: {same}")
+            return {"workspace_limit_bytes": whole_ws // 4, "single_insert_workspace_bytes": whole_ws,
+                    "ranges": info.last_chunks, "insert_ms": t_rng * 1e3,
+                    "insert_kmers_per_s": v_ins / t_rng, "words_equal_single_insert": same}
+
+        ranged = guarded("insert in window ranges", measure_ranges)
         filt.set_sorted_workspace_limit(0)
-        t_rng = statistics.median(rt)
-        ranged = {"workspace_limit_bytes": whole_ws // 4, "single_insert_workspace_bytes": whole_ws,
-                  "ranges": info.last_chunks, "insert_ms": t_rng * 1e3,
-                  "insert_kmers_per_s": v_ins / t_rng, "words_equal_single_insert": same}
-        log(f"[bench] insert in {info.last_chunks} window ranges (workspace limit "
-            f"{whole_ws / 4 / 2**30:.1f} GiB): {t_rng * 1e3:.2f} ms, words equal: {same}")
 
     placement = None
     if args.placement_check:
@@ -1122,47 +1134,52 @@ def main() -> int:
     t_dom, b_dom = (tq, bytes_q) if dom == "query" else (ti, bytes_ins)
     achieved = b_dom / t_dom
 
-    fpr = fpr_scenario(genome, reads, wl, out, sh) if args.fpr == "on" else None
+    fpr = guarded("fpr", fpr_scenario, genome, reads, wl, out, sh) if args.fpr == "on" else None
 
     # serving runs on the filter as inserted, before the probes overwrite its buffer
     serving = None
     if args.serve == "on":
-        serving = serving_scenario(filt, genome, reads, wl, args.serve_requests, args.serve_qps)
+        serving = guarded("serving", serving_scenario, filt, genome, reads, wl, args.serve_requests,
+                          args.serve_qps)
 
     ceilings = None
     if args.probes == "on":
-        from cusbf_amd import probes
+        def measure_ceilings() -> dict:
+            from cusbf_amd import probes
 
-        log("[bench] measuring memory ceilings")
-        words = filt.words_device_ptr()
-        sectors = filt.num_shards()
-        copy_bytes = min(reads.numel(), 1 << 32)
-        copy_bw = probes.stream_copy(reads.data_ptr(), out.data_ptr(), copy_bytes)
-        gather = probes.random_gather(words, sectors, min(s_q, 2_000_000_000))
-        update = probes.random_sector_update(words, sectors, s_ins)
-        # the query's memory traffic without its arithmetic: the reads streamed in and
-        # the answers out in 1 KiB tiles, with the query's shard runs as random sectors
-        tiles = min(reads.numel(), out.numel()) // 1024
-        per_tile = max(1, round(s_q / tiles))
-        traffic_s = probes.query_traffic(reads.data_ptr(), out.data_ptr(), tiles, per_tile, words, sectors)
-        parts_s = {name: probes.query_traffic(reads.data_ptr(), out.data_ptr(), tiles, per_tile, words,
-                                              sectors, parts) * 1e3
-                   for name, parts in (("sectors_only_ms", 0), ("sectors_and_input_ms", 1),
-                                       ("sectors_and_output_ms", 2))}
-        ceilings = {
-            "stream_copy_GBps": copy_bw / 1e9,
-            "random_32B_gather_per_s": gather,
-            "random_32B_gather_GBps": gather * 32 / 1e9,
-            "random_32B_update_per_s": update,
-            "buffer_bytes": sectors * 32,
-            "query_traffic_probe": {"tiles": tiles, "stream_bytes_each_way": tiles * 1024,
-                                    "sectors_per_tile": per_tile, "sectors": tiles * per_tile,
-                                    "ms": traffic_s * 1e3, "query_ms_over_probe_ms": tq / traffic_s,
-                                    **parts_s},
-        }
-        log(f"[bench] copy {copy_bw / 1e9:.0f} GB/s, random gather {gather / 1e9:.2f} G/s, "
-            f"random update {update / 1e9:.2f} G/s; query traffic alone {traffic_s * 1e3:.2f} ms "
-            f"(query {tq * 1e3:.2f} ms)")
+            log("[bench] measuring memory ceilings")
+            words = filt.words_device_ptr()
+            sectors = filt.num_shards()
+            copy_bytes = min(reads.numel(), 1 << 32)
+            copy_bw = probes.stream_copy(reads.data_ptr(), out.data_ptr(), copy_bytes)
+            gather = probes.random_gather(words, sectors, min(s_q, 2_000_000_000))
+            update = probes.random_sector_update(words, sectors, s_ins)
+            # the query's memory traffic without its arithmetic: the reads streamed in and
+            # the answers out in 1 KiB tiles, with the query's shard runs as random sectors
+            tiles = min(reads.numel(), out.numel()) // 1024
+            per_tile = max(1, round(s_q / tiles))
+            traffic_s = probes.query_traffic(reads.data_ptr(), out.data_ptr(), tiles, per_tile, words, sectors)
+            parts_s = {name: probes.query_traffic(reads.data_ptr(), out.data_ptr(), tiles, per_tile, words,
+                                                  sectors, parts) * 1e3
+                       for name, parts in (("sectors_only_ms", 0), ("sectors_and_input_ms", 1),
+                                           ("sectors_and_output_ms", 2))}
+            c = {
+                "stream_copy_GBps": copy_bw / 1e9,
+                "random_32B_gather_per_s": gather,
+                "random_32B_gather_GBps": gather * 32 / 1e9,
+                "random_32B_update_per_s": update,
+                "buffer_bytes": sectors * 32,
+                "query_traffic_probe": {"tiles": tiles, "stream_bytes_each_way": tiles * 1024,
+                                        "sectors_per_tile": per_tile, "sectors": tiles * per_tile,
+                                        "ms": traffic_s * 1e3, "query_ms_over_probe_ms": tq / traffic_s,
+                                        **parts_s},
+            }
+            log(f"[bench] copy {copy_bw / 1e9:.0f} GB/s, random gather {gather / 1e9:.2f} G/s, "
+                f"random update {update / 1e9:.2f} G/s; query traffic alone {traffic_s * 1e3:.2f} ms "
+                f"(query {tq * 1e3:.2f} ms)")
+            return c
+
+        ceilings = guarded("ceilings", measure_ceilings)
     result = {
         "metric": METRIC,
         "value": value,
@@ -1196,9 +1213,9 @@ def main() -> int:
         "insert_roofline_frac": bytes_ins / ti / HBM_PEAK,
         "query_roofline_frac": bytes_q / tq / HBM_PEAK,
         "insert_random_sector_frac": (s_ins / ti / ceilings["random_32B_update_per_s"]
-                                      if ceilings else None),
+                                      if ceilings and "error" not in ceilings else None),
         "query_random_sector_frac": (s_q / tq / ceilings["random_32B_gather_per_s"]
-                                     if ceilings else None),
+                                     if ceilings and "error" not in ceilings else None),
         "query_positive_kmers": positives,
         "query_positive_frac": positives / max(1, v_q),
         "ceilings": ceilings,
@@ -1215,13 +1232,14 @@ def main() -> int:
     if placement is not None:
         result["placement_check"] = placement
     if serving is not None:
-        result["serve_p50_ms"] = serving["server_p50_ms"]
-        result["serve_p99_ms"] = serving["server_p99_ms"]
+        result["serve_p50_ms"] = serving.get("server_p50_ms")
+        result["serve_p99_ms"] = serving.get("server_p99_ms")
         result["serving"] = serving
     if args.dense == "on":
-        result["dense"] = dense_scenario(genome, wl, sh, args.dense_host_bases)
+        result["dense"] = guarded("dense", dense_scenario, genome, wl, sh, args.dense_host_bases)
     if args.fastx == "on":
-        result["fastx"] = fastx_scenario(genome, reads, wl, args.fastx_bases, args.fastx_reads)
+        result["fastx"] = guarded("fastx", fastx_scenario, genome, reads, wl, args.fastx_bases,
+                                  args.fastx_reads)
     if args.batched == "on":
         del out
         filt.close()
@@ -1230,13 +1248,13 @@ def main() -> int:
             return path.replace(".json", f"_{a}.json") if path else ""
 
         result["batched"] = {
-            a: batched_scenario(a, args.batched_records, args.batched_reps,
-                                per_alphabet(args.tuning_out, a), args.batched_baselines == "on",
-                                per_alphabet(args.tuning_in, a))
+            a: guarded(f"batched {a}", batched_scenario, a, args.batched_records, args.batched_reps,
+                       per_alphabet(args.tuning_out, a), args.batched_baselines == "on",
+                       per_alphabet(args.tuning_in, a))
             for a in ("dna", "protein")}
     if args.cpu_baseline == "on":
         log("[bench] CPU baseline on a bounded sample")
-        result["cpu_baseline"] = cpu_baseline(genome, reads, wl, args.cpu_threads)
+        result["cpu_baseline"] = guarded("cpu baseline", cpu_baseline, genome, reads, wl, args.cpu_threads)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
