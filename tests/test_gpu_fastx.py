@@ -272,3 +272,33 @@ def test_fastx_missing_file_is_io_error(gpu, tmp_path):
         f.insert_fastx_file(missing)
     assert e.value.category == ErrorCategory.io
     assert e.value.message == f"Failed to open FASTA/FASTQ file: {missing}"
+
+
+@pytest.mark.parametrize("fmt", ["fasta", "fastq"])
+def test_consumer_stop_on_split_text(gpu, tmp_path, fmt):
+    """A consumer that raises stops a query whose text is split by host threads (a file,
+    several chunks): the records before it arrive in order, the call raises, and the
+    filter keeps working."""
+    from cusbf_amd import fastx
+
+    records = make_records("dna", 600, seed=12, median=400)
+    text = fastq_text(records) if fmt == "fastq" else fasta_text(records, width=60)
+    path = write(tmp_path, "q." + fmt, text)
+    f = new_filter((31, 28, 16, 4, "dna"), bits=1 << 22)
+    f.insert_fastx_file(path)
+    fastx.set_chunk_bytes(f, 16384)  # many chunks
+    seen = []
+
+    def consumer(v):
+        seen.append(v.record_index)
+        if len(seen) == 250:
+            raise RuntimeError("stop here")
+
+    with pytest.raises(RuntimeError, match="stop here"):
+        f.query_fastx_file_records(path, consumer)
+    assert seen == list(range(250))
+    got = []
+    rep = f.query_fastx_file_records(path, lambda v: got.append((v.record_index, v.positive_kmers, v.queriedKmers)))
+    assert [g[0] for g in got] == list(range(len(records)))
+    assert all(p == q for _, p, q in got)  # every record was inserted
+    assert rep.records_queried == len(records)
