@@ -946,6 +946,44 @@ hipError_t request_counts(const KernelConfig& c, const char* seq, uint64_t num_s
     return hipGetLastError();
 }
 
+// bits[i / 64] bit i % 64 = hits[i] (each hit byte is 0 or 1): four hit bytes
+// x = b0 | b1 << 8 | b2 << 16 | b3 << 24 give b0 | b1 << 1 | b2 << 2 | b3 << 3 as
+// (x * 0x10204080) >> 28 in 32-bit arithmetic (b_i lands on bit 28 + i; every
+// cross term lands on a distinct bit below 28 or above 31, so nothing carries).
+__global__ __launch_bounds__(kBlock) void pack_hits_kernel(const uint8_t* __restrict__ hits, uint64_t n,
+                                                           uint64_t* __restrict__ bits) {
+    const uint64_t words = (n + 63) / 64;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; w < words; w += stride) {
+        const uint64_t p0 = 64 * w;
+        uint64_t out = 0;
+        if (p0 + 64 <= n && (reinterpret_cast<uintptr_t>(hits + p0) & 15u) == 0) {
+            const u32x4* src = reinterpret_cast<const u32x4*>(hits + p0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const u32x4 v = src[q];
+                const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    out |= static_cast<uint64_t>((x[c] * 0x10204080u) >> 28) << (16 * q + 4 * c);
+                }
+            }
+        } else {
+            const uint64_t m = n - p0 < 64 ? n - p0 : 64;
+            for (uint64_t i = 0; i < m; ++i) out |= static_cast<uint64_t>(hits[p0 + i] & 1u) << i;
+        }
+        bits[w] = out;
+    }
+}
+
+hipError_t pack_hits(const uint8_t* hits, uint64_t n, uint64_t* bits, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    (void)hipGetLastError();  // report only this launch
+    hipLaunchKernelGGL(pack_hits_kernel, dim3(grid_for((n + 63) / 64, kBlock, kMaxBlocks)), dim3(kBlock), 0,
+                       stream, hits, n, bits);
+    return hipGetLastError();
+}
+
 hipError_t join_records(const char* in, uint64_t num_symbols, const uint64_t* offsets,
                         uint64_t num_records, uint32_t symbol_width, uint8_t separator, char* out,
                         hipStream_t stream) {

@@ -184,6 +184,10 @@ hipError_t request_counts(const KernelConfig& c, const char* seq, uint64_t num_s
                           const uint8_t* hits, const uint64_t* req, uint64_t n_req,
                           uint64_t* counts, hipStream_t stream);
 
+// Hit bytes (0 or 1) packed into bits, LSB first: bits[i / 64] bit i % 64 = hits[i];
+// (n + 63) / 64 words.
+hipError_t pack_hits(const uint8_t* hits, uint64_t n, uint64_t* bits, hipStream_t stream);
+
 // Reductions.
 hipError_t popcount_words(const uint64_t* words, uint64_t n, uint64_t* d_out, hipStream_t stream);
 hipError_t count_ones(const uint8_t* hits, uint64_t n, uint64_t* d_out, hipStream_t stream);

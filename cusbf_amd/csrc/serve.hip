@@ -31,6 +31,7 @@
 #include "../../include/cusbf_hip.h"
 #include "filter_internal.hpp"
 #include "host_pool.hpp"
+#include "launch.hpp"
 
 namespace {
 
@@ -64,7 +65,8 @@ struct Slot {
     char* d_seq = nullptr;
     uint8_t* d_hits = nullptr;
     char* h_seq = nullptr;
-    uint8_t* h_hits = nullptr;
+    uint64_t* d_bits = nullptr;  // the batch's answers packed into bits (PCIe moves these)
+    uint64_t* h_bits = nullptr;
     uint64_t* d_req = nullptr;     // per request: first window in the batch, window count
     uint64_t* h_req = nullptr;
     uint64_t* d_counts = nullptr;  // per request: valid windows, positive windows
@@ -76,6 +78,35 @@ struct Slot {
 };
 
 constexpr uint64_t kBatchTarget = 256ull << 20;
+
+// Answers cross PCIe packed into bits (one eighth of the bytes; the device packs
+// them after the query) and are expanded into the callers' byte buffers on the
+// host pool: kExpand[b] holds bit j of b in byte j.
+uint64_t bits_bytes(uint64_t windows) { return (windows + 63) / 64 * sizeof(uint64_t); }
+
+struct ExpandTable {
+    uint64_t v[256];
+    constexpr ExpandTable() : v{} {
+        for (int b = 0; b < 256; ++b) {
+            uint64_t x = 0;
+            for (int j = 0; j < 8; ++j) x |= static_cast<uint64_t>((b >> j) & 1) << (8 * j);
+            v[b] = x;
+        }
+    }
+};
+constexpr ExpandTable kExpand{};
+
+// Answers [first, first + n) of a batch as bytes 0 / 1 at dst.
+void expand_bits(const uint64_t* bits, uint64_t first, uint64_t n, uint8_t* dst) {
+    uint64_t i = 0;
+    for (; i < n && ((first + i) & 7u) != 0; ++i) dst[i] = static_cast<uint8_t>((bits[(first + i) >> 6] >> ((first + i) & 63u)) & 1u);
+    const uint8_t* bytes = reinterpret_cast<const uint8_t*>(bits);  // little-endian: byte k holds bits 8k..8k+7
+    for (; i + 8 <= n; i += 8) {
+        const uint64_t v = kExpand.v[bytes[(first + i) >> 3]];
+        std::memcpy(dst + i, &v, sizeof(v));
+    }
+    for (; i < n; ++i) dst[i] = static_cast<uint8_t>((bits[(first + i) >> 6] >> ((first + i) & 63u)) & 1u);
+}
 
 // Copy parts of at most 4 MiB.
 constexpr uint64_t kCopyPart = 4ull << 20;
@@ -249,7 +280,11 @@ void launch_batch(cusbf_server* s, Slot& slot) {
             slot.status = rc;
             slot.message = buf;
         }
-        e = hipMemcpyAsync(slot.h_hits, slot.d_hits, windows, hipMemcpyDeviceToHost, slot.stream);
+        e = sbf::pack_hits(slot.d_hits, windows, slot.d_bits, slot.stream);
+        if (e == hipSuccess) {
+            e = hipMemcpyAsync(slot.h_bits, slot.d_bits, (windows + 63) / 64 * sizeof(uint64_t),
+                               hipMemcpyDeviceToHost, slot.stream);
+        }
         if (e == hipSuccess) {
             e = hipMemcpyAsync(slot.h_counts, slot.d_counts, 2 * n_req * sizeof(uint64_t),
                                hipMemcpyDeviceToHost, slot.stream);
@@ -333,7 +368,7 @@ void completion_loop(cusbf_server* s) {
         uint64_t windows = 0;
         struct Part {
             uint8_t* dst;
-            const uint8_t* src;
+            uint64_t first;  // the part's first window in the batch
             uint64_t len;
         };
         std::vector<Part> parts;
@@ -342,9 +377,9 @@ void completion_loop(cusbf_server* s) {
             r->status = slot.status;
             r->message = slot.message;
             if (slot.status == CUSBF_OK && r->windows) {
-                const uint8_t* src = slot.h_hits + r->batch_offset / s->width;
+                const uint64_t first = r->batch_offset / s->width;
                 for (uint64_t done = 0; done < r->windows; done += kCopyPart) {
-                    parts.push_back(Part{r->hits + done, src + done, std::min(kCopyPart, r->windows - done)});
+                    parts.push_back(Part{r->hits + done, first + done, std::min(kCopyPart, r->windows - done)});
                 }
                 r->report.queried_kmers = slot.h_counts[2 * i];
                 r->report.positive_kmers = slot.h_counts[2 * i + 1];
@@ -364,7 +399,7 @@ void completion_loop(cusbf_server* s) {
         task_start.push_back(parts.size());
         s->pool->run(task_start.size() - 1, [&](size_t t) {
             for (size_t j = task_start[t]; j < task_start[t + 1]; ++j) {
-                std::memcpy(parts[j].dst, parts[j].src, parts[j].len);
+                expand_bits(slot.h_bits, parts[j].first, parts[j].len, parts[j].dst);
             }
         });
         const auto now = Clock::now();
@@ -389,7 +424,8 @@ void free_slots(cusbf_server* s) {
         if (slot.d_seq) (void)hipFree(slot.d_seq);
         if (slot.d_hits) (void)hipFree(slot.d_hits);
         if (slot.h_seq) (void)hipHostFree(slot.h_seq);
-        if (slot.h_hits) (void)hipHostFree(slot.h_hits);
+        if (slot.d_bits) (void)hipFree(slot.d_bits);
+        if (slot.h_bits) (void)hipHostFree(slot.h_bits);
         if (slot.d_req) (void)hipFree(slot.d_req);
         if (slot.h_req) (void)hipHostFree(slot.h_req);
         if (slot.d_counts) (void)hipFree(slot.d_counts);
@@ -432,7 +468,8 @@ int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* optio
         if (e == hipSuccess) e = hipMalloc(&slot.d_seq, s->seq_cap);
         if (e == hipSuccess) e = hipMalloc(&slot.d_hits, s->seq_cap);
         if (e == hipSuccess) e = hipHostMalloc(&slot.h_seq, s->seq_cap, hipHostMallocDefault);
-        if (e == hipSuccess) e = hipHostMalloc(&slot.h_hits, s->seq_cap, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc(&slot.d_bits, bits_bytes(s->seq_cap));
+        if (e == hipSuccess) e = hipHostMalloc(&slot.h_bits, bits_bytes(s->seq_cap), hipHostMallocDefault);
         const uint64_t desc = 2 * kMaxBatchRequests * sizeof(uint64_t);
         if (e == hipSuccess) e = hipMalloc(&slot.d_req, desc);
         if (e == hipSuccess) e = hipHostMalloc(&slot.h_req, desc, hipHostMallocDefault);
@@ -450,7 +487,7 @@ int cusbf_server_create(const cusbf_filter* f, const cusbf_server_options* optio
     // page (first-touch faults) inside the batch fills and answer copies
     for (Slot& slot : s->slots) {
         ci::touch_pages(*s->pool, slot.h_seq, s->seq_cap);
-        ci::touch_pages(*s->pool, slot.h_hits, s->seq_cap);
+        ci::touch_pages(*s->pool, slot.h_bits, bits_bytes(s->seq_cap));
     }
     cusbf_server* raw = s.release();
     raw->scheduler = std::thread(scheduler_loop, raw);
