@@ -579,14 +579,61 @@ int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t str
     return rc;
 }
 
-int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
-    f->last_path = CUSBF_INSERT_ATOMIC;
-    f->last_chunks = 0;
-    if (!strip_reads(f, in)) {
-        const hipError_t e = sbf::generic_insert(f->kc, in, view(f), stream);
+// A generic insert on the sorted path: one sorted insert (no window ranges) when
+// forced, or under AUTO when the cost model prefers it and the atomic insert
+// would use direct tiles too, and the workspace fits; else the atomic path. The
+// generic kernels' waves take 64 windows of a block tile at a time, so a region
+// reserves for its wave's share of 64-window tiles, each cut into runs at its
+// edges as well.
+int launch_generic_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
+    const uint64_t windows = in.num_symbols < f->cfg.k ? 0 : in.num_symbols - f->cfg.k + 1;
+    bool use = false;
+    sbf::SortedLayout l;
+    if (windows != 0 && f->insert_path != CUSBF_INSERT_ATOMIC && !capturing(stream)) {
+        const bool forced = f->insert_path == CUSBF_INSERT_SORTED;
+        const uint64_t tiles = (windows + sbf::kGenericWaveWindows - 1) / sbf::kGenericWaveWindows;
+        uint64_t blocks = (tiles + 3) / 4;
+        blocks = std::min<uint64_t>(blocks, 16ull * f->compute_units);
+        blocks = std::min<uint64_t>(blocks, std::max<uint64_t>(4ull * f->compute_units, (tiles + 63) / 64));
+        blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, sbf::kStripMaxBlocks));
+        const uint64_t regions = 4 * blocks;
+        const double w = static_cast<double>(f->cfg.k - f->cfg.m + 1);
+        const uint64_t reserve = std::min<uint64_t>(
+            sbf::kGenericWaveWindows,
+            static_cast<uint64_t>(std::ceil(sbf::kGenericWaveWindows * 2.5 / (w + 1.0))) + 2);
+        const uint64_t cap = std::max<uint64_t>(1, (tiles + regions - 1) / regions * reserve);
+        if (regions * cap < (1ull << 31)) {
+            l = sbf::sorted::sorted_layout(static_cast<uint32_t>(regions), static_cast<uint32_t>(cap),
+                                           f->num_shards, f->slice_bits);
+            // AUTO: only where the atomic insert also uses direct tiles; the sorted
+            // build's direct tiles can cost more than the atomics they save
+            use = (forced || (!sbf::generic_hashed_tiles(f->kc) && sorted_pays(f, windows, l, 1))) &&
+                  workspace_fits(f, l.bytes);
+        }
+        if (forced && !use) {
+            return fail(CUSBF_ERR_RESOURCE, "sorted insert workspace does not fit its limit");
+        }
+    }
+    if (!use) {
+        const hipError_t e = sbf::generic_insert(f->kc, in, view(f), nullptr, stream);
         if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
         return CUSBF_OK;
     }
+    int rc = begin_sorted(f, l.bytes, stream);
+    if (rc != CUSBF_OK) return rc;
+    const hipError_t e = enqueue_sorted(f, l, f->d_sorted, stream, [&](const sbf::RecordSink& sink) {
+        return sbf::generic_insert(f->kc, in, view(f), &sink, stream);
+    });
+    if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
+    f->last_path = CUSBF_INSERT_SORTED;
+    f->last_chunks = 1;
+    return end_sorted(f, stream);
+}
+
+int launch_insert_ordered(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
+    f->last_path = CUSBF_INSERT_ATOMIC;
+    f->last_chunks = 0;
+    if (!strip_reads(f, in)) return launch_generic_insert(f, in, stream);
     const uint64_t windows = in.num_symbols < f->cfg.k ? 0 : in.num_symbols - f->cfg.k + 1;
     SortedChunks c;
     int rc = plan_plain_sorted(f, windows, stream, &c);
@@ -912,7 +959,7 @@ int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
                     : f->strip->batched_contains[variant](f->kc, in, view(f), shape, d_output, stream);
         }
     } else {
-        e = op == CUSBF_BATCH_INSERT ? sbf::generic_insert(f->kc, in, view(f), stream)
+        e = op == CUSBF_BATCH_INSERT ? sbf::generic_insert(f->kc, in, view(f), nullptr, stream)
                                      : sbf::generic_contains(f->kc, in, view(f), d_output, stream);
     }
     if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);

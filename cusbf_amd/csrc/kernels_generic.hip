@@ -322,14 +322,25 @@ __device__ __forceinline__ void tile_windows(const KernelConfig& c, const Sequen
 // over the wave (kernels.cuh:143-166 does the same per warp) leaves each run's
 // masks in its last lane, which issues one 64-bit atomic OR per non-zero word:
 // about one update per run instead of one per window.
-template <class Tile>
+//
+// Sorted = true (a sorted insert): the run's last lane writes the run as a record
+// into its wave's region instead (RecordSink in launch.hpp); runs past the
+// region's capacity still take the atomics. That build uses direct tiles only:
+// with hashed tiles it would spill scalar registers to scratch.
+template <class Tile, bool Sorted>
 __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, SequenceInput in,
-                                                        FilterView f, uint64_t num_windows) {
+                                                        FilterView f, uint64_t num_windows,
+                                                        RecordSink sink) {
     constexpr int R = Tile::kR;
     __shared__ Tile T;
     mark_words_written(f);
-    if (batch_rejected(in)) return;
     const int lane = static_cast<int>(threadIdx.x) & 63;
+    const uint32_t region = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    uint32_t emitted = 0;  // records this wave wrote (the same in every lane)
+    if (batch_rejected(in)) {
+        if (Sorted && lane == 0) sink.counts[region] = 0;
+        return;
+    }
     for (uint64_t first = static_cast<uint64_t>(blockIdx.x) * Tile::kWindows; first < num_windows;
          first += static_cast<uint64_t>(gridDim.x) * Tile::kWindows) {
         __syncthreads();  // the previous tile's readers are done
@@ -358,10 +369,25 @@ __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, Sequence
                     if (lane - d >= start) m[q] |= t;
                 }
             }
-            // the run's last lane applies its masks
+            // the run's last lane applies its masks (or writes them as a record)
             const bool tail = valid && (lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0 ||
                                         ((valids >> (lane + 1)) & 1ull) == 0);
-            if (tail) {
+            bool atomics = tail;
+            if constexpr (Sorted) {
+                const uint64_t tails = __ballot(tail);
+                const uint32_t rank = static_cast<uint32_t>(__popcll(tails & ((1ull << lane) - 1ull)));
+                const uint32_t room = sink.cap - emitted;
+                if (tail && rank < room) {
+                    const uint64_t at = static_cast<uint64_t>(region) * sink.cap + emitted + rank;
+                    sink.pairs[at] = (static_cast<uint64_t>(shard) << 32) | at;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) sink.masks[4 * at + q] = m[q];
+                    atomics = false;
+                }
+                const uint32_t n_tails = static_cast<uint32_t>(__popcll(tails));
+                emitted += n_tails < room ? n_tails : room;
+            }
+            if (atomics) {
                 uint64_t* dst = f.words + 4ull * shard;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -369,6 +395,10 @@ __global__ __launch_bounds__(kBlock) void insert_kernel(KernelConfig c, Sequence
                 }
             }
         }
+    }
+    if (Sorted && lane == 0) {
+        sink.counts[region] = emitted;
+        if (emitted == sink.cap) *sink.overflow = 1u;  // later runs may have used atomics
     }
 }
 
@@ -831,19 +861,26 @@ bool hashed_tiles(const KernelConfig& c) {
     return (c.k - c.m + 1) + (c.k - c.s + 1) * (c.h / 4) >= 24;
 }
 
+bool generic_hashed_tiles(const KernelConfig& c) { return hashed_tiles(c); }
+
 hipError_t generic_insert(const KernelConfig& c, const SequenceInput& in, FilterView f,
-                          hipStream_t stream) {
+                          const RecordSink* sink, hipStream_t stream) {
     const uint64_t n = window_count(c, in);
     if (n == 0) return hipSuccess;
     (void)hipGetLastError();  // report only this launch
-    if (hashed_tiles(c)) {
+    if (sink != nullptr) {
+        using Tile = DirectTile<kDirectR>;
+        const uint64_t most = sink->regions / (kBlock / 64);
+        const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(grid_for(n, Tile::kWindows, kMaxBlocks), most));
+        hipLaunchKernelGGL((insert_kernel<Tile, true>), dim3(grid), dim3(kBlock), 0, stream, c, in, f, n, *sink);
+    } else if (hashed_tiles(c)) {
         using Tile = HashedTile<kHashedR>;
-        hipLaunchKernelGGL(insert_kernel<Tile>, dim3(grid_for(n, Tile::kWindows, kMaxBlocks)), dim3(kBlock), 0,
-                           stream, c, in, f, n);
+        hipLaunchKernelGGL((insert_kernel<Tile, false>), dim3(grid_for(n, Tile::kWindows, kMaxBlocks)), dim3(kBlock),
+                           0, stream, c, in, f, n, RecordSink{});
     } else {
         using Tile = DirectTile<kDirectR>;
-        hipLaunchKernelGGL(insert_kernel<Tile>, dim3(grid_for(n, Tile::kWindows, kMaxBlocks)), dim3(kBlock), 0,
-                           stream, c, in, f, n);
+        hipLaunchKernelGGL((insert_kernel<Tile, false>), dim3(grid_for(n, Tile::kWindows, kMaxBlocks)), dim3(kBlock),
+                           0, stream, c, in, f, n, RecordSink{});
     }
     return hipGetLastError();
 }

@@ -150,9 +150,15 @@ hipError_t sorted_apply(const SortedLayout& l, void* base, FilterView f, hipStre
 
 // Generic kernels: any alphabet / configuration, one window per thread; in the
 // batched form they test the record-start bitmap (or, without it, search the
-// offsets) for each window.
+// offsets) for each window. With a record sink the insert writes its runs there
+// (regions per wave, as for the lane-strip kernels; a wave takes 64 windows of a
+// block tile at a time) for a sorted insert.
+constexpr uint32_t kGenericWaveWindows = 64;
+// Whether the generic kernels use hashed tiles for this configuration (the
+// sorted insert's build always uses direct tiles).
+bool generic_hashed_tiles(const KernelConfig& c);
 hipError_t generic_insert(const KernelConfig& c, const SequenceInput& in, FilterView f,
-                          hipStream_t stream);
+                          const RecordSink* sink, hipStream_t stream);
 hipError_t generic_contains(const KernelConfig& c, const SequenceInput& in, FilterView f,
                             uint8_t* out, hipStream_t stream);
 // d_counts[0] += valid windows, d_counts[1] += shard runs (caller zeroes).

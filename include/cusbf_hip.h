@@ -252,7 +252,11 @@ const char* cusbf_kernel_family(const cusbf_filter* f);
  * reservation, and beyond the 256 LDS run slots of a wave tile, still use
  * atomics. AUTO (the default) takes SORTED when a cost model says it is faster
  * and the workspace fits in half the free device memory, else ATOMIC. Inserts
- * recorded into a stream capture and the generic kernels always use ATOMIC.
+ * recorded into a stream capture always use ATOMIC, and so do batched inserts
+ * on the generic kernels; plain inserts on the generic kernels (any alphabet
+ * or configuration without lane-strip kernels) take SORTED as one sorted insert
+ * (no window ranges) when forced, and under AUTO when the cost model prefers it
+ * and the generic kernels use direct tiles for the configuration.
  * Sorted inserts on one filter share its workspace: one on another stream
  * waits for the previous one (stream-ordered, no host wait).
  */

@@ -134,6 +134,48 @@ def test_sorted_dense_packed(gpu):
     assert (f.words() == w).all()
 
 
+# Generic kernels (no lane strips): DNA triplets, a custom two-symbol alphabet, DNA
+# with a findere span past the runtime-span kernels' limit, a protein k past 12 is not
+# possible (5-bit symbols), so protein stays on the lane strips.
+GENERIC = [
+    (3, 2, 2, 4, "triplet"),
+    (7, 5, 4, 8, "triplet"),
+    (31, 8, 16, 4, "dna"),    # findere span 24
+    (31, 28, 4, 16, "dna"),   # minimizer span 28
+]
+
+
+@pytest.mark.parametrize("cfg_t", GENERIC, ids=lambda c: f"{c[4]}-k{c[0]}s{c[1]}m{c[2]}h{c[3]}")
+def test_sorted_generic_kernels(gpu, cfg_t):
+    """Plain inserts on the generic kernels take the sorted path too: words equal the
+    atomic path's and the oracle's, into zero words and into existing ones."""
+    import cusbf_amd as cs
+
+    k, s, m, h, alphabet = cfg_t
+    width = 3 if alphabet == "triplet" else 1
+    seq = random_dna(300_003 * width, k + m, n_rate=0.002)
+    more = random_dna(200_001 * width, k + m + 1, n_rate=0.002)
+    cfg = make_config(k, s, m, h, alphabet)
+    bits = 1 << 22
+    f = sorted_filter(cfg, bits)
+    assert f.kernel_family() == "generic"
+    f.insert_sequence(seq)
+    assert f.insert_path().last_path == cs.INSERT_SORTED
+    a = atomic_filter(cfg, bits)
+    a.insert_sequence(seq)
+    w = oracle_words(k, s, m, h, alphabet, [seq], f.num_shards())
+    assert (a.words() == w).all()
+    assert (f.words() == w).all()
+    f.insert_sequence(more)
+    a.insert_sequence(more)
+    w2 = oracle_words(k, s, m, h, alphabet, [seq, more], f.num_shards())
+    assert (f.words() == w2).all() and (a.words() == w2).all()
+    g = cs.Filter(cfg, bits)  # AUTO
+    g.insert_sequence(seq)
+    g.insert_sequence(more)
+    assert (g.words() == w2).all()
+
+
 RANGED = [
     (31, 28, 16, 4, "dna"),
     (10, 8, 6, 4, "protein"),
