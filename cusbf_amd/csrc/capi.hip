@@ -347,19 +347,64 @@ uint32_t reserve_per_tile(const cusbf_filter* f) {
     return static_cast<uint32_t>(std::min<double>(f->strip->wave_windows, std::max(1.0, r)));
 }
 
-// Cost model of AUTO, from MI355X measurements: the atomic insert does four
-// memory-side atomic ORs per run at ~25.6 G/s; the sorted insert moves ~45 ps
-// of record traffic per run and, in each of its `chunks` sorted inserts, loads
-// and stores every slice it touches, plus about a dozen launches.
-bool sorted_pays(const cusbf_filter* f, uint64_t windows, const sbf::SortedLayout& l, uint64_t chunks) {
+// Cost model of AUTO, from MI355X measurements.
+//
+// Hot shards: a window's minimizer is the least hash among its w = k - m + 1
+// m-mers, so the runs fall mostly on the ~sigma^m / w m-mers whose hashes are
+// smallest (sigma = symbol count), at most every shard. Atomics on one shard
+// serialise at ~1.2e8 per hot shard per second (fit to DNA (3, 2, 2, 4),
+// (5, 4, 3, 4) and (31, 28, m, h) for m = 4..7 at 1 G windows).
+double hot_shards(const cusbf_filter* f) {
     const double w = static_cast<double>(f->cfg.k - f->cfg.m + 1);
-    const double runs = static_cast<double>(windows) * 2.0 / (w + 1.0);
-    const double atomic_s = runs * 4.0 / 25.6e9;
+    const double mmers = std::pow(static_cast<double>(f->d.symbol_count), f->cfg.m);
+    return std::max(1.0, std::min(static_cast<double>(f->num_shards), mmers / w));
+}
+
+// Shard runs of a random sequence: 2 / (w + 1) per window.
+double expected_runs(const cusbf_filter* f, uint64_t windows) {
+    return static_cast<double>(windows) * 2.0 / static_cast<double>(f->cfg.k - f->cfg.m + 2);
+}
+
+// Seconds the hot shards add to an atomic insert's four atomic ORs per run.
+double contention_seconds(const cusbf_filter* f, double runs) {
+    return runs * 4.0 / (1.2e8 * hot_shards(f));
+}
+
+// Atomic insert: four memory-side atomic ORs per run at ~25.6 G/s, plus the
+// hot shards' serialisation.
+double atomic_seconds(const cusbf_filter* f, double runs) {
+    return runs * 4.0 / 25.6e9 + contention_seconds(f, runs);
+}
+
+// Sorted insert: ~45 ps of record traffic per run and, in each of its
+// `chunks` sorted inserts, a load and store of every slice it touches, plus
+// about a dozen launches.
+double sorted_seconds(double runs, const sbf::SortedLayout& l, uint64_t chunks) {
     const double slices = static_cast<double>(l.num_slices);
     const double n = static_cast<double>(chunks);
     const double touched = slices * (1.0 - std::exp(-runs / n / slices));
     const double slice_bytes = 32.0 * static_cast<double>(1u << l.slice_bits);
-    const double sorted_s = runs * 45e-12 + n * (touched * slice_bytes * 2.0 / 4.0e12 + 80e-6);
+    return runs * 45e-12 + n * (touched * slice_bytes * 2.0 / 4.0e12 + 80e-6);
+}
+
+bool sorted_pays(const cusbf_filter* f, uint64_t windows, const sbf::SortedLayout& l, uint64_t chunks) {
+    const double runs = expected_runs(f, windows);
+    return sorted_seconds(runs, l, chunks) < 0.8 * atomic_seconds(f, runs);
+}
+
+// Generic kernels. Where the atomic insert uses direct tiles both paths do the
+// same per-window work and sorted_pays decides. Where it uses hashed tiles
+// (~25 ps per window, atomics included), the sorted insert's direct tiles cost
+// ~0.8 ps per m-mer and ~0.3 ps per s-mer hash of a window, so the sorted path
+// pays only where hot shards slow the atomics by more than that.
+bool generic_sorted_pays(const cusbf_filter* f, uint64_t windows, const sbf::SortedLayout& l) {
+    if (!sbf::generic_hashed_tiles(f->kc)) return sorted_pays(f, windows, l, 1);
+    const double n = static_cast<double>(windows);
+    const double runs = expected_runs(f, windows);
+    const double direct = 0.8e-12 * static_cast<double>(f->cfg.k - f->cfg.m + 1) +
+                          0.3e-12 * static_cast<double>(f->cfg.k - f->cfg.s + 1) * f->cfg.hash_count;
+    const double atomic_s = n * 25e-12 + contention_seconds(f, runs);
+    const double sorted_s = n * direct + sorted_seconds(runs, l, 1);
     return sorted_s < 0.8 * atomic_s;
 }
 
@@ -605,10 +650,7 @@ int launch_generic_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStre
         if (regions * cap < (1ull << 31)) {
             l = sbf::sorted::sorted_layout(static_cast<uint32_t>(regions), static_cast<uint32_t>(cap),
                                            f->num_shards, f->slice_bits);
-            // AUTO: only where the atomic insert also uses direct tiles; the sorted
-            // build's direct tiles can cost more than the atomics they save
-            use = (forced || (!sbf::generic_hashed_tiles(f->kc) && sorted_pays(f, windows, l, 1))) &&
-                  workspace_fits(f, l.bytes);
+            use = (forced || generic_sorted_pays(f, windows, l)) && workspace_fits(f, l.bytes);
         }
         if (forced && !use) {
             return fail(CUSBF_ERR_RESOURCE, "sorted insert workspace does not fit its limit");

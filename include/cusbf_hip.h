@@ -256,7 +256,8 @@ const char* cusbf_kernel_family(const cusbf_filter* f);
  * on the generic kernels; plain inserts on the generic kernels (any alphabet
  * or configuration without lane-strip kernels) take SORTED as one sorted insert
  * (no window ranges) when forced, and under AUTO when the cost model prefers it
- * and the generic kernels use direct tiles for the configuration.
+ * (it weighs the sorted build's direct tiles against the atomic insert's hashed
+ * tiles where the generic kernels hash whole tiles).
  * Sorted inserts on one filter share its workspace: one on another stream
  * waits for the previous one (stream-ordered, no host wait).
  */
