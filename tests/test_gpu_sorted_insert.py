@@ -176,6 +176,29 @@ def test_sorted_generic_kernels(gpu, cfg_t):
     assert (g.words() == w2).all()
 
 
+@pytest.mark.parametrize("cfg_t,want", [((31, 28, 5, 4), "sorted"), ((31, 28, 7, 4), "atomic"),
+                                         ((31, 8, 16, 4), "atomic")], ids=["m5", "m7", "m16"])
+def test_auto_weighs_hot_shards_on_generic_kernels(gpu, cfg_t, want):
+    """AUTO on configurations whose atomic insert uses the generic kernels' hashed tiles.
+    A short minimizer (m = 5: 4^5 m-mers, most minimizers among the ~38 with the smallest
+    hashes) puts the runs on a few shards whose atomics serialise, so the sorted insert
+    (direct tiles) pays; at m = 7 or m = 16 the atomics are not slowed enough to pay for
+    direct tiles. The words equal the oracle's either way."""
+    import cusbf_amd as cs
+
+    k, s, m, h = cfg_t
+    seq = random_dna(20_000_000, k * m + h)
+    cfg = make_config(k, s, m, h)
+    f = cs.Filter(cfg, 1 << 28)
+    assert f.kernel_family() == "generic"
+    f.insert_sequence(seq)
+    assert f.insert_path().last_path == (cs.INSERT_SORTED if want == "sorted" else cs.INSERT_ATOMIC)
+    o = make_oracle(k, s, m, h, "dna")
+    w = o.new_filter(f.num_shards())
+    o.mt_insert(seq, w)
+    assert (f.words() == w).all()
+
+
 RANGED = [
     (31, 28, 16, 4, "dna"),
     (10, 8, 6, 4, "protein"),
