@@ -630,8 +630,11 @@ int launch_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t str
 // generic kernels' waves take 64 windows of a block tile at a time, so a region
 // reserves for its wave's share of 64-window tiles, each cut into runs at its
 // edges as well.
-int launch_generic_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
-    const uint64_t windows = in.num_symbols < f->cfg.k ? 0 : in.num_symbols - f->cfg.k + 1;
+// Decides the path of a generic-kernel insert over `windows` windows (plain or
+// batched: the records of a batch lie back to back), as one sorted insert;
+// *layout is filled when the sorted path is taken.
+int choose_generic_sorted(const cusbf_filter* f, uint64_t windows, hipStream_t stream,
+                          sbf::SortedLayout* layout, bool* use_out) {
     bool use = false;
     sbf::SortedLayout l;
     if (windows != 0 && f->insert_path != CUSBF_INSERT_ATOMIC && !capturing(stream)) {
@@ -656,6 +659,17 @@ int launch_generic_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStre
             return fail(CUSBF_ERR_RESOURCE, "sorted insert workspace does not fit its limit");
         }
     }
+    *layout = l;
+    *use_out = use;
+    return CUSBF_OK;
+}
+
+int launch_generic_insert(cusbf_filter* f, const sbf::SequenceInput& in, hipStream_t stream) {
+    const uint64_t windows = in.num_symbols < f->cfg.k ? 0 : in.num_symbols - f->cfg.k + 1;
+    bool use = false;
+    sbf::SortedLayout l;
+    const int chosen = choose_generic_sorted(f, windows, stream, &l, &use);
+    if (chosen != CUSBF_OK) return chosen;
     if (!use) {
         const hipError_t e = sbf::generic_insert(f->kc, in, view(f), nullptr, stream);
         if (e != hipSuccess) return fail_hip(e, __FILE__, __LINE__);
@@ -1000,6 +1014,10 @@ int enqueue_batched(const cusbf_filter* f, uint32_t op, const char* d_symbols,
                     ? f->strip->batched_insert[variant](f->kc, in, view(f), shape, nullptr, stream)
                     : f->strip->batched_contains[variant](f->kc, in, view(f), shape, d_output, stream);
         }
+    } else if (op == CUSBF_BATCH_INSERT && sorted) {
+        e = enqueue_sorted(f, *sorted, sorted_base, stream, [&](const sbf::RecordSink& sink) {
+            return sbf::generic_insert(f->kc, in, view(f), &sink, stream);
+        });
     } else {
         e = op == CUSBF_BATCH_INSERT ? sbf::generic_insert(f->kc, in, view(f), nullptr, stream)
                                      : sbf::generic_contains(f->kc, in, view(f), d_output, stream);
@@ -1772,11 +1790,11 @@ int cusbf_insert_batched_async(cusbf_filter* f, const char* d_symbols, uint64_t 
     sbf::SortedLayout l;
     bool sorted = false;
     f->last_path = CUSBF_INSERT_ATOMIC;
-    if (batched_strip(f)) {
-        rc = choose_batched_sorted(f, num_symbols, ts, as_stream(stream), &l, &sorted);
-        if (rc == CUSBF_OK && sorted) rc = begin_sorted(f, l.bytes, as_stream(stream));
-        if (rc != CUSBF_OK) return rc;
-    }
+    rc = batched_strip(f)
+             ? choose_batched_sorted(f, num_symbols, ts, as_stream(stream), &l, &sorted)
+             : choose_generic_sorted(f, batched_windows(f, num_symbols), as_stream(stream), &l, &sorted);
+    if (rc == CUSBF_OK && sorted) rc = begin_sorted(f, l.bytes, as_stream(stream));
+    if (rc != CUSBF_OK) return rc;
     const cusbf_batch_tuning& t =
         sorted ? ts : tuning_for(f, CUSBF_BATCH_INSERT, num_symbols, num_records);
     rc = order_insert(f, as_stream(stream));

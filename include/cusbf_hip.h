@@ -252,10 +252,11 @@ const char* cusbf_kernel_family(const cusbf_filter* f);
  * reservation, and beyond the 256 LDS run slots of a wave tile, still use
  * atomics. AUTO (the default) takes SORTED when a cost model says it is faster
  * and the workspace fits in half the free device memory, else ATOMIC. Inserts
- * recorded into a stream capture always use ATOMIC, and so do batched inserts
- * on the generic kernels; plain inserts on the generic kernels (any alphabet
- * or configuration without lane-strip kernels) take SORTED as one sorted insert
- * (no window ranges) when forced, and under AUTO when the cost model prefers it
+ * recorded into a stream capture always use ATOMIC, and so do batch plans on
+ * the generic kernels; plain and batched inserts on the generic kernels (any
+ * alphabet or configuration without lane-strip kernels) take SORTED as one
+ * sorted insert (no window ranges) when forced, and under AUTO when the cost
+ * model prefers it
  * (it weighs the sorted build's direct tiles against the atomic insert's hashed
  * tiles where the generic kernels hash whole tiles).
  * Sorted inserts on one filter share its workspace: one on another stream

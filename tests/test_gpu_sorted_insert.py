@@ -199,6 +199,47 @@ def test_auto_weighs_hot_shards_on_generic_kernels(gpu, cfg_t, want):
     assert (f.words() == w).all()
 
 
+@pytest.mark.parametrize("cfg_t", [(7, 5, 4, 8, "triplet"), (31, 8, 16, 4, "dna")],
+                         ids=["triplet-k7", "dna-s8"])
+def test_sorted_generic_batched(gpu, cfg_t):
+    """Batched inserts (filter.cuh:338-364 record batches, records back to back with
+    offsets) on the generic kernels take the sorted path: forced, and under AUTO where the
+    cost model prefers it; no window crosses a record start. Words equal the atomic
+    batched insert's and the oracle's over the records one by one."""
+    import torch
+
+    import cusbf_amd as cs
+    from cusbf_amd import batched as bt
+
+    k, s, m, h, alphabet = cfg_t
+    width = 3 if alphabet == "triplet" else 1
+    rng = np.random.default_rng(k + m)
+    lens = rng.integers(0, 400, 30_000)
+    lens[:3] = [0, k - 1, k]
+    data = random_dna(int(lens.sum()) * width, 77, n_rate=0.002)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    d_data = torch.frombuffer(bytearray(data), dtype=torch.uint8).cuda()
+    d_off = torch.from_numpy(off).cuda()
+    cfg = make_config(k, s, m, h, alphabet)
+    o = make_oracle(k, s, m, h, alphabet)
+    filters = []
+    for path in (cs.INSERT_SORTED, cs.INSERT_ATOMIC, cs.INSERT_AUTO):
+        f = cs.Filter(cfg, 1 << 28)
+        f.set_insert_path(path)
+        bt.insert_batched_async(f, d_data, d_off)
+        bt.batched_status(f)
+        filters.append(f)
+    assert filters[0].insert_path().last_path == cs.INSERT_SORTED
+    assert filters[1].insert_path().last_path == cs.INSERT_ATOMIC
+    w = o.new_filter(filters[0].num_shards())
+    for i in range(lens.size):
+        rec = data[off[i] * width:off[i + 1] * width]
+        if lens[i] >= k:
+            o.insert(rec, w)
+    for f in filters:
+        assert (f.words() == w).all()
+
+
 RANGED = [
     (31, 28, 16, 4, "dna"),
     (10, 8, 6, 4, "protein"),
